@@ -1,0 +1,185 @@
+/*
+ * scmc.h -- C-ABI of the MI355X speckle Monte Carlo core (libscmc.so).
+ *
+ * This is the drop-in boundary for the reference's engines.  In the
+ * reference, scmc.m calls the engines by function name with positional
+ * arguments (scmc.m:333,339,351-354,359-362 for MCcov; scmc.m:388,394,
+ * 404-406,412-414 for MCfield); a MEX file of the same name placed beside
+ * MCcov.m / MCfield.m shadows the .m file, and the MEX gateways
+ * (mcspeckle_amd/mex/) translate mxArrays into the plain structs below.
+ *
+ * Conventions
+ *  - Plain C types only.  Arrays are column-major like MATLAB.
+ *  - Directions/positions are dim x n (dim = 2 or 3); the 1 x n angle form
+ *    of MCcov.m:56-70 is converted by the caller (see scmc_angles_to_dirs).
+ *  - Complex outputs are split (re, im) when the imaginary pointer is
+ *    non-NULL (R2017b mxGetPr/mxGetPi) and interleaved (re, im, re, im, ...)
+ *    in the real pointer when the imaginary pointer is NULL (R2018a+).
+ *  - Every entry point returns 0 on success or a negative SCMC_E* code;
+ *    scmc_last_error() returns the message (thread-local).  No entry point
+ *    throws or aborts across the boundary.
+ *  - Random numbers: Philox4x32-10, key = seed, counter = (vertex, group,
+ *    path id); path ids are [path_offset, path_offset + max_itr), so shards
+ *    of one run can be computed by different processes or GPUs and summed.
+ */
+#ifndef SCMC_H
+#define SCMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCMC_ABI_VERSION 1
+
+enum scmc_status {
+    SCMC_OK = 0,
+    SCMC_E_ARG = -1,      /* invalid argument (reference error messages)   */
+    SCMC_E_DEVICE = -2,   /* HIP runtime / no device                        */
+    SCMC_E_NOMEM = -3,    /* device allocation failed / over HBM budget     */
+    SCMC_E_UNSUPPORTED = -4,
+    SCMC_E_CANCELLED = -5,
+    SCMC_E_BUSY = -6,     /* request not finished (scmc_poll)               */
+    SCMC_E_NOTFOUND = -7
+};
+
+enum scmc_precision {
+    SCMC_FP64 = 0,   /* FP64 everywhere (tracer, NEE, covariance on FP64 MFMA)      */
+    SCMC_MIXED = 1   /* FP64 tracer + phase, FP32 NEE, FP16 hi/lo MFMA, FP32 acc      */
+};
+
+enum scmc_sct_type { SCMC_ISOTROPIC = 1, SCMC_TABULATED = 2, SCMC_HG = 3 };
+
+/* ampfunc / ampfunc0 (MCcov.m:31-35): g for HG, a table for tabulated
+ * (interface/scatter/tabulatedAmplitudeScatter.m:89-99: evalAmp, sampleIcdf). */
+typedef struct {
+    int32_t type;            /* scmc_sct_type                                  */
+    double g;                /* HG anisotropy                                  */
+    int32_t n_amp;           /* tabulated: numel(evalAmp)                      */
+    const double *amp_re;    /* evalAmp real part                              */
+    const double *amp_im;    /* evalAmp imaginary part (may be NULL = real)    */
+    int32_t n_icdf;          /* tabulated: numel(sampleIcdf)                   */
+    const double *icdf;      /* sampleIcdf                                     */
+} scmc_phase;
+
+/* The positional arguments of MCcov.m:1 / MCfield.m:1 plus the RNG key. */
+typedef struct {
+    int32_t dim;             /* 2 or 3 (numel(box_min))                        */
+    double sigt;             /* extinction coefficient                         */
+    double albedo;
+    double lambda;
+    double box_min[3];
+    double box_max[3];
+    int32_t nl, nv;
+    const double *l;         /* dim x nl : light directions or positions       */
+    const double *v;         /* dim x nv : view directions or positions        */
+    int32_t is_ff_l, is_ff_v;
+    int32_t do_cbs;
+    int32_t smp_flg;         /* 1 uniform, 2 exponential first scatter         */
+    scmc_phase amp;          /* sct_type + ampfunc                             */
+    int32_t has_amp0;        /* ampfunc0 given                                 */
+    scmc_phase amp0;         /* type field ignored: sct_type is amp.type       */
+    int32_t has_lmean0;      /* MCfield's optional lmean0                      */
+    double lmean0[3];
+    uint64_t seed;
+    uint64_t path_offset;    /* first Philox path id                           */
+} scmc_problem;
+
+/* Per-solve statistics returned by the engines (may be passed as NULL). */
+typedef struct {
+    int64_t paths;
+    int64_t vertices;        /* total scattering vertices K                    */
+    int64_t chunks;
+    double ms_trace;         /* tracer (count + emit)                          */
+    double ms_nee;           /* next-event estimation factors                  */
+    double ms_accum;         /* covariance update / field accumulation         */
+    double ms_total;         /* wall time of the call                          */
+    double fp16_scale_log2;  /* mixed: last chunk's power-of-two scale         */
+} scmc_stats;
+
+/* Replayable path set (vertex-major), the format of the oracle's sampler and
+ * of scmc_trace.  offset has n_paths + 1 entries. */
+typedef struct {
+    int64_t n_paths, n_vertices;
+    const int64_t *offset;
+    const double *x1, *px, *w1, *w0p_re, *w0p_im;     /* per path  */
+    const double *x, *ow, *weight, *xi;               /* per vertex */
+} scmc_pathset;
+
+/* ---- engines ---------------------------------------------------------- */
+
+/* [Ms,Mm] = MCcov(...) -- code/MCcov.m:1-299.  Outputs nv x nv x nl x nl.
+ * Normalised exactly as MCcov.m:292-299 (CBS halving, V*sigt/maxItr). */
+int scmc_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision,
+               double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats);
+
+/* u = MCfield(...) -- code/MCfield.m:1-254.  Output nv x nl, normalised as
+ * MCfield.m:252-254. */
+int scmc_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision,
+                 double *u_re, double *u_im, scmc_stats *stats);
+
+/* u = MCfieldFourierQuad(sigt,albedo,box_min,box_max,l,v_max,v_stp,maxItr,
+ * lambda,doCBS,smpFlg,lmean0,vsign) -- code/MCfieldFourierQuad.m:1-303.
+ * 3-D only.  pr supplies sigt, albedo, box, l (3 x nl), lambda, do_cbs,
+ * smp_flg, lmean0 (has_lmean0), seed; pr->amp selects the scattering
+ * function of the walk (isotropic reproduces the reference; HG/tabulated are
+ * an extension).  Output nv0 x nv0 x nl with nv0 from scmc_quad_grid. */
+int scmc_mcfield_fourier_quad(const scmc_problem *pr, double v_max, double v_stp, int64_t max_itr,
+                              double vsign, int32_t precision, double *u_re, double *u_im,
+                              scmc_stats *stats);
+/* Grid sizes of MCfieldFourierQuad.m:80-121 for the given arguments. */
+int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp,
+                   double lambda, int32_t *nv0, int32_t *nx, int32_t *nz);
+
+/* ---- replay / export (path-set parity harness) ------------------------ */
+
+/* Counts vertices of paths [path_offset, path_offset + n_paths) and, when
+ * the output arrays are non-NULL, writes the path set sampled on the GPU.
+ * *n_vertices receives the total.  Arrays sized as scmc_pathset. */
+int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int64_t *offset,
+               double *x1, double *px, double *w1, double *w0p_re, double *w0p_im,
+               double *x, double *ow, double *weight, double *xi);
+/* MCcov contributions (MCcov.m:181-264) of a given path set, normalised by
+ * max_itr; the GPU covariance kernels with the tracer bypassed. */
+int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t max_itr, int32_t precision,
+                      double *ms_re, double *ms_im, double *mm_re, double *mm_im);
+/* MCfield contributions (MCfield.m:194-222) of a given path set. */
+int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t max_itr, int32_t precision,
+                        double *u_re, double *u_im);
+
+/* ---- serving: device-resident continuous batching ---------------------- */
+
+typedef struct {
+    int32_t device;          /* HIP device ordinal                             */
+    int32_t max_active;      /* max requests in flight                          */
+    double hbm_budget_gb;    /* admission budget for request buffers            */
+    int64_t paths_per_slice; /* paths per request per scheduling round (0=auto) */
+} scmc_server_config;
+
+int scmc_server_start(const scmc_server_config *cfg);
+int scmc_server_stop(void);
+/* Submit an MCcov request (mixed or FP64).  The problem's arrays are copied
+ * before return.  *request_id receives the handle. */
+int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id);
+/* 0 when finished (results copied out, handle released), SCMC_E_BUSY when
+ * still running.  Output layout as scmc_mccov; timing in *stats (ms_total =
+ * submit-to-finish latency). */
+int scmc_poll(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, double *mm_im,
+              scmc_stats *stats);
+int scmc_wait(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, double *mm_im,
+              scmc_stats *stats);
+int scmc_cancel(int64_t request_id);
+/* Bytes of device memory a request would hold while admitted. */
+int64_t scmc_request_footprint(int32_t nv, int32_t nl, int32_t precision);
+
+/* ---- misc -------------------------------------------------------------- */
+
+const char *scmc_last_error(void);
+const char *scmc_version(void);
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int scmc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

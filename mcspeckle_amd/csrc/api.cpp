@@ -1,0 +1,670 @@
+// api.cpp -- C-ABI of libscmc.so (include/scmc.h): argument checks, device
+// buffers and the chunked solve pipeline
+//     trace(count) -> scan -> trace(emit) -> NEE factors -> MFMA update
+// for MCcov, and trace -> coefficients -> field accumulation for MCfield.
+#include "kernels.h"
+#include "scmc.h"
+#include "scmc_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace scmc {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+bool hip_ok(hipError_t e, const char *what, int *rc)
+{
+    if (e == hipSuccess) return true;
+    *rc = set_error(SCMC_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+// ------------------------------------------------------------ buffers
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+bool DevBuf::ensure(size_t b, int *rc)
+{
+    if (b <= bytes && p) return true;
+    release();
+    size_t want = b ? b : 16;
+    want = (want + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        *rc = set_error(SCMC_E_NOMEM, "device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+        return false;
+    }
+    bytes = want;
+    return true;
+}
+
+int ensure_device(int *dev_out)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return set_error(SCMC_E_DEVICE, "no HIP device visible");
+    int dev = 0;
+    const char *env = getenv("SCMC_DEVICE");
+    if (env) dev = atoi(env);
+    if (dev < 0 || dev >= n) dev = 0;
+    e = hipSetDevice(dev);
+    if (e != hipSuccess) return set_error(SCMC_E_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    if (dev_out) *dev_out = dev;
+    return SCMC_OK;
+}
+
+// ------------------------------------------------------ validation
+static int check_phase(const scmc_phase &ph, const char *name, bool needs_type)
+{
+    if (needs_type && (ph.type < 1 || ph.type > 3)) return set_error(SCMC_E_ARG, "Invalid %s scattering type %d", name, ph.type);
+    if (ph.type == SCMC_TABULATED) {
+        if (!ph.amp_re || ph.n_amp < 2) return set_error(SCMC_E_ARG, "%s: tabulated evalAmp missing", name);
+        if (!ph.icdf || ph.n_icdf < 2) return set_error(SCMC_E_ARG, "%s: tabulated sampleIcdf missing", name);
+    }
+    if (ph.type == SCMC_HG && !(ph.g >= -1 && ph.g <= 1)) return set_error(SCMC_E_ARG, "Invalid %s g input", name);
+    return SCMC_OK;
+}
+
+int validate_problem(const scmc_problem *pr, int64_t max_itr, bool need_views)
+{
+    if (!pr) return set_error(SCMC_E_ARG, "null problem");
+    if (pr->dim != 2 && pr->dim != 3) return set_error(SCMC_E_ARG, "Invalid box size");   // MCcov.m:48-51
+    for (int j = 0; j < pr->dim; ++j)
+        if (!(pr->box_max[j] >= pr->box_min[j])) return set_error(SCMC_E_ARG, "Invalid box size");
+    if (pr->nl < 1 || !pr->l) return set_error(SCMC_E_ARG, "Invalid light source input");   // MCcov.m:60
+    if (need_views && (pr->nv < 1 || !pr->v)) return set_error(SCMC_E_ARG, "Invalid view source input");   // MCcov.m:68
+    if (!(pr->sigt > 0)) return set_error(SCMC_E_ARG, "sigt must be positive");
+    if (!(pr->albedo >= 0 && pr->albedo <= 1)) return set_error(SCMC_E_ARG, "Invalid albedo input");
+    if (!(pr->lambda > 0)) return set_error(SCMC_E_ARG, "lambda must be positive");
+    if (pr->smp_flg != 1 && pr->smp_flg != 2) return set_error(SCMC_E_ARG, "smpFlg must be 1 or 2");
+    if (max_itr < 1) return set_error(SCMC_E_ARG, "Invalid photons number");   // scmc.m:82-84
+    int rc = check_phase(pr->amp, "ampfunc", true);
+    if (rc) return rc;
+    if (pr->has_amp0) {
+        scmc_phase a0 = pr->amp0;
+        a0.type = pr->amp.type;
+        rc = check_phase(a0, "ampfunc0", false);
+        if (rc) return rc;
+    }
+    return SCMC_OK;
+}
+
+// ---------------------------------------------------- device problem
+int ProblemState::setup(const scmc_problem *pr, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    const int dim = pr->dim;
+    DevProblem &P = dp;
+    memset(&P, 0, sizeof P);
+    P.dim = dim;
+    P.sigt = pr->sigt;
+    P.albedo = pr->albedo;
+    P.lambda = pr->lambda;
+    P.k = M_PI * 2.0 / pr->lambda;
+    P.inv_lambda = 1.0 / pr->lambda;
+    for (int j = 0; j < 3; ++j) {
+        P.bmin[j] = j < dim ? pr->box_min[j] : 0.0;
+        P.bmax[j] = j < dim ? pr->box_max[j] : 0.0;
+    }
+    P.nl = pr->nl;
+    P.nv = pr->nv;
+    P.is_ff_l = pr->is_ff_l ? 1 : 0;
+    P.is_ff_v = pr->is_ff_v ? 1 : 0;
+    P.do_cbs = pr->do_cbs ? 1 : 0;
+    P.smp_flg = pr->smp_flg;
+    P.seed = pr->seed;
+    P.path_offset = pr->path_offset;
+    P.w0p_iso = 1.0 / sqrt(pow(2.0, dim - 1) * M_PI);
+    P.ff_sign = pr->is_ff_v ? -1.0 : 1.0;
+    // unmeanl / meanl: MCcov.m:82-83, MCfield.m:77-82
+    double um[3] = {0, 0, 0};
+    if (pr->has_lmean0) {
+        for (int j = 0; j < dim; ++j) um[j] = pr->lmean0[j];
+    } else {
+        for (int i = 0; i < pr->nl; ++i)
+            for (int j = 0; j < dim; ++j) um[j] += pr->l[(size_t)i * dim + j];
+        for (int j = 0; j < dim; ++j) um[j] /= pr->nl;
+    }
+    double ss = 0;
+    for (int j = 0; j < dim; ++j) ss += um[j] * um[j];
+    double nrm = sqrt(ss);
+    for (int j = 0; j < 3; ++j) {
+        P.unmean[j] = um[j];
+        P.meanl[j] = j < dim ? um[j] / nrm : 0.0;
+    }
+    // directions / positions
+    size_t lb = sizeof(double) * (size_t)dim * pr->nl, vb = sizeof(double) * (size_t)dim * std::max(pr->nv, 1);
+    if (!dl.ensure(lb, &rc) || !dv.ensure(vb, &rc)) return rc;
+    if (!hip_ok(hipMemcpyAsync(dl.p, pr->l, lb, hipMemcpyHostToDevice, s), "copy l", &rc)) return rc;
+    if (pr->nv > 0 && pr->v)
+        if (!hip_ok(hipMemcpyAsync(dv.p, pr->v, sizeof(double) * (size_t)dim * pr->nv, hipMemcpyHostToDevice, s),
+                    "copy v", &rc))
+            return rc;
+    P.l = (const double *)dl.p;
+    P.v = (const double *)dv.p;
+    rc = upload_phase(pr->amp, pr->amp.type, tab_amp, tab_icdf, P.amp, s);
+    if (rc) return rc;
+    P.amp.iso = P.w0p_iso;
+    P.has_amp0 = pr->has_amp0 ? 1 : 0;
+    if (pr->has_amp0) {
+        rc = upload_phase(pr->amp0, pr->amp.type, tab_amp0, tab_icdf0, P.amp0, s);
+        if (rc) return rc;
+        P.amp0.iso = P.w0p_iso;
+    }
+    // hold host copies alive until the async copies complete
+    return hip_ok(hipStreamSynchronize(s), "setup sync", &rc) ? SCMC_OK : rc;
+}
+
+int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevBuf &icdf, DevPhase &out, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    out.type = type;
+    out.g = ph.g;
+    out.n_amp = 0;
+    out.amp = nullptr;
+    out.n_icdf = 0;
+    out.icdf = nullptr;
+    if (type != SCMC_TABULATED) return SCMC_OK;
+    std::vector<double> a((size_t)ph.n_amp * 2);
+    for (int i = 0; i < ph.n_amp; ++i) {
+        a[2 * i] = ph.amp_re[i];
+        a[2 * i + 1] = ph.amp_im ? ph.amp_im[i] : 0.0;
+    }
+    if (!amp.ensure(a.size() * sizeof(double), &rc) || !icdf.ensure(sizeof(double) * ph.n_icdf, &rc)) return rc;
+    if (!hip_ok(hipMemcpy(amp.p, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice), "copy evalAmp", &rc))
+        return rc;
+    if (!hip_ok(hipMemcpy(icdf.p, ph.icdf, sizeof(double) * ph.n_icdf, hipMemcpyHostToDevice), "copy sampleIcdf", &rc))
+        return rc;
+    out.n_amp = ph.n_amp;
+    out.amp = (const double2 *)amp.p;
+    out.n_icdf = ph.n_icdf;
+    out.icdf = (const double *)icdf.p;
+    (void)s;
+    return SCMC_OK;
+}
+
+double box_volume(const scmc_problem *pr)
+{
+    double V = 1;
+    for (int j = 0; j < pr->dim; ++j) V *= pr->box_max[j] - pr->box_min[j];
+    return V;
+}
+
+// ------------------------------------------------------ traced chunk
+int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    np = n;
+    if (!counts.ensure(sizeof(int32_t) * (size_t)n, &rc) || !offsets.ensure(sizeof(int64_t) * (size_t)(n + 1), &rc) ||
+        !paths.ensure(sizeof(PathRec) * (size_t)n, &rc))
+        return rc;
+    launch_trace_count(P, p0, n, (int32_t *)counts.p, s);
+    launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, s);
+    int64_t total = 0;
+    if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)offsets.p + n, sizeof total, hipMemcpyDeviceToHost, s), "copy total",
+                &rc))
+        return rc;
+    if (!hip_ok(hipStreamSynchronize(s), "trace sync", &rc)) return rc;
+    nvert = total;
+    if (!verts.ensure(sizeof(VertexRec) * (size_t)std::max<int64_t>(total, 1), &rc) ||
+        !vpath.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), &rc))
+        return rc;
+    launch_trace_emit(P, p0, n, (const int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
+                      (int32_t *)vpath.p, s);
+    return hip_ok(hipGetLastError(), "trace launch", &rc) ? SCMC_OK : rc;
+}
+
+// Upload a host path set (SoA) as device records.
+int TraceChunk::load(const scmc_pathset *ps, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    np = ps->n_paths;
+    nvert = ps->n_vertices;
+    std::vector<PathRec> hp((size_t)np);
+    std::vector<VertexRec> hv((size_t)std::max<int64_t>(nvert, 1));
+    std::vector<int32_t> hvp((size_t)std::max<int64_t>(nvert, 1));
+    for (int64_t i = 0; i < np; ++i) {
+        PathRec &r = hp[(size_t)i];
+        for (int j = 0; j < 3; ++j) {
+            r.x1[j] = ps->x1[3 * i + j];
+            r.w1[j] = ps->w1[3 * i + j];
+        }
+        r.px = ps->px[i];
+        r.pad = 0;
+        r.w0p = make_double2(ps->w0p_re[i], ps->w0p_im ? ps->w0p_im[i] : 0.0);
+        for (int64_t k = ps->offset[i]; k < ps->offset[i + 1]; ++k) {
+            VertexRec &v = hv[(size_t)k];
+            for (int j = 0; j < 3; ++j) {
+                v.x[j] = ps->x[3 * k + j];
+                v.ow[j] = ps->ow[3 * k + j];
+            }
+            v.weight = ps->weight[k];
+            v.xi = ps->xi ? ps->xi[k] : 0.0;
+            hvp[(size_t)k] = (int32_t)i;
+        }
+    }
+    if (!offsets.ensure(sizeof(int64_t) * (size_t)(np + 1), &rc) || !paths.ensure(sizeof(PathRec) * (size_t)np, &rc) ||
+        !verts.ensure(sizeof(VertexRec) * hv.size(), &rc) || !vpath.ensure(sizeof(int32_t) * hvp.size(), &rc))
+        return rc;
+    if (!hip_ok(hipMemcpy(offsets.p, ps->offset, sizeof(int64_t) * (size_t)(np + 1), hipMemcpyHostToDevice), "copy offsets", &rc) ||
+        !hip_ok(hipMemcpy(paths.p, hp.data(), sizeof(PathRec) * (size_t)np, hipMemcpyHostToDevice), "copy paths", &rc) ||
+        !hip_ok(hipMemcpy(verts.p, hv.data(), sizeof(VertexRec) * hv.size(), hipMemcpyHostToDevice), "copy vertices", &rc) ||
+        !hip_ok(hipMemcpy(vpath.p, hvp.data(), sizeof(int32_t) * hvp.size(), hipMemcpyHostToDevice), "copy vpath", &rc))
+        return rc;
+    (void)s;
+    return SCMC_OK;
+}
+
+// ---------------------------------------------------- covariance engine
+int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    mixed = mixed_;
+    nv = ps.dp.nv;
+    nl = ps.dp.nl;
+    n = nv * nl;
+    tile = mixed ? kTileMixed : kTileF64;
+    if (!bound.ensure(16, &rc)) return rc;
+    (void)s;
+    return SCMC_OK;
+}
+
+static int pick_ksplit(int64_t k, int n, int tile)
+{
+    const int nt = (n + tile - 1) / tile;
+    const int64_t tiles = (int64_t)nt * (nt + 1) / 2;
+    // aim for >= 4 waves of workgroups over 256 CUs, >= 1024 vertices per item
+    int64_t want = (2048 + tiles - 1) / tiles;
+    int64_t maxs = std::max<int64_t>(1, k / 1024);
+    int64_t ks = std::min<int64_t>(want, maxs);
+    if (ks >= 8) ks = (ks / 8) * 8;   // keep splits aligned with the 8 XCDs
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ks, 32));
+}
+
+int CovEngine::ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    if (acc.p && ks <= ks_have) return SCMC_OK;
+    // grow: allocate a larger set of copies, copy the existing sums over
+    DevBuf nb;
+    size_t one = sizeof(double2) * (size_t)n * n;
+    if (!nb.ensure(one * ks, &rc)) return rc;
+    launch_zero(nb.p, one * ks, s);
+    if (acc.p && ks_have > 0)
+        if (!hip_ok(hipMemcpyAsync(nb.p, acc.p, one * ks_have, hipMemcpyDeviceToDevice, s), "grow acc", &rc)) return rc;
+    if (!hip_ok(hipStreamSynchronize(s), "acc sync", &rc)) return rc;
+    std::swap(acc.p, nb.p);
+    std::swap(acc.bytes, nb.bytes);
+    ks_have = ks;
+    return SCMC_OK;
+}
+
+int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee)
+{
+    int rc = SCMC_OK;
+    const DevProblem &P = ps.dp;
+    const int64_t np = tc.np, nvert = tc.nvert;
+    const int64_t k2 = nvert - np;
+    const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
+    const bool cbs = P.do_cbs != 0;
+    if (!A1.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nv, &rc) || !B1.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nl, &rc) ||
+        !SA.ensure(cs * (size_t)np * nv, &rc) || !SB.ensure(cs * (size_t)np * nl, &rc) ||
+        !Gp.ensure(cs * (size_t)np * nl, &rc) || !maxG.ensure(sizeof(float) * (size_t)np, &rc) ||
+        !maxF2.ensure(sizeof(float) * (size_t)np, &rc))
+        return rc;
+    if (cbs)
+        if (!A2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nv, &rc) ||
+            !B2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nl, &rc) || !Fp2.ensure(cs * (size_t)np * nv, &rc))
+            return rc;
+    launch_zero(bound.p, 16, s);
+    CovFactors f;
+    f.A1 = A1.p;
+    f.B1 = B1.p;
+    f.A2 = cbs ? A2.p : nullptr;
+    f.B2 = cbs ? B2.p : nullptr;
+    f.SA = SA.p;
+    f.SB = SB.p;
+    f.Gp = Gp.p;
+    f.Fp2 = cbs ? Fp2.p : nullptr;
+    f.maxG = (float *)maxG.p;
+    f.maxF2 = (float *)maxF2.p;
+    f.bound = (unsigned *)bound.p;
+    hipEvent_t e0, e1, e2;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventCreate(&e2);
+    (void)hipEventRecord(e0, s);
+    launch_nee_cov(P, np, nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p, (const int32_t *)tc.vpath.p,
+                   (const int64_t *)tc.offsets.p, f, mixed, s);
+    (void)hipEventRecord(e1, s);
+    const int ksm = pick_ksplit(k2, n, tile), kss = pick_ksplit(np, n, tile);
+    if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), s))) return rc;
+    if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), s))) return rc;
+    HerkProblem hp;
+    hp.nv = nv;
+    hp.nl = nl;
+    hp.n = n;
+    // multiple scattering
+    hp.k = k2;
+    hp.A1 = A1.p;
+    hp.B1 = B1.p;
+    hp.A2 = cbs ? A2.p : nullptr;
+    hp.B2 = cbs ? B2.p : nullptr;
+    hp.acc = (double2 *)acc_m.p;
+    hp.bound = (const unsigned *)bound.p;
+    hp.ksplit = ksm;
+    if (mixed)
+        launch_herk_mixed(hp, s);
+    else
+        launch_herk_fp64(hp, s);
+    // single scattering
+    hp.k = np;
+    hp.A1 = SA.p;
+    hp.B1 = SB.p;
+    hp.A2 = nullptr;
+    hp.B2 = nullptr;
+    hp.acc = (double2 *)acc_s.p;
+    hp.bound = (const unsigned *)bound.p + 1;
+    hp.ksplit = kss;
+    if (mixed)
+        launch_herk_mixed(hp, s);
+    else
+        launch_herk_fp64(hp, s);
+    (void)hipEventRecord(e2, s);
+    if (!hip_ok(hipGetLastError(), "covariance launch", &rc)) return rc;
+    if (!hip_ok(hipEventSynchronize(e2), "covariance sync", &rc)) return rc;
+    float t1 = 0, t2 = 0;
+    (void)hipEventElapsedTime(&t1, e0, e1);
+    (void)hipEventElapsedTime(&t2, e1, e2);
+    *ms_nee += t1;
+    *ms_accum += t2;
+    if (mixed) {
+        unsigned b = 0;
+        (void)hipMemcpy(&b, bound.p, sizeof b, hipMemcpyDeviceToHost);
+        float bf;
+        memcpy(&bf, &b, sizeof bf);
+        int ex = 0;
+        if (bf > 0) frexpf(bf, &ex);
+        last_scale_log2 = 15 - ex;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return SCMC_OK;
+}
+
+int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
+                        double *mm_re, double *mm_im, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    const size_t ne = (size_t)n * n;
+    DevBuf out, af;
+    if (!out.ensure(sizeof(double) * 2 * ne, &rc) || !af.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
+    const double scale = 1.0 / (double)max_itr * box_volume(pr) * pr->sigt;
+    launch_af_table(ps.dp, (double2 *)af.p, s);
+    double *o = (double *)out.p;
+    struct Dst {
+        double *re, *im;
+    };
+    auto emit = [&](const DevBuf &acc, int ks, double sc, const double2 *rowf, Dst d) -> int {
+        if (!d.re) return SCMC_OK;
+        if (!acc.p) {
+            launch_zero(out.p, sizeof(double) * 2 * ne, s);
+        } else {
+            launch_cov_finalize((const double2 *)acc.p, ks, nv, nl, tile, sc, rowf, o, d.im ? o + ne : nullptr, s);
+        }
+        int r2 = SCMC_OK;
+        if (d.im) {
+            if (!hip_ok(hipMemcpyAsync(d.re, o, sizeof(double) * ne, hipMemcpyDeviceToHost, s), "copy re", &r2) ||
+                !hip_ok(hipMemcpyAsync(d.im, o + ne, sizeof(double) * ne, hipMemcpyDeviceToHost, s), "copy im", &r2))
+                return r2;
+        } else {
+            if (!hip_ok(hipMemcpyAsync(d.re, o, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy out", &r2))
+                return r2;
+        }
+        return hip_ok(hipStreamSynchronize(s), "finalize sync", &r2) ? SCMC_OK : r2;
+    };
+    // MCcov.m:293-299: Mm/2 with CBS, then /maxItr*V*sigt; Ms /maxItr*V*sigt
+    if ((rc = emit(acc_s, ks_s, scale, (const double2 *)af.p, Dst{ms_re, ms_im}))) return rc;
+    if ((rc = emit(acc_m, ks_m, pr->do_cbs ? scale / 2 : scale, nullptr, Dst{mm_re, mm_im}))) return rc;
+    return SCMC_OK;
+}
+
+int64_t chunk_paths(int n, int64_t remaining, double mean_len, bool mixed)
+{
+    // vertices per chunk: large enough to fill the GPU, bounded by memory
+    double per_vertex = (double)n;   // factor + update cost proxy
+    int64_t target_vertices = (int64_t)std::max(65536.0, std::min(4.0e6, 6.0e10 / std::max(per_vertex, 1.0) / 64));
+    if (n >= 1024) target_vertices = mixed ? 400000 : 200000;
+    int64_t p = (int64_t)(target_vertices / std::max(mean_len, 1.0));
+    p = std::max<int64_t>(256, std::min<int64_t>(p, 1 << 20));
+    return std::min(p, remaining);
+}
+
+} // namespace scmc
+
+using namespace scmc;
+
+// ======================================================= C entry points
+extern "C" {
+
+const char *scmc_last_error(void) { return g_err.c_str(); }
+
+const char *scmc_version(void) { return "scmc-mi355x 0.1 (abi 1, gfx950)"; }
+
+int scmc_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static bool far_field_only(const scmc_problem *pr, int *rc)
+{
+    if (!pr->is_ff_l || !pr->is_ff_v) {
+        *rc = set_error(SCMC_E_UNSUPPORTED, "near-field sources are not supported by the GPU engines");
+        return false;
+    }
+    return true;
+}
+
+int scmc_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *ms_re, double *ms_im,
+               double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (!far_field_only(pr, &rc)) return rc;
+    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
+    if ((rc = ensure_device(nullptr))) return rc;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        CovEngine ce;
+        scmc_stats st;
+        memset(&st, 0, sizeof st);
+        const bool mixed = precision == SCMC_MIXED;
+        if (!(rc = ps.setup(pr, s)) && !(rc = ce.init(ps, mixed, s))) {
+            double mean_len = 20;
+            for (int64_t p0 = 0; p0 < max_itr && !rc;) {
+                int64_t np = chunk_paths(ce.n, max_itr - p0, mean_len, mixed);
+                auto a = std::chrono::steady_clock::now();
+                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
+                if (!hip_ok(hipStreamSynchronize(s), "trace", &rc)) break;
+                st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+                if ((rc = ce.chunk(ps, tc, s, &st.ms_accum, &st.ms_nee))) break;
+                st.paths += np;
+                st.vertices += tc.nvert;
+                st.chunks += 1;
+                mean_len = (double)st.vertices / (double)st.paths;
+                p0 += np;
+            }
+            if (!rc) rc = ce.finalize(ps, pr, max_itr, ms_re, ms_im, mm_re, mm_im, s);
+        }
+        st.fp16_scale_log2 = ce.last_scale_log2;
+        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = st;
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int scmc_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *u_re, double *u_im,
+                 scmc_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (!far_field_only(pr, &rc)) return rc;
+    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
+    if ((rc = ensure_device(nullptr))) return rc;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        FieldEngine fe;
+        scmc_stats st;
+        memset(&st, 0, sizeof st);
+        if (!(rc = ps.setup(pr, s)) && !(rc = fe.init(ps, s))) {
+            double mean_len = 20;
+            for (int64_t p0 = 0; p0 < max_itr && !rc;) {
+                int64_t np = field_chunk_paths(pr->nv, pr->nl, max_itr - p0, mean_len);
+                auto a = std::chrono::steady_clock::now();
+                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
+                if (!hip_ok(hipStreamSynchronize(s), "trace", &rc)) break;
+                st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+                if ((rc = fe.chunk(ps, tc, precision == SCMC_MIXED, s, &st.ms_accum))) break;
+                st.paths += np;
+                st.vertices += tc.nvert;
+                st.chunks += 1;
+                mean_len = (double)st.vertices / (double)st.paths;
+                p0 += np;
+            }
+            if (!rc) rc = fe.finalize(pr, max_itr, u_re, u_im, s);
+        }
+        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = st;
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int64_t *offset, double *x1, double *px,
+               double *w1, double *w0p_re, double *w0p_im, double *x, double *ow, double *weight, double *xi)
+{
+    int rc = validate_problem(pr, n_paths, false);
+    if (rc) return rc;
+    if ((rc = ensure_device(nullptr))) return rc;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        if (!(rc = ps.setup(pr, s)) && !(rc = tc.run(ps.dp, 0, n_paths, s))) {
+            if (n_vertices) *n_vertices = tc.nvert;
+            if (offset && x) {
+                std::vector<int64_t> off((size_t)n_paths + 1);
+                std::vector<PathRec> hp((size_t)n_paths);
+                std::vector<VertexRec> hv((size_t)std::max<int64_t>(tc.nvert, 1));
+                if (hip_ok(hipMemcpy(off.data(), tc.offsets.p, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost), "copy", &rc) &&
+                    hip_ok(hipMemcpy(hp.data(), tc.paths.p, sizeof(PathRec) * hp.size(), hipMemcpyDeviceToHost), "copy", &rc) &&
+                    hip_ok(hipMemcpy(hv.data(), tc.verts.p, sizeof(VertexRec) * (size_t)tc.nvert, hipMemcpyDeviceToHost), "copy", &rc)) {
+                    for (int64_t i = 0; i <= n_paths; ++i) offset[i] = off[(size_t)i];
+                    for (int64_t i = 0; i < n_paths; ++i) {
+                        for (int j = 0; j < 3; ++j) {
+                            x1[3 * i + j] = hp[(size_t)i].x1[j];
+                            w1[3 * i + j] = hp[(size_t)i].w1[j];
+                        }
+                        px[i] = hp[(size_t)i].px;
+                        w0p_re[i] = hp[(size_t)i].w0p.x;
+                        w0p_im[i] = hp[(size_t)i].w0p.y;
+                    }
+                    for (int64_t k = 0; k < tc.nvert; ++k) {
+                        for (int j = 0; j < 3; ++j) {
+                            x[3 * k + j] = hv[(size_t)k].x[j];
+                            ow[3 * k + j] = hv[(size_t)k].ow[j];
+                        }
+                        weight[k] = hv[(size_t)k].weight;
+                        xi[k] = hv[(size_t)k].xi;
+                    }
+                }
+            }
+        }
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t max_itr, int32_t precision,
+                      double *ms_re, double *ms_im, double *mm_re, double *mm_im)
+{
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (!far_field_only(pr, &rc)) return rc;
+    if (!pset || pset->n_paths < 1) return set_error(SCMC_E_ARG, "empty path set");
+    if ((rc = ensure_device(nullptr))) return rc;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        CovEngine ce;
+        double a = 0, b = 0;
+        if (!(rc = ps.setup(pr, s)) && !(rc = ce.init(ps, precision == SCMC_MIXED, s)) && !(rc = tc.load(pset, s)) &&
+            !(rc = ce.chunk(ps, tc, s, &a, &b)))
+            rc = ce.finalize(ps, pr, max_itr, ms_re, ms_im, mm_re, mm_im, s);
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t max_itr, int32_t precision,
+                        double *u_re, double *u_im)
+{
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (!far_field_only(pr, &rc)) return rc;
+    if (!pset || pset->n_paths < 1) return set_error(SCMC_E_ARG, "empty path set");
+    if ((rc = ensure_device(nullptr))) return rc;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        FieldEngine fe;
+        double a = 0;
+        if (!(rc = ps.setup(pr, s)) && !(rc = fe.init(ps, s)) && !(rc = tc.load(pset, s)) &&
+            !(rc = fe.chunk(ps, tc, precision == SCMC_MIXED, s, &a)))
+            rc = fe.finalize(pr, max_itr, u_re, u_im, s);
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+} // extern "C"

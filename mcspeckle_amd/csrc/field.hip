@@ -1,0 +1,280 @@
+// field.hip -- MCfield field accumulation (code/MCfield.m:190-222).
+//
+// The field is a sum over "view-side entries" e of
+//     u(v,l) += E(x_e, v) * a_e(v,l) * coef_e(l),   E = evalphaseatt(x, -v)
+// with three entry kinds:
+//   first vertex (pL == 1, MCfield.m:209): x = x1, a = af_ang_vl(v,l),
+//       coef = c_1 e_l0(l)
+//   vertex pL > 1 (MCfield.m:211):         x = x_n, a = a(ow . v),
+//       coef = c_n e_l0_ms(l) [/sqrt 2 with CBS]
+//   reversed path, CBS (MCfield.m:213):    x = x1, a = a(-w1 . v)/w0p,
+//       coef = sum_n c_n e_l_ms(x_n)(l) / sqrt 2
+// where c_n = sqrt(W_n/px) exp(2 pi i xi_n) (MCfield.m:218-220).  The CBS
+// term of every vertex of a path shares x1, so it is summed per path first.
+//
+// Layout: one lane per view, a workgroup per 256 views x a slice of the
+// entries; entries are staged through LDS in batches and broadcast to all
+// lanes; FP64 accumulators in registers; per-slice partial sums are reduced
+// in a fixed order (deterministic).  The mixed variant forms the phase k x.v
+// and its range reduction in FP64 and evaluates sin/cos (hardware, input in
+// revolutions), the attenuation exponential and the amplitude in FP32.
+#include "kernels.h"
+
+namespace scmc {
+
+constexpr int kFieldViews = 256;   // views per workgroup
+constexpr int kFieldBatch = 64;    // entries per LDS batch
+constexpr int kFieldNL = 4;        // lights per launch pass
+
+// Per-path / per-vertex coefficients.  Block per path, lanes over lights.
+template <int DIM>
+__global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np, const PathRec *paths,
+                                                        const VertexRec *verts, const int64_t *offsets,
+                                                        double2 *coef, double2 *coefR)
+{
+    const int64_t p = blockIdx.x;
+    if (p >= np) return;
+    const PathRec pr = paths[p];
+    const int64_t o = offsets[p], e = offsets[p + 1];
+    const double rs2 = 1 / sqrt(2.0);
+    for (int l = threadIdx.x; l < P.nl; l += blockDim.x) {
+        const double *lc = P.l + (size_t)l * DIM;
+        double2 el0 = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
+        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, lc), P.amp), pr.w0p);
+        double2 el0ms = cmul(el0, af);
+        double2 R = make_double2(0.0, 0.0);
+        for (int64_t vi = o; vi < e; ++vi) {
+            const VertexRec r = verts[vi];
+            double amp = sqrt(r.weight / pr.px);
+            double s, c;
+            sincos(2 * kPi * r.xi, &s, &c);
+            double2 cn = make_double2(amp * c, amp * s);
+            double2 t;
+            if (vi == o) {
+                t = cmul(el0, cn);
+            } else {
+                t = cmul(el0ms, cn);
+                if (P.do_cbs) {
+                    t.x *= rs2;
+                    t.y *= rs2;
+                    double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
+                    double2 el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l),
+                                      eval_amp<DIM>(dot<DIM>(mow, lc), P.amp));
+                    double2 q = cmul(el, cn);
+                    R.x += q.x * rs2;
+                    R.y += q.y * rs2;
+                }
+            }
+            coef[vi * P.nl + l] = t;
+        }
+        if (P.do_cbs) coefR[p * P.nl + l] = R;
+    }
+}
+
+struct FEntry {
+    double x[3];
+    double ad[3];     // amplitude direction: ow (vertex) or -w1 (reversed)
+    double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
+    int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
+    int pad;
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_entry(int64_t e, int64_t nvert, const PathRec *paths, const VertexRec *verts,
+                                           const int32_t *vpath, const int64_t *offsets, FEntry &f)
+{
+    if (e < nvert) {
+        const VertexRec r = verts[e];
+        const int p = vpath[e];
+        for (int j = 0; j < 3; ++j) {
+            f.x[j] = r.x[j];
+            f.ad[j] = r.ow[j];
+        }
+        f.kind = (e == offsets[p]) ? 0 : 1;
+        f.div = make_double2(1.0, 0.0);
+    } else {
+        const PathRec pr = paths[e - nvert];
+        for (int j = 0; j < 3; ++j) {
+            f.x[j] = pr.x1[j];
+            f.ad[j] = -pr.w1[j];
+        }
+        f.kind = 2;
+        double den = pr.w0p.x * pr.w0p.x + pr.w0p.y * pr.w0p.y;
+        f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
+    }
+}
+
+// HG / isotropic / tabulated amplitude in FP32 (mixed variant).
+template <int DIM>
+__device__ __forceinline__ float2 amp_f32(float c, const DevPhase &ph, float hg_k, float g2p1, float g2)
+{
+    if (ph.type == 1) return make_float2((float)ph.iso, 0.f);
+    if (ph.type == 3) {
+        float t = fmaxf(g2p1 - g2 * c, 1e-30f);   // 1 + g^2 - 2 g c
+        float a = (DIM == 3) ? hg_k * __builtin_amdgcn_exp2f(-0.75f * __builtin_amdgcn_logf(t))
+                             : hg_k * __builtin_amdgcn_rsqf(t);
+        return make_float2(a, 0.f);
+    }
+    float cc = fminf(fmaxf(c, -1.f), 1.f);
+    float ang = acosf(cc);
+    int n = ph.n_amp;
+    float scale = (DIM == 2) ? (float)((n - 1) / (2 * kPi)) : (float)((n - 1) / kPi);
+    int idx = (int)rintf(ang * scale);
+    idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
+    double2 a = ph.amp[idx];
+    return make_float2((float)a.x, (float)a.y);
+}
+
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
+                                                            int nsplit, int l0, int nlp, const PathRec *paths,
+                                                            const VertexRec *verts, const int32_t *vpath,
+                                                            const int64_t *offsets, const double2 *coef,
+                                                            const double2 *coefR, const double2 *aftab,
+                                                            double2 *part)
+{
+    __shared__ FEntry se[kFieldBatch];
+    __shared__ double2 sc[kFieldBatch][kFieldNL];
+    const int v = blockIdx.x * kFieldViews + threadIdx.x;
+    const bool vok = v < P.nv;
+    const int split = blockIdx.y;
+    const int64_t e0 = nent * split / nsplit, e1 = nent * (split + 1) / nsplit;
+
+    // per-view constants
+    double vd[3] = {0, 0, 0};
+    if (vok)
+        for (int j = 0; j < DIM; ++j) vd[j] = P.v[(size_t)v * DIM + j];
+    double mv[3] = {-vd[0], -vd[1], -vd[2]};   // ff_sign * v (far field)
+    float vf[3] = {(float)vd[0], (float)vd[1], (float)vd[2]};
+    float inv[3] = {0, 0, 0};
+    bool up[3] = {false, false, false};   // nd_j > 0 -> distance to the max face
+    for (int j = 0; j < DIM; ++j) {
+        double nd = -mv[j];
+        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
+        up[j] = nd > 0;
+    }
+    double2 afv[kFieldNL];
+    for (int q = 0; q < kFieldNL; ++q)
+        afv[q] = (vok && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)] : make_double2(0.0, 0.0);
+    const float g = (float)P.amp.g;
+    const float hg_k = (float)sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
+    const float g2p1 = 1.f + g * g, g2 = 2.f * g;
+    const float hs = (float)(P.sigt * 0.5);
+
+    double2 acc[kFieldNL];
+    for (int q = 0; q < kFieldNL; ++q) acc[q] = make_double2(0.0, 0.0);
+
+    for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
+        const int cnt = (int)((e1 - eb) < kFieldBatch ? (e1 - eb) : kFieldBatch);
+        __syncthreads();
+        if (threadIdx.x < cnt) load_entry<DIM>(eb + threadIdx.x, nvert, paths, verts, vpath, offsets, se[threadIdx.x]);
+        for (int t = threadIdx.x; t < cnt * kFieldNL; t += blockDim.x) {
+            const int ei = t / kFieldNL, q = t % kFieldNL;
+            const int64_t e = eb + ei;
+            double2 c = make_double2(0.0, 0.0);
+            if (q < nlp)
+                c = (e < nvert) ? coef[e * P.nl + l0 + q] : coefR[(e - nvert) * P.nl + l0 + q];
+            sc[ei][q] = c;
+        }
+        __syncthreads();
+        if (!vok) continue;
+        for (int ei = 0; ei < cnt; ++ei) {
+            const FEntry &f = se[ei];
+            double2 ev;
+            float2 am = make_float2(1.f, 0.f);
+            if (MIXED) {
+                double pv = dot<DIM>(f.x, mv);
+                double cyc = pv * P.inv_lambda;
+                float fr = (float)(cyc - rint(cyc));
+                float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
+                float best = INFINITY;
+                for (int j = 0; j < DIM; ++j) {
+                    float dist = (float)(up[j] ? (P.bmax[j] - f.x[j]) : (f.x[j] - P.bmin[j]));
+                    best = fminf(best, dist * inv[j]);
+                }
+                float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
+                ev = make_double2(mag * cs, mag * sn);
+                if (f.kind != 0) {
+                    float c = (float)f.ad[0] * vf[0] + (float)f.ad[1] * vf[1] + (DIM == 3 ? (float)f.ad[2] * vf[2] : 0.f);
+                    am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
+                }
+            } else {
+                ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+                if (f.kind != 0) {
+                    double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)v * DIM), P.amp);
+                    am = make_float2(0.f, 0.f);
+                    ev = cmul(ev, a);
+                }
+            }
+            if (MIXED && f.kind != 0) ev = cmul(ev, make_double2(am.x, am.y));
+            if (f.kind == 2) ev = cmul(ev, f.div);
+            for (int q = 0; q < kFieldNL; ++q) {
+                double2 t = (f.kind == 0) ? cmul(ev, afv[q]) : ev;
+                double2 c = sc[ei][q];
+                acc[q].x += t.x * c.x - t.y * c.y;
+                acc[q].y += t.x * c.y + t.y * c.x;
+            }
+        }
+    }
+    if (vok)
+        for (int q = 0; q < nlp; ++q) part[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+}
+
+__global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, double2 *acc)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 s = acc[i];
+    for (int k = 0; k < nsplit; ++k) {
+        double2 t = part[(size_t)k * n + i];
+        s.x += t.x;
+        s.y += t.y;
+    }
+    acc[i] = s;
+}
+
+template <int DIM>
+static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths,
+                           const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work,
+                           double2 *acc, bool mixed, hipStream_t s)
+{
+    // work layout: coef[nvert*nl] | coefR[np*nl] | aftab[nv*nl] | part[nsplit*nv*nl]
+    double2 *coef = work;
+    double2 *coefR = coef + (size_t)nvert * P.nl;
+    double2 *aftab = coefR + (size_t)np * P.nl;
+    double2 *part = aftab + (size_t)P.nv * P.nl;
+    hipLaunchKernelGGL(field_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts, offsets, coef,
+                       coefR);
+    launch_af_table(P, aftab, s);
+    const int64_t nent = nvert + (P.do_cbs ? np : 0);
+    const int vt = (P.nv + kFieldViews - 1) / kFieldViews;
+    int nsplit = (2048 + vt - 1) / vt;
+    const int64_t maxsplit = (nent + kFieldBatch - 1) / kFieldBatch;
+    if (nsplit > maxsplit) nsplit = (int)maxsplit;
+    if (nsplit > 64) nsplit = 64;
+    if (nsplit < 1) nsplit = 1;
+    for (int l0 = 0; l0 < P.nl; l0 += kFieldNL) {
+        const int nlp = (P.nl - l0) < kFieldNL ? (P.nl - l0) : kFieldNL;
+        dim3 g((unsigned)vt, (unsigned)nsplit);
+        if (mixed)
+            hipLaunchKernelGGL((field_kernel<DIM, true>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
+                               nlp, paths, verts, vpath, offsets, coef, coefR, aftab, part);
+        else
+            hipLaunchKernelGGL((field_kernel<DIM, false>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
+                               nlp, paths, verts, vpath, offsets, coef, coefR, aftab, part);
+    }
+    const int64_t n = (int64_t)P.nv * P.nl;
+    hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nsplit, n, acc);
+}
+
+void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
+                  const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc, bool mixed,
+                  hipStream_t s)
+{
+    if (P.dim == 3)
+        field_dispatch<3>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, s);
+    else
+        field_dispatch<2>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, s);
+}
+
+} // namespace scmc

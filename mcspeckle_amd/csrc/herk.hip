@@ -1,0 +1,418 @@
+// herk.hip -- complex Hermitian rank-K covariance update on the gfx950 matrix
+// cores, with the Kronecker factors expanded on load.
+//
+// Computes C += sum_k u_k u_k^H for the covariance of code/MCcov.m:232-264
+// (the four-term CBS update of :244-254 equals (f+r)(f+r)^H with
+// u = A1 (x) B1 + A2 (x) B2).  Row index i of C is (v,l) with i = v + nv*l,
+// the "matrix view" of Ms(v1,v2,l1,l2).
+//
+// Real formulation.  Interleave (re, im) along the reduction axis: with
+// Z[i][2k] = Re u_k(i), Z[i][2k+1] = Im u_k(i) and the rotated rows
+// Z'[i] = (Im, -Re) (i.e. -i*u):  Re C = Z Z^T and Im C = Z' Z^T.  One tile
+// of C therefore needs two real GEMMs sharing the B operand.
+//
+// Mixed kernel: u is formed in FP32 from the factors, scaled by a power of
+// two so that max|u| < 2^15, split as u = hi + lo with FP16 hi and lo, and
+// accumulated as hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_f16 into FP32
+// registers (~22-bit products, FP32 sums).  Each (tile, k-split) work item then
+// adds its FP32 tile, unscaled, into its own FP64 accumulator copy, so the
+// result is deterministic and the per-item FP32 sum length stays short.
+//
+// FP64 kernel: the same structure with v_mfma_f64_16x16x4f64 and no split.
+//
+// Only lower-triangle tiles (ti >= tj) are computed; the finaliser restores
+// the upper triangle by Hermitian symmetry.
+#include "kernels.h"
+
+namespace scmc {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void tri_decode(int t, int &ti, int &tj)
+{
+    int a = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    while (a * (a + 1) / 2 > t) --a;
+    ti = a;
+    tj = t - a * (a + 1) / 2;
+}
+
+__device__ __forceinline__ float2 fcmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// (re, im) pairs -> (im, -re): the rows of -i*u.
+__device__ __forceinline__ half8 rot_neg_i(half8 a)
+{
+    half8 r;
+    r[0] = a[1]; r[1] = -a[0];
+    r[2] = a[3]; r[3] = -a[2];
+    r[4] = a[5]; r[5] = -a[4];
+    r[6] = a[7]; r[7] = -a[6];
+    return r;
+}
+
+// ------------------------------------------------------------ mixed
+namespace mixed {
+constexpr int TM = kTileMixed;   // 128x128 tile per workgroup
+constexpr int KS = 8;            // vertices per MFMA step (K = 16 halfs)
+constexpr int NKS = 2;           // MFMA steps per LDS stage
+constexpr int VPS = KS * NKS;    // vertices per stage
+// LDS: [stage 2][tile 2][plane 2 (hi,lo)][NKS][TM rows][2 x 16 B]
+constexpr int ROWQ = 2;                          // uint4 per row per step
+constexpr int PLANE = NKS * TM * ROWQ;           // uint4 per plane
+constexpr int TILEQ = 2 * PLANE;
+constexpr int STAGEQ = 2 * TILEQ;
+__device__ __forceinline__ int lidx(int stage, int tile, int plane, int ks, int row, int q)
+{
+    return stage * STAGEQ + tile * TILEQ + plane * PLANE + (ks * TM + row) * ROWQ + q;
+}
+} // namespace mixed
+
+template <bool CBS>
+__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
+                                                            const float2 *__restrict__ B1, const float2 *__restrict__ A2,
+                                                            const float2 *__restrict__ B2, double2 *__restrict__ acc,
+                                                            const unsigned *__restrict__ bound, int ksplit)
+{
+    using namespace mixed;
+    __shared__ uint4 lds[2 * STAGEQ];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int item = blockIdx.x;
+    const int s = item % ksplit;
+    int ti, tj;
+    tri_decode(item / ksplit, ti, tj);
+    const bool diag = (ti == tj);
+    const int64_t kb = k * s / ksplit, ke = k * (s + 1) / ksplit;
+
+    // power-of-two scale: max|u| * scale < 2^15
+    float b = bound ? __uint_as_float(*bound) : 0.f;
+    int ex = 0;
+    if (b > 0.f) frexpf(b, &ex);
+    const int sh = 15 - ex;
+    const float scale = ldexpf(1.f, sh);
+    const double unscale = ldexp(1.0, -2 * sh);
+
+    // staging role: one row of the row tile (T=0) or the column tile (T=1)
+    const int T = tid >> 7, r = tid & 127;
+    const int gi = (T ? tj : ti) * TM + r;
+    const bool rvalid = gi < n && !(diag && T == 1);
+    const int vv = rvalid ? gi % nv : 0, ll = rvalid ? gi / nv : 0;
+
+    auto stage_fill = [&](int st, int buf) {
+        const int64_t k0 = kb + (int64_t)st * VPS;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            half8 h[2], lo[2];
+#pragma unroll
+            for (int q = 0; q < KS; ++q) {
+                const int64_t kk = k0 + ks * KS + q;
+                float2 u = make_float2(0.f, 0.f);
+                if (rvalid && kk < ke) {
+                    u = fcmul(A1[kk * nv + vv], B1[kk * nl + ll]);
+                    if (CBS) {
+                        float2 u2 = fcmul(A2[kk * nv + vv], B2[kk * nl + ll]);
+                        u.x += u2.x;
+                        u.y += u2.y;
+                    }
+                    u.x *= scale;
+                    u.y *= scale;
+                }
+                _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
+                _Float16 lr = (_Float16)(u.x - (float)hr), li = (_Float16)(u.y - (float)hi_);
+                h[q >> 2][(q & 3) * 2] = hr;
+                h[q >> 2][(q & 3) * 2 + 1] = hi_;
+                lo[q >> 2][(q & 3) * 2] = lr;
+                lo[q >> 2][(q & 3) * 2 + 1] = li;
+            }
+            if (!(diag && T == 1)) {
+                uint4 *dh = &lds[lidx(buf, T, 0, ks, r, 0)];
+                uint4 *dl = &lds[lidx(buf, T, 1, ks, r, 0)];
+                dh[0] = __builtin_bit_cast(uint4, h[0]);
+                dh[1] = __builtin_bit_cast(uint4, h[1]);
+                dl[0] = __builtin_bit_cast(uint4, lo[0]);
+                dl[1] = __builtin_bit_cast(uint4, lo[1]);
+            }
+        }
+    };
+
+    f32x16 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                accR[a][c][e] = 0.f;
+                accI[a][c][e] = 0.f;
+            }
+
+    const int wr = wave >> 1, wc = wave & 1;
+    const int half = lane >> 5, lrow = lane & 31;
+    const int btile = diag ? 0 : 1;
+    const int nstage = (int)((ke - kb + VPS - 1) / VPS);
+
+    if (nstage > 0) stage_fill(0, 0);
+    __syncthreads();
+    for (int st = 0; st < nstage; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nstage) stage_fill(st + 1, buf ^ 1);
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            half8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                const int row = wr * 64 + rb * 32 + lrow;
+                ah[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 0, ks, row, half)]);
+                al[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 1, ks, row, half)]);
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int col = wc * 64 + cb * 32 + lrow;
+                bh[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 0, ks, col, half)]);
+                bl[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 1, ks, col, half)]);
+            }
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                const half8 rh = rot_neg_i(ah[rb]), rl = rot_neg_i(al[rb]);
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bh[cb], accR[rb][cb], 0, 0, 0);
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bh[cb], accI[rb][cb], 0, 0, 0);
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bl[cb], accR[rb][cb], 0, 0, 0);
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bl[cb], accI[rb][cb], 0, 0, 0);
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rb], bh[cb], accR[rb][cb], 0, 0, 0);
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, bh[cb], accI[rb][cb], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: add the unscaled FP32 tile into accumulator copy s
+    double2 *C = acc + (size_t)s * n * n;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int i = ti * TM + wr * 64 + rb * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                const int j = tj * TM + wc * 64 + cb * 32 + lrow;
+                if (i < n && j < n) {
+                    double2 *p = &C[(size_t)i * n + j];
+                    double2 o = *p;
+                    o.x += (double)accR[rb][cb][e] * unscale;
+                    o.y += (double)accI[rb][cb][e] * unscale;
+                    *p = o;
+                }
+            }
+}
+
+// ------------------------------------------------------------- fp64
+namespace f64 {
+constexpr int TM = kTileF64;   // 64x64 tile per workgroup, 4 waves of 32x32
+constexpr int KS = 2;          // vertices per MFMA step (K = 4 doubles)
+constexpr int NKS = 4;         // steps per stage
+constexpr int VPS = KS * NKS;  // 8 vertices per stage
+// LDS: [stage 2][tile 2][NKS][TM rows][4 doubles]
+__device__ __forceinline__ int lidx(int stage, int tile, int ks, int row, int e)
+{
+    return (((stage * 2 + tile) * NKS + ks) * TM + row) * 4 + e;
+}
+} // namespace f64
+
+template <bool CBS>
+__global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, int64_t k, const double2 *__restrict__ A1,
+                                                        const double2 *__restrict__ B1, const double2 *__restrict__ A2,
+                                                        const double2 *__restrict__ B2, double2 *__restrict__ acc,
+                                                        int ksplit)
+{
+    using namespace f64;
+    __shared__ double lds[2 * 2 * NKS * TM * 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int item = blockIdx.x;
+    const int s = item % ksplit;
+    int ti, tj;
+    tri_decode(item / ksplit, ti, tj);
+    const bool diag = (ti == tj);
+    const int64_t kb = k * s / ksplit, ke = k * (s + 1) / ksplit;
+
+    // staging: 128 row slots (64 row-tile + 64 column-tile), 2 threads each
+    const int T = tid >> 7, r = (tid & 127) >> 1, hp = tid & 1;
+    const int gi = (T ? tj : ti) * TM + r;
+    const bool rvalid = gi < n && !(diag && T == 1);
+    const int vv = rvalid ? gi % nv : 0, ll = rvalid ? gi / nv : 0;
+
+    auto stage_fill = [&](int st, int buf) {
+        const int64_t k0 = kb + (int64_t)st * VPS;
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+            const int q = hp * 4 + q4;
+            const int64_t kk = k0 + q;
+            double2 u = make_double2(0.0, 0.0);
+            if (rvalid && kk < ke) {
+                u = cmul(A1[kk * nv + vv], B1[kk * nl + ll]);
+                if (CBS) {
+                    double2 u2 = cmul(A2[kk * nv + vv], B2[kk * nl + ll]);
+                    u.x += u2.x;
+                    u.y += u2.y;
+                }
+            }
+            if (!(diag && T == 1)) {
+                double *d = &lds[lidx(buf, T, q >> 1, r, (q & 1) * 2)];
+                d[0] = u.x;
+                d[1] = u.y;
+            }
+        }
+    };
+
+    f64x4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                accR[a][c][e] = 0.0;
+                accI[a][c][e] = 0.0;
+            }
+
+    const int wr = wave >> 1, wc = wave & 1;
+    const int kk = lane >> 4, l16 = lane & 15;
+    const int btile = diag ? 0 : 1;
+    const int nstage = (int)((ke - kb + VPS - 1) / VPS);
+
+    if (nstage > 0) stage_fill(0, 0);
+    __syncthreads();
+    for (int st = 0; st < nstage; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nstage) stage_fill(st + 1, buf ^ 1);
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            double a[2], ar[2], bb[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                const int row = wr * 32 + rb * 16 + l16;
+                a[rb] = lds[lidx(buf, 0, ks, row, kk)];
+                const double partner = lds[lidx(buf, 0, ks, row, kk ^ 1)];
+                ar[rb] = (kk & 1) ? -partner : partner;
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) bb[cb] = lds[lidx(buf, btile, ks, wc * 32 + cb * 16 + l16, kk)];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bb[cb], accR[rb][cb], 0, 0, 0);
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rb], bb[cb], accI[rb][cb], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+    double2 *C = acc + (size_t)s * n * n;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = ti * TM + wr * 32 + rb * 16 + 4 * kk + e;
+                const int j = tj * TM + wc * 32 + cb * 16 + l16;
+                if (i < n && j < n) {
+                    double2 *p = &C[(size_t)i * n + j];
+                    double2 o = *p;
+                    o.x += accR[rb][cb][e];
+                    o.y += accI[rb][cb][e];
+                    *p = o;
+                }
+            }
+}
+
+static int ntiles(int n, int tm) { return (n + tm - 1) / tm; }
+
+void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
+{
+    if (hp.k <= 0) return;
+    const int nt = ntiles(hp.n, kTileMixed);
+    const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
+    if (hp.A2)
+        hipLaunchKernelGGL(herk_mixed_kernel<true>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,
+                           (const float2 *)hp.B2, hp.acc, hp.bound, hp.ksplit);
+    else
+        hipLaunchKernelGGL(herk_mixed_kernel<false>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const float2 *)hp.A1, (const float2 *)hp.B1, nullptr, nullptr, hp.acc, hp.bound,
+                           hp.ksplit);
+}
+
+void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
+{
+    if (hp.k <= 0) return;
+    const int nt = ntiles(hp.n, kTileF64);
+    const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
+    if (hp.A2)
+        hipLaunchKernelGGL(herk_fp64_kernel<true>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const double2 *)hp.A1, (const double2 *)hp.B1, (const double2 *)hp.A2,
+                           (const double2 *)hp.B2, hp.acc, hp.ksplit);
+    else
+        hipLaunchKernelGGL(herk_fp64_kernel<false>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const double2 *)hp.A1, (const double2 *)hp.B1, nullptr, nullptr, hp.acc, hp.ksplit);
+}
+
+// ------------------------------------------------------------ finaliser
+__global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale,
+                                    const double2 *rowf, double *out_re, double *out_im)
+{
+    const int n = nv * nl;
+    const int64_t total = (int64_t)n * n;
+    int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= total) return;
+    // o = v1 + nv*(v2 + nv*(l1 + nl*l2))  (MATLAB column-major Nv x Nv x Nl x Nl)
+    int64_t t = o;
+    const int v1 = (int)(t % nv); t /= nv;
+    const int v2 = (int)(t % nv); t /= nv;
+    const int l1 = (int)(t % nl); t /= nl;
+    const int l2 = (int)t;
+    const int i = v1 + nv * l1, j = v2 + nv * l2;
+    const bool lower = (i / tile) >= (j / tile);
+    const int a = lower ? i : j, b = lower ? j : i;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < ksplit; ++s) {
+        double2 c = acc[(size_t)s * n * n + (size_t)a * n + b];
+        re += c.x;
+        im += c.y;
+    }
+    if (!lower) im = -im;
+    if (rowf) {
+        // * rowf[i] * conj(rowf[j])
+        double2 fi = rowf[i], fj = rowf[j];
+        double pr = fi.x * fj.x + fi.y * fj.y, pi = fi.y * fj.x - fi.x * fj.y;
+        double nr = re * pr - im * pi, ni = re * pi + im * pr;
+        re = nr;
+        im = ni;
+    }
+    re *= scale;
+    im *= scale;
+    if (out_im) {
+        out_re[o] = re;
+        out_im[o] = im;
+    } else {
+        out_re[2 * o] = re;
+        out_re[2 * o + 1] = im;
+    }
+}
+
+void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale, const double2 *rowf,
+                         double *out_re, double *out_im, hipStream_t s)
+{
+    const int64_t total = (int64_t)nv * nl * nv * nl;
+    dim3 g((unsigned)((total + 255) / 256));
+    hipLaunchKernelGGL(cov_finalize_kernel, g, dim3(256), 0, s, acc, ksplit, nv, nl, tile, scale, rowf, out_re, out_im);
+}
+
+} // namespace scmc

@@ -1,0 +1,80 @@
+// kernels.h -- host-side launch interfaces of the speckle Monte Carlo kernels.
+#pragma once
+#include "device_common.h"
+
+namespace scmc {
+
+// One scattering vertex (64 B): position, incoming direction, albedo weight
+// W_n and the random-phase uniform xi_n of MCfield.m:220.
+struct VertexRec {
+    double x[3];
+    double ow[3];
+    double weight;
+    double xi;
+};
+
+// Per-path quantities of the first vertex (MCcov.m:114-153).
+struct PathRec {
+    double x1[3];
+    double px;
+    double w1[3];
+    double pad;
+    double2 w0p;
+};
+
+// ---------------------------------------------------------------- tracer
+void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, hipStream_t s);
+// offsets[np + 1] = exclusive scan of counts (offsets[np] = total).
+void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, hipStream_t s);
+void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64_t *offsets, PathRec *paths,
+                       VertexRec *verts, int32_t *vpath, hipStream_t s);
+
+// -------------------------------------------------- covariance factors
+// Factor arrays for the rank-K update C += sum_k u_k u_k^H with
+//   u_k(v,l) = A1[k][v] B1[k][l] + A2[k][v] B2[k][l]   (A2/B2 null without CBS)
+// Single scattering uses A1 = sqrt(albedo/px) e_v0, B1 = e_l0 (the
+// amplitude product a(l.v) is a per-row factor applied when finalising).
+struct CovFactors {
+    void *A1, *B1, *A2, *B2;   // float2* (mixed) or double2* (fp64)
+    void *SA, *SB;             // single-scatter factors per path
+    void *Gp, *Fp2;            // path-level e_l0_ms, e_v0_ms
+    float *maxG, *maxF2;       // path-level maxima (mixed scaling)
+    unsigned *bound;           // [0] multi, [1] single: float bits of max |u|
+};
+void launch_nee_cov(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
+                    const int32_t *vpath, const int64_t *offsets, const CovFactors &f, bool mixed, hipStream_t s);
+
+// ------------------------------------------------------------ HERK
+struct HerkProblem {
+    int nv, nl, n;
+    int64_t k;             // number of rank-1 terms
+    const void *A1, *B1, *A2, *B2;
+    double2 *acc;          // [ksplit][n][n] row-major FP64 accumulators
+    const unsigned *bound; // mixed: float bits of max |u| (may be null)
+    int ksplit;
+};
+// Tile edge of the mixed and FP64 kernels (the finaliser needs it).
+constexpr int kTileMixed = 128;
+constexpr int kTileF64 = 64;
+void launch_herk_mixed(const HerkProblem &hp, hipStream_t s);
+void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
+
+// Sum the ksplit accumulators, restore the upper triangle, scale, apply the
+// optional per-row factor rowf[(v,l)] (single scattering amplitude), and write
+// the MATLAB nv x nv x nl x nl column-major layout (split or interleaved).
+void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale,
+                         const double2 *rowf, double *out_re, double *out_im, hipStream_t s);
+// Single-scatter amplitude table af_ang_vl(v,l) = a(l_l . v_v) (far field).
+void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s);
+
+// --------------------------------------------------------------- field
+// u(v,l) += sum over vertices of the MCfield.m:207-222 contributions,
+// FP64 accumulators acc[nv*nl] (column-major, v fastest).
+void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
+                  const int32_t *vpath, const int64_t *offsets, double2 *work_l, double2 *acc, bool mixed,
+                  hipStream_t s);
+
+// Device-wide helpers.
+void launch_zero(void *p, size_t bytes, hipStream_t s);
+
+} // namespace scmc

@@ -1,0 +1,189 @@
+// nee_cov.hip -- next-event estimation factors for the covariance update.
+//
+// For every path and vertex this evaluates the complex volumetric
+// throughputs of code/MCcov.m:157-222 (evalphaseatt x evalampfunc_general)
+// and writes them as the Kronecker factors of the rank-1 term
+//   u(v,l) = sqrt(W/px) * [ e_v_ms(v) e_l0_ms(l) + e_v0_ms(v) e_l_ms(l) ]
+// whose outer product u u^H expands to the four M00/Mnn/Mn0/M0n terms of
+// MCcov.m:244-254 (the second term only with CBS; MCcov.m:256-262 otherwise).
+// Single scattering (MCcov.m:234-241) is u_s(v,l) = sqrt(albedo/px) e_v0(v)
+// e_l0(l) af_ang_vl(v,l); the amplitude table is a per-row factor applied when
+// the covariance is finalised.
+//
+// Evaluation is FP64 in both precision modes (it is <1% of the work at the
+// headline size); the mixed mode stores FP32 factors and records the largest
+// |u| bound of the chunk for the FP16 power-of-two scaling.
+#include "kernels.h"
+
+namespace scmc {
+
+template <typename T> __device__ __forceinline__ T to_c(double2 v);
+template <> __device__ __forceinline__ float2 to_c<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <> __device__ __forceinline__ double2 to_c<double2>(double2 v) { return v; }
+
+__device__ __forceinline__ float cmag(double2 v) { return (float)fmax(fabs(v.x), fabs(v.y)); }
+
+__device__ __forceinline__ float block_max(float v, float *sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    float r = 0.f;
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) r = fmaxf(r, sh[i]);
+    return r;
+}
+
+// Positive floats order like their bit patterns.
+__device__ __forceinline__ void atomic_max_pos(unsigned *p, float v)
+{
+    if (v > 0.f) atomicMax(p, __float_as_uint(v));
+}
+
+// MCcov.m:157-179 : path-level factors at the first vertex (far field).
+template <int DIM, typename T>
+__global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t np, const PathRec *paths, T *Gp,
+                                                           T *Fp2, T *SA, T *SB, float *maxG, float *maxF2,
+                                                           unsigned *bound)
+{
+    __shared__ float sh[4];
+    const int64_t p = blockIdx.x;
+    if (p >= np) return;
+    const PathRec pr = paths[p];
+    const double ssw = sqrt(P.albedo / pr.px);
+    float mG = 0.f, mF = 0.f, mSA = 0.f, mSB = 0.f;
+    for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
+        const double *lc = P.l + (size_t)i * DIM;
+        double2 e = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
+        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, lc), P.amp), pr.w0p);
+        double2 g = cmul(e, af);
+        Gp[p * P.nl + i] = to_c<T>(g);
+        SB[p * P.nl + i] = to_c<T>(e);
+        mG = fmaxf(mG, cmag(g));
+        mSB = fmaxf(mSB, cmag(e));
+    }
+    for (int i = threadIdx.x; i < P.nv; i += blockDim.x) {
+        const double *vc = P.v + (size_t)i * DIM;
+        double2 e = phase_att64<DIM>(P, pr.x1, vc, P.ff_sign, P.is_ff_v);
+        double2 se = make_double2(ssw * e.x, ssw * e.y);
+        SA[p * P.nv + i] = to_c<T>(se);
+        mSA = fmaxf(mSA, cmag(se));
+        if (P.do_cbs) {
+            double mw[3] = {-pr.w1[0], -pr.w1[1], -pr.w1[2]};
+            double2 af = cdiv(eval_amp<DIM>(dot<DIM>(mw, vc), P.amp), pr.w0p);
+            double2 f = cmul(e, af);
+            Fp2[p * P.nv + i] = to_c<T>(f);
+            mF = fmaxf(mF, cmag(f));
+        }
+    }
+    mG = block_max(mG, sh);
+    mF = block_max(mF, sh);
+    mSA = block_max(mSA, sh);
+    mSB = block_max(mSB, sh);
+    if (threadIdx.x == 0) {
+        maxG[p] = mG;
+        maxF2[p] = mF;
+        atomic_max_pos(&bound[1], 2.f * mSA * mSB);
+    }
+}
+
+// MCcov.m:205-222 : vertex-level factors for pL > 1, written at the
+// multiple-scattering index m = vertex - path - 1 (path-major order).
+template <int DIM, typename T>
+__global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64_t nvert, const PathRec *paths,
+                                                             const VertexRec *verts, const int32_t *vpath,
+                                                             const int64_t *offsets, const T *Gp, const T *Fp2,
+                                                             const float *maxG, const float *maxF2, T *A1, T *B1,
+                                                             T *A2, T *B2, unsigned *bound)
+{
+    __shared__ float sh[4];
+    const int64_t vi = blockIdx.x;
+    if (vi >= nvert) return;
+    const int p = vpath[vi];
+    if (vi == offsets[p]) return;   // first vertex: single scattering
+    const int64_t m = vi - p - 1;
+    const VertexRec r = verts[vi];
+    const double sw = sqrt(r.weight / paths[p].px);
+    float mA = 0.f, mB = 0.f;
+    for (int i = threadIdx.x; i < P.nv; i += blockDim.x) {
+        const double *vc = P.v + (size_t)i * DIM;
+        double2 e = phase_att64<DIM>(P, r.x, vc, P.ff_sign, P.is_ff_v);
+        double2 af = eval_amp<DIM>(dot<DIM>(r.ow, vc), P.amp);
+        double2 f = cmul(e, af);
+        f.x *= sw;
+        f.y *= sw;
+        A1[m * P.nv + i] = to_c<T>(f);
+        mA = fmaxf(mA, cmag(f));
+        if (P.do_cbs) A2[m * P.nv + i] = Fp2[(int64_t)p * P.nv + i];
+    }
+    for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
+        B1[m * P.nl + i] = Gp[(int64_t)p * P.nl + i];
+        if (P.do_cbs) {
+            const double *lc = P.l + (size_t)i * DIM;
+            double2 e = phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l);
+            double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
+            double2 af = eval_amp<DIM>(dot<DIM>(mow, lc), P.amp);
+            double2 g = cmul(e, af);
+            g.x *= sw;
+            g.y *= sw;
+            B2[m * P.nl + i] = to_c<T>(g);
+            mB = fmaxf(mB, cmag(g));
+        }
+    }
+    mA = block_max(mA, sh);
+    mB = block_max(mB, sh);
+    if (threadIdx.x == 0) atomic_max_pos(&bound[0], 2.f * (mA * maxG[p] + maxF2[p] * mB));
+}
+
+template <int DIM, typename T>
+static void nee_cov_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths,
+                             const VertexRec *verts, const int32_t *vpath, const int64_t *offsets,
+                             const CovFactors &f, hipStream_t s)
+{
+    if (np > 0)
+        hipLaunchKernelGGL((nee_cov_path_kernel<DIM, T>), dim3((unsigned)np), dim3(128), 0, s, P, np, paths,
+                           (T *)f.Gp, (T *)f.Fp2, (T *)f.SA, (T *)f.SB, f.maxG, f.maxF2, f.bound);
+    if (nvert > 0)
+        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T>), dim3((unsigned)nvert), dim3(128), 0, s, P, nvert, paths,
+                           verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1,
+                           (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound);
+}
+
+void launch_nee_cov(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
+                    const int32_t *vpath, const int64_t *offsets, const CovFactors &f, bool mixed, hipStream_t s)
+{
+    if (P.dim == 3) {
+        if (mixed)
+            nee_cov_dispatch<3, float2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+        else
+            nee_cov_dispatch<3, double2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+    } else {
+        if (mixed)
+            nee_cov_dispatch<2, float2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+        else
+            nee_cov_dispatch<2, double2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+    }
+}
+
+// af_ang_vl(v,l) = evalampfunc_general(l(:,j)'*v) (MCcov.m:91-96), far field.
+template <int DIM>
+__global__ void af_table_kernel(DevProblem P, double2 *af)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)P.nv * P.nl) return;
+    int v = (int)(idx % P.nv), l = (int)(idx / P.nv);
+    af[idx] = eval_amp<DIM>(dot<DIM>(P.l + (size_t)l * DIM, P.v + (size_t)v * DIM), P.amp);
+}
+
+void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s)
+{
+    int64_t n = (int64_t)P.nv * P.nl;
+    dim3 g((unsigned)((n + 255) / 256));
+    if (P.dim == 3)
+        hipLaunchKernelGGL(af_table_kernel<3>, g, dim3(256), 0, s, P, af);
+    else
+        hipLaunchKernelGGL(af_table_kernel<2>, g, dim3(256), 0, s, P, af);
+}
+
+} // namespace scmc

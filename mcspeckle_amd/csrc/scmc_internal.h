@@ -1,0 +1,71 @@
+// scmc_internal.h -- host-side state shared by the C-ABI entry points.
+#pragma once
+#include "kernels.h"
+#include "scmc.h"
+
+#include <cstddef>
+#include <cstdint>
+
+namespace scmc {
+
+int set_error(int code, const char *fmt, ...);
+bool hip_ok(hipError_t e, const char *what, int *rc);
+int ensure_device(int *dev_out);
+int validate_problem(const scmc_problem *pr, int64_t max_itr, bool need_views);
+double box_volume(const scmc_problem *pr);
+int64_t chunk_paths(int n, int64_t remaining, double mean_len, bool mixed);
+int64_t field_chunk_paths(int nv, int nl, int64_t remaining, double mean_len);
+
+// Owning device allocation that only grows.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    void release();
+    bool ensure(size_t b, int *rc);
+};
+
+// Device copy of the problem description (directions, tables, constants).
+struct ProblemState {
+    DevProblem dp;
+    DevBuf dl, dv, tab_amp, tab_icdf, tab_amp0, tab_icdf0;
+    int setup(const scmc_problem *pr, hipStream_t s);
+    int upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevBuf &icdf, DevPhase &out, hipStream_t s);
+};
+
+// One traced chunk of paths: offsets, path records and vertex records.
+struct TraceChunk {
+    int64_t np = 0, nvert = 0;
+    DevBuf counts, offsets, paths, verts, vpath;
+    int run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s);
+    int load(const scmc_pathset *ps, hipStream_t s);
+};
+
+// Covariance accumulation state across chunks.
+struct CovEngine {
+    bool mixed = false;
+    int nv = 0, nl = 0, n = 0, tile = 0;
+    int ks_m = 0, ks_s = 0;
+    int last_scale_log2 = 0;
+    DevBuf acc_m, acc_s, bound;
+    DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2;
+    int init(const ProblemState &ps, bool mixed, hipStream_t s);
+    int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
+    int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee);
+    int finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
+                 double *mm_re, double *mm_im, hipStream_t s);
+};
+
+// Field accumulation state across chunks.
+struct FieldEngine {
+    int nv = 0, nl = 0;
+    DevBuf acc, work, out;
+    int init(const ProblemState &ps, hipStream_t s);
+    int chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum);
+    int finalize(const scmc_problem *pr, int64_t max_itr, double *u_re, double *u_im, hipStream_t s);
+};
+
+} // namespace scmc

@@ -1,0 +1,317 @@
+"""The speckle Monte Carlo engines with the reference's positional-argument
+conventions, running on the MI355X through libscmc.so.
+
+    [Ms, Mm] = MCcov(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v,
+                     maxItr, lambda, doCBS, smpFlg, sct_type, ampfunc[, ampfunc0])
+                                                      (code/MCcov.m:1, :45-70)
+    u = MCfield(...same..., [ampfunc0[, lmean0]])     (code/MCfield.m:1, :41-66)
+    u = MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp,
+                           maxItr, lambda, doCBS, smpFlg, lmean0, vsign)
+                                                      (code/MCfieldFourierQuad.m:1)
+
+Differences from MATLAB are keyword-only: ``seed`` and ``path_offset``
+(Philox key and first path id; MATLAB uses the global ``rng``), ``precision``
+("fp64" or "mixed") and ``stats`` (a dict receiving timing and counts).
+Outputs are numpy complex128 arrays in MATLAB's column-major layout
+(``Ms[v1, v2, l1, l2]``, ``u[v, l]``).
+"""
+from dataclasses import dataclass, field
+from typing import Optional
+
+import ctypes as C
+import numpy as np
+
+from . import _native as N
+
+
+class ScmcArgumentError(ValueError):
+    """Raised with the reference's error message for invalid arguments."""
+
+
+@dataclass
+class PhaseSpec:
+    type: int                       # 1 isotropic, 2 tabulated, 3 HG
+    g: float = 0.0
+    eval_amp: Optional[np.ndarray] = None      # complex, tabulated
+    sample_icdf: Optional[np.ndarray] = None   # real, tabulated
+
+
+@dataclass
+class ProblemSpec:
+    dim: int
+    sigt: float
+    albedo: float
+    lam: float
+    box_min: np.ndarray
+    box_max: np.ndarray
+    l: np.ndarray                   # dim x nl
+    v: np.ndarray                   # dim x nv (may be empty for the quad engine)
+    is_ff_l: bool
+    is_ff_v: bool
+    do_cbs: bool
+    smp_flg: int
+    amp: PhaseSpec
+    amp0: Optional[PhaseSpec] = None
+    lmean0: Optional[np.ndarray] = None
+    seed: int = 0
+    path_offset: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def nl(self):
+        return self.l.shape[1]
+
+    @property
+    def nv(self):
+        return self.v.shape[1]
+
+
+def _phase_from_arg(sct_type, ampfunc, name):
+    if sct_type == 1:
+        return PhaseSpec(1)
+    if sct_type == 3:
+        g = float(np.asarray(ampfunc).reshape(-1)[0])
+        return PhaseSpec(3, g=g)
+    if sct_type == 2:
+        try:
+            ea = ampfunc["evalAmp"]
+            ic = ampfunc["sampleIcdf"]
+        except (TypeError, KeyError):
+            ea = getattr(ampfunc, "evalAmp", None)
+            ic = getattr(ampfunc, "sampleIcdf", None)
+        if ea is None or ic is None:
+            raise ScmcArgumentError(f"{name} must hold evalAmp and sampleIcdf for a tabulated scatterer")
+        return PhaseSpec(2, eval_amp=np.asarray(ea, dtype=np.complex128).reshape(-1),
+                         sample_icdf=np.asarray(ic, dtype=np.float64).reshape(-1))
+    raise ScmcArgumentError("sct_type must be 1 (isotropic), 2 (tabulated) or 3 (HG)")
+
+
+def _sources(x, dim, what):
+    """MCcov.m:56-70: dim x N, or a 1 x N row of angles in 2-D."""
+    a = np.asarray(x, dtype=np.float64)
+    if a.ndim == 1:
+        a = a.reshape(1, -1)
+    if a.shape[0] != dim:
+        if dim == 2 and a.shape[0] == 1:
+            a = np.vstack([np.sin(a[0]), np.cos(a[0])])
+        else:
+            raise ScmcArgumentError(f"Invalid {what} source input")
+    return np.ascontiguousarray(a)
+
+
+def make_problem(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS, smpFlg, sct_type, ampfunc,
+                 ampfunc0=None, lmean0=None, seed=0, path_offset=0, need_views=True):
+    bmin = np.asarray(box_min, dtype=np.float64).reshape(-1)
+    bmax = np.asarray(box_max, dtype=np.float64).reshape(-1)
+    # MCcov.m:48-51
+    if bmax.size not in (2, 3) or bmax.size != bmin.size:
+        raise ScmcArgumentError("Invalid box size")
+    dim = bmax.size
+    L = _sources(l, dim, "light")
+    V = _sources(v, dim, "view") if need_views else np.zeros((dim, 0))
+    if smpFlg not in (1, 2):
+        raise ScmcArgumentError("smpFlg must be 1 or 2")
+    amp = _phase_from_arg(int(sct_type), ampfunc, "ampfunc")
+    amp0 = _phase_from_arg(int(sct_type), ampfunc0, "ampfunc0") if ampfunc0 is not None else None
+    lm = None
+    if lmean0 is not None and np.asarray(lmean0).size:
+        lm = np.asarray(lmean0, dtype=np.float64).reshape(-1)
+    return ProblemSpec(dim=dim, sigt=float(sigt), albedo=float(albedo), lam=float(lambda_), box_min=bmin,
+                       box_max=bmax, l=L, v=V, is_ff_l=bool(is_ff_l), is_ff_v=bool(is_ff_v), do_cbs=bool(doCBS),
+                       smp_flg=int(smpFlg), amp=amp, amp0=amp0, lmean0=lm, seed=int(seed),
+                       path_offset=int(path_offset))
+
+
+def _c_phase(ph, keep):
+    out = N.Phase()
+    if ph is None:
+        return out
+    out.type = ph.type
+    out.g = ph.g
+    if ph.type == 2:
+        re = np.ascontiguousarray(ph.eval_amp.real, dtype=np.float64)
+        im = np.ascontiguousarray(ph.eval_amp.imag, dtype=np.float64)
+        ic = np.ascontiguousarray(ph.sample_icdf, dtype=np.float64)
+        keep += [re, im, ic]
+        out.n_amp = re.size
+        out.amp_re = N.dptr(re)
+        out.amp_im = N.dptr(im)
+        out.n_icdf = ic.size
+        out.icdf = N.dptr(ic)
+    return out
+
+
+def c_problem(spec):
+    """ctypes scmc_problem for a ProblemSpec (arrays kept alive on spec)."""
+    keep = []
+    p = N.Problem()
+    p.dim = spec.dim
+    p.sigt, p.albedo, p.lam = spec.sigt, spec.albedo, spec.lam
+    for j in range(spec.dim):
+        p.box_min[j] = spec.box_min[j]
+        p.box_max[j] = spec.box_max[j]
+    lc = np.ascontiguousarray(spec.l.T)   # dim x nl column-major
+    vc = np.ascontiguousarray(spec.v.T)
+    keep += [lc, vc]
+    p.nl, p.nv = spec.l.shape[1], spec.v.shape[1]
+    p.l = N.dptr(lc)
+    p.v = N.dptr(vc) if vc.size else None
+    p.is_ff_l, p.is_ff_v = int(spec.is_ff_l), int(spec.is_ff_v)
+    p.do_cbs, p.smp_flg = int(spec.do_cbs), spec.smp_flg
+    p.amp = _c_phase(spec.amp, keep)
+    p.has_amp0 = int(spec.amp0 is not None)
+    p.amp0 = _c_phase(spec.amp0, keep)
+    if spec.lmean0 is not None:
+        p.has_lmean0 = 1
+        for j in range(spec.dim):
+            p.lmean0[j] = spec.lmean0[j]
+    p.seed = spec.seed
+    p.path_offset = spec.path_offset
+    spec._keep = keep
+    return p
+
+
+def _prec(precision):
+    if isinstance(precision, str):
+        if precision not in N.PRECISIONS:
+            raise ScmcArgumentError(f"precision must be one of {sorted(N.PRECISIONS)}")
+        return N.PRECISIONS[precision]
+    return int(precision)
+
+
+def mccov_spec(spec, max_itr, precision="fp64", stats=None):
+    """Run MCcov for a ProblemSpec; returns (Ms, Mm) shaped (nv, nv, nl, nl)."""
+    L = N.lib()
+    nv, nl = spec.nv, spec.nl
+    ne = (nv * nl) ** 2
+    ms_re, ms_im, mm_re, mm_im = (np.empty(ne) for _ in range(4))
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(L.scmc_mccov(C.byref(p), int(max_itr), _prec(precision), N.dptr(ms_re), N.dptr(ms_im),
+                         N.dptr(mm_re), N.dptr(mm_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    shape = (nv, nv, nl, nl)
+    return ((ms_re + 1j * ms_im).reshape(shape, order="F"), (mm_re + 1j * mm_im).reshape(shape, order="F"))
+
+
+def mcfield_spec(spec, max_itr, precision="fp64", stats=None):
+    L = N.lib()
+    nv, nl = spec.nv, spec.nl
+    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(L.scmc_mcfield(C.byref(p), int(max_itr), _prec(precision), N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+
+
+def MCcov(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg, sct_type,
+          ampfunc, ampfunc0=None, *, seed=0, path_offset=0, precision="fp64", stats=None):
+    """code/MCcov.m:1-299 on the GPU. Returns (Ms, Mm)."""
+    spec = make_problem(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS, smpFlg, sct_type,
+                        ampfunc, ampfunc0, seed=seed, path_offset=path_offset)
+    if not (np.isscalar(maxItr) and maxItr >= 1):
+        raise ScmcArgumentError("Invalid photons number")
+    return mccov_spec(spec, int(maxItr), precision, stats)
+
+
+def MCfield(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg, sct_type,
+            ampfunc, ampfunc0=None, lmean0=None, *, seed=0, path_offset=0, precision="fp64", stats=None):
+    """code/MCfield.m:1-254 on the GPU. Returns u (nv x nl)."""
+    spec = make_problem(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS, smpFlg, sct_type,
+                        ampfunc, ampfunc0, lmean0=lmean0, seed=seed, path_offset=path_offset)
+    if not (np.isscalar(maxItr) and maxItr >= 1):
+        raise ScmcArgumentError("Invalid photons number")
+    return mcfield_spec(spec, int(maxItr), precision, stats)
+
+
+def quad_grid(box_min, box_max, v_max, v_stp, lambda_):
+    """Grid sizes (nv0, nx, nz) of MCfieldFourierQuad.m:80-121."""
+    L = N.lib()
+    bmin = np.ascontiguousarray(np.asarray(box_min, dtype=np.float64).reshape(3))
+    bmax = np.ascontiguousarray(np.asarray(box_max, dtype=np.float64).reshape(3))
+    nv0, nx, nz = C.c_int32(), C.c_int32(), C.c_int32()
+    N.check(L.scmc_quad_grid(N.dptr(bmin), N.dptr(bmax), float(v_max), float(v_stp), float(lambda_),
+                             C.byref(nv0), C.byref(nx), C.byref(nz)))
+    return nv0.value, nx.value, nz.value
+
+
+def MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp, maxItr, lambda_, doCBS, smpFlg, lmean0=None,
+                       vsign=-1, *, sct_type=1, ampfunc=0, seed=0, path_offset=0, precision="mixed", stats=None):
+    """code/MCfieldFourierQuad.m:1-303 on the GPU. Returns u (nv0 x nv0 x nl).
+
+    ``sct_type``/``ampfunc`` select the walk's phase function; the default
+    (isotropic) is the reference's only case."""
+    spec = make_problem(sigt, albedo, box_min, box_max, l, np.zeros((3, 0)), 1, 1, lambda_, doCBS, smpFlg,
+                        sct_type, ampfunc, None, lmean0=lmean0, seed=seed, path_offset=path_offset,
+                        need_views=False)
+    if spec.dim != 3:
+        raise ScmcArgumentError("MCfieldFourierQuad is 3-D only")
+    nv0, _, _ = quad_grid(spec.box_min, spec.box_max, v_max, v_stp, lambda_)
+    nl = spec.nl
+    u_re, u_im = np.empty(nv0 * nv0 * nl), np.empty(nv0 * nv0 * nl)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(N.lib().scmc_mcfield_fourier_quad(C.byref(p), float(v_max), float(v_stp), int(maxItr), float(vsign),
+                                              _prec(precision), N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return (u_re + 1j * u_im).reshape((nv0, nv0, nl), order="F")
+
+
+# ------------------------------------------------------------ path sets
+
+def trace(spec, n_paths):
+    """Path set sampled on the GPU for paths [path_offset, path_offset+n)."""
+    L = N.lib()
+    p = c_problem(spec)
+    nvert = C.c_int64()
+    N.check(L.scmc_trace(C.byref(p), int(n_paths), C.byref(nvert), None, None, None, None, None, None, None, None,
+                         None, None))
+    K = nvert.value
+    ps = {
+        "offset": np.empty(n_paths + 1, dtype=np.int64),
+        "x1": np.empty(3 * n_paths), "px": np.empty(n_paths), "w1": np.empty(3 * n_paths),
+        "w0p_re": np.empty(n_paths), "w0p_im": np.empty(n_paths),
+        "x": np.empty(3 * K), "ow": np.empty(3 * K), "weight": np.empty(K), "xi": np.empty(K),
+    }
+    N.check(L.scmc_trace(C.byref(p), int(n_paths), C.byref(nvert), N.i64ptr(ps["offset"]), N.dptr(ps["x1"]),
+                         N.dptr(ps["px"]), N.dptr(ps["w1"]), N.dptr(ps["w0p_re"]), N.dptr(ps["w0p_im"]),
+                         N.dptr(ps["x"]), N.dptr(ps["ow"]), N.dptr(ps["weight"]), N.dptr(ps["xi"])))
+    return ps
+
+
+def c_pathset(ps):
+    c = N.PathSet()
+    c.n_paths = ps["offset"].size - 1
+    c.n_vertices = int(ps["offset"][-1])
+    c.offset = N.i64ptr(ps["offset"])
+    for k in ("x1", "px", "w1", "w0p_re", "w0p_im", "x", "ow", "weight", "xi"):
+        setattr(c, k, N.dptr(ps[k]))
+    return c
+
+
+def mccov_replay(spec, ps, max_itr, precision="fp64"):
+    L = N.lib()
+    nv, nl = spec.nv, spec.nl
+    ne = (nv * nl) ** 2
+    ms_re, ms_im, mm_re, mm_im = (np.empty(ne) for _ in range(4))
+    p = c_problem(spec)
+    cps = c_pathset(ps)
+    N.check(L.scmc_mccov_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(ms_re),
+                                N.dptr(ms_im), N.dptr(mm_re), N.dptr(mm_im)))
+    shape = (nv, nv, nl, nl)
+    return ((ms_re + 1j * ms_im).reshape(shape, order="F"), (mm_re + 1j * mm_im).reshape(shape, order="F"))
+
+
+def mcfield_replay(spec, ps, max_itr, precision="fp64"):
+    L = N.lib()
+    nv, nl = spec.nv, spec.nl
+    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    p = c_problem(spec)
+    cps = c_pathset(ps)
+    N.check(L.scmc_mcfield_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(u_re),
+                                  N.dptr(u_im)))
+    return (u_re + 1j * u_im).reshape((nv, nl), order="F")

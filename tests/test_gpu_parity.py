@@ -1,0 +1,156 @@
+"""GPU parity: the HIP engines (through the C-ABI) against the FP64 oracle.
+
+Tolerances (error metric for covariances: max_ij |C_ij - C*_ij| /
+sqrt(C*_ii C*_jj), the Cauchy-Schwarz scale, so near-zero off-diagonal
+entries do not blow up the relative error):
+  * replayed path sets, FP64 GPU path ............ <= 1e-12
+  * replayed path sets, mixed (FP16 hi/lo MFMA) .. <= 1e-4
+  * full runs, same seed, FP64 .................... <= 1e-10 (the GPU tracer
+    reproduces the oracle's walks; only transcendental ulps differ)
+  * fields: max |u - u*| / max |u*| with the same bounds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import cone_dirs, cov_matrix, cs_scale, far_dirs
+from mcspeckle_amd import engines as E
+from test_oracle_engines import small_spec, tab3d
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cs_err(M, Mref):
+    C, R = cov_matrix(M), cov_matrix(Mref)
+    s = cs_scale(R)
+    s = np.maximum(s, 1e-300)
+    return float(np.max(np.abs(C - R) / s))
+
+
+def field_err(u, uref):
+    return float(np.abs(u - uref).max() / np.abs(uref).max())
+
+
+def spec_cases():
+    out = []
+    for kind, dim, cbs, smp, amp0 in [("iso", 3, True, 2, None), ("hg", 3, False, 2, None), ("hg", 3, True, 1, 0.4),
+                                      ("tab", 3, True, 2, None), ("hg", 2, True, 2, None), ("iso", 2, False, 1, None)]:
+        out.append(pytest.param(kind, dim, cbs, smp, amp0, id=f"{kind}-{dim}d-cbs{int(cbs)}-smp{smp}-a0{amp0}"))
+    return out
+
+
+def build(kind, dim, cbs, smp, amp0, nv=5, nl=3, seed=11):
+    tab = tab3d() if kind == "tab" else None
+    return small_spec(kind, dim, cbs, nv=nv, nl=nl, seed=seed, smp=smp, amp0=amp0, tab=tab)
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
+def test_gpu_trace_reproduces_oracle_walks(oracle, kind, dim, cbs, smp, amp0):
+    spec = build(kind, dim, cbs, smp, amp0)
+    say("trace", kind, dim, cbs, smp, amp0)
+    n = 3000
+    g = E.trace(spec, n)
+    o = oracle.trace(spec, n)
+    same = np.diff(g["offset"]) == np.diff(o["offset"])
+    assert same.mean() > 0.999
+    assert np.array_equal(g["offset"], o["offset"])
+    np.testing.assert_allclose(g["x"], o["x"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(g["ow"], o["ow"], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(g["weight"], o["weight"])
+    np.testing.assert_array_equal(g["xi"], o["xi"])
+    np.testing.assert_allclose(g["px"], o["px"], rtol=1e-12)
+    np.testing.assert_allclose(g["w0p_re"], o["w0p_re"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_gpu_mccov_replay(oracle, kind, dim, cbs, smp, amp0, precision, tol):
+    spec = build(kind, dim, cbs, smp, amp0)
+    say("mccov replay", kind, dim, cbs, smp, amp0, precision)
+    n = 400
+    ps = oracle.trace(spec, n)
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, n)
+    Ms, Mm = E.mccov_replay(spec, ps, n, precision)
+    assert cs_err(Ms, Ms0) <= tol
+    assert cs_err(Mm, Mm0) <= tol
+    assert cs_err(Ms + Mm, Ms0 + Mm0) <= tol
+
+
+@pytest.mark.parametrize("nv,nl", [(1, 1), (7, 5), (64, 3), (130, 1), (20, 13)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_gpu_mccov_tile_edges(oracle, nv, nl, precision, tol):
+    """Sizes around the 64/128 tile edges, incl. N=1 and N not a multiple."""
+    rng = np.random.default_rng(nv * 100 + nl)
+    l, v = far_dirs(nl, 15, rng), far_dirs(nv, 15, rng)
+    spec = E.make_problem(6.0, 0.95, [-50, -50, 0], [50, 50, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.8, seed=5)
+    say("tile edges", nv, nl, precision)
+    n = 60
+    ps = oracle.trace(spec, n)
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, n)
+    Ms, Mm = E.mccov_replay(spec, ps, n, precision)
+    assert cs_err(Ms, Ms0) <= tol
+    assert cs_err(Mm, Mm0) <= tol
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
+def test_gpu_mccov_full_run_same_seed(oracle, kind, dim, cbs, smp, amp0):
+    spec = build(kind, dim, cbs, smp, amp0)
+    say("mccov full", kind, dim, cbs, smp, amp0)
+    n = 2000
+    Ms0, Mm0 = oracle.mccov(spec, n)
+    Ms, Mm = E.mccov_spec(spec, n, "fp64")
+    assert cs_err(Ms, Ms0) <= 1e-10
+    assert cs_err(Mm, Mm0) <= 1e-10
+    Ms1, Mm1 = E.mccov_spec(spec, n, "mixed")
+    assert cs_err(Ms1 + Mm1, Ms0 + Mm0) <= 1e-4
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+def test_gpu_mcfield_replay(oracle, kind, dim, cbs, smp, amp0, precision, tol):
+    spec = build(kind, dim, cbs, smp, amp0, nv=37, nl=3)
+    say("mcfield replay", kind, dim, cbs, smp, amp0, precision)
+    n = 300
+    ps = oracle.trace(spec, n)
+    u0 = oracle.mcfield_replay(spec, ps, n)
+    u = E.mcfield_replay(spec, ps, n, precision)
+    assert field_err(u, u0) <= tol
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
+def test_gpu_mcfield_full_run_same_seed(oracle, kind, dim, cbs, smp, amp0):
+    spec = build(kind, dim, cbs, smp, amp0, nv=300, nl=2)
+    say("mcfield full", kind, dim, cbs, smp, amp0)
+    n = 1000
+    u0 = oracle.mcfield(spec, n)
+    u = E.mcfield_spec(spec, n, "fp64")
+    assert field_err(u, u0) <= 1e-10
+    u1 = E.mcfield_spec(spec, n, "mixed")
+    assert field_err(u1, u0) <= 2e-5
+
+
+def test_gpu_headline_size_mixed_vs_fp64(oracle):
+    """4096^2 covariance (64 lights x 64 views on a 15 degree cone, HG 0.9,
+    slab OD 20): mixed vs FP64 on the same paths <= 1e-4, and the FP64 GPU
+    path vs the oracle on a few replayed paths <= 1e-12."""
+    rng = np.random.default_rng(2024)
+    l, v = cone_dirs(64, 15, rng), cone_dirs(64, 15, rng)
+    spec = E.make_problem(20.0, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.9, seed=0)
+    say("headline 4096 replay fp64 vs oracle (3 paths)")
+    ps = oracle.trace(spec, 3)
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, 3)
+    Ms, Mm = E.mccov_replay(spec, ps, 3, "fp64")
+    assert cs_err(Mm, Mm0) <= 1e-12
+    assert cs_err(Ms, Ms0) <= 1e-12
+    say("headline 4096 mixed vs fp64 (300 paths)")
+    Msd, Mmd = E.mccov_spec(spec, 300, "fp64")
+    Msm, Mmm = E.mccov_spec(spec, 300, "mixed")
+    err = cs_err(Msm + Mmm, Msd + Mmd)
+    say("headline mixed cs err", err)
+    assert err <= 1e-4
