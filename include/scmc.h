@@ -11,7 +11,7 @@
  * Conventions
  *  - Plain C types only.  Arrays are column-major like MATLAB.
  *  - Directions/positions are dim x n (dim = 2 or 3); the 1 x n angle form
- *    of MCcov.m:56-70 is converted by the caller (see scmc_angles_to_dirs).
+ *    of MCcov.m:56-70 is converted to [sin; cos] rows by the caller.
  *  - Complex outputs are split (re, im) when the imaginary pointer is
  *    non-NULL (R2017b mxGetPr/mxGetPi) and interleaved (re, im, re, im, ...)
  *    in the real pointer when the imaginary pointer is NULL (R2018a+).
@@ -95,6 +95,10 @@ typedef struct {
     double ms_accum;         /* covariance update / field accumulation         */
     double ms_total;         /* wall time of the call                          */
     double fp16_scale_log2;  /* mixed: last chunk's power-of-two scale         */
+    double ms_herk;          /* multiple-scattering covariance update kernels,
+                                HIP-event time on the solve stream             */
+    int64_t herk_launches;   /* number of those launches                        */
+    double herk_flops;       /* their algorithmic flops, 8 N^2 K                */
 } scmc_stats;
 
 /* Replayable path set (vertex-major), the format of the oracle's sampler and
@@ -130,6 +134,19 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr, double v_max, double v_stp
 /* Grid sizes of MCfieldFourierQuad.m:80-121 for the given arguments. */
 int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp,
                    double lambda, int32_t *nv0, int32_t *nx, int32_t *nz);
+
+/* ---- sessions: accumulate path ranges, read results at any time -------- */
+/* A session owns the device accumulators of one MCcov (kind 0) or MCfield
+ * (kind 1) problem.  scmc_session_run adds paths [first, first + n) (ids
+ * relative to pr->path_offset) and returns when they are accumulated;
+ * scmc_session_result normalises by max_itr without consuming the sums, so a
+ * run can be checkpointed, resumed or sharded (the parfor repeats of
+ * scmc.m:343-367 are disjoint path ranges of one session). */
+int scmc_session_create(const scmc_problem *pr, int32_t kind, int32_t precision, int64_t *session);
+int scmc_session_run(int64_t session, int64_t first, int64_t n_paths);
+int scmc_session_result(int64_t session, int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);
+int scmc_session_stats(int64_t session, scmc_stats *stats);
+int scmc_session_destroy(int64_t session);
 
 /* ---- replay / export (path-set parity harness) ------------------------ */
 

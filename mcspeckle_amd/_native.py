@@ -27,6 +27,8 @@ PRECISIONS = {"fp64": FP64, "mixed": MIXED}
 EXPORTS = [
     "scmc_mccov", "scmc_mcfield", "scmc_mcfield_fourier_quad", "scmc_quad_grid",
     "scmc_trace", "scmc_mccov_replay", "scmc_mcfield_replay",
+    "scmc_session_create", "scmc_session_run", "scmc_session_result", "scmc_session_stats",
+    "scmc_session_destroy",
     "scmc_server_start", "scmc_server_stop", "scmc_submit_mccov", "scmc_poll",
     "scmc_wait", "scmc_cancel", "scmc_request_footprint",
     "scmc_last_error", "scmc_version", "scmc_device_count",
@@ -61,6 +63,7 @@ class Stats(C.Structure):
         ("paths", C.c_int64), ("vertices", C.c_int64), ("chunks", C.c_int64),
         ("ms_trace", C.c_double), ("ms_nee", C.c_double), ("ms_accum", C.c_double),
         ("ms_total", C.c_double), ("fp16_scale_log2", C.c_double),
+        ("ms_herk", C.c_double), ("herk_launches", C.c_int64), ("herk_flops", C.c_double),
     ]
 
     def as_dict(self):
@@ -111,6 +114,11 @@ def lib():
         "scmc_trace": ([P, C.c_int64, _i64p, _i64p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "scmc_mccov_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp, _dp, _dp], C.c_int),
         "scmc_mcfield_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp], C.c_int),
+        "scmc_session_create": ([P, C.c_int32, C.c_int32, _i64p], C.c_int),
+        "scmc_session_run": ([C.c_int64, C.c_int64, C.c_int64], C.c_int),
+        "scmc_session_result": ([C.c_int64, C.c_int64, _dp, _dp, _dp, _dp], C.c_int),
+        "scmc_session_stats": ([C.c_int64, S], C.c_int),
+        "scmc_session_destroy": ([C.c_int64], C.c_int),
         "scmc_server_start": ([C.POINTER(ServerConfig)], C.c_int),
         "scmc_server_stop": ([], C.c_int),
         "scmc_submit_mccov": ([P, C.c_int64, C.c_int32, _i64p], C.c_int),

@@ -8,6 +8,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -376,10 +379,15 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     hp.acc = (double2 *)acc_m.p;
     hp.bound = (const unsigned *)bound.p;
     hp.ksplit = ksm;
+    hipEvent_t h0, h1;
+    (void)hipEventCreate(&h0);
+    (void)hipEventCreate(&h1);
+    (void)hipEventRecord(h0, s);
     if (mixed)
         launch_herk_mixed(hp, s);
     else
         launch_herk_fp64(hp, s);
+    (void)hipEventRecord(h1, s);
     // single scattering
     hp.k = np;
     hp.A1 = SA.p;
@@ -396,11 +404,19 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     (void)hipEventRecord(e2, s);
     if (!hip_ok(hipGetLastError(), "covariance launch", &rc)) return rc;
     if (!hip_ok(hipEventSynchronize(e2), "covariance sync", &rc)) return rc;
-    float t1 = 0, t2 = 0;
+    float t1 = 0, t2 = 0, th = 0;
     (void)hipEventElapsedTime(&t1, e0, e1);
     (void)hipEventElapsedTime(&t2, e1, e2);
+    (void)hipEventElapsedTime(&th, h0, h1);
     *ms_nee += t1;
     *ms_accum += t2;
+    if (k2 > 0) {
+        ms_herk += th;
+        herk_launches += 1;
+        herk_flops += 8.0 * (double)n * (double)n * (double)k2;
+    }
+    (void)hipEventDestroy(h0);
+    (void)hipEventDestroy(h1);
     if (mixed) {
         unsigned b = 0;
         (void)hipMemcpy(&b, bound.p, sizeof b, hipMemcpyDeviceToHost);
@@ -491,88 +507,196 @@ static bool far_field_only(const scmc_problem *pr, int *rc)
     return true;
 }
 
-int scmc_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *ms_re, double *ms_im,
-               double *mm_re, double *mm_im, scmc_stats *stats)
+} // extern "C"
+
+namespace scmc {
+
+void ProblemCopy::assign(const scmc_problem *src)
+{
+    pr = *src;
+    const int dim = src->dim;
+    l.assign(src->l, src->l + (size_t)dim * src->nl);
+    pr.l = l.data();
+    if (src->v && src->nv > 0) {
+        v.assign(src->v, src->v + (size_t)dim * src->nv);
+        pr.v = v.data();
+    }
+    auto cp = [](const scmc_phase &ph, scmc_phase &out, std::vector<double> &re, std::vector<double> &im,
+                 std::vector<double> &ic) {
+        if (ph.type != SCMC_TABULATED) return;
+        re.assign(ph.amp_re, ph.amp_re + ph.n_amp);
+        if (ph.amp_im)
+            im.assign(ph.amp_im, ph.amp_im + ph.n_amp);
+        else
+            im.assign((size_t)ph.n_amp, 0.0);
+        ic.assign(ph.icdf, ph.icdf + ph.n_icdf);
+        out.amp_re = re.data();
+        out.amp_im = im.data();
+        out.icdf = ic.data();
+    };
+    cp(src->amp, pr.amp, a_re, a_im, a_ic);
+    if (src->has_amp0) {
+        scmc_phase a0 = src->amp0;
+        a0.type = src->amp.type;
+        cp(a0, pr.amp0, b_re, b_im, b_ic);
+    }
+}
+
+int Session::init(const scmc_problem *pr, int kind_, int precision)
+{
+    int rc = validate_problem(pr, 1, true);
+    if (rc) return rc;
+    if (!far_field_only(pr, &rc)) return rc;
+    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
+    if (kind_ != 0 && kind_ != 1) return set_error(SCMC_E_ARG, "invalid session kind %d", kind_);
+    if ((rc = ensure_device(nullptr))) return rc;
+    kind = kind_;
+    mixed = precision == SCMC_MIXED;
+    pc.assign(pr);
+    if (!hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
+    if ((rc = ps.setup(&pc.pr, stream))) return rc;
+    memset(&st, 0, sizeof st);
+    return kind == 0 ? ce.init(ps, mixed, stream) : fe.init(ps, stream);
+}
+
+Session::~Session()
+{
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Session::run(int64_t first, int64_t n)
+{
+    int rc = SCMC_OK;
+    if (first < 0 || n < 0) return set_error(SCMC_E_ARG, "invalid path range");
+    for (int64_t p0 = first; p0 < first + n;) {
+        const int64_t rem = first + n - p0;
+        const int64_t np = kind == 0 ? chunk_paths(ce.n, rem, mean_len, mixed)
+                                     : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
+        auto a = std::chrono::steady_clock::now();
+        if ((rc = tc.run(ps.dp, (uint64_t)p0, np, stream))) return rc;
+        if (!hip_ok(hipStreamSynchronize(stream), "trace", &rc)) return rc;
+        st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+        if (kind == 0)
+            rc = ce.chunk(ps, tc, stream, &st.ms_accum, &st.ms_nee);
+        else
+            rc = fe.chunk(ps, tc, mixed, stream, &st.ms_accum);
+        if (rc) return rc;
+        st.paths += np;
+        st.vertices += tc.nvert;
+        st.chunks += 1;
+        mean_len = (double)st.vertices / (double)st.paths;
+        p0 += np;
+    }
+    return SCMC_OK;
+}
+
+int Session::result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im)
+{
+    if (max_itr < 1) return set_error(SCMC_E_ARG, "Invalid photons number");
+    if (kind == 0) return ce.finalize(ps, &pc.pr, max_itr, a_re, a_im, b_re, b_im, stream);
+    return fe.finalize(&pc.pr, max_itr, a_re, a_im, stream);
+}
+
+void Session::fill_stats(scmc_stats *out) const
+{
+    *out = st;
+    if (kind == 0) {
+        out->fp16_scale_log2 = ce.last_scale_log2;
+        out->ms_herk = ce.ms_herk;
+        out->herk_launches = ce.herk_launches;
+        out->herk_flops = ce.herk_flops;
+    }
+}
+
+static std::mutex g_sess_mu;
+static std::map<int64_t, std::unique_ptr<Session>> g_sessions;
+static int64_t g_next_session = 1;
+
+static Session *find_session(int64_t h)
+{
+    std::lock_guard<std::mutex> g(g_sess_mu);
+    auto it = g_sessions.find(h);
+    return it == g_sessions.end() ? nullptr : it->second.get();
+}
+
+} // namespace scmc
+
+extern "C" {
+
+int scmc_session_create(const scmc_problem *pr, int32_t kind, int32_t precision, int64_t *session)
+{
+    if (!session) return set_error(SCMC_E_ARG, "null session handle");
+    std::unique_ptr<Session> s(new Session());
+    int rc = s->init(pr, kind, precision);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_sess_mu);
+    *session = g_next_session++;
+    g_sessions[*session] = std::move(s);
+    return SCMC_OK;
+}
+
+int scmc_session_run(int64_t session, int64_t first, int64_t n_paths)
+{
+    Session *s = find_session(session);
+    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    return s->run(first, n_paths);
+}
+
+int scmc_session_result(int64_t session, int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im)
+{
+    Session *s = find_session(session);
+    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    return s->result(max_itr, a_re, a_im, b_re, b_im);
+}
+
+int scmc_session_stats(int64_t session, scmc_stats *stats)
+{
+    Session *s = find_session(session);
+    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    if (stats) s->fill_stats(stats);
+    return SCMC_OK;
+}
+
+int scmc_session_destroy(int64_t session)
+{
+    std::unique_ptr<Session> victim;
+    {
+        std::lock_guard<std::mutex> g(g_sess_mu);
+        auto it = g_sessions.find(session);
+        if (it == g_sessions.end()) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+        victim = std::move(it->second);
+        g_sessions.erase(it);
+    }
+    return SCMC_OK;
+}
+
+static int one_shot(const scmc_problem *pr, int kind, int64_t max_itr, int32_t precision, double *a_re, double *a_im,
+                    double *b_re, double *b_im, scmc_stats *stats)
 {
     auto t0 = std::chrono::steady_clock::now();
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
-    if (!far_field_only(pr, &rc)) return rc;
-    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
-    if ((rc = ensure_device(nullptr))) return rc;
-    hipStream_t s;
-    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
-    {
-        ProblemState ps;
-        TraceChunk tc;
-        CovEngine ce;
-        scmc_stats st;
-        memset(&st, 0, sizeof st);
-        const bool mixed = precision == SCMC_MIXED;
-        if (!(rc = ps.setup(pr, s)) && !(rc = ce.init(ps, mixed, s))) {
-            double mean_len = 20;
-            for (int64_t p0 = 0; p0 < max_itr && !rc;) {
-                int64_t np = chunk_paths(ce.n, max_itr - p0, mean_len, mixed);
-                auto a = std::chrono::steady_clock::now();
-                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
-                if (!hip_ok(hipStreamSynchronize(s), "trace", &rc)) break;
-                st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-                if ((rc = ce.chunk(ps, tc, s, &st.ms_accum, &st.ms_nee))) break;
-                st.paths += np;
-                st.vertices += tc.nvert;
-                st.chunks += 1;
-                mean_len = (double)st.vertices / (double)st.paths;
-                p0 += np;
-            }
-            if (!rc) rc = ce.finalize(ps, pr, max_itr, ms_re, ms_im, mm_re, mm_im, s);
-        }
-        st.fp16_scale_log2 = ce.last_scale_log2;
-        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (stats) *stats = st;
+    Session s;
+    if ((rc = s.init(pr, kind, precision))) return rc;
+    if ((rc = s.run(0, max_itr))) return rc;
+    rc = s.result(max_itr, a_re, a_im, b_re, b_im);
+    if (stats) {
+        s.fill_stats(stats);
+        stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    (void)hipStreamDestroy(s);
     return rc;
+}
+
+int scmc_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *ms_re, double *ms_im,
+               double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    return one_shot(pr, 0, max_itr, precision, ms_re, ms_im, mm_re, mm_im, stats);
 }
 
 int scmc_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *u_re, double *u_im,
                  scmc_stats *stats)
 {
-    auto t0 = std::chrono::steady_clock::now();
-    int rc = validate_problem(pr, max_itr, true);
-    if (rc) return rc;
-    if (!far_field_only(pr, &rc)) return rc;
-    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
-    if ((rc = ensure_device(nullptr))) return rc;
-    hipStream_t s;
-    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
-    {
-        ProblemState ps;
-        TraceChunk tc;
-        FieldEngine fe;
-        scmc_stats st;
-        memset(&st, 0, sizeof st);
-        if (!(rc = ps.setup(pr, s)) && !(rc = fe.init(ps, s))) {
-            double mean_len = 20;
-            for (int64_t p0 = 0; p0 < max_itr && !rc;) {
-                int64_t np = field_chunk_paths(pr->nv, pr->nl, max_itr - p0, mean_len);
-                auto a = std::chrono::steady_clock::now();
-                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
-                if (!hip_ok(hipStreamSynchronize(s), "trace", &rc)) break;
-                st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-                if ((rc = fe.chunk(ps, tc, precision == SCMC_MIXED, s, &st.ms_accum))) break;
-                st.paths += np;
-                st.vertices += tc.nvert;
-                st.chunks += 1;
-                mean_len = (double)st.vertices / (double)st.paths;
-                p0 += np;
-            }
-            if (!rc) rc = fe.finalize(pr, max_itr, u_re, u_im, s);
-        }
-        st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (stats) *stats = st;
-    }
-    (void)hipStreamDestroy(s);
-    return rc;
+    return one_shot(pr, 1, max_itr, precision, u_re, u_im, nullptr, nullptr, stats);
 }
 
 int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int64_t *offset, double *x1, double *px,

@@ -115,13 +115,16 @@ __device__ __forceinline__ float2 amp_f32(float c, const DevPhase &ph, float hg_
                              : hg_k * __builtin_amdgcn_rsqf(t);
         return make_float2(a, 0.f);
     }
-    float cc = fminf(fmaxf(c, -1.f), 1.f);
-    float ang = acosf(cc);
-    int n = ph.n_amp;
-    float scale = (DIM == 2) ? (float)((n - 1) / (2 * kPi)) : (float)((n - 1) / kPi);
-    int idx = (int)rintf(ang * scale);
-    idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
-    double2 a = ph.amp[idx];
+    return make_float2(0.f, 0.f);   // tabulated: see amp_tab (FP64 bin index)
+}
+
+// Tabulated amplitude with the nearest-bin index formed in FP64, so the bin
+// matches evalampfunc_amplitude.m:8 exactly (an FP32 acos flips bins near
+// their edges).
+template <int DIM>
+__device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
+{
+    double2 a = eval_amp<DIM>(c, ph);
     return make_float2((float)a.x, (float)a.y);
 }
 
@@ -195,8 +198,13 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                 float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
                 ev = make_double2(mag * cs, mag * sn);
                 if (f.kind != 0) {
-                    float c = (float)f.ad[0] * vf[0] + (float)f.ad[1] * vf[1] + (DIM == 3 ? (float)f.ad[2] * vf[2] : 0.f);
-                    am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
+                    if (P.amp.type == 2) {
+                        am = amp_tab<DIM>(dot<DIM>(f.ad, vd), P.amp);
+                    } else {
+                        float c = (float)f.ad[0] * vf[0] + (float)f.ad[1] * vf[1] +
+                                  (DIM == 3 ? (float)f.ad[2] * vf[2] : 0.f);
+                        am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
+                    }
                 }
             } else {
                 ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);

@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
         for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int i = ti * TM + wr * 32 + rb * 16 + 4 * kk + e;
+                // v_mfma_f64_16x16x4f64 result: register e of lane l holds row 4e + l/16, column l%16
+                const int i = ti * TM + wr * 32 + rb * 16 + 4 * e + kk;
                 const int j = tj * TM + wc * 32 + cb * 16 + l16;
                 if (i < n && j < n) {
                     double2 *p = &C[(size_t)i * n + j];

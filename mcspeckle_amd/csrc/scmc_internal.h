@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace scmc {
 
@@ -50,6 +51,8 @@ struct CovEngine {
     int nv = 0, nl = 0, n = 0, tile = 0;
     int ks_m = 0, ks_s = 0;
     int last_scale_log2 = 0;
+    double ms_herk = 0, herk_flops = 0;
+    int64_t herk_launches = 0;
     DevBuf acc_m, acc_s, bound;
     DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2;
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
@@ -66,6 +69,32 @@ struct FieldEngine {
     int init(const ProblemState &ps, hipStream_t s);
     int chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum);
     int finalize(const scmc_problem *pr, int64_t max_itr, double *u_re, double *u_im, hipStream_t s);
+};
+
+// Deep copy of a problem description (the session owns its arrays).
+struct ProblemCopy {
+    scmc_problem pr;
+    std::vector<double> l, v, a_re, a_im, a_ic, b_re, b_im, b_ic;
+    void assign(const scmc_problem *src);
+};
+
+// A solve in progress: device accumulators plus the traced-chunk buffers.
+struct Session {
+    int kind = 0;   // 0 MCcov, 1 MCfield
+    bool mixed = false;
+    ProblemCopy pc;
+    hipStream_t stream = nullptr;
+    ProblemState ps;
+    TraceChunk tc;
+    CovEngine ce;
+    FieldEngine fe;
+    scmc_stats st{};
+    double mean_len = 20;
+    int init(const scmc_problem *pr, int kind, int precision);
+    int run(int64_t first, int64_t n);
+    int result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);
+    void fill_stats(scmc_stats *out) const;
+    ~Session();
 };
 
 } // namespace scmc

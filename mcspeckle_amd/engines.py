@@ -315,3 +315,54 @@ def mcfield_replay(spec, ps, max_itr, precision="fp64"):
     N.check(L.scmc_mcfield_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(u_re),
                                   N.dptr(u_im)))
     return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+
+
+class Session:
+    """Device-resident accumulation of one MCcov (kind "cov") or MCfield
+    (kind "field") problem over path ranges; see scmc_session_* in scmc.h."""
+
+    def __init__(self, spec, kind="cov", precision="fp64"):
+        self.spec = spec
+        self.kind = kind
+        self._p = c_problem(spec)
+        h = C.c_int64()
+        N.check(N.lib().scmc_session_create(C.byref(self._p), 0 if kind == "cov" else 1, _prec(precision),
+                                            C.byref(h)))
+        self.handle = h.value
+
+    def run(self, first, n_paths):
+        N.check(N.lib().scmc_session_run(self.handle, int(first), int(n_paths)))
+
+    def stats(self):
+        st = N.Stats()
+        N.check(N.lib().scmc_session_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def result(self, max_itr):
+        nv, nl = self.spec.nv, self.spec.nl
+        if self.kind == "cov":
+            ne = (nv * nl) ** 2
+            a = [np.empty(ne) for _ in range(4)]
+            N.check(N.lib().scmc_session_result(self.handle, int(max_itr), *[N.dptr(x) for x in a]))
+            shape = (nv, nv, nl, nl)
+            return (a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F")
+        re, im = np.empty(nv * nl), np.empty(nv * nl)
+        N.check(N.lib().scmc_session_result(self.handle, int(max_itr), N.dptr(re), N.dptr(im), None, None))
+        return (re + 1j * im).reshape((nv, nl), order="F")
+
+    def close(self):
+        if self.handle:
+            N.lib().scmc_session_destroy(self.handle)
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
