@@ -124,7 +124,7 @@ def main():
                    "paths_per_step": P, "vertices_per_path": verts / (args.steps * P)},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP16, "traffic": None,
-                     "kernel": "herk_mixed_kernel<true>", "launches": herk_launches,
+                     "kernel": "herk_mixed_kernel", "expand_ms_per_launch": (st1["ms_expand"] - st0["ms_expand"]) / max(herk_launches, 1), "launches": herk_launches,
                      "avg_launch_ms": herk_ms / max(herk_launches, 1),
                      "flops_per_launch": herk_flops / max(herk_launches, 1),
                      "issued_mfma_flops_factor": 1.5},

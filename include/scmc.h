@@ -99,6 +99,7 @@ typedef struct {
                                 HIP-event time on the solve stream             */
     int64_t herk_launches;   /* number of those launches                        */
     double herk_flops;       /* their algorithmic flops, 8 N^2 K                */
+    double ms_expand;        /* mixed: FP16 hi/lo row expansion before them     */
 } scmc_stats;
 
 /* Replayable path set (vertex-major), the format of the oracle's sampler and

@@ -299,9 +299,10 @@ static int pick_ksplit(int64_t k, int n, int tile)
 {
     const int nt = (n + tile - 1) / tile;
     const int64_t tiles = (int64_t)nt * (nt + 1) / 2;
-    // aim for >= 4 waves of workgroups over 256 CUs, >= 1024 vertices per item
-    int64_t want = (2048 + tiles - 1) / tiles;
-    int64_t maxs = std::max<int64_t>(1, k / 1024);
+    // aim for >= 16 rounds of workgroups over 256 CUs x 2 (load balance) with
+    // >= 2048 vertices per work item (FP32 sums stay short, launch amortised)
+    int64_t want = (8192 + tiles - 1) / tiles;
+    int64_t maxs = std::max<int64_t>(1, k / 2048);
     int64_t ks = std::min<int64_t>(want, maxs);
     if (ks >= 8) ks = (ks / 8) * 8;   // keep splits aligned with the 8 XCDs
     return (int)std::max<int64_t>(1, std::min<int64_t>(ks, 32));
@@ -379,14 +380,21 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     hp.acc = (double2 *)acc_m.p;
     hp.bound = (const unsigned *)bound.p;
     hp.ksplit = ksm;
-    hipEvent_t h0, h1;
+    if (mixed && !zbuf.ensure(mixed_zbuf_bytes(std::max<int64_t>(k2, np), n), &rc)) return rc;
+    hp.zbuf = zbuf.p;
+    hipEvent_t h0, hm, h1;
     (void)hipEventCreate(&h0);
+    (void)hipEventCreate(&hm);
     (void)hipEventCreate(&h1);
     (void)hipEventRecord(h0, s);
-    if (mixed)
+    if (mixed) {
+        hp.ev_mid = hm;
         launch_herk_mixed(hp, s);
-    else
+        hp.ev_mid = nullptr;
+    } else {
         launch_herk_fp64(hp, s);
+        (void)hipEventRecord(hm, s);
+    }
     (void)hipEventRecord(h1, s);
     // single scattering
     hp.k = np;
@@ -404,18 +412,21 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     (void)hipEventRecord(e2, s);
     if (!hip_ok(hipGetLastError(), "covariance launch", &rc)) return rc;
     if (!hip_ok(hipEventSynchronize(e2), "covariance sync", &rc)) return rc;
-    float t1 = 0, t2 = 0, th = 0;
+    float t1 = 0, t2 = 0, th = 0, tx = 0;
     (void)hipEventElapsedTime(&t1, e0, e1);
     (void)hipEventElapsedTime(&t2, e1, e2);
-    (void)hipEventElapsedTime(&th, h0, h1);
+    (void)hipEventElapsedTime(&tx, h0, hm);
+    (void)hipEventElapsedTime(&th, mixed ? hm : h0, h1);
     *ms_nee += t1;
     *ms_accum += t2;
     if (k2 > 0) {
         ms_herk += th;
+        if (mixed) ms_expand += tx;
         herk_launches += 1;
         herk_flops += 8.0 * (double)n * (double)n * (double)k2;
     }
     (void)hipEventDestroy(h0);
+    (void)hipEventDestroy(hm);
     (void)hipEventDestroy(h1);
     if (mixed) {
         unsigned b = 0;
@@ -471,10 +482,10 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
 
 int64_t chunk_paths(int n, int64_t remaining, double mean_len, bool mixed)
 {
-    // vertices per chunk: large enough to fill the GPU, bounded by memory
-    double per_vertex = (double)n;   // factor + update cost proxy
-    int64_t target_vertices = (int64_t)std::max(65536.0, std::min(4.0e6, 6.0e10 / std::max(per_vertex, 1.0) / 64));
-    if (n >= 1024) target_vertices = mixed ? 400000 : 200000;
+    // vertices per chunk: large enough to fill the GPU, bounded by the
+    // expanded-row buffer (8 B per row and vertex, ~4 GB) and factor memory
+    double cap = mixed ? 4.0e9 / (8.0 * n) : 2.5e9 / (16.0 * n);
+    int64_t target_vertices = (int64_t)std::max(65536.0, std::min(4.0e6, cap));
     int64_t p = (int64_t)(target_vertices / std::max(mean_len, 1.0));
     p = std::max<int64_t>(256, std::min<int64_t>(p, 1 << 20));
     return std::min(p, remaining);
@@ -605,6 +616,7 @@ void Session::fill_stats(scmc_stats *out) const
         out->ms_herk = ce.ms_herk;
         out->herk_launches = ce.herk_launches;
         out->herk_flops = ce.herk_flops;
+        out->ms_expand = ce.ms_expand;
     }
 }
 

@@ -72,70 +72,103 @@ __device__ __forceinline__ int lidx(int stage, int tile, int plane, int ks, int 
 }
 } // namespace mixed
 
+// Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
+// for K-step ks (vertices 8ks..8ks+7) and row i = v + nv*l, scaled by the
+// chunk's power of two.  One thread per (row, K-step).
 template <bool CBS>
-__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
-                                                            const float2 *__restrict__ B1, const float2 *__restrict__ A2,
-                                                            const float2 *__restrict__ B2, double2 *__restrict__ acc,
-                                                            const unsigned *__restrict__ bound, int ksplit)
+__global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
+                                                           const float2 *__restrict__ B1, const float2 *__restrict__ A2,
+                                                           const float2 *__restrict__ B2, const unsigned *__restrict__ bound,
+                                                           uint4 *__restrict__ Z)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nks = (k + 7) / 8;
+    if (idx >= nks * n) return;
+    const int i = (int)(idx % n);
+    const int64_t ks = idx / n;
+    const int v = i % nv, l = i / nv;
+    float b = bound ? __uint_as_float(*bound) : 0.f;
+    int ex = 0;
+    if (b > 0.f) frexpf(b, &ex);
+    const float scale = ldexpf(1.f, 15 - ex);
+    half8 h[2], lo[2];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int64_t kk = ks * 8 + q;
+        float2 u = make_float2(0.f, 0.f);
+        if (kk < k) {
+            u = fcmul(A1[kk * nv + v], B1[kk * nl + l]);
+            if (CBS) {
+                float2 u2 = fcmul(A2[kk * nv + v], B2[kk * nl + l]);
+                u.x += u2.x;
+                u.y += u2.y;
+            }
+            u.x *= scale;
+            u.y *= scale;
+        }
+        _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
+        h[q >> 2][(q & 3) * 2] = hr;
+        h[q >> 2][(q & 3) * 2 + 1] = hi_;
+        lo[q >> 2][(q & 3) * 2] = (_Float16)(u.x - (float)hr);
+        lo[q >> 2][(q & 3) * 2 + 1] = (_Float16)(u.y - (float)hi_);
+    }
+    uint4 *d = Z + (ks * n + i) * 4;
+    d[0] = __builtin_bit_cast(uint4, h[0]);
+    d[1] = __builtin_bit_cast(uint4, h[1]);
+    d[2] = __builtin_bit_cast(uint4, lo[0]);
+    d[3] = __builtin_bit_cast(uint4, lo[1]);
+}
+
+// The Hermitian update over pre-expanded rows: a streaming GEMM.
+__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_total, const uint4 *__restrict__ Z,
+                                                            double2 *__restrict__ acc, const unsigned *__restrict__ bound,
+                                                            int ksplit)
 {
     using namespace mixed;
     __shared__ uint4 lds[2 * STAGEQ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int item = blockIdx.x;
-    const int s = item % ksplit;
+    const int s = item % ksplit;   // consecutive items -> consecutive XCDs share a k-range
     int ti, tj;
     tri_decode(item / ksplit, ti, tj);
     const bool diag = (ti == tj);
-    const int64_t kb = k * s / ksplit, ke = k * (s + 1) / ksplit;
+    const int64_t kb = nks_total * s / ksplit, ke = nks_total * (s + 1) / ksplit;   // in K-steps
 
-    // power-of-two scale: max|u| * scale < 2^15
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
     if (b > 0.f) frexpf(b, &ex);
-    const int sh = 15 - ex;
-    const float scale = ldexpf(1.f, sh);
-    const double unscale = ldexp(1.0, -2 * sh);
+    const double unscale = ldexp(1.0, -2 * (15 - ex));
 
-    // staging role: one row of the row tile (T=0) or the column tile (T=1)
+    // loading role: row r of the row tile (T=0) or the column tile (T=1)
     const int T = tid >> 7, r = tid & 127;
     const int gi = (T ? tj : ti) * TM + r;
     const bool rvalid = gi < n && !(diag && T == 1);
-    const int vv = rvalid ? gi % nv : 0, ll = rvalid ? gi / nv : 0;
+    const bool writer = !(diag && T == 1);
 
-    auto stage_fill = [&](int st, int buf) {
-        const int64_t k0 = kb + (int64_t)st * VPS;
+    uint4 pre[NKS][4];
+    auto load = [&](int st) {
 #pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-            half8 h[2], lo[2];
+        for (int q = 0; q < NKS; ++q) {
+            const int64_t ks = kb + (int64_t)st * NKS + q;
+            if (rvalid && ks < ke) {
+                const uint4 *src = Z + (ks * n + gi) * 4;
+                pre[q][0] = src[0];
+                pre[q][1] = src[1];
+                pre[q][2] = src[2];
+                pre[q][3] = src[3];
+            } else {
+                pre[q][0] = pre[q][1] = pre[q][2] = pre[q][3] = make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        if (!writer) return;
 #pragma unroll
-            for (int q = 0; q < KS; ++q) {
-                const int64_t kk = k0 + ks * KS + q;
-                float2 u = make_float2(0.f, 0.f);
-                if (rvalid && kk < ke) {
-                    u = fcmul(A1[kk * nv + vv], B1[kk * nl + ll]);
-                    if (CBS) {
-                        float2 u2 = fcmul(A2[kk * nv + vv], B2[kk * nl + ll]);
-                        u.x += u2.x;
-                        u.y += u2.y;
-                    }
-                    u.x *= scale;
-                    u.y *= scale;
-                }
-                _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
-                _Float16 lr = (_Float16)(u.x - (float)hr), li = (_Float16)(u.y - (float)hi_);
-                h[q >> 2][(q & 3) * 2] = hr;
-                h[q >> 2][(q & 3) * 2 + 1] = hi_;
-                lo[q >> 2][(q & 3) * 2] = lr;
-                lo[q >> 2][(q & 3) * 2 + 1] = li;
-            }
-            if (!(diag && T == 1)) {
-                uint4 *dh = &lds[lidx(buf, T, 0, ks, r, 0)];
-                uint4 *dl = &lds[lidx(buf, T, 1, ks, r, 0)];
-                dh[0] = __builtin_bit_cast(uint4, h[0]);
-                dh[1] = __builtin_bit_cast(uint4, h[1]);
-                dl[0] = __builtin_bit_cast(uint4, lo[0]);
-                dl[1] = __builtin_bit_cast(uint4, lo[1]);
-            }
+        for (int q = 0; q < NKS; ++q) {
+            lds[lidx(buf, T, 0, q, r, 0)] = pre[q][0];
+            lds[lidx(buf, T, 0, q, r, 1)] = pre[q][1];
+            lds[lidx(buf, T, 1, q, r, 0)] = pre[q][2];
+            lds[lidx(buf, T, 1, q, r, 1)] = pre[q][3];
         }
     };
 
@@ -153,13 +186,16 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int nv, int nl, int 
     const int wr = wave >> 1, wc = wave & 1;
     const int half = lane >> 5, lrow = lane & 31;
     const int btile = diag ? 0 : 1;
-    const int nstage = (int)((ke - kb + VPS - 1) / VPS);
+    const int nstage = (int)((ke - kb + NKS - 1) / NKS);
 
-    if (nstage > 0) stage_fill(0, 0);
+    if (nstage > 0) {
+        load(0);
+        store(0);
+    }
     __syncthreads();
     for (int st = 0; st < nstage; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nstage) stage_fill(st + 1, buf ^ 1);
+        if (st + 1 < nstage) load(st + 1);
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
             half8 ah[2], al[2], bh[2], bl[2];
@@ -189,6 +225,7 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int nv, int nl, int 
                 }
             }
         }
+        if (st + 1 < nstage) store(buf ^ 1);
         __syncthreads();
     }
 
@@ -339,17 +376,25 @@ static int ntiles(int n, int tm) { return (n + tm - 1) / tm; }
 void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
+    const int64_t nks = (hp.k + 7) / 8;
+    const int64_t nthr = nks * hp.n;
+    dim3 ge((unsigned)((nthr + 255) / 256));
+    if (hp.A2)
+        hipLaunchKernelGGL(expand_mixed_kernel<true>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,
+                           (const float2 *)hp.B2, hp.bound, (uint4 *)hp.zbuf);
+    else
+        hipLaunchKernelGGL(expand_mixed_kernel<false>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const float2 *)hp.A1, (const float2 *)hp.B1, nullptr, nullptr, hp.bound,
+                           (uint4 *)hp.zbuf);
+    if (hp.ev_mid) (void)hipEventRecord(hp.ev_mid, s);
     const int nt = ntiles(hp.n, kTileMixed);
     const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
-    if (hp.A2)
-        hipLaunchKernelGGL(herk_mixed_kernel<true>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
-                           (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,
-                           (const float2 *)hp.B2, hp.acc, hp.bound, hp.ksplit);
-    else
-        hipLaunchKernelGGL(herk_mixed_kernel<false>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
-                           (const float2 *)hp.A1, (const float2 *)hp.B1, nullptr, nullptr, hp.acc, hp.bound,
-                           hp.ksplit);
+    hipLaunchKernelGGL(herk_mixed_kernel, dim3(grid), dim3(256), 0, s, hp.n, nks, (const uint4 *)hp.zbuf, hp.acc,
+                       hp.bound, hp.ksplit);
 }
+
+size_t mixed_zbuf_bytes(int64_t k, int n) { return (size_t)((k + 7) / 8) * (size_t)n * 64; }
 
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
 {

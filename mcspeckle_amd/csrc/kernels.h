@@ -52,7 +52,11 @@ struct HerkProblem {
     double2 *acc;          // [ksplit][n][n] row-major FP64 accumulators
     const unsigned *bound; // mixed: float bits of max |u| (may be null)
     int ksplit;
+    void *zbuf = nullptr;          // mixed: expanded FP16 hi/lo rows (mixed_zbuf_bytes)
+    hipEvent_t ev_mid = nullptr;   // mixed: recorded between expansion and update
 };
+// Bytes of the expanded-row buffer of the mixed update for k terms.
+size_t mixed_zbuf_bytes(int64_t k, int n);
 // Tile edge of the mixed and FP64 kernels (the finaliser needs it).
 constexpr int kTileMixed = 128;
 constexpr int kTileF64 = 64;

@@ -51,9 +51,9 @@ struct CovEngine {
     int nv = 0, nl = 0, n = 0, tile = 0;
     int ks_m = 0, ks_s = 0;
     int last_scale_log2 = 0;
-    double ms_herk = 0, herk_flops = 0;
+    double ms_herk = 0, herk_flops = 0, ms_expand = 0;
     int64_t herk_launches = 0;
-    DevBuf acc_m, acc_s, bound;
+    DevBuf acc_m, acc_s, bound, zbuf;
     DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2;
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
