@@ -70,6 +70,10 @@ __device__ __forceinline__ int lidx(int stage, int tile, int plane, int ks, int 
 {
     return stage * STAGEQ + tile * TILEQ + plane * PLANE + (ks * TM + row) * ROWQ + q;
 }
+// The expanded buffer is padded with zeros to whole tiles and whole stages so
+// the update kernel's loads are unconditional (no waits at branch joins).
+__host__ __device__ __forceinline__ int pad_rows(int n) { return (n + TM - 1) / TM * TM; }
+__host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + NKS - 1) / NKS * NKS; }
 } // namespace mixed
 
 // Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
@@ -82,11 +86,13 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
                                                            uint4 *__restrict__ Z)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nks = (k + 7) / 8;
-    if (idx >= nks * n) return;
-    const int i = (int)(idx % n);
-    const int64_t ks = idx / n;
-    const int v = i % nv, l = i / nv;
+    const int npad = mixed::pad_rows(n);
+    const int64_t nks = mixed::pad_ksteps(k);
+    if (idx >= nks * npad) return;
+    const int i = (int)(idx % npad);
+    const int64_t ks = idx / npad;
+    const bool ok = i < n;
+    const int v = ok ? i % nv : 0, l = ok ? i / nv : 0;
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
     if (b > 0.f) frexpf(b, &ex);
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
     for (int q = 0; q < 8; ++q) {
         const int64_t kk = ks * 8 + q;
         float2 u = make_float2(0.f, 0.f);
-        if (kk < k) {
+        if (ok && kk < k) {
             u = fcmul(A1[kk * nv + v], B1[kk * nl + l]);
             if (CBS) {
                 float2 u2 = fcmul(A2[kk * nv + v], B2[kk * nl + l]);
@@ -112,7 +118,7 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
         lo[q >> 2][(q & 3) * 2] = (_Float16)(u.x - (float)hr);
         lo[q >> 2][(q & 3) * 2 + 1] = (_Float16)(u.y - (float)hi_);
     }
-    uint4 *d = Z + (ks * n + i) * 4;
+    uint4 *d = Z + (ks * npad + i) * 4;
     d[0] = __builtin_bit_cast(uint4, h[0]);
     d[1] = __builtin_bit_cast(uint4, h[1]);
     d[2] = __builtin_bit_cast(uint4, lo[0]);
@@ -132,45 +138,19 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
     int ti, tj;
     tri_decode(item / ksplit, ti, tj);
     const bool diag = (ti == tj);
-    const int64_t kb = nks_total * s / ksplit, ke = nks_total * (s + 1) / ksplit;   // in K-steps
+    const int npad = pad_rows(n);
+    const int64_t nst = nks_total / NKS;                                           // stages
+    const int64_t sb = nst * s / ksplit, se = nst * (s + 1) / ksplit;
 
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
     if (b > 0.f) frexpf(b, &ex);
     const double unscale = ldexp(1.0, -2 * (15 - ex));
 
-    // loading role: row r of the row tile (T=0) or the column tile (T=1)
+    // loading role: row r of the row tile (T=0) or the column tile (T=1);
+    // every load is in bounds (padded rows and K-steps are zero)
     const int T = tid >> 7, r = tid & 127;
     const int gi = (T ? tj : ti) * TM + r;
-    const bool rvalid = gi < n && !(diag && T == 1);
-    const bool writer = !(diag && T == 1);
-
-    uint4 pre[NKS][4];
-    auto load = [&](int st) {
-#pragma unroll
-        for (int q = 0; q < NKS; ++q) {
-            const int64_t ks = kb + (int64_t)st * NKS + q;
-            if (rvalid && ks < ke) {
-                const uint4 *src = Z + (ks * n + gi) * 4;
-                pre[q][0] = src[0];
-                pre[q][1] = src[1];
-                pre[q][2] = src[2];
-                pre[q][3] = src[3];
-            } else {
-                pre[q][0] = pre[q][1] = pre[q][2] = pre[q][3] = make_uint4(0, 0, 0, 0);
-            }
-        }
-    };
-    auto store = [&](int buf) {
-        if (!writer) return;
-#pragma unroll
-        for (int q = 0; q < NKS; ++q) {
-            lds[lidx(buf, T, 0, q, r, 0)] = pre[q][0];
-            lds[lidx(buf, T, 0, q, r, 1)] = pre[q][1];
-            lds[lidx(buf, T, 1, q, r, 0)] = pre[q][2];
-            lds[lidx(buf, T, 1, q, r, 1)] = pre[q][3];
-        }
-    };
 
     f32x16 accR[2][2], accI[2][2];
 #pragma unroll
@@ -186,16 +166,44 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
     const int wr = wave >> 1, wc = wave & 1;
     const int half = lane >> 5, lrow = lane & 31;
     const int btile = diag ? 0 : 1;
-    const int nstage = (int)((ke - kb + NKS - 1) / NKS);
+    const int nstage = (int)(se - sb);
+    static_assert(NKS == 2, "the prefetch below holds two K-steps");
+    // global source of this thread's row for stage st: Z[(st*NKS + q)*npad + gi]
+    const uint4 *zrow = Z + (size_t)gi * 4;
+    const size_t kstride = (size_t)npad * 4;   // uint4 per K-step
+    // LDS destinations of this thread's row (stage buffer 0)
+    const int d00 = lidx(0, T, 0, 0, r, 0), d01 = lidx(0, T, 1, 0, r, 0);
+    const int d10 = lidx(0, T, 0, 1, r, 0), d11 = lidx(0, T, 1, 1, r, 0);
+
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7;
+#define SCMC_LOAD(st)                                                   \
+    do {                                                                \
+        const uint4 *z0 = zrow + (size_t)(st) * NKS * kstride;          \
+        const uint4 *z1 = z0 + kstride;                                 \
+        p0 = z0[0]; p1 = z0[1]; p2 = z0[2]; p3 = z0[3];                 \
+        p4 = z1[0]; p5 = z1[1]; p6 = z1[2]; p7 = z1[3];                 \
+    } while (0)
+#define SCMC_STORE(buf)                                                 \
+    do {                                                                \
+        const int o = (buf) * STAGEQ;                                   \
+        lds[o + d00] = p0; lds[o + d00 + 1] = p1;                       \
+        lds[o + d01] = p2; lds[o + d01 + 1] = p3;                       \
+        lds[o + d10] = p4; lds[o + d10 + 1] = p5;                       \
+        lds[o + d11] = p6; lds[o + d11 + 1] = p7;                       \
+    } while (0)
 
     if (nstage > 0) {
-        load(0);
-        store(0);
+        SCMC_LOAD(sb);
+        SCMC_STORE(0);
     }
     __syncthreads();
     for (int st = 0; st < nstage; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nstage) load(st + 1);
+        // branch-free prefetch (the last iteration reloads a valid stage); the
+        // scheduling barrier keeps the loads ahead of the MFMA block so their
+        // latency overlaps the matrix work.
+        SCMC_LOAD(sb + (st + 1 < nstage ? st + 1 : st));
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
             half8 ah[2], al[2], bh[2], bl[2];
@@ -225,9 +233,12 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
                 }
             }
         }
-        if (st + 1 < nstage) store(buf ^ 1);
+        // (on a diagonal tile the column half of LDS is written but not read)
+        SCMC_STORE(buf ^ 1);
         __syncthreads();
     }
+#undef SCMC_LOAD
+#undef SCMC_STORE
 
     // epilogue: add the unscaled FP32 tile into accumulator copy s
     double2 *C = acc + (size_t)s * n * n;
@@ -376,8 +387,8 @@ static int ntiles(int n, int tm) { return (n + tm - 1) / tm; }
 void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
-    const int64_t nks = (hp.k + 7) / 8;
-    const int64_t nthr = nks * hp.n;
+    const int64_t nks = mixed::pad_ksteps(hp.k);
+    const int64_t nthr = nks * mixed::pad_rows(hp.n);
     dim3 ge((unsigned)((nthr + 255) / 256));
     if (hp.A2)
         hipLaunchKernelGGL(expand_mixed_kernel<true>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
@@ -394,7 +405,7 @@ void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
                        hp.bound, hp.ksplit);
 }
 
-size_t mixed_zbuf_bytes(int64_t k, int n) { return (size_t)((k + 7) / 8) * (size_t)n * 64; }
+size_t mixed_zbuf_bytes(int64_t k, int n) { return (size_t)mixed::pad_ksteps(k) * (size_t)mixed::pad_rows(n) * 64; }
 
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
 {
