@@ -382,6 +382,13 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     hp.ksplit = ksm;
     if (mixed && !zbuf.ensure(mixed_zbuf_bytes(std::max<int64_t>(k2, np), n), &rc)) return rc;
     hp.zbuf = zbuf.p;
+    if (mixed && !order.p) {
+        std::vector<int> o = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
+        if (!order.ensure(sizeof(int) * o.size(), &rc)) return rc;
+        if (!hip_ok(hipMemcpy(order.p, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice), "tile order", &rc))
+            return rc;
+    }
+    hp.order = order.p;
     hipEvent_t h0, hm, h1;
     (void)hipEventCreate(&h0);
     (void)hipEventCreate(&hm);

@@ -24,6 +24,9 @@
 // the upper triangle by Hermitian symmetry.
 #include "kernels.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace scmc {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -37,6 +40,23 @@ __device__ __forceinline__ void tri_decode(int t, int &ti, int &tj)
     while (a * (a + 1) / 2 > t) --a;
     ti = a;
     tj = t - a * (a + 1) / 2;
+}
+
+// Work item -> (k-split s, tile rank t).  Workgroups are dispatched to the 8
+// XCDs round robin (XCD = blockIdx % 8).  When ksplit is a multiple of 8 each
+// XCD gets its own k-splits and walks all tiles of one split in the blocked
+// order of `order` before the next, so the workgroups resident on one XCD
+// share a k-range and an 8x8 block of tiles: their rows stay in that XCD's L2.
+__device__ __forceinline__ void item_decode(int b, int ksplit, int ntile, int &s, int &t)
+{
+    if ((ksplit & 7) == 0) {
+        const int x = b & 7, j = b >> 3;
+        s = x + 8 * (j / ntile);
+        t = j % ntile;
+    } else {
+        s = b % ksplit;
+        t = b / ksplit;
+    }
 }
 
 __device__ __forceinline__ float2 fcmul(float2 a, float2 b)
@@ -73,7 +93,8 @@ __device__ __forceinline__ int lidx(int stage, int tile, int plane, int ks, int 
 // The expanded buffer is padded with zeros to whole tiles and whole stages so
 // the update kernel's loads are unconditional (no waits at branch joins).
 __host__ __device__ __forceinline__ int pad_rows(int n) { return (n + TM - 1) / TM * TM; }
-__host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + NKS - 1) / NKS * NKS; }
+// K-steps are padded to whole stage pairs (the update loop is unrolled by two).
+__host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + 2 * NKS - 1) / (2 * NKS) * (2 * NKS); }
 } // namespace mixed
 
 // Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
@@ -128,19 +149,18 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
 // The Hermitian update over pre-expanded rows: a streaming GEMM.
 __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_total, const uint4 *__restrict__ Z,
                                                             double2 *__restrict__ acc, const unsigned *__restrict__ bound,
-                                                            int ksplit)
+                                                            int ksplit, const int2 *__restrict__ order, int ntile)
 {
     using namespace mixed;
     __shared__ uint4 lds[2 * STAGEQ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int item = blockIdx.x;
-    const int s = item % ksplit;   // consecutive items -> consecutive XCDs share a k-range
-    int ti, tj;
-    tri_decode(item / ksplit, ti, tj);
+    int s, t;
+    item_decode(blockIdx.x, ksplit, ntile, s, t);
+    const int ti = order[t].x, tj = order[t].y;
     const bool diag = (ti == tj);
     const int npad = pad_rows(n);
-    const int64_t nst = nks_total / NKS;                                           // stages
-    const int64_t sb = nst * s / ksplit, se = nst * (s + 1) / ksplit;
+    const int64_t npair = nks_total / (2 * NKS);                                   // stage pairs
+    const int64_t sb = 2 * (npair * s / ksplit), se = 2 * (npair * (s + 1) / ksplit);
 
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
@@ -175,68 +195,80 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
     const int d00 = lidx(0, T, 0, 0, r, 0), d01 = lidx(0, T, 1, 0, r, 0);
     const int d10 = lidx(0, T, 0, 1, r, 0), d11 = lidx(0, T, 1, 1, r, 0);
 
-    uint4 p0, p1, p2, p3, p4, p5, p6, p7;
-#define SCMC_LOAD(st)                                                   \
+    // Two register sets (X, Y) hold the next two stages: loads run two MFMA
+    // blocks ahead of their LDS store, which covers HBM latency under load.
+    uint4 x0, x1, x2, x3, x4, x5, x6, x7, y0, y1, y2, y3, y4, y5, y6, y7;
+#define SCMC_LOAD(R, st)                                                \
     do {                                                                \
         const uint4 *z0 = zrow + (size_t)(st) * NKS * kstride;          \
         const uint4 *z1 = z0 + kstride;                                 \
-        p0 = z0[0]; p1 = z0[1]; p2 = z0[2]; p3 = z0[3];                 \
-        p4 = z1[0]; p5 = z1[1]; p6 = z1[2]; p7 = z1[3];                 \
+        R##0 = z0[0]; R##1 = z0[1]; R##2 = z0[2]; R##3 = z0[3];         \
+        R##4 = z1[0]; R##5 = z1[1]; R##6 = z1[2]; R##7 = z1[3];         \
     } while (0)
-#define SCMC_STORE(buf)                                                 \
+#define SCMC_STORE(R, buf)                                              \
     do {                                                                \
         const int o = (buf) * STAGEQ;                                   \
-        lds[o + d00] = p0; lds[o + d00 + 1] = p1;                       \
-        lds[o + d01] = p2; lds[o + d01 + 1] = p3;                       \
-        lds[o + d10] = p4; lds[o + d10 + 1] = p5;                       \
-        lds[o + d11] = p6; lds[o + d11 + 1] = p7;                       \
+        lds[o + d00] = R##0; lds[o + d00 + 1] = R##1;                   \
+        lds[o + d01] = R##2; lds[o + d01 + 1] = R##3;                   \
+        lds[o + d10] = R##4; lds[o + d10 + 1] = R##5;                   \
+        lds[o + d11] = R##6; lds[o + d11 + 1] = R##7;                   \
+    } while (0)
+#define SCMC_MFMA_STAGE(buf)                                                                                    \
+    do {                                                                                                        \
+        _Pragma("unroll") for (int ks = 0; ks < NKS; ++ks)                                                      \
+        {                                                                                                       \
+            half8 ah[2], al[2], bh[2], bl[2];                                                                   \
+            _Pragma("unroll") for (int rb = 0; rb < 2; ++rb)                                                    \
+            {                                                                                                   \
+                const int row = wr * 64 + rb * 32 + lrow;                                                       \
+                ah[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 0, ks, row, half)]);                        \
+                al[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 1, ks, row, half)]);                        \
+            }                                                                                                   \
+            _Pragma("unroll") for (int cb = 0; cb < 2; ++cb)                                                    \
+            {                                                                                                   \
+                const int col = wc * 64 + cb * 32 + lrow;                                                       \
+                bh[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 0, ks, col, half)]);                    \
+                bl[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 1, ks, col, half)]);                    \
+            }                                                                                                   \
+            _Pragma("unroll") for (int rb = 0; rb < 2; ++rb)                                                    \
+            {                                                                                                   \
+                const half8 rh = rot_neg_i(ah[rb]), rl = rot_neg_i(al[rb]);                                     \
+                _Pragma("unroll") for (int cb = 0; cb < 2; ++cb)                                                \
+                {                                                                                               \
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bh[cb], accR[rb][cb], 0, 0, 0); \
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bh[cb], accI[rb][cb], 0, 0, 0);     \
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bl[cb], accR[rb][cb], 0, 0, 0); \
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bl[cb], accI[rb][cb], 0, 0, 0);     \
+                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rb], bh[cb], accR[rb][cb], 0, 0, 0); \
+                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, bh[cb], accI[rb][cb], 0, 0, 0);     \
+                }                                                                                               \
+            }                                                                                                   \
+        }                                                                                                       \
     } while (0)
 
+    // nstage is even; stages are clamped so every load stays inside the item
+    const int last = nstage - 1;
     if (nstage > 0) {
-        SCMC_LOAD(sb);
-        SCMC_STORE(0);
+        SCMC_LOAD(x, sb);
+        SCMC_STORE(x, 0);
+        SCMC_LOAD(x, sb + (1 < last ? 1 : last));
     }
     __syncthreads();
-    for (int st = 0; st < nstage; ++st) {
-        const int buf = st & 1;
-        // branch-free prefetch (the last iteration reloads a valid stage); the
-        // scheduling barrier keeps the loads ahead of the MFMA block so their
-        // latency overlaps the matrix work.
-        SCMC_LOAD(sb + (st + 1 < nstage ? st + 1 : st));
+    for (int st = 0; st < nstage; st += 2) {
+        // stage st is in LDS buffer 0, stage st+1 in registers X
+        SCMC_LOAD(y, sb + (st + 2 < last ? st + 2 : last));
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-            half8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                const int row = wr * 64 + rb * 32 + lrow;
-                ah[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 0, ks, row, half)]);
-                al[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 1, ks, row, half)]);
-            }
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int col = wc * 64 + cb * 32 + lrow;
-                bh[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 0, ks, col, half)]);
-                bl[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 1, ks, col, half)]);
-            }
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                const half8 rh = rot_neg_i(ah[rb]), rl = rot_neg_i(al[rb]);
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bh[cb], accR[rb][cb], 0, 0, 0);
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bh[cb], accI[rb][cb], 0, 0, 0);
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bl[cb], accR[rb][cb], 0, 0, 0);
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bl[cb], accI[rb][cb], 0, 0, 0);
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rb], bh[cb], accR[rb][cb], 0, 0, 0);
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, bh[cb], accI[rb][cb], 0, 0, 0);
-                }
-            }
-        }
-        // (on a diagonal tile the column half of LDS is written but not read)
-        SCMC_STORE(buf ^ 1);
+        SCMC_MFMA_STAGE(0);
+        SCMC_STORE(x, 1);
+        __syncthreads();
+        // stage st+1 is in LDS buffer 1, stage st+2 in registers Y
+        SCMC_LOAD(x, sb + (st + 3 < last ? st + 3 : last));
+        __builtin_amdgcn_sched_barrier(0);
+        SCMC_MFMA_STAGE(1);
+        SCMC_STORE(y, 0);
         __syncthreads();
     }
+#undef SCMC_MFMA_STAGE
 #undef SCMC_LOAD
 #undef SCMC_STORE
 
@@ -400,9 +432,27 @@ void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
                            (uint4 *)hp.zbuf);
     if (hp.ev_mid) (void)hipEventRecord(hp.ev_mid, s);
     const int nt = ntiles(hp.n, kTileMixed);
-    const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
+    const int ntile = nt * (nt + 1) / 2;
+    const unsigned grid = (unsigned)(ntile * hp.ksplit);
     hipLaunchKernelGGL(herk_mixed_kernel, dim3(grid), dim3(256), 0, s, hp.n, nks, (const uint4 *)hp.zbuf, hp.acc,
-                       hp.bound, hp.ksplit);
+                       hp.bound, hp.ksplit, (const int2 *)hp.order, ntile);
+}
+
+// Lower-triangle tiles (ti >= tj) enumerated by 8x8 super-blocks.
+std::vector<int> blocked_tile_order(int nt)
+{
+    const int SB = 8;
+    std::vector<int> o;
+    const int ns = (nt + SB - 1) / SB;
+    for (int I = 0; I < ns; ++I)
+        for (int J = 0; J <= I; ++J)
+            for (int ti = I * SB; ti < std::min(nt, I * SB + SB); ++ti)
+                for (int tj = J * SB; tj < std::min(nt, J * SB + SB); ++tj)
+                    if (ti >= tj) {
+                        o.push_back(ti);
+                        o.push_back(tj);
+                    }
+    return o;
 }
 
 size_t mixed_zbuf_bytes(int64_t k, int n) { return (size_t)mixed::pad_ksteps(k) * (size_t)mixed::pad_rows(n) * 64; }

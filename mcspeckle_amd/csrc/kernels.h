@@ -2,6 +2,8 @@
 #pragma once
 #include "device_common.h"
 
+#include <vector>
+
 namespace scmc {
 
 // One scattering vertex (64 B): position, incoming direction, albedo weight
@@ -53,8 +55,12 @@ struct HerkProblem {
     const unsigned *bound; // mixed: float bits of max |u| (may be null)
     int ksplit;
     void *zbuf = nullptr;          // mixed: expanded FP16 hi/lo rows (mixed_zbuf_bytes)
+    const void *order = nullptr;   // mixed: int2 tile order (blocked_tile_order)
     hipEvent_t ev_mid = nullptr;   // mixed: recorded between expansion and update
 };
+// (ti, tj) pairs of the lower-triangle tiles in the blocked order used by
+// the mixed update, flattened.
+std::vector<int> blocked_tile_order(int nt);
 // Bytes of the expanded-row buffer of the mixed update for k terms.
 size_t mixed_zbuf_bytes(int64_t k, int n);
 // Tile edge of the mixed and FP64 kernels (the finaliser needs it).

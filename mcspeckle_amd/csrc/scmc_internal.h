@@ -53,7 +53,7 @@ struct CovEngine {
     int last_scale_log2 = 0;
     double ms_herk = 0, herk_flops = 0, ms_expand = 0;
     int64_t herk_launches = 0;
-    DevBuf acc_m, acc_s, bound, zbuf;
+    DevBuf acc_m, acc_s, bound, zbuf, order;
     DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2;
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
