@@ -4,15 +4,6 @@
 using namespace scmc;
 
 extern "C" {
-int scmc_mcfield_fourier_quad(const scmc_problem *, double, double, int64_t, double, int32_t, double *, double *,
-                              scmc_stats *)
-{
-    return set_error(SCMC_E_UNSUPPORTED, "MCfieldFourierQuad engine not built");
-}
-int scmc_quad_grid(const double *, const double *, double, double, double, int32_t *, int32_t *, int32_t *)
-{
-    return set_error(SCMC_E_UNSUPPORTED, "MCfieldFourierQuad engine not built");
-}
 int scmc_server_start(const scmc_server_config *) { return set_error(SCMC_E_UNSUPPORTED, "server not built"); }
 int scmc_server_stop(void) { return SCMC_OK; }
 int scmc_submit_mccov(const scmc_problem *, int64_t, int32_t, int64_t *)

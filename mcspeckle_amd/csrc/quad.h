@@ -1,0 +1,34 @@
+// quad.h -- geometry of the Fourier-accelerated field (MCfieldFourierQuad.m:80-138).
+#pragma once
+#include "kernels.h"
+
+namespace scmc {
+
+constexpr int kQuadMaxSlabs = 4096;
+
+// Grid geometry shared by the host set-up and the scatter kernel.
+struct QuadGeom {
+    int nv0;          // views per axis (odd)
+    int nx;           // lateral cells per axis (odd)
+    int nfft;         // padded transform size, max(nx, nv0)
+    int nx_pad;       // zero padding of the cell grid on each side
+    int nva_pad;      // crop of the transform on each side
+    int nz;           // number of slab boundaries (slabs = nz - 1)
+    double box_stp, z_bin, gx0, gy0;
+    double l_diratt, l_idiratt;   // 1/|l_z| by sign of l(end) (MCfieldFourierQuad.m:127-128)
+    const double *gz;             // nz slab boundaries (device)
+};
+
+// Host restatement of MCfieldFourierQuad.m:80-121; fills the enlarged box.
+int quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp, double lambda,
+                    QuadGeom &g, double *bmin_out, double *bmax_out, std::vector<double> *gz_out);
+
+void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
+                         const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,
+                         hipStream_t s);
+void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
+                  int batch, bool transB, bool fp32, hipStream_t s);
+void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,
+                         bool fp32, hipStream_t s);
+
+} // namespace scmc

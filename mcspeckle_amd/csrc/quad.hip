@@ -1,0 +1,217 @@
+// quad.hip -- Fourier-accelerated field (code/MCfieldFourierQuad.m).
+//
+// Per stored vertex (pxL = sqrt(W/px) e^{2 pi i xi}, /sqrt 2 for pL > 1 with
+// CBS, MCfieldFourierQuad.m:181-182) the reference adds, for every light,
+//     pxL * exp(-i k (tz - z_s)) * exp(i k l.x_L - sigma/2 att(x_L))
+// into the xy cell of the "scatter" position x_s in the z-slab of z_s
+// (MCfieldFourierQuad.m:263-273): pass 1 x_s = x1, x_L = x_n; pass 2 (CBS)
+// swaps them (:224-232).  Each slab grid is then Fourier transformed
+// (fftshift/flip/crop of :276-280) and summed with the slab phase
+// exp(zexp0 tz) (:281-282).
+//
+// GPU mapping: one lane per vertex scatters with hardware atomics into a
+// per-(slab, light) grid resident in HBM (FP64 or FP32); the 2-D DFT of every
+// slab is two batched complex GEMMs H = W X W^T with W[o][ix] =
+// exp(+2 pi i (f_o q_ix mod n)/n) (the shift/flip/crop folded into the
+// indices, twiddles reduced exactly in integers); a combine kernel applies
+// the slab phases and sums over slabs.
+#include "kernels.h"
+#include "quad.h"
+
+namespace scmc {
+
+template <typename T> struct cx;
+template <> struct cx<double> { typedef double2 t; };
+template <> struct cx<float> { typedef float2 t; };
+
+__device__ __forceinline__ void atomic_add_c(double2 *p, double re, double im)
+{
+    unsafeAtomicAdd(&p->x, re);
+    unsafeAtomicAdd(&p->y, im);
+}
+__device__ __forceinline__ void atomic_add_c(float2 *p, double re, double im)
+{
+    unsafeAtomicAdd(&p->x, (float)re);
+    unsafeAtomicAdd(&p->y, (float)im);
+}
+
+// First slab j with z < gz[j+1] (the while loop of MCfieldFourierQuad.m:249);
+// -1 when z is at or above the last boundary.
+__device__ __forceinline__ int slab_of(double z, const QuadGeom &g)
+{
+    if (!(z < g.gz[g.nz - 1])) return -1;
+    int j = (int)floor((z - g.gz[0]) / g.z_bin);
+    j = j < 0 ? 0 : (j > g.nz - 2 ? g.nz - 2 : j);
+    while (j > 0 && z < g.gz[j]) --j;
+    while (j < g.nz - 2 && !(z < g.gz[j + 1])) ++j;
+    return j;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeom g, int64_t nvert, const PathRec *paths,
+                                                           const VertexRec *verts, const int32_t *vpath,
+                                                           const int64_t *offsets, typename cx<T>::t *grid)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvert) return;
+    const VertexRec r = verts[e];
+    const int p = vpath[e];
+    const PathRec pr = paths[p];
+    const bool first = (e == offsets[p]);
+    double amp = sqrt(r.weight / pr.px);
+    if (P.do_cbs && !first) amp *= 1 / sqrt(2.0);
+    double sx, cxv;
+    sincos(2 * kPi * r.xi, &sx, &cxv);
+    const double pre = amp * cxv, pim = amp * sx;
+    for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
+        const double *xs = pass == 0 ? pr.x1 : r.x;   // scatter (view-side) position
+        const double *xl = pass == 0 ? r.x : pr.x1;   // light-side position
+        const int j = slab_of(xs[2], g);
+        if (j < 0) continue;
+        long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
+        ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
+        iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
+        const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
+        double s1, c1;
+        sincos(-P.k * (-xs[2] + tz), &s1, &c1);
+        const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
+        for (int l = 0; l < P.nl; ++l) {
+            const double *lc = P.l + 3 * l;
+            double ph = P.k * dot<3>(lc, xl);
+            double s2, c2;
+            sincos(ph, &s2, &c2);
+            const double m2 = exp(att);
+            // pxL * e1 * e2
+            double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
+            double br = m2 * c2, bi = m2 * s2;
+            double tr = ar * br - ai * bi, tim = ar * bi + ai * br;
+            atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], tr, tim);
+        }
+    }
+}
+
+// Batched complex GEMM C[b] = A[b] * B[b] (column-major, lda = M, ldb = K,
+// ldc = M), strides in elements (0 = shared operand).  64x64 tile per
+// workgroup, 4x4 outputs per thread, 16-deep K panels in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void cgemm_kernel(int M, int N, int K, const typename cx<T>::t *A, int64_t sa,
+                                                    const typename cx<T>::t *B, int64_t sb, typename cx<T>::t *C,
+                                                    int64_t sc, bool transB)
+{
+    typedef typename cx<T>::t C2;
+    __shared__ C2 As[16][64 + 1];
+    __shared__ C2 Bs[16][64 + 1];
+    const int b = blockIdx.z;
+    const C2 *Ab = A + (size_t)b * sa;
+    const C2 *Bb = B + (size_t)b * sb;
+    C2 *Cb = C + (size_t)b * sc;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    T accr[4][4], acci[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) accr[i][j] = acci[i][j] = 0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        for (int t = threadIdx.x; t < 16 * 64; t += 256) {
+            const int kk = t / 64, mm = t % 64;
+            const int gm = m0 + mm, gk = k0 + kk;
+            C2 a;
+            a.x = a.y = 0;
+            if (gm < M && gk < K) a = Ab[(size_t)gk * M + gm];
+            As[kk][mm] = a;
+            const int gn = n0 + mm;
+            C2 bb;
+            bb.x = bb.y = 0;
+            if (gn < N && gk < K) bb = transB ? Bb[(size_t)gk * N + gn] : Bb[(size_t)gn * K + gk];
+            Bs[kk][mm] = bb;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 16; ++kk) {
+            C2 av[4], bv[4];
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][tx + 16 * i];
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][ty + 16 * j];
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    accr[i][j] += av[i].x * bv[j].x - av[i].y * bv[j].y;
+                    acci[i][j] += av[i].x * bv[j].y + av[i].y * bv[j].x;
+                }
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const int gm = m0 + tx + 16 * i, gn = n0 + ty + 16 * j;
+            if (gm < M && gn < N) {
+                C2 c;
+                c.x = accr[i][j];
+                c.y = acci[i][j];
+                Cb[(size_t)gn * M + gm] = c;
+            }
+        }
+}
+
+// u[o, l] += sum_s H[s, l][o] * exp(zexp0[o] * tz_s)   (MCfieldFourierQuad.m:281-282)
+template <typename T>
+__global__ void quad_combine_kernel(int nv0, int nl, int nz, const typename cx<T>::t *H, const double2 *zexp0,
+                                    const double *gz, double2 *u)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nvv = (int64_t)nv0 * nv0;
+    if (idx >= nvv * nl) return;
+    const int64_t o = idx % nvv;
+    const int l = (int)(idx / nvv);
+    const double2 ze = zexp0[o];
+    double ur = 0, ui = 0;
+    for (int s = 0; s < nz - 1; ++s) {
+        const double tz = 0.5 * (gz[s] + gz[s + 1]);
+        const typename cx<T>::t h = H[((size_t)s * nl + l) * nvv + o];
+        const double m = exp(ze.x * tz);
+        double sn, cs;
+        sincos(ze.y * tz, &sn, &cs);
+        const double er = m * cs, ei = m * sn;
+        ur += (double)h.x * er - (double)h.y * ei;
+        ui += (double)h.x * ei + (double)h.y * er;
+    }
+    u[idx].x += ur;
+    u[idx].y += ui;
+}
+
+void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
+                         const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,
+                         hipStream_t s)
+{
+    if (nvert <= 0) return;
+    dim3 gr((unsigned)((nvert + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(quad_scatter_kernel<float>, gr, dim3(256), 0, s, P, g, nvert, paths, verts, vpath, offsets,
+                           (float2 *)grid);
+    else
+        hipLaunchKernelGGL(quad_scatter_kernel<double>, gr, dim3(256), 0, s, P, g, nvert, paths, verts, vpath,
+                           offsets, (double2 *)grid);
+}
+
+void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
+                  int batch, bool transB, bool fp32, hipStream_t s)
+{
+    dim3 gr((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)batch);
+    if (fp32)
+        hipLaunchKernelGGL(cgemm_kernel<float>, gr, dim3(256), 0, s, M, N, K, (const float2 *)A, sa, (const float2 *)B,
+                           sb, (float2 *)C, sc, transB);
+    else
+        hipLaunchKernelGGL(cgemm_kernel<double>, gr, dim3(256), 0, s, M, N, K, (const double2 *)A, sa,
+                           (const double2 *)B, sb, (double2 *)C, sc, transB);
+}
+
+void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,
+                         bool fp32, hipStream_t s)
+{
+    const int64_t n = (int64_t)nv0 * nv0 * nl;
+    dim3 gr((unsigned)((n + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(quad_combine_kernel<float>, gr, dim3(256), 0, s, nv0, nl, nz, (const float2 *)H, zexp0, gz,
+                           u);
+    else
+        hipLaunchKernelGGL(quad_combine_kernel<double>, gr, dim3(256), 0, s, nv0, nl, nz, (const double2 *)H, zexp0,
+                           gz, u);
+}
+
+} // namespace scmc

@@ -1,0 +1,237 @@
+// quad_engine.cpp -- host side of MCfieldFourierQuad (code/MCfieldFourierQuad.m:1-303).
+#include "quad.h"
+#include "scmc_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace scmc {
+
+static int colon_len(double a, double s, double b)
+{
+    // element count of MATLAB's a:s:b (with its relative tolerance)
+    double q = (b - a) / s;
+    return (int)floor(q + q * 1e-12 + 1e-10) + 1;
+}
+
+int quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp, double lambda,
+                    QuadGeom &g, double *bmin, double *bmax, std::vector<double> *gz)
+{
+    if (!(v_stp > 0) || !(v_max > 0) || !(lambda > 0))
+        return set_error(SCMC_E_ARG, "v_max, v_stp and lambda must be positive");
+    // MCfieldFourierQuad.m:80-98
+    g.nv0 = colon_len(-v_max, v_stp, v_max);
+    g.box_stp = lambda / (v_max * 2 + v_stp);
+    const double bw0 = box_max[0] - box_min[0];
+    g.nx = (int)ceil(bw0 / g.box_stp / 2) * 2 + 1;
+    const double bwn = g.nx * g.box_stp;
+    for (int j = 0; j < 3; ++j) {
+        bmin[j] = box_min[j];
+        bmax[j] = box_max[j];
+    }
+    for (int j = 0; j < 2; ++j) {
+        bmin[j] = box_min[j] - (bwn - bw0) / 2;
+        bmax[j] = box_max[j] + (bwn - bw0) / 2;
+    }
+    g.gx0 = bmin[0] + g.box_stp / 2;
+    g.gy0 = bmin[1] + g.box_stp / 2;
+    // MCfieldFourierQuad.m:103-106
+    g.nx_pad = std::max((g.nv0 - g.nx) / 2, 0);
+    g.nva_pad = std::max((g.nx - g.nv0) / 2, 0);
+    g.nfft = g.nx + 2 * g.nx_pad;
+    // MCfieldFourierQuad.m:108-121: z_bin from the widest view of the grid
+    double worst = 0;
+    for (int i = 0; i < g.nv0; ++i)
+        for (int j = 0; j < g.nv0; ++j) {
+            double vx = -(-v_max + i * v_stp), vy = -(-v_max + j * v_stp);
+            double q = 1 - fabs(sqrt(1 - vx * vx - vy * vy));
+            worst = std::max(worst, q);
+        }
+    if (!(worst > 0)) return set_error(SCMC_E_ARG, "view grid has no oblique direction (v_max too small)");
+    g.z_bin = lambda / worst / 5;
+    g.z_bin = g.z_bin / 2;
+    const int nb = colon_len(bmin[2], g.z_bin, bmax[2]);
+    g.nz = nb + 1;
+    if (g.nz > kQuadMaxSlabs) return set_error(SCMC_E_ARG, "too many z slabs (%d)", g.nz);
+    if (gz) {
+        gz->resize((size_t)g.nz);
+        for (int i = 0; i < nb; ++i) (*gz)[(size_t)i] = bmin[2] + i * g.z_bin;
+        (*gz)[(size_t)g.nz - 1] = bmax[2];
+    }
+    return SCMC_OK;
+}
+
+} // namespace scmc
+
+using namespace scmc;
+
+extern "C" {
+
+int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp, double lambda,
+                   int32_t *nv0, int32_t *nx, int32_t *nz)
+{
+    QuadGeom g;
+    double bmin[3], bmax[3];
+    int rc = quad_grid_setup(box_min, box_max, v_max, v_stp, lambda, g, bmin, bmax, nullptr);
+    if (rc) return rc;
+    if (nv0) *nv0 = g.nv0;
+    if (nx) *nx = g.nx;
+    if (nz) *nz = g.nz;
+    return SCMC_OK;
+}
+
+int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_stp, int64_t max_itr, double vsign,
+                              int32_t precision, double *u_re, double *u_im, scmc_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!pr_in) return set_error(SCMC_E_ARG, "null problem");
+    if (pr_in->dim != 3) return set_error(SCMC_E_ARG, "MCfieldFourierQuad is 3-D only");
+    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
+    int rc = validate_problem(pr_in, max_itr, false);
+    if (rc) return rc;
+    QuadGeom g;
+    double bmin[3], bmax[3];
+    std::vector<double> gz;
+    if ((rc = quad_grid_setup(pr_in->box_min, pr_in->box_max, v_max, v_stp, pr_in->lambda, g, bmin, bmax, &gz)))
+        return rc;
+    if ((rc = ensure_device(nullptr))) return rc;
+    const bool fp32 = precision == SCMC_MIXED;
+    // the walk runs in the enlarged box with lmean0 (default [0;0;1]) and no
+    // ampfunc0 (MCfieldFourierQuad.m:59-64, 96-98, 166-172)
+    scmc_problem pr = *pr_in;
+    for (int j = 0; j < 3; ++j) {
+        pr.box_min[j] = bmin[j];
+        pr.box_max[j] = bmax[j];
+    }
+    pr.is_ff_l = 1;
+    pr.is_ff_v = 1;
+    pr.nv = 0;
+    pr.v = nullptr;
+    pr.has_amp0 = 0;
+    if (!pr.has_lmean0) {
+        pr.has_lmean0 = 1;
+        pr.lmean0[0] = 0;
+        pr.lmean0[1] = 0;
+        pr.lmean0[2] = 1;
+    }
+    const double lz = pr.l[3 * (pr.nl - 1) + 2];   // l(end)
+    g.l_diratt = lz > 0 ? 1 / fabs(lz) : 0;
+    g.l_idiratt = lz < 0 ? 1 / fabs(lz) : 0;
+    hipStream_t s;
+    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    scmc_stats st;
+    memset(&st, 0, sizeof st);
+    {
+        ProblemState ps;
+        TraceChunk tc;
+        DevBuf dgz, grid, W, T1, H, zexp, u;
+        const int nl = pr.nl, nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
+        const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+        const size_t ngrid = (size_t)nsl * nl * nx * nx;
+        do {
+            if ((rc = ps.setup(&pr, s))) break;
+            if (!dgz.ensure(sizeof(double) * gz.size(), &rc) || !grid.ensure(cs * ngrid, &rc)) break;
+            if (!hip_ok(hipMemcpy(dgz.p, gz.data(), sizeof(double) * gz.size(), hipMemcpyHostToDevice), "gz", &rc)) break;
+            g.gz = (const double *)dgz.p;
+            launch_zero(grid.p, (cs * ngrid + 15) / 16 * 16 > grid.bytes ? grid.bytes : (cs * ngrid + 15) / 16 * 16, s);
+            int64_t nstore = 0;
+            double mean_len = 20;
+            for (int64_t p0 = 0; p0 < max_itr;) {
+                int64_t np = std::min<int64_t>(max_itr - p0, std::max<int64_t>(256, (int64_t)(4.0e6 / mean_len)));
+                auto a = std::chrono::steady_clock::now();
+                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
+                launch_quad_scatter(ps.dp, g, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
+                                    (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, grid.p, fp32, s);
+                if (!hip_ok(hipStreamSynchronize(s), "quad scatter", &rc)) break;
+                st.ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+                nstore += tc.nvert;
+                st.paths += np;
+                st.vertices += tc.nvert;
+                st.chunks += 1;
+                mean_len = (double)st.vertices / (double)st.paths;
+                p0 += np;
+            }
+            if (rc) break;
+            // DFT matrix W[o][ix] = exp(+2 pi i ((o + nva_pad - m)(ix + nx_pad - m) mod n)/n)
+            const int n = g.nfft, m = (n - 1) / 2;
+            std::vector<double> w((size_t)nv0 * nx * 2);
+            for (int ix = 0; ix < nx; ++ix)
+                for (int o = 0; o < nv0; ++o) {
+                    long long f = o + g.nva_pad - m, q = ix + g.nx_pad - m;
+                    long long r = ((f * q) % n + n) % n;
+                    double ang = 2 * M_PI * (double)r / n;
+                    w[2 * ((size_t)ix * nv0 + o)] = cos(ang);
+                    w[2 * ((size_t)ix * nv0 + o) + 1] = sin(ang);
+                }
+            if (!W.ensure(cs * (size_t)nv0 * nx, &rc) || !T1.ensure(cs * (size_t)nsl * nl * nv0 * nx, &rc) ||
+                !H.ensure(cs * (size_t)nsl * nl * nv0 * nv0, &rc) ||
+                !zexp.ensure(sizeof(double2) * (size_t)nv0 * nv0, &rc) ||
+                !u.ensure(sizeof(double2) * (size_t)nv0 * nv0 * nl, &rc))
+                break;
+            if (fp32) {
+                std::vector<float> wf(w.begin(), w.end());
+                if (!hip_ok(hipMemcpy(W.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice), "W", &rc)) break;
+            } else if (!hip_ok(hipMemcpy(W.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice), "W", &rc)) {
+                break;
+            }
+            // zexp0 and boxexp0 (MCfieldFourierQuad.m:124-138)
+            std::vector<double> ze((size_t)nv0 * nv0 * 2);
+            std::vector<std::complex<double>> be((size_t)nv0 * nv0);
+            for (int j = 0; j < nv0; ++j)
+                for (int i = 0; i < nv0; ++i) {
+                    double vx = -(-v_max + i * v_stp), vy = -(-v_max + j * v_stp);
+                    double vz = vsign * sqrt(1 - vx * vx - vy * vy);
+                    double dir = vz < 0 ? 1 / fabs(vz) : 0, idir = vz > 0 ? 1 / fabs(vz) : 0;
+                    size_t o = (size_t)i + (size_t)nv0 * j;
+                    ze[2 * o] = -pr.sigt / 2 * (dir - idir);
+                    ze[2 * o + 1] = -2 * M_PI / pr.lambda * vz * vsign;
+                    be[o] = exp(-pr.sigt / 2 * (-bmin[2] * dir + bmax[2] * idir)) *
+                            std::exp(std::complex<double>(0, 2 * M_PI / pr.lambda *
+                                                                 (-bmin[2] * (vz < 0) * vz + bmax[2] * (vz > 0) * vz)));
+                }
+            if (!hip_ok(hipMemcpy(zexp.p, ze.data(), sizeof(double) * ze.size(), hipMemcpyHostToDevice), "zexp", &rc))
+                break;
+            auto a = std::chrono::steady_clock::now();
+            // T1[b] = W (nv0 x nx) * X[b] (nx x nx);  H[b] = T1[b] * W^T
+            launch_cgemm(nv0, nx, nx, W.p, 0, grid.p, (int64_t)nx * nx, T1.p, (int64_t)nv0 * nx, nsl * nl, false, fp32, s);
+            launch_cgemm(nv0, nv0, nx, T1.p, (int64_t)nv0 * nx, W.p, 0, H.p, (int64_t)nv0 * nv0, nsl * nl, true, fp32, s);
+            launch_zero(u.p, sizeof(double2) * (size_t)nv0 * nv0 * nl, s);
+            launch_quad_combine(nv0, nl, g.nz, H.p, (const double2 *)zexp.p, g.gz, (double2 *)u.p, fp32, s);
+            std::vector<double> hu((size_t)nv0 * nv0 * nl * 2);
+            if (!hip_ok(hipMemcpyAsync(hu.data(), u.p, sizeof(double) * hu.size(), hipMemcpyDeviceToHost, s), "copy u",
+                        &rc) ||
+                !hip_ok(hipStreamSynchronize(s), "quad sync", &rc))
+                break;
+            st.ms_nee += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+            // boxexp0 and MCfieldFourierQuad.m:302-303 (maxItr = stored vertices)
+            const double bwn = g.nx * g.box_stp;
+            const double V = bwn * bwn * (bmax[2] - bmin[2]);
+            const double sc = nstore ? sqrt(1.0 / (double)nstore * V * pr.sigt) : 0.0;
+            const size_t nvv = (size_t)nv0 * nv0;
+            for (int l = 0; l < nl; ++l)
+                for (size_t o = 0; o < nvv; ++o) {
+                    size_t k = o + nvv * l;
+                    std::complex<double> z(hu[2 * k], hu[2 * k + 1]);
+                    z *= be[o];
+                    z *= sc;
+                    if (u_im) {
+                        u_re[k] = z.real();
+                        u_im[k] = z.imag();
+                    } else {
+                        u_re[2 * k] = z.real();
+                        u_re[2 * k + 1] = z.imag();
+                    }
+                }
+        } while (0);
+    }
+    (void)hipStreamDestroy(s);
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,97 @@
+"""Pins of the oracle's MCfieldFourierQuad restatement (CPU only).
+
+A second restatement written with numpy's FFT routines (fftshift, ifftshift,
+fft2 and flips exactly as MCfieldFourierQuad.m:276-280 spells them) and the
+oracle's own walks (traced in the enlarged box) must reproduce the oracle's
+direct-DFT result; the grid sizes must match the C-ABI's scmc_quad_grid."""
+import numpy as np
+import pytest
+
+from mcspeckle_amd import engines as E
+
+
+def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, cbs, smp, vsign, seed):
+    g = oracle.quad_grid(np.array(bmin, float), np.array(bmax, float), v_max, v_stp, lam)
+    nv0, nx = g.nv0, g.nx
+    Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
+    spec = E.make_problem(sigt, albedo, Bmin, Bmax, l, np.zeros((3, 0)), 1, 1, lam, cbs, smp, 1, 0,
+                          lmean0=[0, 0, 1], seed=seed, need_views=False)
+    ps = oracle.trace(spec, n_paths)
+    off = ps["offset"]
+    xL, x1L, pxL = [], [], []
+    for p in range(n_paths):
+        for k in range(off[p], off[p + 1]):
+            xL.append(ps["x"][3 * k:3 * k + 3])
+            x1L.append(ps["x1"][3 * p:3 * p + 3])
+            c = np.sqrt(ps["weight"][k] / ps["px"][p]) * np.exp(2j * np.pi * ps["xi"][k])
+            if cbs and k > off[p]:
+                c /= np.sqrt(2)
+            pxL.append(c)
+    xL, x1L, pxL = np.array(xL).T, np.array(x1L).T, np.array(pxL)
+    K = pxL.size
+    theta_v = -v_max + np.arange(nv0) * v_stp
+    vx, vy = np.meshgrid(-theta_v, -theta_v, indexing="ij")
+    vz = vsign * np.sqrt(1 - vx ** 2 - vy ** 2)
+    gz = np.concatenate([Bmin[2] + np.arange(g.nz - 1) * g.z_bin, [Bmax[2]]])
+    diratt0 = np.where(vz < 0, 1 / np.abs(vz), 0)
+    idiratt0 = np.where(vz > 0, 1 / np.abs(vz), 0)
+    lz = l[2, -1]
+    l_diratt = 1 / abs(lz) if lz > 0 else 0
+    l_idiratt = 1 / abs(lz) if lz < 0 else 0
+    zexp0 = -2j * np.pi / lam * vz * vsign - sigt / 2 * (diratt0 - idiratt0)
+    boxexp0 = np.exp(-sigt / 2 * (-Bmin[2] * diratt0 + Bmax[2] * idiratt0))
+    boxexp0 = boxexp0 * np.exp(2j * np.pi / lam * (-Bmin[2] * (vz < 0) * vz + Bmax[2] * (vz > 0) * vz))
+    nl = l.shape[1]
+    u = np.zeros((nv0, nv0, nl), complex)
+    pad, crop = g.nx_pad, g.nva_pad
+    for tt in range(2):
+        if tt == 1:
+            if not cbs:
+                continue
+            xL, x1L = x1L, xL
+        order = np.argsort(x1L[2], kind="stable")
+        xs, xl, pw = x1L[:, order], xL[:, order], pxL[order]
+        slab = np.searchsorted(gz[1:], xs[2], side="right")   # first j with z < gz[j+1]
+        for j1 in range(g.nz - 1):
+            sel = np.flatnonzero(slab == j1)
+            if sel.size == 0:
+                continue
+            tz = (gz[j1] + gz[j1 + 1]) / 2
+            ix = np.clip(np.floor((xs[0, sel] - g.gx0) / g.box_stp + 0.5).astype(int), 0, nx - 1)
+            iy = np.clip(np.floor((xs[1, sel] - g.gy0) / g.box_stp + 0.5).astype(int), 0, nx - 1)
+            for j2 in range(nl):
+                term = pw[sel] * np.exp(-2j * np.pi / lam * (-xs[2, sel] + tz)) * np.exp(
+                    2j * np.pi / lam * (l[:, j2] @ xl[:, sel])
+                    - sigt / 2 * ((xl[2, sel] - Bmin[2]) * l_diratt + (Bmax[2] - xl[2, sel]) * l_idiratt))
+                uxy = np.zeros((nx, nx), complex)
+                np.add.at(uxy, (ix, iy), term)
+                uxy = np.pad(uxy, pad)
+                tfu = np.fliplr(np.flipud(np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(uxy)))))
+                if crop:
+                    tfu = tfu[crop:-crop, crop:-crop]
+                u[:, :, j2] += tfu * np.exp(zexp0 * tz)
+    u *= boxexp0[:, :, None]
+    V = (nx * g.box_stp) ** 2 * (Bmax[2] - Bmin[2])
+    return u * np.sqrt(V * sigt / K), K
+
+
+@pytest.mark.parametrize("cbs,vmax,vsign", [(False, 0.3, -1.0), (True, 0.3, -1.0), (True, 0.25, 1.0)])
+def test_oracle_quad_matches_numpy_fft_restatement(oracle, cbs, vmax, vsign):
+    l = np.array([[0.05, -0.1], [0.0, 0.08], [np.sqrt(1 - 0.05 ** 2), np.sqrt(1 - 0.1 ** 2 - 0.08 ** 2)]])
+    args = dict(sigt=3.0, albedo=0.95, bmin=[-2.0, -2.0, 0.0], bmax=[2.0, 2.0, 0.6], l=l, v_max=vmax, v_stp=0.05,
+                n_paths=150, lam=0.5, cbs=cbs, smp=2, vsign=vsign, seed=4)
+    u_np, K = quad_numpy(oracle, **args)
+    u_or, K2 = oracle.mcfield_fourier_quad(args["sigt"], args["albedo"], args["bmin"], args["bmax"], l, vmax, 0.05,
+                                           150, 0.5, cbs, 2, None, vsign, 4)
+    assert K == K2
+    np.testing.assert_allclose(u_or, u_np, rtol=0, atol=1e-10 * np.abs(u_np).max())
+
+
+@pytest.mark.parametrize("bw,vmax,vstp,lam", [(4.0, 0.3, 0.05, 0.5), (10.0, 0.25, 0.5 / 510, 0.0098),
+                                              (2.0, 0.5, 0.01, 1.0)])
+def test_quad_grid_sizes_match(oracle, bw, vmax, vstp, lam):
+    bmin, bmax = [-bw / 2, -bw / 2, 0.0], [bw / 2, bw / 2, 1.0]
+    g = oracle.quad_grid(np.array(bmin), np.array(bmax), vmax, vstp, lam)
+    nv0, nx, nz = E.quad_grid(bmin, bmax, vmax, vstp, lam)
+    assert (nv0, nx, nz) == (g.nv0, g.nx, g.nz)
+    assert nv0 % 2 == 1 and nx % 2 == 1
