@@ -1,0 +1,298 @@
+// serve.cpp -- device-resident continuous batching of covariance solves.
+//
+// Requests (MCcov problems of any size, precision and path count) arrive with
+// scmc_submit_mccov.  The server keeps `max_active` slots; a slot takes the
+// first pending request whose device footprint fits the remaining HBM budget
+// (FIFO with backfill), so requests are admitted as slots and memory free up.
+// Each admitted request owns a session (its own HIP stream and accumulators)
+// and advances in slices of paths, so the kernels of all admitted requests
+// overlap on the GPU; a cancel takes effect at the next slice boundary and
+// frees the slot.  Results are normalised exactly as scmc_mccov and depend
+// only on the request (Philox paths, fixed slice schedule), not on the mix.
+#include "scmc_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace scmc {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    int64_t id = 0;
+    ProblemCopy pc;
+    int64_t max_itr = 0;
+    int precision = 0;
+    int64_t footprint = 0;
+    int state = 0;   // 0 pending, 1 running, 2 done
+    int rc = SCMC_OK;
+    std::string err;
+    std::atomic<bool> cancel{false};
+    std::vector<double> ms_re, ms_im, mm_re, mm_im;
+    scmc_stats st{};
+    clk::time_point t_submit, t_start, t_done;
+};
+
+struct Server {
+    scmc_server_config cfg{};
+    int64_t budget = 0, used = 0, slice = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> pending;
+    std::map<int64_t, std::unique_ptr<Request>> all;
+    std::vector<std::thread> slots;
+    int64_t next_id = 1;
+    bool stop = false;
+};
+
+static std::mutex g_srv_mu;
+static std::unique_ptr<Server> g_srv;
+
+int64_t request_footprint(int nv, int nl, int precision)
+{
+    const double n = (double)nv * nl;
+    const bool mixed = precision == SCMC_MIXED;
+    const double tile = mixed ? kTileMixed : kTileF64;
+    const double nt = std::ceil(n / tile);
+    const double tiles = nt * (nt + 1) / 2;
+    // accumulator copies (multiple + single scattering), at most 32 + 8
+    const double ks = std::min(32.0, std::max(1.0, std::ceil(8192.0 / tiles)));
+    const double acc = (ks + std::min(ks, 8.0)) * n * n * 16.0;
+    // chunk buffers: expanded rows, factors, vertex records
+    const double cap = mixed ? 4.0e9 / (8.0 * n) : 2.5e9 / (16.0 * n);
+    const double kchunk = std::max(65536.0, std::min(4.0e6, cap));
+    const double cs = mixed ? 8.0 : 16.0;
+    const double chunk = kchunk * ((mixed ? 8.0 * n : 0.0) + 2.0 * (nv + nl) * cs * 2.0 + 80.0);
+    return (int64_t)(acc + chunk + 64.0 * 1024 * 1024);
+}
+
+static void run_request(Server *S, Request *r)
+{
+    Session sess;
+    int rc = sess.init(&r->pc.pr, 0, r->precision);
+    const int64_t slice = S->slice > 0 ? S->slice : std::max<int64_t>(1000, r->max_itr / 8);
+    for (int64_t p0 = 0; !rc && p0 < r->max_itr; p0 += slice) {
+        if (r->cancel.load()) {
+            rc = set_error(SCMC_E_CANCELLED, "request %lld cancelled", (long long)r->id);
+            break;
+        }
+        rc = sess.run(p0, std::min(slice, r->max_itr - p0));
+    }
+    if (!rc) {
+        const size_t ne = (size_t)r->pc.pr.nv * r->pc.pr.nv * r->pc.pr.nl * r->pc.pr.nl;
+        r->ms_re.resize(ne);
+        r->ms_im.resize(ne);
+        r->mm_re.resize(ne);
+        r->mm_im.resize(ne);
+        rc = sess.result(r->max_itr, r->ms_re.data(), r->ms_im.data(), r->mm_re.data(), r->mm_im.data());
+    }
+    sess.fill_stats(&r->st);
+    r->rc = rc;
+    if (rc) r->err = scmc_last_error();
+}
+
+static void slot_loop(Server *S)
+{
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] {
+                if (S->stop) return true;
+                for (Request *q : S->pending)
+                    if (S->used + q->footprint <= S->budget || S->used == 0) return true;
+                return false;
+            });
+            if (S->stop) return;
+            for (auto it = S->pending.begin(); it != S->pending.end(); ++it)
+                if (S->used + (*it)->footprint <= S->budget || S->used == 0) {
+                    r = *it;
+                    S->pending.erase(it);
+                    break;
+                }
+            if (!r) continue;
+            S->used += r->footprint;
+            r->state = 1;
+            r->t_start = clk::now();
+        }
+        run_request(S, r);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->used -= r->footprint;
+            r->state = 2;
+            r->t_done = clk::now();
+        }
+        S->cv_work.notify_all();
+        S->cv_done.notify_all();
+    }
+}
+
+static int copy_out(Request *r, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    if (stats) {
+        *stats = r->st;
+        stats->ms_total = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+        // queueing delay before admission in ms_trace's place is not reported; ms_total is latency
+    }
+    if (r->rc) return set_error(r->rc, "%s", r->err.c_str());
+    const size_t ne = r->ms_re.size();
+    auto put = [&](double *re, double *im, const std::vector<double> &a, const std::vector<double> &b) {
+        if (!re) return;
+        if (im) {
+            memcpy(re, a.data(), ne * sizeof(double));
+            memcpy(im, b.data(), ne * sizeof(double));
+        } else {
+            for (size_t i = 0; i < ne; ++i) {
+                re[2 * i] = a[i];
+                re[2 * i + 1] = b[i];
+            }
+        }
+    };
+    put(ms_re, ms_im, r->ms_re, r->ms_im);
+    put(mm_re, mm_im, r->mm_re, r->mm_im);
+    return SCMC_OK;
+}
+
+} // namespace scmc
+
+using namespace scmc;
+
+extern "C" {
+
+int64_t scmc_request_footprint(int32_t nv, int32_t nl, int32_t precision)
+{
+    if (nv < 1 || nl < 1) return -1;
+    return request_footprint(nv, nl, precision);
+}
+
+int scmc_server_start(const scmc_server_config *cfg)
+{
+    std::lock_guard<std::mutex> g(g_srv_mu);
+    if (g_srv) return set_error(SCMC_E_BUSY, "server already running");
+    int rc = ensure_device(nullptr);
+    if (rc) return rc;
+    std::unique_ptr<Server> S(new Server());
+    if (cfg) S->cfg = *cfg;
+    if (S->cfg.max_active <= 0) S->cfg.max_active = 4;
+    size_t freeb = 0, total = 0;
+    (void)hipMemGetInfo(&freeb, &total);
+    S->budget = S->cfg.hbm_budget_gb > 0 ? (int64_t)(S->cfg.hbm_budget_gb * 1e9) : (int64_t)(0.8 * (double)freeb);
+    S->slice = S->cfg.paths_per_slice;
+    Server *raw = S.get();
+    for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);
+    g_srv = std::move(S);
+    return SCMC_OK;
+}
+
+int scmc_server_stop(void)
+{
+    std::unique_ptr<Server> S;
+    {
+        std::lock_guard<std::mutex> g(g_srv_mu);
+        if (!g_srv) return SCMC_OK;
+        S = std::move(g_srv);
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        for (auto &kv : S->all) kv.second->cancel = true;
+    }
+    S->cv_work.notify_all();
+    for (auto &t : S->slots) t.join();
+    return SCMC_OK;
+}
+
+int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id)
+{
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
+    if (!pr->is_ff_l || !pr->is_ff_v)
+        return set_error(SCMC_E_UNSUPPORTED, "near-field sources are not supported by the GPU engines");
+    std::lock_guard<std::mutex> g(g_srv_mu);
+    if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
+    Server *S = g_srv.get();
+    std::unique_ptr<Request> r(new Request());
+    r->pc.assign(pr);
+    r->max_itr = max_itr;
+    r->precision = precision;
+    r->footprint = request_footprint(pr->nv, pr->nl, precision);
+    r->t_submit = clk::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->id = S->next_id++;
+        if (request_id) *request_id = r->id;
+        S->pending.push_back(r.get());
+        S->all[r->id] = std::move(r);
+    }
+    S->cv_work.notify_all();
+    return SCMC_OK;
+}
+
+static int finish(int64_t id, bool block, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    Server *S;
+    {
+        std::lock_guard<std::mutex> g(g_srv_mu);
+        if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
+        S = g_srv.get();
+    }
+    std::unique_ptr<Request> done;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->all.find(id);
+        if (it == S->all.end()) return set_error(SCMC_E_NOTFOUND, "unknown request %lld", (long long)id);
+        Request *r = it->second.get();
+        if (block)
+            S->cv_done.wait(lk, [&] { return r->state == 2; });
+        else if (r->state != 2)
+            return SCMC_E_BUSY;
+        done = std::move(it->second);
+        S->all.erase(it);
+    }
+    return copy_out(done.get(), ms_re, ms_im, mm_re, mm_im, stats);
+}
+
+int scmc_poll(int64_t id, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    return finish(id, false, ms_re, ms_im, mm_re, mm_im, stats);
+}
+
+int scmc_wait(int64_t id, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
+{
+    return finish(id, true, ms_re, ms_im, mm_re, mm_im, stats);
+}
+
+int scmc_cancel(int64_t id)
+{
+    std::lock_guard<std::mutex> g(g_srv_mu);
+    if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
+    Server *S = g_srv.get();
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return set_error(SCMC_E_NOTFOUND, "unknown request %lld", (long long)id);
+    Request *r = it->second.get();
+    r->cancel = true;
+    if (r->state == 0) {
+        // never admitted: drop it from the queue and finish it as cancelled
+        S->pending.erase(std::remove(S->pending.begin(), S->pending.end(), r), S->pending.end());
+        r->rc = SCMC_E_CANCELLED;
+        r->err = "request cancelled before admission";
+        r->state = 2;
+        r->t_done = clk::now();
+        S->cv_done.notify_all();
+    }
+    return SCMC_OK;
+}
+
+} // extern "C"

@@ -366,3 +366,60 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching covariance server (scmc_server_* in scmc.h)."""
+
+    def __init__(self, max_active=4, hbm_budget_gb=0.0, paths_per_slice=0, device=0):
+        cfg = N.ServerConfig()
+        cfg.device, cfg.max_active = device, max_active
+        cfg.hbm_budget_gb, cfg.paths_per_slice = hbm_budget_gb, paths_per_slice
+        N.check(N.lib().scmc_server_start(C.byref(cfg)))
+        self._specs = {}
+
+    def submit(self, spec, max_itr, precision="mixed"):
+        p = c_problem(spec)
+        h = C.c_int64()
+        N.check(N.lib().scmc_submit_mccov(C.byref(p), int(max_itr), _prec(precision), C.byref(h)))
+        self._specs[h.value] = spec
+        return h.value
+
+    def _finish(self, rid, block):
+        spec = self._specs[rid]
+        nv, nl = spec.nv, spec.nl
+        ne = (nv * nl) ** 2
+        a = [np.empty(ne) for _ in range(4)]
+        st = N.Stats()
+        fn = N.lib().scmc_wait if block else N.lib().scmc_poll
+        rc = fn(int(rid), *[N.dptr(x) for x in a], C.byref(st))
+        if rc == N.SCMC_E_BUSY:
+            return None
+        self._specs.pop(rid, None)
+        N.check(rc)
+        shape = (nv, nv, nl, nl)
+        return ((a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F"),
+                st.as_dict())
+
+    def poll(self, rid):
+        """(Ms, Mm, stats) when finished, None while running."""
+        return self._finish(rid, False)
+
+    def wait(self, rid):
+        return self._finish(rid, True)
+
+    def cancel(self, rid):
+        N.check(N.lib().scmc_cancel(int(rid)))
+
+    def stop(self):
+        N.lib().scmc_server_stop()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+
+def request_footprint(nv, nl, precision="mixed"):
+    return int(N.lib().scmc_request_footprint(int(nv), int(nl), _prec(precision)))

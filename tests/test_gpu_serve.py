@@ -1,0 +1,90 @@
+"""Serving: continuous batching of covariance requests on one GPU.
+
+* Every result is the same alone and in the mix: a request's result depends
+  only on its own Philox paths and slice schedule.  Tolerance: identical to
+  1e-12 (Cauchy-Schwarz scaled) between the mixed-in-a-batch and the alone run
+  through the same server, and within the precision's bound (1e-4 mixed,
+  1e-12 FP64) against the one-shot engine.
+* Admission by footprint: with a budget that fits one request at a time, the
+  requests still all complete, one after another.
+* Cancel frees the slot and reports SCMC_E_CANCELLED.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from conftest import cone_dirs, cov_matrix, cs_scale
+from mcspeckle_amd import _native as N
+from mcspeckle_amd import engines as E
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cs_err(M, R):
+    C, Rr = cov_matrix(M), cov_matrix(R)
+    return float(np.max(np.abs(C - Rr) / np.maximum(cs_scale(Rr), 1e-300)))
+
+
+def req(i, nv, nl, od, g, seed):
+    rng = np.random.default_rng(100 + i)
+    return E.make_problem(od, 0.99, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(nl, 15, rng), cone_dirs(nv, 15, rng),
+                          1, 1, 1.0, True, 2, 3, g, seed=seed)
+
+
+MIX = [(0, 8, 8, 5.0, 0.5, 2000, "mixed"), (1, 33, 7, 12.0, 0.9, 1500, "mixed"), (2, 5, 4, 2.0, 0.2, 4000, "fp64"),
+       (3, 64, 16, 8.0, 0.8, 600, "mixed"), (4, 16, 12, 15.0, 0.0, 1000, "fp64"), (5, 11, 3, 1.0, 0.95, 3000, "mixed")]
+
+
+def test_results_same_alone_and_in_the_mix():
+    alone = {}
+    with E.Server(max_active=1, paths_per_slice=500) as S:
+        for i, nv, nl, od, g, n, prec in MIX:
+            say("alone", i)
+            rid = S.submit(req(i, nv, nl, od, g, seed=i), n, prec)
+            alone[i] = S.wait(rid)
+    with E.Server(max_active=4, paths_per_slice=500) as S:
+        ids = {}
+        for i, nv, nl, od, g, n, prec in MIX:
+            ids[i] = S.submit(req(i, nv, nl, od, g, seed=i), n, prec)
+        say("mix submitted")
+        mixed = {i: S.wait(r) for i, r in ids.items()}
+    for i, nv, nl, od, g, n, prec in MIX:
+        Ms_a, Mm_a, _ = alone[i]
+        Ms_m, Mm_m, _ = mixed[i]
+        assert cs_err(Ms_m + Mm_m, Ms_a + Mm_a) <= 1e-12, i
+        Ms1, Mm1 = E.mccov_spec(req(i, nv, nl, od, g, seed=i), n, prec)
+        tol = 1e-4 if prec == "mixed" else 1e-12
+        assert cs_err(Ms_m + Mm_m, Ms1 + Mm1) <= tol, i
+
+
+def test_admission_by_footprint_serialises_big_requests():
+    fp = E.request_footprint(64, 64, "mixed")
+    assert fp > 1e9
+    with E.Server(max_active=4, hbm_budget_gb=fp * 1.5 / 1e9, paths_per_slice=200) as S:
+        ids = [S.submit(req(i, 64, 64, 10.0, 0.9, seed=i), 400, "mixed") for i in range(3)]
+        say("big submitted")
+        res = [S.wait(r) for r in ids]
+    for Ms, Mm, st in res:
+        C = cov_matrix(Ms + Mm)
+        assert np.all(np.isfinite(C)) and np.real(np.diag(C)).min() > 0
+        assert st["paths"] == 400
+
+
+def test_cancel_frees_the_slot():
+    with E.Server(max_active=1, paths_per_slice=100) as S:
+        big = S.submit(req(0, 64, 32, 10.0, 0.9, seed=1), 200000, "fp64")
+        small = S.submit(req(1, 4, 4, 3.0, 0.5, seed=2), 500, "mixed")
+        time.sleep(0.5)
+        S.cancel(big)
+        with pytest.raises(N.ScmcError) as ei:
+            S.wait(big)
+        assert ei.value.code == N.SCMC_E_CANCELLED
+        Ms, Mm, st = S.wait(small)
+        assert np.all(np.isfinite(Ms + Mm))
