@@ -68,6 +68,113 @@ def cpu_baseline(spec, n_paths, threads):
     return n_paths * N * N / dt, dt
 
 
+def serving_stream(n_req, rate, path_scale, seed=5):
+    """Requests shaped like the serving configuration: Nv, Nl ~ U{4..64},
+    optical depth ~ U[1,20] (T = 1), HG g ~ U[0,0.95], paths ~ logU[1e5,1e7]
+    (times path_scale), Poisson arrivals at `rate` per second."""
+    from mcspeckle_amd import engines as E
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(n_req):
+        nv, nl = int(rng.integers(4, 65)), int(rng.integers(4, 65))
+        od, g = rng.uniform(1, 20), rng.uniform(0, 0.95)
+        paths = max(1, int(round(np.exp(rng.uniform(np.log(1e5), np.log(1e7))) * path_scale)))
+
+        def cone(n):
+            ct = rng.uniform(np.cos(np.deg2rad(15)), 1.0, n)
+            st = np.sqrt(1 - ct ** 2)
+            ph = rng.uniform(0, 2 * np.pi, n)
+            return np.vstack([st * np.cos(ph), st * np.sin(ph), ct])
+
+        spec = E.make_problem(od, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone(nl), cone(nv), 1, 1, 1.0, True, 2, 3, g,
+                              seed=1000 + i)
+        out.append((t, spec, paths))
+        t += rng.exponential(1.0 / rate)
+    return out
+
+
+def serve(n_req, rate, path_scale, max_active):
+    import threading
+    from mcspeckle_amd import engines as E
+    stream = serving_stream(n_req, rate, path_scale)
+    lat, work = [], 0.0
+    S = E.Server(max_active=max_active)
+    ids = []
+    t0 = time.perf_counter()
+
+    def submitter():
+        for (ta, spec, paths) in stream:
+            d = ta - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids.append((S.submit(spec, paths, "mixed"), spec, paths))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    done = 0
+    while done < n_req:
+        while len(ids) <= done:
+            time.sleep(0.001)
+        rid, spec, paths = ids[done]
+        Ms, Mm, st = S.wait(rid)
+        lat.append(st["ms_total"])
+        work += paths * float(spec.nv * spec.nl) ** 2
+        done += 1
+    t_end = time.perf_counter() - t0
+    th.join()
+    S.stop()
+    lat = np.array(lat)
+    return {"requests": n_req, "arrival_rate_per_s": rate, "path_scale": path_scale, "max_active": max_active,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "throughput": work / t_end, "unit": "paths*entries/s",
+            "makespan_s": t_end, "offered_span_s": stream[-1][0]}
+
+
+def field_leg(n_paths, precision="mixed"):
+    """MCfield speckle image: slab T=2 (x, y in +-5e4), sigma_s = 10,
+    sigma_a = 0.05, HG g = 0.8, one plane wave at normal incidence, 128 x 128
+    far-field views on a uniform grid over +-10 degrees, CBS off."""
+    from mcspeckle_amd import engines as E
+    th = np.deg2rad(np.linspace(-10, 10, 128))
+    X, Y = np.meshgrid(th, th)
+    v = np.vstack([np.sin(X.ravel()), np.sin(Y.ravel()), np.ones(X.size)])
+    v /= np.linalg.norm(v, axis=0)
+    sigt = 10.05
+    spec = E.make_problem(sigt, 10 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 2], [[0], [0], [1]], v, 1, 1, 1.0, False, 2,
+                          3, 0.8, seed=0)
+    st = {}
+    t0 = time.perf_counter()
+    E.mcfield_spec(spec, n_paths, precision, st)
+    dt = time.perf_counter() - t0
+    return {"paths": n_paths, "views": v.shape[1], "value": n_paths * v.shape[1] / dt, "unit": "paths*entries/s",
+            "seconds": dt, "ms_accum": st["ms_accum"], "vertices": st["vertices"], "precision": precision}
+
+
+def quad_leg(n_paths, precision="mixed", seed=3):
+    """Fourier-accelerated field: box 10 x 10 x 1, sigma_s = 8, sigma_a = 0.02,
+    tabulated phase function (256 angle bins, mixture of 3 HG lobes with
+    g ~ U[-0.3, 0.95] and Dirichlet(1) weights, seeded), one light at normal
+    incidence, 511 x 511 view grid (the reference's grids are odd)."""
+    from mcspeckle_amd import engines as E
+    from mcspeckle_amd.interface import tabulatedAmplitudeScatter
+    rng = np.random.default_rng(seed)
+    gs, ws = rng.uniform(-0.3, 0.95, 3), rng.dirichlet([1, 1, 1])
+    th = np.linspace(0, np.pi, 256)
+    hg = sum(w * (1 - g * g) / (4 * np.pi * (1 + g * g - 2 * g * np.cos(th)) ** 1.5) for g, w in zip(gs, ws))
+    tab = tabulatedAmplitudeScatter(th, np.sqrt(hg))["ampfunc"]
+    v_max, v_stp = 0.25, 0.5 / 510
+    lam = 10 * (2 * v_max + v_stp) / 511
+    sigt = 8.02
+    st = {}
+    t0 = time.perf_counter()
+    u = E.MCfieldFourierQuad(sigt, 8 / sigt, [-5, -5, 0], [5, 5, 1], [[0], [0], [1]], v_max, v_stp, n_paths, lam,
+                             False, 2, None, -1, sct_type=2, ampfunc=tab, seed=0, precision=precision, stats=st)
+    dt = time.perf_counter() - t0
+    return {"paths": n_paths, "grid": list(u.shape[:2]), "value": n_paths * u.shape[0] * u.shape[1] / dt,
+            "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_scatter": st["ms_accum"],
+            "ms_dft": st["ms_nee"], "precision": precision}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +186,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-path-scale", type=float, default=0.01)
+    ap.add_argument("--serve-active", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--field-paths", type=int, default=1000000)
+    ap.add_argument("--quad-paths", type=int, default=5000000)
+    ap.add_argument("--no-legs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -155,6 +270,15 @@ def main():
         }
         out["mixed_vs_fp64_speedup"] = value / out["fp64_plain"]["value"]
         out["mixed_vs_fp64_max_cs_rel_err"] = cs_err(Msm + Mmm, Ms64 + Mm64)
+
+    # ---- the field and Fourier-field configurations
+    if not args.no_legs:
+        out["mcfield"] = field_leg(args.field_paths)
+        out["mcfield_fourier_quad"] = quad_leg(args.quad_paths)
+
+    # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
+    if not args.no_serve:
+        out["serving"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale, args.serve_active)
 
     # ---- CPU baseline: FP64 restatement with the reference's loop structure
     if not args.no_cpu:
