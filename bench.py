@@ -51,6 +51,19 @@ def workload(seed_dirs=2024):
     return E.make_problem(20.0, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.9, seed=0)
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the mixed update kernel, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/herk_mixed_4096.md
+    (scaled by the copy-kernel calibration recorded there)."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", "herk_mixed_4096.md")):
+            if line.startswith("| traffic_bytes_per_launch |"):
+                return int(line.split("|")[2])
+    except OSError:
+        pass
+    return None
+
+
 def cs_err(A, B):
     nv, _, nl, _ = B.shape
     C = np.transpose(A, (0, 2, 1, 3)).reshape(nv * nl, nv * nl, order="F")
@@ -175,6 +188,23 @@ def quad_leg(n_paths, precision="mixed", seed=3):
             "ms_dft": st["ms_nee"], "precision": precision}
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of `mib` MiB through the HIP runtime, the
+    reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +224,11 @@ def main():
     ap.add_argument("--field-paths", type=int, default=1000000)
     ap.add_argument("--quad-paths", type=int, default=5000000)
     ap.add_argument("--no-legs", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0)
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(paths would shard by Philox path id with one final reduce).", file=sys.stderr)
@@ -238,7 +272,7 @@ def main():
                                "64 lights x 64 views on a 15 deg cone, CBS on",
                    "paths_per_step": P, "vertices_per_path": verts / (args.steps * P)},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP16, "traffic": None,
+                     "frac": achieved / PEAK_FP16, "traffic": profiled_traffic(),
                      "kernel": "herk_mixed_kernel", "expand_ms_per_launch": (st1["ms_expand"] - st0["ms_expand"]) / max(herk_launches, 1), "launches": herk_launches,
                      "avg_launch_ms": herk_ms / max(herk_launches, 1),
                      "flops_per_launch": herk_flops / max(herk_launches, 1),
