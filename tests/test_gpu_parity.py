@@ -154,3 +154,49 @@ def test_gpu_headline_size_mixed_vs_fp64(oracle):
     err = cs_err(Msm + Mmm, Msd + Mmd)
     say("headline mixed cs err", err)
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_agrees_with_oracle_within_3_standard_errors(oracle, precision):
+    """Independent seeds: per covariance entry (real and imaginary parts of the
+    64 x 64 plain-MCcov configuration: slab T=1, sigma_s=5, sigma_a=0.1,
+    isotropic, 8 lights x 8 views in [0, 20] deg, CBS), the GPU estimate and
+    the oracle estimate differ by less than 3 combined standard errors
+    (batch means, 20 batches each) for >= 98% of the entries (the t_19
+    distribution puts ~0.7% beyond 3)."""
+    rng = np.random.default_rng(0)
+    l, v = far_dirs(8, 20, rng), far_dirs(8, 20, rng)
+    sigt = 5.1
+    spec = E.make_problem(sigt, 5 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 1, 0, seed=1)
+    B, ng, no = 20, 4000, 1000
+    say("3-SE gpu", precision)
+    gb = []
+    with E.Session(spec, "cov", precision) as s:
+        for b in range(B):
+            s2 = E.Session(spec, "cov", precision)
+            s2.run(b * ng, ng)
+            Ms, Mm = s2.result(ng)
+            s2.close()
+            gb.append(cov_matrix(Ms + Mm))
+    say("3-SE oracle")
+    ospec = E.make_problem(sigt, 5 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 1, 0, seed=2)
+    ob = []
+    V = np.prod(ospec.box_max - ospec.box_min)
+    for b in range(B):
+        Ms, Mm = oracle.mccov_sums(ospec, b * no, no, 16)
+        ob.append(cov_matrix(Ms + Mm / 2) * V * sigt / no)
+    gb, ob = np.array(gb), np.array(ob)
+    mg, mo = gb.mean(0), ob.mean(0)
+    # arithmetic floor of the precision mode (entries that are exactly zero in
+    # FP64, e.g. Im C_ii, carry ~1e-7 FP32 rounding in mixed mode)
+    floor = (1e-5 if precision == "mixed" else 1e-12) * cs_scale(mo)
+    z = []
+    for part in (np.real, np.imag):
+        seg = part(gb).std(0, ddof=1) / np.sqrt(B)
+        seo = part(ob).std(0, ddof=1) / np.sqrt(B)
+        se = np.sqrt(seg ** 2 + seo ** 2 + floor ** 2)
+        z.append((np.abs(part(mg) - part(mo)) / se).ravel())
+    z = np.concatenate(z)
+    say("  fraction within 3 SE", np.mean(z < 3), "max z", z.max())
+    assert np.mean(z < 3) >= 0.98
+    assert z.max() < 6
