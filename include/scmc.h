@@ -187,7 +187,8 @@ int scmc_poll(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, d
 int scmc_wait(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, double *mm_im,
               scmc_stats *stats);
 int scmc_cancel(int64_t request_id);
-/* Bytes of device memory a request would hold while admitted. */
+/* Bytes of device memory a request with far-field sources would hold while
+ * admitted (a near-field request also holds its single-scatter rows). */
 int64_t scmc_request_footprint(int32_t nv, int32_t nl, int32_t precision);
 
 /* ---- misc -------------------------------------------------------------- */

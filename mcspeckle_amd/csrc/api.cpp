@@ -343,6 +343,9 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
         if (!A2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nv, &rc) ||
             !B2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nl, &rc) || !Fp2.ensure(cs * (size_t)np * nv, &rc))
             return rc;
+    const bool near = !(P.is_ff_l && P.is_ff_v);
+    if (near)
+        if (!U.ensure(cs * (size_t)np * n, &rc) || !ones.ensure(cs * (size_t)np, &rc)) return rc;
     launch_zero(bound.p, 16, s);
     CovFactors f;
     f.A1 = A1.p;
@@ -353,6 +356,8 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     f.SB = SB.p;
     f.Gp = Gp.p;
     f.Fp2 = cbs ? Fp2.p : nullptr;
+    f.U = near ? U.p : nullptr;
+    f.ones = near ? ones.p : nullptr;
     f.maxG = (float *)maxG.p;
     f.maxF2 = (float *)maxF2.p;
     f.bound = (unsigned *)bound.p;
@@ -403,10 +408,15 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
         (void)hipEventRecord(hm, s);
     }
     (void)hipEventRecord(h1, s);
-    // single scattering
+    // single scattering: Kronecker factors (far field) or whole rows with a
+    // unit second factor (near field)
     hp.k = np;
-    hp.A1 = SA.p;
-    hp.B1 = SB.p;
+    hp.A1 = near ? U.p : SA.p;
+    hp.B1 = near ? ones.p : SB.p;
+    if (near) {
+        hp.nv = n;
+        hp.nl = 1;
+    }
     hp.A2 = nullptr;
     hp.B2 = nullptr;
     hp.acc = (double2 *)acc_s.p;
@@ -458,7 +468,10 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
     DevBuf out, af;
     if (!out.ensure(sizeof(double) * 2 * ne, &rc) || !af.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
     const double scale = 1.0 / (double)max_itr * box_volume(pr) * pr->sigt;
-    launch_af_table(ps.dp, (double2 *)af.p, s);
+    // far field: the single-scatter amplitude table is a per-row factor; near
+    // field it is already inside the rows
+    const bool far = ps.dp.is_ff_l && ps.dp.is_ff_v;
+    if (far) launch_af_table(ps.dp, (double2 *)af.p, s);
     double *o = (double *)out.p;
     struct Dst {
         double *re, *im;
@@ -482,7 +495,7 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
         return hip_ok(hipStreamSynchronize(s), "finalize sync", &r2) ? SCMC_OK : r2;
     };
     // MCcov.m:293-299: Mm/2 with CBS, then /maxItr*V*sigt; Ms /maxItr*V*sigt
-    if ((rc = emit(acc_s, ks_s, scale, (const double2 *)af.p, Dst{ms_re, ms_im}))) return rc;
+    if ((rc = emit(acc_s, ks_s, scale, far ? (const double2 *)af.p : nullptr, Dst{ms_re, ms_im}))) return rc;
     if ((rc = emit(acc_m, ks_m, pr->do_cbs ? scale / 2 : scale, nullptr, Dst{mm_re, mm_im}))) return rc;
     return SCMC_OK;
 }
@@ -516,14 +529,6 @@ int scmc_device_count(void)
     return n;
 }
 
-static bool far_field_only(const scmc_problem *pr, int *rc)
-{
-    if (!pr->is_ff_l || !pr->is_ff_v) {
-        *rc = set_error(SCMC_E_UNSUPPORTED, "near-field sources are not supported by the GPU engines");
-        return false;
-    }
-    return true;
-}
 
 } // extern "C"
 
@@ -564,7 +569,6 @@ int Session::init(const scmc_problem *pr, int kind_, int precision)
 {
     int rc = validate_problem(pr, 1, true);
     if (rc) return rc;
-    if (!far_field_only(pr, &rc)) return rc;
     if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
     if (kind_ != 0 && kind_ != 1) return set_error(SCMC_E_ARG, "invalid session kind %d", kind_);
     if ((rc = ensure_device(nullptr))) return rc;
@@ -769,7 +773,6 @@ int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t 
 {
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
-    if (!far_field_only(pr, &rc)) return rc;
     if (!pset || pset->n_paths < 1) return set_error(SCMC_E_ARG, "empty path set");
     if ((rc = ensure_device(nullptr))) return rc;
     hipStream_t s;
@@ -792,7 +795,6 @@ int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_
 {
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
-    if (!far_field_only(pr, &rc)) return rc;
     if (!pset || pset->n_paths < 1) return set_error(SCMC_E_ARG, "empty path set");
     if ((rc = ensure_device(nullptr))) return rc;
     hipStream_t s;

@@ -122,6 +122,53 @@ __device__ __forceinline__ double2 eval_amp(double c, const DevPhase &ph)
     return make_double2(sqrt(evaluate_hg<DIM>(c, ph.g)), 0.0);
 }
 
+// Entrance directions at a path's first vertex x1 (MCcov.m:99-105,128-135;
+// MCfield.m:121-136): in the far field the direction columns themselves; in
+// the near field rv_i = (v_i - x1)/|v_i - x1| and rl_j = (x1 - l_j)/|x1 - l_j|.
+// Every vertex of the path reuses them (the reference computes them once per
+// path).  r must hold 3 doubles.
+template <int DIM>
+__device__ __forceinline__ void view_dir(const DevProblem &P, const double *x1, int i, double *r)
+{
+    const double *c = P.v + (size_t)i * DIM;
+    r[2] = 0.0;
+    if (P.is_ff_v) {
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) r[j] = c[j];
+        return;
+    }
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+        r[j] = c[j] - x1[j];
+        s += r[j] * r[j];
+    }
+    const double nrm = sqrt(s);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) r[j] = r[j] / nrm;
+}
+
+template <int DIM>
+__device__ __forceinline__ void light_dir(const DevProblem &P, const double *x1, int i, double *r)
+{
+    const double *c = P.l + (size_t)i * DIM;
+    r[2] = 0.0;
+    if (P.is_ff_l) {
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) r[j] = c[j];
+        return;
+    }
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+        r[j] = x1[j] - c[j];
+        s += r[j] * r[j];
+    }
+    const double nrm = sqrt(s);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) r[j] = r[j] / nrm;
+}
+
 __device__ __forceinline__ double2 cdiv(double2 a, double2 b)
 {
     // a / b with the textbook formula; b is an amplitude (never tiny).

@@ -39,8 +39,10 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np
     const double rs2 = 1 / sqrt(2.0);
     for (int l = threadIdx.x; l < P.nl; l += blockDim.x) {
         const double *lc = P.l + (size_t)l * DIM;
+        double rl[3];
+        light_dir<DIM>(P, pr.x1, l, rl);   // MCfield.m:134-137 (near field: from x1)
         double2 el0 = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
-        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, lc), P.amp), pr.w0p);
+        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, rl), P.amp), pr.w0p);
         double2 el0ms = cmul(el0, af);
         double2 R = make_double2(0.0, 0.0);
         for (int64_t vi = o; vi < e; ++vi) {
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np
                     t.y *= rs2;
                     double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
                     double2 el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l),
-                                      eval_amp<DIM>(dot<DIM>(mow, lc), P.amp));
+                                      eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
                     double2 q = cmul(el, cn);
                     R.x += q.x * rs2;
                     R.y += q.y * rs2;
@@ -228,6 +230,114 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
         for (int q = 0; q < nlp; ++q) part[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
 }
 
+// Near-field sources (either side): the entrance directions rl/rv depend on
+// the path's first vertex x1 (MCfield.m:129-143), so the amplitude of every
+// entry is evaluated per (entry, view) from x1, and the first-vertex
+// amplitude table af_ang_vl per (entry, view, light).  Evaluated in FP64 in
+// both precision modes.
+struct NEntry {
+    double x[3];
+    double x1[3];
+    double ad[3];
+    double2 div;
+    int kind;
+    int pad;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
+                                                                 int nsplit, int l0, int nlp, const PathRec *paths,
+                                                                 const VertexRec *verts, const int32_t *vpath,
+                                                                 const int64_t *offsets, const double2 *coef,
+                                                                 const double2 *coefR, double2 *part)
+{
+    __shared__ NEntry se[kFieldBatch];
+    __shared__ double2 sc[kFieldBatch][kFieldNL];
+    __shared__ double srl[kFieldBatch][kFieldNL][3];
+    const int v = blockIdx.x * kFieldViews + threadIdx.x;
+    const bool vok = v < P.nv;
+    const int split = blockIdx.y;
+    const int64_t e0 = nent * split / nsplit, e1 = nent * (split + 1) / nsplit;
+    double2 acc[kFieldNL];
+    for (int q = 0; q < kFieldNL; ++q) acc[q] = make_double2(0.0, 0.0);
+
+    for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
+        const int cnt = (int)((e1 - eb) < kFieldBatch ? (e1 - eb) : kFieldBatch);
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const int64_t e = eb + threadIdx.x;
+            NEntry &f = se[threadIdx.x];
+            if (e < nvert) {
+                const VertexRec r = verts[e];
+                const int p = vpath[e];
+                for (int j = 0; j < 3; ++j) {
+                    f.x[j] = r.x[j];
+                    f.x1[j] = paths[p].x1[j];
+                    f.ad[j] = r.ow[j];
+                }
+                f.kind = (e == offsets[p]) ? 0 : 1;
+                f.div = make_double2(1.0, 0.0);
+            } else {
+                const PathRec pr = paths[e - nvert];
+                for (int j = 0; j < 3; ++j) {
+                    f.x[j] = pr.x1[j];
+                    f.x1[j] = pr.x1[j];
+                    f.ad[j] = -pr.w1[j];
+                }
+                f.kind = 2;
+                double den = pr.w0p.x * pr.w0p.x + pr.w0p.y * pr.w0p.y;
+                f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
+            }
+        }
+        for (int t = threadIdx.x; t < cnt * kFieldNL; t += blockDim.x) {
+            const int ei = t / kFieldNL, q = t % kFieldNL;
+            const int64_t e = eb + ei;
+            double2 c = make_double2(0.0, 0.0);
+            if (q < nlp)
+                c = (e < nvert) ? coef[e * P.nl + l0 + q] : coefR[(e - nvert) * P.nl + l0 + q];
+            sc[ei][q] = c;
+        }
+        __syncthreads();
+        // light entrance directions at each entry's x1 (used by first vertices)
+        for (int t = threadIdx.x; t < cnt * kFieldNL; t += blockDim.x) {
+            const int ei = t / kFieldNL, q = t % kFieldNL;
+            double rl[3] = {0.0, 0.0, 0.0};
+            if (q < nlp && se[ei].kind == 0) light_dir<DIM>(P, se[ei].x1, l0 + q, rl);
+            for (int j = 0; j < 3; ++j) srl[ei][q][j] = rl[j];
+        }
+        __syncthreads();
+        if (!vok) continue;
+        for (int ei = 0; ei < cnt; ++ei) {
+            const NEntry &f = se[ei];
+            double x1[3] = {f.x1[0], f.x1[1], f.x1[2]};
+            double rv[3];
+            view_dir<DIM>(P, x1, v, rv);
+            double xx[3] = {f.x[0], f.x[1], f.x[2]};
+            double2 ev = phase_att64<DIM>(P, xx, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+            if (f.kind == 0) {
+                for (int q = 0; q < kFieldNL; ++q) {
+                    double rl[3] = {srl[ei][q][0], srl[ei][q][1], srl[ei][q][2]};
+                    double2 t = cmul(ev, eval_amp<DIM>(dot<DIM>(rl, rv), P.amp));
+                    double2 c = sc[ei][q];
+                    acc[q].x += t.x * c.x - t.y * c.y;
+                    acc[q].y += t.x * c.y + t.y * c.x;
+                }
+            } else {
+                double ad[3] = {f.ad[0], f.ad[1], f.ad[2]};
+                double2 t = cmul(ev, eval_amp<DIM>(dot<DIM>(ad, rv), P.amp));
+                if (f.kind == 2) t = cmul(t, f.div);
+                for (int q = 0; q < kFieldNL; ++q) {
+                    double2 c = sc[ei][q];
+                    acc[q].x += t.x * c.x - t.y * c.y;
+                    acc[q].y += t.x * c.y + t.y * c.x;
+                }
+            }
+        }
+    }
+    if (vok)
+        for (int q = 0; q < nlp; ++q) part[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+}
+
 __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, double2 *acc)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,7 +363,8 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     double2 *part = aftab + (size_t)P.nv * P.nl;
     hipLaunchKernelGGL(field_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts, offsets, coef,
                        coefR);
-    launch_af_table(P, aftab, s);
+    const bool near = !(P.is_ff_l && P.is_ff_v);
+    if (!near) launch_af_table(P, aftab, s);
     const int64_t nent = nvert + (P.do_cbs ? np : 0);
     const int vt = (P.nv + kFieldViews - 1) / kFieldViews;
     int nsplit = (2048 + vt - 1) / vt;
@@ -264,7 +375,10 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     for (int l0 = 0; l0 < P.nl; l0 += kFieldNL) {
         const int nlp = (P.nl - l0) < kFieldNL ? (P.nl - l0) : kFieldNL;
         dim3 g((unsigned)vt, (unsigned)nsplit);
-        if (mixed)
+        if (near)
+            hipLaunchKernelGGL((field_near_kernel<DIM>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
+                               nlp, paths, verts, vpath, offsets, coef, coefR, part);
+        else if (mixed)
             hipLaunchKernelGGL((field_kernel<DIM, true>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
                                nlp, paths, verts, vpath, offsets, coef, coefR, aftab, part);
         else

@@ -35,11 +35,13 @@ void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64
 // Factor arrays for the rank-K update C += sum_k u_k u_k^H with
 //   u_k(v,l) = A1[k][v] B1[k][l] + A2[k][v] B2[k][l]   (A2/B2 null without CBS)
 // Single scattering uses A1 = sqrt(albedo/px) e_v0, B1 = e_l0 (the
-// amplitude product a(l.v) is a per-row factor applied when finalising).
+// amplitude product a(l.v) is a per-row factor applied when finalising) in
+// the far field, and whole rows U with B = 1 in the near field.
 struct CovFactors {
     void *A1, *B1, *A2, *B2;   // float2* (mixed) or double2* (fp64)
     void *SA, *SB;             // single-scatter factors per path
     void *Gp, *Fp2;            // path-level e_l0_ms, e_v0_ms
+    void *U, *ones;            // near field: single-scatter rows [np][nv*nl], ones[np]
     float *maxG, *maxF2;       // path-level maxima (mixed scaling)
     unsigned *bound;           // [0] multi, [1] single: float bits of max |u|
 };

@@ -7,8 +7,9 @@
 // whose outer product u u^H expands to the four M00/Mnn/Mn0/M0n terms of
 // MCcov.m:244-254 (the second term only with CBS; MCcov.m:256-262 otherwise).
 // Single scattering (MCcov.m:234-241) is u_s(v,l) = sqrt(albedo/px) e_v0(v)
-// e_l0(l) af_ang_vl(v,l); the amplitude table is a per-row factor applied when
-// the covariance is finalised.
+// e_l0(l) af_ang_vl(v,l); in the far field the amplitude table is a per-row
+// factor applied when the covariance is finalised, in the near field it
+// depends on the first vertex and the rows are written out whole.
 //
 // Evaluation is FP64 in both precision modes (it is <1% of the work at the
 // headline size); the mixed mode stores FP32 factors and records the largest
@@ -41,7 +42,12 @@ __device__ __forceinline__ void atomic_max_pos(unsigned *p, float v)
     if (v > 0.f) atomicMax(p, __float_as_uint(v));
 }
 
-// MCcov.m:157-179 : path-level factors at the first vertex (far field).
+// MCcov.m:157-179 : path-level factors at the first vertex.  The entrance
+// directions rl/rv are the source columns in the far field and depend on x1
+// in the near field (MCcov.m:128-135).  Far field only, the single-scatter
+// bound is the product of the factor maxima (the amplitude table is applied
+// when finalising); near field the single-scatter rows carry the amplitude
+// and nee_single_near_kernel records their bound.
 template <int DIM, typename T>
 __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t np, const PathRec *paths, T *Gp,
                                                            T *Fp2, T *SA, T *SB, float *maxG, float *maxF2,
@@ -55,8 +61,10 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t
     float mG = 0.f, mF = 0.f, mSA = 0.f, mSB = 0.f;
     for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
         const double *lc = P.l + (size_t)i * DIM;
+        double rl[3];
+        light_dir<DIM>(P, pr.x1, i, rl);
         double2 e = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
-        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, lc), P.amp), pr.w0p);
+        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, rl), P.amp), pr.w0p);
         double2 g = cmul(e, af);
         Gp[p * P.nl + i] = to_c<T>(g);
         SB[p * P.nl + i] = to_c<T>(e);
@@ -70,8 +78,10 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t
         SA[p * P.nv + i] = to_c<T>(se);
         mSA = fmaxf(mSA, cmag(se));
         if (P.do_cbs) {
+            double rv[3];
+            view_dir<DIM>(P, pr.x1, i, rv);
             double mw[3] = {-pr.w1[0], -pr.w1[1], -pr.w1[2]};
-            double2 af = cdiv(eval_amp<DIM>(dot<DIM>(mw, vc), P.amp), pr.w0p);
+            double2 af = cdiv(eval_amp<DIM>(dot<DIM>(mw, rv), P.amp), pr.w0p);
             double2 f = cmul(e, af);
             Fp2[p * P.nv + i] = to_c<T>(f);
             mF = fmaxf(mF, cmag(f));
@@ -84,7 +94,41 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t
     if (threadIdx.x == 0) {
         maxG[p] = mG;
         maxF2[p] = mF;
-        atomic_max_pos(&bound[1], 2.f * mSA * mSB);
+        if (P.is_ff_l && P.is_ff_v) atomic_max_pos(&bound[1], 2.f * mSA * mSB);
+    }
+}
+
+// Near-field single scattering (MCcov.m:138-142, 234-241): the amplitude
+// af(v,l) = a(rl_l . rv_v) depends on x1, so the rank-1 term is not a
+// Kronecker product; the rows u_s[p][v + nv*l] = SA[p][v] SB[p][l] af(v,l)
+// are written out and fed to the update as an (n x 1) "Kronecker" pair with
+// B = 1 (ones[p]).
+template <int DIM, typename T>
+__global__ __launch_bounds__(256) void nee_single_near_kernel(DevProblem P, int64_t np, const PathRec *paths,
+                                                              const T *SA, const T *SB, T *U, T *ones,
+                                                              unsigned *bound)
+{
+    __shared__ float sh[4];
+    const int64_t p = blockIdx.x;
+    if (p >= np) return;
+    const PathRec pr = paths[p];
+    const int n = P.nv * P.nl;
+    float mU = 0.f;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const int v = r % P.nv, l = r / P.nv;
+        double rl[3], rv[3];
+        light_dir<DIM>(P, pr.x1, l, rl);
+        view_dir<DIM>(P, pr.x1, v, rv);
+        double2 af = eval_amp<DIM>(dot<DIM>(rl, rv), P.amp);
+        T a = SA[p * P.nv + v], b = SB[p * P.nl + l];
+        double2 u = cmul(cmul(make_double2(a.x, a.y), make_double2(b.x, b.y)), af);
+        U[p * n + r] = to_c<T>(u);
+        mU = fmaxf(mU, cmag(u));
+    }
+    mU = block_max(mU, sh);
+    if (threadIdx.x == 0) {
+        ones[p] = to_c<T>(make_double2(1.0, 0.0));
+        atomic_max_pos(&bound[1], mU);
     }
 }
 
@@ -104,12 +148,15 @@ __global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64
     if (vi == offsets[p]) return;   // first vertex: single scattering
     const int64_t m = vi - p - 1;
     const VertexRec r = verts[vi];
-    const double sw = sqrt(r.weight / paths[p].px);
+    const PathRec pr = paths[p];
+    const double sw = sqrt(r.weight / pr.px);
     float mA = 0.f, mB = 0.f;
     for (int i = threadIdx.x; i < P.nv; i += blockDim.x) {
         const double *vc = P.v + (size_t)i * DIM;
+        double rv[3];
+        view_dir<DIM>(P, pr.x1, i, rv);
         double2 e = phase_att64<DIM>(P, r.x, vc, P.ff_sign, P.is_ff_v);
-        double2 af = eval_amp<DIM>(dot<DIM>(r.ow, vc), P.amp);
+        double2 af = eval_amp<DIM>(dot<DIM>(r.ow, rv), P.amp);
         double2 f = cmul(e, af);
         f.x *= sw;
         f.y *= sw;
@@ -121,9 +168,11 @@ __global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64
         B1[m * P.nl + i] = Gp[(int64_t)p * P.nl + i];
         if (P.do_cbs) {
             const double *lc = P.l + (size_t)i * DIM;
+            double rl[3];
+            light_dir<DIM>(P, pr.x1, i, rl);
             double2 e = phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l);
             double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
-            double2 af = eval_amp<DIM>(dot<DIM>(mow, lc), P.amp);
+            double2 af = eval_amp<DIM>(dot<DIM>(mow, rl), P.amp);
             double2 g = cmul(e, af);
             g.x *= sw;
             g.y *= sw;
@@ -144,6 +193,9 @@ static void nee_cov_dispatch(const DevProblem &P, int64_t np, int64_t nvert, con
     if (np > 0)
         hipLaunchKernelGGL((nee_cov_path_kernel<DIM, T>), dim3((unsigned)np), dim3(128), 0, s, P, np, paths,
                            (T *)f.Gp, (T *)f.Fp2, (T *)f.SA, (T *)f.SB, f.maxG, f.maxF2, f.bound);
+    if (np > 0 && !(P.is_ff_l && P.is_ff_v))
+        hipLaunchKernelGGL((nee_single_near_kernel<DIM, T>), dim3((unsigned)np), dim3(256), 0, s, P, np, paths,
+                           (const T *)f.SA, (const T *)f.SB, (T *)f.U, (T *)f.ones, f.bound);
     if (nvert > 0)
         hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T>), dim3((unsigned)nvert), dim3(128), 0, s, P, nvert, paths,
                            verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1,

@@ -54,7 +54,7 @@ struct CovEngine {
     double ms_herk = 0, herk_flops = 0, ms_expand = 0;
     int64_t herk_launches = 0;
     DevBuf acc_m, acc_s, bound, zbuf, order;
-    DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2;
+    DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2, U, ones;
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
     int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee);

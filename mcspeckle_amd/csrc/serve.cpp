@@ -58,7 +58,7 @@ struct Server {
 static std::mutex g_srv_mu;
 static std::unique_ptr<Server> g_srv;
 
-int64_t request_footprint(int nv, int nl, int precision)
+int64_t request_footprint(int nv, int nl, int precision, bool near)
 {
     const double n = (double)nv * nl;
     const bool mixed = precision == SCMC_MIXED;
@@ -72,7 +72,9 @@ int64_t request_footprint(int nv, int nl, int precision)
     const double cap = mixed ? 4.0e9 / (8.0 * n) : 2.5e9 / (16.0 * n);
     const double kchunk = std::max(65536.0, std::min(4.0e6, cap));
     const double cs = mixed ? 8.0 : 16.0;
-    const double chunk = kchunk * ((mixed ? 8.0 * n : 0.0) + 2.0 * (nv + nl) * cs * 2.0 + 80.0);
+    double chunk = kchunk * ((mixed ? 8.0 * n : 0.0) + 2.0 * (nv + nl) * cs * 2.0 + 80.0);
+    // near field: whole single-scatter rows, one per path (<= one per vertex)
+    if (near) chunk += kchunk * n * cs;
     return (int64_t)(acc + chunk + 64.0 * 1024 * 1024);
 }
 
@@ -172,7 +174,7 @@ extern "C" {
 int64_t scmc_request_footprint(int32_t nv, int32_t nl, int32_t precision)
 {
     if (nv < 1 || nl < 1) return -1;
-    return request_footprint(nv, nl, precision);
+    return request_footprint(nv, nl, precision, false);
 }
 
 int scmc_server_start(const scmc_server_config *cfg)
@@ -217,8 +219,6 @@ int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
     if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
-    if (!pr->is_ff_l || !pr->is_ff_v)
-        return set_error(SCMC_E_UNSUPPORTED, "near-field sources are not supported by the GPU engines");
     std::lock_guard<std::mutex> g(g_srv_mu);
     if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
     Server *S = g_srv.get();
@@ -226,7 +226,7 @@ int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision
     r->pc.assign(pr);
     r->max_itr = max_itr;
     r->precision = precision;
-    r->footprint = request_footprint(pr->nv, pr->nl, precision);
+    r->footprint = request_footprint(pr->nv, pr->nl, precision, !(pr->is_ff_l && pr->is_ff_v));
     r->t_submit = clk::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);

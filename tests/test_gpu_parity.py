@@ -200,3 +200,76 @@ def test_gpu_agrees_with_oracle_within_3_standard_errors(oracle, precision):
     say("  fraction within 3 SE", np.mean(z < 3), "max z", z.max())
     assert np.mean(z < 3) >= 0.98
     assert z.max() < 6
+
+
+# ------------------------------------------------------ near-field sources
+NEAR_CASES = [pytest.param(*c, id="-".join(map(str, c))) for c in
+              [("hg", 3, True, "both"), ("iso", 3, False, "l"), ("tab", 3, True, "v"), ("hg", 2, True, "both"),
+               ("iso", 2, False, "v")]]
+
+
+def near_spec(kind, dim, cbs, which, nv=5, nl=3, seed=11, smp=2):
+    """small_spec with point sources: lights 1-3 below the slab, views 1-3
+    above it (MCcov.m:127-142, evalphaseatt.m:10-16)."""
+    f = build(kind, dim, cbs, smp, None, nv=nv, nl=nl, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    top = f.box_max[-1]
+
+    def points(n, z):
+        return np.vstack([rng.uniform(-2, 2, (dim - 1, n)), z])
+
+    near_l, near_v = which in ("l", "both"), which in ("v", "both")
+    l = points(nl, -rng.uniform(1, 3, nl)) if near_l else f.l
+    v = points(nv, top + rng.uniform(1, 3, nv)) if near_v else f.v
+    amp = f.amp.g if f.amp.type == 3 else (0 if f.amp.type == 1 else tab3d())
+    return E.make_problem(f.sigt, f.albedo, f.box_min, f.box_max, l, v, not near_l, not near_v, f.lam, cbs, smp,
+                          f.amp.type, amp, seed=seed)
+
+
+@pytest.mark.parametrize("kind,dim,cbs,which", NEAR_CASES)
+def test_gpu_near_field_trace_and_replay(oracle, kind, dim, cbs, which):
+    spec = near_spec(kind, dim, cbs, which)
+    say("near trace+replay", kind, dim, cbs, which)
+    n = 400
+    g = E.trace(spec, n)
+    ps = oracle.trace(spec, n)
+    assert np.array_equal(g["offset"], ps["offset"])
+    np.testing.assert_allclose(g["x"], ps["x"], rtol=0, atol=1e-10)
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, n)
+    for precision, tol in (("fp64", 1e-12), ("mixed", 1e-4)):
+        Ms, Mm = E.mccov_replay(spec, ps, n, precision)
+        assert cs_err(Ms, Ms0) <= tol, precision
+        assert cs_err(Mm, Mm0) <= tol, precision
+    u0 = oracle.mcfield_replay(spec, ps, n)
+    for precision, tol in (("fp64", 1e-12), ("mixed", 2e-5)):
+        u = E.mcfield_replay(spec, ps, n, precision)
+        assert field_err(u, u0) <= tol, precision
+
+
+@pytest.mark.parametrize("kind,dim,cbs,which", NEAR_CASES)
+def test_gpu_near_field_full_run_same_seed(oracle, kind, dim, cbs, which):
+    spec = near_spec(kind, dim, cbs, which, nv=6, nl=4)
+    say("near full", kind, dim, cbs, which)
+    n = 1500
+    Ms0, Mm0 = oracle.mccov(spec, n)
+    Ms, Mm = E.mccov_spec(spec, n, "fp64")
+    assert cs_err(Ms, Ms0) <= 1e-10
+    assert cs_err(Mm, Mm0) <= 1e-10
+    Ms1, Mm1 = E.mccov_spec(spec, n, "mixed")
+    assert cs_err(Ms1 + Mm1, Ms0 + Mm0) <= 1e-4
+    u0 = oracle.mcfield(spec, n)
+    assert field_err(E.mcfield_spec(spec, n, "fp64"), u0) <= 1e-10
+    assert field_err(E.mcfield_spec(spec, n, "mixed"), u0) <= 2e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_gpu_near_field_tile_edges(oracle, precision, tol):
+    """Near-field single-scatter rows at N = 130 (crosses the 64/128 tiles)."""
+    spec = near_spec("hg", 3, True, "both", nv=26, nl=5, seed=3)
+    say("near tile edges", precision)
+    n = 60
+    ps = oracle.trace(spec, n)
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, n)
+    Ms, Mm = E.mccov_replay(spec, ps, n, precision)
+    assert cs_err(Ms, Ms0) <= tol
+    assert cs_err(Mm, Mm0) <= tol

@@ -9,6 +9,9 @@
 * MCfield second moment: over the random phases, E[u u^H] equals the
   covariance of the same paths (MCfield.m:207-222 vs MCcov.m:232-264).
 * Invariants: Hermitian, positive semi-definite.
+* Near-field sources: placed at distance R along the far-field directions,
+  the near-field covariance and field tend to the far-field ones (factor
+  R^(dim-1) per near side, e^{ikR} per near side in the field).
 """
 import numpy as np
 import pytest
@@ -235,3 +238,56 @@ def test_mcfield_first_vertex_term_exact(oracle):
         want += np.sqrt(q["weight"][p] / q["px"][p]) * np.exp(2j * np.pi * q["xi"][p]) * s
     want *= np.sqrt(np.prod(spec.box_max - spec.box_min) * spec.sigt / 7)
     np.testing.assert_allclose(u, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
+def near_far_pair(kind, dim, cbs, R, which="both", nv=3, nl=2, seed=31):
+    """A far-field problem and the same problem with its sources placed at
+    distance R along the directions (lights at -R l, views at +R v).  Uniform
+    first vertices and an isotropic first direction make the walks of the two
+    identical (neither depends on the sources)."""
+    f = small_spec(kind, dim, cbs, nv=nv, nl=nl, seed=seed, smp=1, tab=tab3d() if kind == "tab" else None)
+    near_l = which in ("both", "l")
+    near_v = which in ("both", "v")
+    l = -R * f.l if near_l else f.l
+    v = R * f.v if near_v else f.v
+    amp = f.amp.g if f.amp.type == 3 else (0 if f.amp.type == 1 else tab3d())
+    n = make_problem(f.sigt, f.albedo, f.box_min, f.box_max, l, v, not near_l, not near_v, f.lam, cbs, 1,
+                     f.amp.type, amp, seed=f.seed)
+    return f, n
+
+
+@pytest.mark.parametrize("kind,dim,cbs,which", [("hg", 3, True, "both"), ("iso", 3, False, "l"),
+                                                ("tab", 3, True, "v"), ("hg", 2, True, "both")])
+def test_near_field_sources_tend_to_far_field(oracle, kind, dim, cbs, which):
+    """Sources at distance R: every near-field factor is the far-field one times
+    e^{ikR} / R^((dim-1)/2) up to O(|x|/R) in directions and O(k|x|^2/R) in phase,
+    so R^(dim-1) per near side times the covariance tends to the far-field one
+    (the e^{ikR} cancel in u u^H), and the field picks up e^{ikR} per near side."""
+    R = 1e8
+    f, nr = near_far_pair(kind, dim, cbs, R, which)
+    sides = 2 if which == "both" else 1
+    n = 30
+    psf, psn = oracle.trace(f, n), oracle.trace(nr, n)
+    for key in ("offset", "x", "ow", "weight", "x1", "w1", "px"):
+        np.testing.assert_array_equal(psf[key], psn[key])
+    Msf, Mmf = oracle.mccov_replay(f, psf, n)
+    Msn, Mmn = oracle.mccov_replay(nr, psn, n)
+    sc = (R + 0.01 * f.lam) ** ((dim - 1) * sides)
+    for near, far in ((Msn, Msf), (Mmn, Mmf)):
+        C, Cf = cov_matrix(near) * sc, cov_matrix(far)
+        assert np.max(np.abs(C - Cf) / cs_scale(Cf)) < 2e-5
+    uf = oracle.mcfield_replay(f, psf, n)
+    un = oracle.mcfield_replay(nr, psn, n)
+    k = 2 * np.pi / f.lam
+    un = un * np.exp(-1j * k * R * sides) * (R + 0.01 * f.lam) ** ((dim - 1) / 2 * sides)
+    assert np.abs(un - uf).max() < 2e-5 * np.abs(uf).max()
+
+
+def test_near_field_covariance_hermitian_psd(oracle):
+    _, nr = near_far_pair("hg", 3, True, 6.0, "both", nv=4, nl=3)
+    Ms, Mm = oracle.mccov(nr, 300)
+    for M in (Ms, Mm, Ms + Mm):
+        C = cov_matrix(M)
+        np.testing.assert_allclose(C, C.conj().T, atol=1e-13 * np.abs(C).max())
+        ev = np.linalg.eigvalsh((C + C.conj().T) / 2)
+        assert ev.min() > -1e-10 * ev.max()

@@ -76,6 +76,41 @@ def test_phase_att_transmission_closed_form(oracle):
     np.testing.assert_allclose(e / np.abs(e), np.exp(1j * ph), atol=1e-9)
 
 
+@pytest.mark.parametrize("dim", [3, 2])
+def test_phase_att_near_field_closed_form(oracle, dim):
+    """Point source p below the slab z in [0, T], x inside: the straight
+    segment x -> p runs inside the medium for the fraction z/(z + h) of its
+    length nd, so |E| = exp(-sigma/2 * nd z/(z + h)) / (nd + 0.01 lambda)^((dim-1)/2)
+    and arg E = k nd.  A source inside the box: the whole segment is in the
+    medium (|E| uses nd itself)."""
+    L = oracle.lib()
+    sigma, lam, T = 2.5, 0.6, 1.0
+    W = 1e5
+    bmin, pmin = _d([-W, 0.0] if dim == 2 else [-W, -W, 0.0])
+    bmax, pmax = _d([W, T] if dim == 2 else [W, W, T])
+    rng = np.random.default_rng(8)
+    for inside in (False, True):
+        pts = []
+        for _ in range(40):
+            lat = rng.uniform(-3, 3, dim - 1)
+            h = rng.uniform(0.5, 4.0)
+            pts.append(np.r_[lat, rng.uniform(0.05, T - 0.05) if inside else -h])
+        P = np.array(pts)
+        pp, ppt = _d(P.reshape(-1))
+        x = np.r_[rng.uniform(-1, 1, dim - 1), rng.uniform(0.1, T - 0.1)]
+        xx, px = _d(x)
+        re, pre = _d(np.zeros(len(P)))
+        im, pim = _d(np.zeros(len(P)))
+        L.orc_eval_phase_att(px, ppt, len(P), dim, 0, 1.0, sigma, lam, pmin, pmax, pre, pim)
+        e = re + 1j * im
+        nd = np.linalg.norm(x - P, axis=1)
+        zf = x[-1]
+        path_in = nd if inside else nd * zf / (zf - P[:, -1])
+        mag = np.exp(-sigma / 2 * path_in) / (nd + 0.01 * lam) ** ((dim - 1) / 2)
+        np.testing.assert_allclose(np.abs(e), mag, rtol=1e-12)
+        np.testing.assert_allclose(e / np.abs(e), np.exp(2j * np.pi / lam * nd), atol=1e-9)
+
+
 @pytest.mark.parametrize("g", [-0.5, 0.0, 0.3, 0.9, 0.95])
 def test_hg_normalisation(oracle, g):
     L = oracle.lib()
