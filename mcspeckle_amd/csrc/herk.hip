@@ -77,24 +77,18 @@ __device__ __forceinline__ half8 rot_neg_i(half8 a)
 
 // ------------------------------------------------------------ mixed
 namespace mixed {
-constexpr int TM = kTileMixed;   // 128x128 tile per workgroup
-constexpr int KS = 8;            // vertices per MFMA step (K = 16 halfs)
-constexpr int NKS = 2;           // MFMA steps per LDS stage
-constexpr int VPS = KS * NKS;    // vertices per stage
-// LDS: [stage 2][tile 2][plane 2 (hi,lo)][NKS][TM rows][2 x 16 B]
-constexpr int ROWQ = 2;                          // uint4 per row per step
-constexpr int PLANE = NKS * TM * ROWQ;           // uint4 per plane
-constexpr int TILEQ = 2 * PLANE;
-constexpr int STAGEQ = 2 * TILEQ;
-__device__ __forceinline__ int lidx(int stage, int tile, int plane, int ks, int row, int q)
+constexpr int TM = kTileMixed;   // 128x128 tile per workgroup, 4 waves of 64x64
+constexpr int NBUF = 5;          // LDS stages (16 KB each, 80 KB: two workgroups per CU)
+// LDS stage: [tile 2 (rows, columns)][plane 2 (hi, lo)][TM rows][32 B]
+constexpr int STAGEB = 2 * 2 * TM * 32;
+__device__ __forceinline__ int loff(int tile, int plane, int row, int q)
 {
-    return stage * STAGEQ + tile * TILEQ + plane * PLANE + (ks * TM + row) * ROWQ + q;
+    return ((tile * 2 + plane) * TM + row) * 32 + q * 16;
 }
-// The expanded buffer is padded with zeros to whole tiles and whole stages so
-// the update kernel's loads are unconditional (no waits at branch joins).
+// The expanded buffer is padded with zeros to whole tiles and to a multiple
+// of four K-steps so the update kernel's loads are unconditional.
 __host__ __device__ __forceinline__ int pad_rows(int n) { return (n + TM - 1) / TM * TM; }
-// K-steps are padded to whole stage pairs (the update loop is unrolled by two).
-__host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + 2 * NKS - 1) / (2 * NKS) * (2 * NKS); }
+__host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + 3) / 4 * 4; }
 } // namespace mixed
 
 // Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
@@ -146,31 +140,31 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
     d[3] = __builtin_bit_cast(uint4, lo[1]);
 }
 
-// The Hermitian update over pre-expanded rows: a streaming GEMM.
+// The Hermitian update over pre-expanded rows: a streaming GEMM.  The rows
+// of each K-step go from HBM straight into LDS (global_load_lds_dwordx4, no
+// register staging), NBUF - 1 K-steps ahead of the matrix cores; each wave
+// reads its 64-row A and 64-column B fragments one K-step ahead and issues
+// 24 MFMAs per K-step (Re and Im, hi*hi + hi*lo + lo*hi).  Each K-step costs
+// one workgroup barrier.
 __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_total, const uint4 *__restrict__ Z,
                                                             double2 *__restrict__ acc, const unsigned *__restrict__ bound,
                                                             int ksplit, const int2 *__restrict__ order, int ntile)
 {
     using namespace mixed;
-    __shared__ uint4 lds[2 * STAGEQ];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int s, t;
     item_decode(blockIdx.x, ksplit, ntile, s, t);
     const int ti = order[t].x, tj = order[t].y;
-    const bool diag = (ti == tj);
     const int npad = pad_rows(n);
-    const int64_t npair = nks_total / (2 * NKS);                                   // stage pairs
+    const int64_t npair = nks_total / 2;   // K-steps come in pairs (padded to 4)
     const int64_t sb = 2 * (npair * s / ksplit), se = 2 * (npair * (s + 1) / ksplit);
+    const int nstage = (int)(se - sb);
 
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
     if (b > 0.f) frexpf(b, &ex);
     const double unscale = ldexp(1.0, -2 * (15 - ex));
-
-    // loading role: row r of the row tile (T=0) or the column tile (T=1);
-    // every load is in bounds (padded rows and K-steps are zero)
-    const int T = tid >> 7, r = tid & 127;
-    const int gi = (T ? tj : ti) * TM + r;
 
     f32x16 accR[2][2], accI[2][2];
 #pragma unroll
@@ -183,94 +177,101 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
                 accI[a][c][e] = 0.f;
             }
 
+    // Loading role: wave w moves tile T = w/2 (rows of ti, then columns of
+    // tj; a diagonal tile loads the same rows twice), plane P = w%2, four
+    // chunks of 32 rows x 32 B per K-step.  Lane l covers row l/2, 16 B
+    // half l%2: the chunk lands contiguously at M0 + 16 l, which is exactly
+    // the [row][32 B] layout of the stage.
+    const int T = wave >> 1, P = wave & 1;
+    const uint4 *src = Z + (size_t)((T ? tj : ti) * TM + (lane >> 1)) * 4 + P * 2 + (lane & 1);
+    const size_t kstride = (size_t)npad * 4;   // uint4 per K-step
+    const int ldst = loff(T, P, 0, 0);
+    typedef __attribute__((address_space(3))) void *lds_ptr;
+#define SCMC_ISSUE(kstep, buf)                                                                           \
+    do {                                                                                                 \
+        const uint4 *g = src + (size_t)(kstep) * kstride;                                                \
+        unsigned char *d = lds + (buf) * STAGEB + ldst;                                                  \
+        __builtin_amdgcn_global_load_lds((const void *)g, (lds_ptr)d, 16, 0, 0);                         \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 128), (lds_ptr)(d + 1024), 16, 0, 0);       \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_ptr)(d + 2048), 16, 0, 0);       \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 384), (lds_ptr)(d + 3072), 16, 0, 0);       \
+    } while (0)
+
     const int wr = wave >> 1, wc = wave & 1;
     const int half = lane >> 5, lrow = lane & 31;
-    const int btile = diag ? 0 : 1;
-    const int nstage = (int)(se - sb);
-    static_assert(NKS == 2, "the prefetch below holds two K-steps");
-    // global source of this thread's row for stage st: Z[(st*NKS + q)*npad + gi]
-    const uint4 *zrow = Z + (size_t)gi * 4;
-    const size_t kstride = (size_t)npad * 4;   // uint4 per K-step
-    // LDS destinations of this thread's row (stage buffer 0)
-    const int d00 = lidx(0, T, 0, 0, r, 0), d01 = lidx(0, T, 1, 0, r, 0);
-    const int d10 = lidx(0, T, 0, 1, r, 0), d11 = lidx(0, T, 1, 1, r, 0);
-
-    // Two register sets (X, Y) hold the next two stages: loads run two MFMA
-    // blocks ahead of their LDS store, which covers HBM latency under load.
-    uint4 x0, x1, x2, x3, x4, x5, x6, x7, y0, y1, y2, y3, y4, y5, y6, y7;
-#define SCMC_LOAD(R, st)                                                \
-    do {                                                                \
-        const uint4 *z0 = zrow + (size_t)(st) * NKS * kstride;          \
-        const uint4 *z1 = z0 + kstride;                                 \
-        R##0 = z0[0]; R##1 = z0[1]; R##2 = z0[2]; R##3 = z0[3];         \
-        R##4 = z1[0]; R##5 = z1[1]; R##6 = z1[2]; R##7 = z1[3];         \
+    const int oa0 = loff(0, 0, wr * 64 + lrow, half), oa1 = oa0 + 32 * 32;   // A rows, blocks 0/1
+    const int ob0 = loff(1, 0, wc * 64 + lrow, half), ob1 = ob0 + 32 * 32;   // B columns
+    constexpr int PL = TM * 32;                                               // hi -> lo plane
+    // Fragments of one K-step: A hi/lo of the two 32-row blocks, B hi/lo of
+    // the two 32-column blocks.  Two sets (F, G) alternate so the next
+    // K-step's fragments are read while the current one's MFMAs run.
+    half8 Fah0, Fah1, Fal0, Fal1, Fbh0, Fbh1, Fbl0, Fbl1;
+    half8 Gah0, Gah1, Gal0, Gal1, Gbh0, Gbh1, Gbl0, Gbl1;
+#define SCMC_READ(X, bufi)                                                  \
+    do {                                                                    \
+        const unsigned char *bp = lds + (bufi) * STAGEB;                    \
+        X##ah0 = *(const half8 *)(bp + oa0);                                \
+        X##ah1 = *(const half8 *)(bp + oa1);                                \
+        X##bh0 = *(const half8 *)(bp + ob0);                                \
+        X##bh1 = *(const half8 *)(bp + ob1);                                \
+        X##al0 = *(const half8 *)(bp + oa0 + PL);                           \
+        X##al1 = *(const half8 *)(bp + oa1 + PL);                           \
+        X##bl0 = *(const half8 *)(bp + ob0 + PL);                           \
+        X##bl1 = *(const half8 *)(bp + ob1 + PL);                           \
     } while (0)
-#define SCMC_STORE(R, buf)                                              \
-    do {                                                                \
-        const int o = (buf) * STAGEQ;                                   \
-        lds[o + d00] = R##0; lds[o + d00 + 1] = R##1;                   \
-        lds[o + d01] = R##2; lds[o + d01 + 1] = R##3;                   \
-        lds[o + d10] = R##4; lds[o + d10 + 1] = R##5;                   \
-        lds[o + d11] = R##6; lds[o + d11 + 1] = R##7;                   \
+#define SCMC_MFMA_ROW(X, rb)                                                                               \
+    do {                                                                                                   \
+        const half8 rh = rot_neg_i(X##ah##rb), rl = rot_neg_i(X##al##rb);                                  \
+        accR[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##ah##rb, X##bh0, accR[rb][0], 0, 0, 0);      \
+        accI[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, X##bh0, accI[rb][0], 0, 0, 0);            \
+        accR[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##ah##rb, X##bh1, accR[rb][1], 0, 0, 0);      \
+        accI[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, X##bh1, accI[rb][1], 0, 0, 0);            \
+        accR[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##ah##rb, X##bl0, accR[rb][0], 0, 0, 0);      \
+        accI[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, X##bl0, accI[rb][0], 0, 0, 0);            \
+        accR[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##ah##rb, X##bl1, accR[rb][1], 0, 0, 0);      \
+        accI[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, X##bl1, accI[rb][1], 0, 0, 0);            \
+        accR[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##al##rb, X##bh0, accR[rb][0], 0, 0, 0);      \
+        accI[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, X##bh0, accI[rb][0], 0, 0, 0);            \
+        accR[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##al##rb, X##bh1, accR[rb][1], 0, 0, 0);      \
+        accI[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, X##bh1, accI[rb][1], 0, 0, 0);            \
     } while (0)
-#define SCMC_MFMA_STAGE(buf)                                                                                    \
-    do {                                                                                                        \
-        _Pragma("unroll") for (int ks = 0; ks < NKS; ++ks)                                                      \
-        {                                                                                                       \
-            half8 ah[2], al[2], bh[2], bl[2];                                                                   \
-            _Pragma("unroll") for (int rb = 0; rb < 2; ++rb)                                                    \
-            {                                                                                                   \
-                const int row = wr * 64 + rb * 32 + lrow;                                                       \
-                ah[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 0, ks, row, half)]);                        \
-                al[rb] = __builtin_bit_cast(half8, lds[lidx(buf, 0, 1, ks, row, half)]);                        \
-            }                                                                                                   \
-            _Pragma("unroll") for (int cb = 0; cb < 2; ++cb)                                                    \
-            {                                                                                                   \
-                const int col = wc * 64 + cb * 32 + lrow;                                                       \
-                bh[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 0, ks, col, half)]);                    \
-                bl[cb] = __builtin_bit_cast(half8, lds[lidx(buf, btile, 1, ks, col, half)]);                    \
-            }                                                                                                   \
-            _Pragma("unroll") for (int rb = 0; rb < 2; ++rb)                                                    \
-            {                                                                                                   \
-                const half8 rh = rot_neg_i(ah[rb]), rl = rot_neg_i(al[rb]);                                     \
-                _Pragma("unroll") for (int cb = 0; cb < 2; ++cb)                                                \
-                {                                                                                               \
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bh[cb], accR[rb][cb], 0, 0, 0); \
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bh[cb], accI[rb][cb], 0, 0, 0);     \
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rb], bl[cb], accR[rb][cb], 0, 0, 0); \
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rh, bl[cb], accI[rb][cb], 0, 0, 0);     \
-                    accR[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rb], bh[cb], accR[rb][cb], 0, 0, 0); \
-                    accI[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, bh[cb], accI[rb][cb], 0, 0, 0);     \
-                }                                                                                               \
-            }                                                                                                   \
-        }                                                                                                       \
-    } while (0)
-
-    // nstage is even; stages are clamped so every load stays inside the item
+    // nstage is even.  Every load stays inside the item: stages past the
+    // last reload the last one into a buffer that is never read again.
     const int last = nstage - 1;
     if (nstage > 0) {
-        SCMC_LOAD(x, sb);
-        SCMC_STORE(x, 0);
-        SCMC_LOAD(x, sb + (1 < last ? 1 : last));
+        SCMC_ISSUE(sb, 0);
+        SCMC_ISSUE(sb + (1 < last ? 1 : last), 1);
+        SCMC_ISSUE(sb + (2 < last ? 2 : last), 2);
+        SCMC_ISSUE(sb + (3 < last ? 3 : last), 3);
+        __builtin_amdgcn_s_waitcnt(0xF7C);   // vmcnt(12): stage 0
+        __builtin_amdgcn_s_barrier();
+        SCMC_READ(F, 0);
     }
-    __syncthreads();
+    // K-step st with fragments X: publish stage st + 1 (its loads were issued
+    // three steps ago), issue stage st + 4 into the buffer of stage st - 1
+    // (its fragments were read during step st - 2 and consumed by step
+    // st - 1, which every wave finished before this barrier), run the MFMAs
+    // of stage st and read stage st + 1's fragments into Y between them.
+#define SCMC_STEP(X, Y, st)                                                                              \
+    do {                                                                                                 \
+        __builtin_amdgcn_s_waitcnt(0xF78); /* vmcnt(8): stage st+1 landed */                            \
+        __builtin_amdgcn_s_barrier();                                                                    \
+        SCMC_ISSUE(sb + ((st) + 4 < last ? (st) + 4 : last), ((st) + 4) % NBUF);                         \
+        SCMC_MFMA_ROW(X, 0);                                                                             \
+        SCMC_READ(Y, ((st) + 1) % NBUF);                                                                 \
+        SCMC_MFMA_ROW(X, 1);                                                                             \
+    } while (0)
     for (int st = 0; st < nstage; st += 2) {
-        // stage st is in LDS buffer 0, stage st+1 in registers X
-        SCMC_LOAD(y, sb + (st + 2 < last ? st + 2 : last));
-        __builtin_amdgcn_sched_barrier(0);
-        SCMC_MFMA_STAGE(0);
-        SCMC_STORE(x, 1);
-        __syncthreads();
-        // stage st+1 is in LDS buffer 1, stage st+2 in registers Y
-        SCMC_LOAD(x, sb + (st + 3 < last ? st + 3 : last));
-        __builtin_amdgcn_sched_barrier(0);
-        SCMC_MFMA_STAGE(1);
-        SCMC_STORE(y, 0);
-        __syncthreads();
+        SCMC_STEP(F, G, st);
+        SCMC_STEP(G, F, st + 1);
     }
-#undef SCMC_MFMA_STAGE
-#undef SCMC_LOAD
-#undef SCMC_STORE
+#undef SCMC_STEP
+#undef SCMC_MFMA_ROW
+#undef SCMC_READ
+#undef SCMC_ISSUE
+    // the clamped loads past the last stage may still be writing LDS: drain
+    // them before the workgroup can end and its LDS be handed to another
+    __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
 
     // epilogue: add the unscaled FP32 tile into accumulator copy s
     double2 *C = acc + (size_t)s * n * n;
