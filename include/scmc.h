@@ -136,6 +136,29 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr, double v_max, double v_stp
 int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp,
                    double lambda, int32_t *nv0, int32_t *nx, int32_t *nz);
 
+/* u = MCfieldOnWave(sigt,albedo,box_min,box_max,l,v,is_ff_l,is_ff_v,maxItr,
+ * lambda,doCBS,smpFlg,sct_type,ampfunc,ampfunc0,lmean0,Wl) --
+ * code/MCfieldOnWave.m:1-267: the field of the illumination wave
+ * sum_l Wl(l) * light l.  wl_re/wl_im hold nl weights (wl_im may be NULL).
+ * Output nv x 1.  The single-scattering term sums over lights for every
+ * scattering type (the reference's isotropic branch, MCfieldOnWave.m:214,
+ * only does so for Nl = 1). */
+int scmc_mcfield_wave(const scmc_problem *pr, const double *wl_re, const double *wl_im, int64_t max_itr,
+                      int32_t precision, double *u_re, double *u_im, scmc_stats *stats);
+
+/* u = MCfieldFourierQuadOnWave(sigt,albedo,box_min,box_max,l,v_max,v_stp,
+ * maxItr,lambda,doCBS,smpFlg,lmean0,Wl,vsign) --
+ * code/MCfieldFourierQuadOnWave.m:1-337.  As scmc_mcfield_fourier_quad with
+ * the lights combined per vertex with the weights Wl (:281-302), the grid
+ * parity fix of :100-102 and the sign conventions of :119, :143, :302.
+ * Output nv0 x nv0 with nv0 from scmc_quad_grid_wave. */
+int scmc_mcfield_fourier_quad_wave(const scmc_problem *pr, const double *wl_re, const double *wl_im,
+                                   double v_max, double v_stp, int64_t max_itr, double vsign,
+                                   int32_t precision, double *u_re, double *u_im, scmc_stats *stats);
+/* Grid sizes of MCfieldFourierQuadOnWave.m:84-131. */
+int scmc_quad_grid_wave(const double box_min[3], const double box_max[3], double v_max, double v_stp,
+                        double lambda, int32_t *nv0, int32_t *nx, int32_t *nz);
+
 /* ---- sessions: accumulate path ranges, read results at any time -------- */
 /* A session owns the device accumulators of one MCcov (kind 0) or MCfield
  * (kind 1) problem.  scmc_session_run adds paths [first, first + n) (ids

@@ -565,7 +565,7 @@ void ProblemCopy::assign(const scmc_problem *src)
     }
 }
 
-int Session::init(const scmc_problem *pr, int kind_, int precision)
+int Session::init(const scmc_problem *pr, int kind_, int precision, const double *wl_re, const double *wl_im)
 {
     int rc = validate_problem(pr, 1, true);
     if (rc) return rc;
@@ -578,7 +578,7 @@ int Session::init(const scmc_problem *pr, int kind_, int precision)
     if (!hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
     if ((rc = ps.setup(&pc.pr, stream))) return rc;
     memset(&st, 0, sizeof st);
-    return kind == 0 ? ce.init(ps, mixed, stream) : fe.init(ps, stream);
+    return kind == 0 ? ce.init(ps, mixed, stream) : fe.init(ps, stream, wl_re, wl_im);
 }
 
 Session::~Session()
@@ -720,6 +720,25 @@ int scmc_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision, dou
                  scmc_stats *stats)
 {
     return one_shot(pr, 1, max_itr, precision, u_re, u_im, nullptr, nullptr, stats);
+}
+
+
+int scmc_mcfield_wave(const scmc_problem *pr, const double *wl_re, const double *wl_im, int64_t max_itr,
+                      int32_t precision, double *u_re, double *u_im, scmc_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = validate_problem(pr, max_itr, true);
+    if (rc) return rc;
+    if (!wl_re) return set_error(SCMC_E_ARG, "Wl must have one weight per light");
+    Session s;
+    if ((rc = s.init(pr, 1, precision, wl_re, wl_im))) return rc;
+    if ((rc = s.run(0, max_itr))) return rc;
+    rc = s.result(max_itr, u_re, u_im, nullptr, nullptr);
+    if (stats) {
+        s.fill_stats(stats);
+        stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
 }
 
 int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int64_t *offset, double *x1, double *px,

@@ -12,6 +12,12 @@
 // where c_n = sqrt(W_n/px) exp(2 pi i xi_n) (MCfield.m:218-220).  The CBS
 // term of every vertex of a path shares x1, so it is summed per path first.
 //
+// Illumination wave (code/MCfieldOnWave.m): the lights are combined with the
+// weights Wl before the outer product, so every light-side coefficient becomes
+// a sum over lights and u has one column; the first-vertex entry keeps its
+// per-light coefficients because af_ang_vl(v,l) couples view and light
+// (MCfieldOnWave.m:214).
+//
 // Layout: one lane per view, a workgroup per 256 views x a slice of the
 // entries; entries are staged through LDS in batches and broadcast to all
 // lanes; FP64 accumulators in registers; per-slice partial sums are reduced
@@ -73,12 +79,97 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np
     }
 }
 
+__device__ __forceinline__ double2 wave_sum(double2 v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o);
+        v.y += __shfl_xor(v.y, o);
+    }
+    return v;
+}
+
+// MCfieldOnWave.m:165-221 per path (one wave, lanes over lights):
+//   S0 = sum_l e_l0(l) af_l(l) Wl(l)                       (:177)
+//   coef[n] = c_n S0 [/sqrt 2]      for vertices n >= 2     (:211, :213)
+//   coefR   = sum_n c_n SL_n / sqrt 2, SL_n = sum_l e_l(x_n, l) a(-ow_n . rl_l) Wl(l)   (:199, :213)
+//   cl0[l]  = c_1 e_l0(l) Wl(l)     (first vertex, contracted with af_ang_vl in the
+//                                    view kernel, :214)
+template <int DIM>
+__global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64_t np, const PathRec *paths,
+                                                             const VertexRec *verts, const int64_t *offsets,
+                                                             const double2 *wl, double2 *coef, double2 *coefR,
+                                                             double2 *cl0)
+{
+    const int64_t p = blockIdx.x;
+    if (p >= np) return;
+    const PathRec pr = paths[p];
+    const int64_t o = offsets[p], e = offsets[p + 1];
+    const double rs2 = 1 / sqrt(2.0);
+    const int lane = threadIdx.x;
+    auto cn_of = [&](int64_t vi) {
+        const VertexRec r = verts[vi];
+        double amp = sqrt(r.weight / pr.px);
+        double sn, cs;
+        sincos(2 * kPi * r.xi, &sn, &cs);
+        return make_double2(amp * cs, amp * sn);
+    };
+    const double2 c1 = cn_of(o);
+    double2 s0 = make_double2(0.0, 0.0);
+    for (int l = lane; l < P.nl; l += 64) {
+        const double *lc = P.l + (size_t)l * DIM;
+        double rl[3];
+        light_dir<DIM>(P, pr.x1, l, rl);
+        double2 el0 = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
+        double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, rl), P.amp), pr.w0p);
+        double2 t = cmul(cmul(el0, af), wl[l]);
+        s0.x += t.x;
+        s0.y += t.y;
+        cl0[p * P.nl + l] = cmul(cmul(el0, wl[l]), c1);
+    }
+    s0 = wave_sum(s0);
+    for (int64_t vi = o + lane; vi < e; vi += 64) {
+        double2 t = make_double2(0.0, 0.0);
+        if (vi != o) {
+            t = cmul(cn_of(vi), s0);
+            if (P.do_cbs) {
+                t.x *= rs2;
+                t.y *= rs2;
+            }
+        }
+        coef[vi] = t;
+    }
+    if (P.do_cbs) {
+        double2 R = make_double2(0.0, 0.0);
+        for (int64_t vi = o + 1; vi < e; ++vi) {
+            const VertexRec r = verts[vi];
+            const double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
+            double2 sl = make_double2(0.0, 0.0);
+            for (int l = lane; l < P.nl; l += 64) {
+                const double *lc = P.l + (size_t)l * DIM;
+                double rl[3];
+                light_dir<DIM>(P, pr.x1, l, rl);
+                double2 el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l),
+                                  eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
+                double2 t = cmul(el, wl[l]);
+                sl.x += t.x;
+                sl.y += t.y;
+            }
+            sl = wave_sum(sl);
+            double2 q = cmul(cn_of(vi), sl);
+            R.x += q.x * rs2;
+            R.y += q.y * rs2;
+        }
+        if (lane == 0) coefR[p] = R;
+    }
+}
+
 struct FEntry {
     double x[3];
     double ad[3];     // amplitude direction: ow (vertex) or -w1 (reversed)
     double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
     int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
-    int pad;
+    int p;            // path of the entry
 };
 
 template <int DIM>
@@ -93,6 +184,7 @@ __device__ __forceinline__ void load_entry(int64_t e, int64_t nvert, const PathR
             f.ad[j] = r.ow[j];
         }
         f.kind = (e == offsets[p]) ? 0 : 1;
+        f.p = p;
         f.div = make_double2(1.0, 0.0);
     } else {
         const PathRec pr = paths[e - nvert];
@@ -101,6 +193,7 @@ __device__ __forceinline__ void load_entry(int64_t e, int64_t nvert, const PathR
             f.ad[j] = -pr.w1[j];
         }
         f.kind = 2;
+        f.p = (int)(e - nvert);
         double den = pr.w0p.x * pr.w0p.x + pr.w0p.y * pr.w0p.y;
         f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
     }
@@ -130,14 +223,17 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
     return make_float2((float)a.x, (float)a.y);
 }
 
-template <int DIM, bool MIXED>
+// WAVE: one output column; coef/coefR hold the light-summed coefficients and
+// the first-vertex entry contracts af_ang_vl(v, :) with cl0[p, :].
+template <int DIM, bool MIXED, bool WAVE>
 __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
                                                             const int64_t *offsets, const double2 *coef,
                                                             const double2 *coefR, const double2 *aftab,
-                                                            double2 *part)
+                                                            const double2 *cl0, double2 *part)
 {
+    const int ncol = WAVE ? 1 : P.nl;   // columns of coef / coefR / part
     __shared__ FEntry se[kFieldBatch];
     __shared__ double2 sc[kFieldBatch][kFieldNL];
     const int v = blockIdx.x * kFieldViews + threadIdx.x;
@@ -160,7 +256,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
     }
     double2 afv[kFieldNL];
     for (int q = 0; q < kFieldNL; ++q)
-        afv[q] = (vok && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)] : make_double2(0.0, 0.0);
+        afv[q] = (!WAVE && vok && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)] : make_double2(0.0, 0.0);
     const float g = (float)P.amp.g;
     const float hg_k = (float)sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
     const float g2p1 = 1.f + g * g, g2 = 2.f * g;
@@ -178,13 +274,27 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
             const int64_t e = eb + ei;
             double2 c = make_double2(0.0, 0.0);
             if (q < nlp)
-                c = (e < nvert) ? coef[e * P.nl + l0 + q] : coefR[(e - nvert) * P.nl + l0 + q];
+                c = (e < nvert) ? coef[e * ncol + l0 + q] : coefR[(e - nvert) * ncol + l0 + q];
             sc[ei][q] = c;
         }
         __syncthreads();
         if (!vok) continue;
         for (int ei = 0; ei < cnt; ++ei) {
             const FEntry &f = se[ei];
+            if (WAVE && f.kind == 0) {
+                // sum_l af_ang_vl(v, l) cl0[p, l], then the first-vertex E(x1, v)
+                double2 a = make_double2(0.0, 0.0);
+                for (int l = 0; l < P.nl; ++l) {
+                    double2 t = cmul(aftab[(size_t)v + (size_t)P.nv * l], cl0[(size_t)f.p * P.nl + l]);
+                    a.x += t.x;
+                    a.y += t.y;
+                }
+                double2 ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+                double2 t = cmul(ev, a);
+                acc[0].x += t.x;
+                acc[0].y += t.y;
+                continue;
+            }
             double2 ev;
             float2 am = make_float2(1.f, 0.f);
             if (MIXED) {
@@ -227,7 +337,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
         }
     }
     if (vok)
-        for (int q = 0; q < nlp; ++q) part[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+        for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
 }
 
 // Near-field sources (either side): the entrance directions rl/rv depend on
@@ -241,16 +351,18 @@ struct NEntry {
     double ad[3];
     double2 div;
     int kind;
-    int pad;
+    int p;
 };
 
-template <int DIM>
+template <int DIM, bool WAVE>
 __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
                                                                  int nsplit, int l0, int nlp, const PathRec *paths,
                                                                  const VertexRec *verts, const int32_t *vpath,
                                                                  const int64_t *offsets, const double2 *coef,
-                                                                 const double2 *coefR, double2 *part)
+                                                                 const double2 *coefR, const double2 *cl0,
+                                                                 double2 *part)
 {
+    const int ncol = WAVE ? 1 : P.nl;
     __shared__ NEntry se[kFieldBatch];
     __shared__ double2 sc[kFieldBatch][kFieldNL];
     __shared__ double srl[kFieldBatch][kFieldNL][3];
@@ -276,6 +388,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
                     f.ad[j] = r.ow[j];
                 }
                 f.kind = (e == offsets[p]) ? 0 : 1;
+                f.p = p;
                 f.div = make_double2(1.0, 0.0);
             } else {
                 const PathRec pr = paths[e - nvert];
@@ -285,6 +398,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
                     f.ad[j] = -pr.w1[j];
                 }
                 f.kind = 2;
+                f.p = (int)(e - nvert);
                 double den = pr.w0p.x * pr.w0p.x + pr.w0p.y * pr.w0p.y;
                 f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
             }
@@ -294,7 +408,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
             const int64_t e = eb + ei;
             double2 c = make_double2(0.0, 0.0);
             if (q < nlp)
-                c = (e < nvert) ? coef[e * P.nl + l0 + q] : coefR[(e - nvert) * P.nl + l0 + q];
+                c = (e < nvert) ? coef[e * ncol + l0 + q] : coefR[(e - nvert) * ncol + l0 + q];
             sc[ei][q] = c;
         }
         __syncthreads();
@@ -302,7 +416,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
         for (int t = threadIdx.x; t < cnt * kFieldNL; t += blockDim.x) {
             const int ei = t / kFieldNL, q = t % kFieldNL;
             double rl[3] = {0.0, 0.0, 0.0};
-            if (q < nlp && se[ei].kind == 0) light_dir<DIM>(P, se[ei].x1, l0 + q, rl);
+            if (!WAVE && q < nlp && se[ei].kind == 0) light_dir<DIM>(P, se[ei].x1, l0 + q, rl);
             for (int j = 0; j < 3; ++j) srl[ei][q][j] = rl[j];
         }
         __syncthreads();
@@ -314,7 +428,20 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
             view_dir<DIM>(P, x1, v, rv);
             double xx[3] = {f.x[0], f.x[1], f.x[2]};
             double2 ev = phase_att64<DIM>(P, xx, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
-            if (f.kind == 0) {
+            if (WAVE && f.kind == 0) {
+                // sum_l a(rl_l . rv) cl0[p, l] (MCfieldOnWave.m:143-148, 214)
+                double2 a = make_double2(0.0, 0.0);
+                for (int l = 0; l < P.nl; ++l) {
+                    double rl[3];
+                    light_dir<DIM>(P, x1, l, rl);
+                    double2 t = cmul(eval_amp<DIM>(dot<DIM>(rl, rv), P.amp), cl0[(size_t)f.p * P.nl + l]);
+                    a.x += t.x;
+                    a.y += t.y;
+                }
+                double2 t = cmul(ev, a);
+                acc[0].x += t.x;
+                acc[0].y += t.y;
+            } else if (f.kind == 0) {
                 for (int q = 0; q < kFieldNL; ++q) {
                     double rl[3] = {srl[ei][q][0], srl[ei][q][1], srl[ei][q][2]};
                     double2 t = cmul(ev, eval_amp<DIM>(dot<DIM>(rl, rv), P.amp));
@@ -335,7 +462,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
         }
     }
     if (vok)
-        for (int q = 0; q < nlp; ++q) part[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+        for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
 }
 
 __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, double2 *acc)
@@ -354,15 +481,22 @@ __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, 
 template <int DIM>
 static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths,
                            const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work,
-                           double2 *acc, bool mixed, hipStream_t s)
+                           double2 *acc, bool mixed, const double2 *wl, hipStream_t s)
 {
-    // work layout: coef[nvert*nl] | coefR[np*nl] | aftab[nv*nl] | part[nsplit*nv*nl]
+    const bool wave = wl != nullptr;
+    const int ncol = wave ? 1 : P.nl;
+    // work layout: coef[nvert*ncol] | coefR[np*ncol] | aftab[nv*nl] | cl0[np*nl] (wave) | part[nsplit*nv*ncol]
     double2 *coef = work;
-    double2 *coefR = coef + (size_t)nvert * P.nl;
-    double2 *aftab = coefR + (size_t)np * P.nl;
-    double2 *part = aftab + (size_t)P.nv * P.nl;
-    hipLaunchKernelGGL(field_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts, offsets, coef,
-                       coefR);
+    double2 *coefR = coef + (size_t)nvert * ncol;
+    double2 *aftab = coefR + (size_t)np * ncol;
+    double2 *cl0 = aftab + (size_t)P.nv * P.nl;
+    double2 *part = cl0 + (wave ? (size_t)np * P.nl : 0);
+    if (wave)
+        hipLaunchKernelGGL(field_wave_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts,
+                           offsets, wl, coef, coefR, cl0);
+    else
+        hipLaunchKernelGGL(field_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts, offsets,
+                           coef, coefR);
     const bool near = !(P.is_ff_l && P.is_ff_v);
     if (!near) launch_af_table(P, aftab, s);
     const int64_t nent = nvert + (P.do_cbs ? np : 0);
@@ -372,31 +506,53 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     if (nsplit > maxsplit) nsplit = (int)maxsplit;
     if (nsplit > 64) nsplit = 64;
     if (nsplit < 1) nsplit = 1;
-    for (int l0 = 0; l0 < P.nl; l0 += kFieldNL) {
-        const int nlp = (P.nl - l0) < kFieldNL ? (P.nl - l0) : kFieldNL;
+    for (int l0 = 0; l0 < ncol; l0 += kFieldNL) {
+        const int nlp = (ncol - l0) < kFieldNL ? (ncol - l0) : kFieldNL;
         dim3 g((unsigned)vt, (unsigned)nsplit);
-        if (near)
-            hipLaunchKernelGGL((field_near_kernel<DIM>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
-                               nlp, paths, verts, vpath, offsets, coef, coefR, part);
-        else if (mixed)
-            hipLaunchKernelGGL((field_kernel<DIM, true>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
-                               nlp, paths, verts, vpath, offsets, coef, coefR, aftab, part);
-        else
-            hipLaunchKernelGGL((field_kernel<DIM, false>), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0,
-                               nlp, paths, verts, vpath, offsets, coef, coefR, aftab, part);
+#define SCMC_FIELD(KERNEL)                                                                                      \
+    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
+                       vpath, offsets, coef, coefR, aftab, cl0, part)
+#define SCMC_FIELD_NEAR(KERNEL)                                                                                 \
+    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
+                       vpath, offsets, coef, coefR, cl0, part)
+        if (near) {
+            if (wave)
+                SCMC_FIELD_NEAR((field_near_kernel<DIM, true>));
+            else
+                SCMC_FIELD_NEAR((field_near_kernel<DIM, false>));
+        } else if (mixed) {
+            if (wave)
+                SCMC_FIELD((field_kernel<DIM, true, true>));
+            else
+                SCMC_FIELD((field_kernel<DIM, true, false>));
+        } else {
+            if (wave)
+                SCMC_FIELD((field_kernel<DIM, false, true>));
+            else
+                SCMC_FIELD((field_kernel<DIM, false, false>));
+        }
+#undef SCMC_FIELD
+#undef SCMC_FIELD_NEAR
     }
-    const int64_t n = (int64_t)P.nv * P.nl;
+    const int64_t n = (int64_t)P.nv * ncol;
     hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nsplit, n, acc);
+}
+
+size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave)
+{
+    const int ncol = wave ? 1 : nl;
+    return (size_t)nvert * ncol + (size_t)np * ncol + (size_t)nv * nl + (wave ? (size_t)np * nl : 0) +
+           (size_t)64 * nv * ncol;
 }
 
 void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
                   const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc, bool mixed,
-                  hipStream_t s)
+                  const double2 *wl, hipStream_t s)
 {
     if (P.dim == 3)
-        field_dispatch<3>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, s);
+        field_dispatch<3>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
     else
-        field_dispatch<2>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, s);
+        field_dispatch<2>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
 }
 
 } // namespace scmc

@@ -17,21 +17,34 @@ int64_t field_chunk_paths(int nv, int nl, int64_t remaining, double mean_len)
     return std::min(p, remaining);
 }
 
-int FieldEngine::init(const ProblemState &ps, hipStream_t s)
+int FieldEngine::init(const ProblemState &ps, hipStream_t s, const double *wl_re, const double *wl_im)
 {
     int rc = SCMC_OK;
     nv = ps.dp.nv;
     nl = ps.dp.nl;
-    if (!acc.ensure(sizeof(double2) * (size_t)nv * nl, &rc)) return rc;
-    launch_zero(acc.p, sizeof(double2) * (size_t)nv * nl, s);
+    wave = wl_re != nullptr;
+    ncol = wave ? 1 : nl;
+    if (!acc.ensure(sizeof(double2) * (size_t)nv * ncol, &rc)) return rc;
+    launch_zero(acc.p, sizeof(double2) * (size_t)nv * ncol, s);
+    if (wave) {
+        // the light weights Wl of MCfieldOnWave.m (nl complex)
+        std::vector<double> h((size_t)nl * 2);
+        for (int l = 0; l < nl; ++l) {
+            h[2 * (size_t)l] = wl_re[l];
+            h[2 * (size_t)l + 1] = wl_im ? wl_im[l] : 0.0;
+        }
+        if (!wl.ensure(sizeof(double2) * (size_t)nl, &rc)) return rc;
+        if (!hip_ok(hipMemcpyAsync(wl.p, h.data(), sizeof(double2) * (size_t)nl, hipMemcpyHostToDevice, s), "wl", &rc) ||
+            !hip_ok(hipStreamSynchronize(s), "wl sync", &rc))
+            return rc;
+    }
     return SCMC_OK;
 }
 
 int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum)
 {
     int rc = SCMC_OK;
-    // coef[nvert*nl] | coefR[np*nl] | aftab[nv*nl] | part[64*nv*nl]
-    size_t need = (size_t)tc.nvert * nl + (size_t)tc.np * nl + (size_t)nv * nl + (size_t)64 * nv * nl;
+    const size_t need = field_work_elems(tc.np, tc.nvert, nv, nl, wave);
     if (!work.ensure(sizeof(double2) * need, &rc)) return rc;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
@@ -39,7 +52,7 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
     (void)hipEventRecord(e0, s);
     launch_field(ps.dp, tc.np, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
                  (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, (double2 *)work.p, (double2 *)acc.p, mixed,
-                 s);
+                 wave ? (const double2 *)wl.p : nullptr, s);
     (void)hipEventRecord(e1, s);
     if (!hip_ok(hipGetLastError(), "field launch", &rc)) return rc;
     if (!hip_ok(hipEventSynchronize(e1), "field sync", &rc)) return rc;
@@ -54,12 +67,12 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
 int FieldEngine::finalize(const scmc_problem *pr, int64_t max_itr, double *u_re, double *u_im, hipStream_t s)
 {
     int rc = SCMC_OK;
-    const size_t ne = (size_t)nv * nl;
+    const size_t ne = (size_t)nv * ncol;
     std::vector<double> h(ne * 2);
     if (!hip_ok(hipMemcpyAsync(h.data(), acc.p, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy field", &rc) ||
         !hip_ok(hipStreamSynchronize(s), "field sync", &rc))
         return rc;
-    // MCfield.m:253-254
+    // MCfield.m:253-254 (MCfieldOnWave.m:265-267)
     const double sc = sqrt(1.0 / (double)max_itr * box_volume(pr) * pr->sigt);
     for (size_t i = 0; i < ne; ++i) {
         if (u_im) {

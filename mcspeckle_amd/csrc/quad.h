@@ -8,8 +8,8 @@ constexpr int kQuadMaxSlabs = 4096;
 
 // Grid geometry shared by the host set-up and the scatter kernel.
 struct QuadGeom {
-    int nv0;          // views per axis (odd)
-    int nx;           // lateral cells per axis (odd)
+    int nv0;          // views per axis
+    int nx;           // lateral cells per axis (odd; OnWave: the parity of nv0)
     int nfft;         // padded transform size, max(nx, nv0)
     int nx_pad;       // zero padding of the cell grid on each side
     int nva_pad;      // crop of the transform on each side
@@ -17,11 +17,17 @@ struct QuadGeom {
     double box_stp, z_bin, gx0, gy0;
     double l_diratt, l_idiratt;   // 1/|l_z| by sign of l(end) (MCfieldFourierQuad.m:127-128)
     const double *gz;             // nz slab boundaries (device)
+    // illumination wave (MCfieldFourierQuadOnWave.m): lights combined per
+    // vertex with the weights wl, one output plane, vsign conventions
+    int wave = 0;
+    double vsign = -1.0;
+    const double2 *wl = nullptr;  // nl weights (device)
 };
 
-// Host restatement of MCfieldFourierQuad.m:80-121; fills the enlarged box.
+// Host restatement of MCfieldFourierQuad.m:80-121 (with parity_fix, of
+// MCfieldFourierQuadOnWave.m:84-131); fills the enlarged box.
 int quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp, double lambda,
-                    QuadGeom &g, double *bmin_out, double *bmax_out, std::vector<double> *gz_out);
+                    bool parity_fix, QuadGeom &g, double *bmin_out, double *bmax_out, std::vector<double> *gz_out);
 
 void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
                          const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,

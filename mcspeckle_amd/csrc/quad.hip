@@ -1,4 +1,5 @@
-// quad.hip -- Fourier-accelerated field (code/MCfieldFourierQuad.m).
+// quad.hip -- Fourier-accelerated field (code/MCfieldFourierQuad.m and its
+// illumination-wave form code/MCfieldFourierQuadOnWave.m).
 //
 // Per stored vertex (pxL = sqrt(W/px) e^{2 pi i xi}, /sqrt 2 for pL > 1 with
 // CBS, MCfieldFourierQuad.m:181-182) the reference adds, for every light,
@@ -13,8 +14,8 @@
 // per-(slab, light) grid resident in HBM (FP64 or FP32); the 2-D DFT of every
 // slab is two batched complex GEMMs H = W X W^T with W[o][ix] =
 // exp(+2 pi i (f_o q_ix mod n)/n) (the shift/flip/crop folded into the
-// indices, twiddles reduced exactly in integers); a combine kernel applies
-// the slab phases and sums over slabs.
+// indices, twiddles reduced exactly in integers, odd or even n); a combine
+// kernel applies the slab phases and sums over slabs.
 #include "kernels.h"
 #include "quad.h"
 
@@ -72,9 +73,28 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
         ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
         iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
         const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
+        const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
+        if (g.wave) {
+            // MCfieldFourierQuadOnWave.m:281-302: pxL exp(i vsign k (tz - z_s)) twl atl,
+            // twl = sum_l Wl(l) exp(i vsign k l.x_L)
+            double twr = 0, twi = 0;
+            for (int l = 0; l < P.nl; ++l) {
+                double s2, c2;
+                sincos(g.vsign * P.k * dot<3>(P.l + 3 * l, xl), &s2, &c2);
+                const double2 w = g.wl[l];
+                twr += w.x * c2 - w.y * s2;
+                twi += w.x * s2 + w.y * c2;
+            }
+            double s1, c1;
+            sincos(g.vsign * P.k * (-xs[2] + tz), &s1, &c1);
+            const double m = exp(att);
+            double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
+            double tr = (ar * twr - ai * twi) * m, tim = (ar * twi + ai * twr) * m;
+            atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], tr, tim);
+            continue;
+        }
         double s1, c1;
         sincos(-P.k * (-xs[2] + tz), &s1, &c1);
-        const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
         for (int l = 0; l < P.nl; ++l) {
             const double *lc = P.l + 3 * l;
             double ph = P.k * dot<3>(lc, xl);

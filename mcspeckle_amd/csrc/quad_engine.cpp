@@ -1,4 +1,5 @@
-// quad_engine.cpp -- host side of MCfieldFourierQuad (code/MCfieldFourierQuad.m:1-303).
+// quad_engine.cpp -- host side of MCfieldFourierQuad (code/MCfieldFourierQuad.m:1-303)
+// and MCfieldFourierQuadOnWave (code/MCfieldFourierQuadOnWave.m:1-337).
 #include "quad.h"
 #include "scmc_internal.h"
 
@@ -19,7 +20,7 @@ static int colon_len(double a, double s, double b)
 }
 
 int quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp, double lambda,
-                    QuadGeom &g, double *bmin, double *bmax, std::vector<double> *gz)
+                    bool parity_fix, QuadGeom &g, double *bmin, double *bmax, std::vector<double> *gz)
 {
     if (!(v_stp > 0) || !(v_max > 0) || !(lambda > 0))
         return set_error(SCMC_E_ARG, "v_max, v_stp and lambda must be positive");
@@ -28,6 +29,12 @@ int quad_grid_setup(const double *box_min, const double *box_max, double v_max, 
     g.box_stp = lambda / (v_max * 2 + v_stp);
     const double bw0 = box_max[0] - box_min[0];
     g.nx = (int)ceil(bw0 / g.box_stp / 2) * 2 + 1;
+    // MCfieldFourierQuadOnWave.m:100-102: the cell grid takes the parity of
+    // the view grid (the transform size may then be even)
+    if (parity_fix && (g.nv0 - g.nx) % 2 != 0) g.nx += 1;
+    // the padding / crop of MCfieldFourierQuad.m:103-106 must be whole cells
+    if ((g.nv0 - g.nx) % 2 != 0)
+        return set_error(SCMC_E_ARG, "view grid (%d) and cell grid (%d) differ by an odd number of cells", g.nv0, g.nx);
     const double bwn = g.nx * g.box_stp;
     for (int j = 0; j < 3; ++j) {
         bmin[j] = box_min[j];
@@ -71,12 +78,12 @@ using namespace scmc;
 
 extern "C" {
 
-int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp, double lambda,
-                   int32_t *nv0, int32_t *nx, int32_t *nz)
+static int quad_grid_query(const double box_min[3], const double box_max[3], double v_max, double v_stp,
+                           double lambda, bool wave, int32_t *nv0, int32_t *nx, int32_t *nz)
 {
     QuadGeom g;
     double bmin[3], bmax[3];
-    int rc = quad_grid_setup(box_min, box_max, v_max, v_stp, lambda, g, bmin, bmax, nullptr);
+    int rc = quad_grid_setup(box_min, box_max, v_max, v_stp, lambda, wave, g, bmin, bmax, nullptr);
     if (rc) return rc;
     if (nv0) *nv0 = g.nv0;
     if (nx) *nx = g.nx;
@@ -84,10 +91,25 @@ int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_ma
     return SCMC_OK;
 }
 
-int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_stp, int64_t max_itr, double vsign,
-                              int32_t precision, double *u_re, double *u_im, scmc_stats *stats)
+int scmc_quad_grid(const double box_min[3], const double box_max[3], double v_max, double v_stp, double lambda,
+                   int32_t *nv0, int32_t *nx, int32_t *nz)
+{
+    return quad_grid_query(box_min, box_max, v_max, v_stp, lambda, false, nv0, nx, nz);
+}
+
+int scmc_quad_grid_wave(const double box_min[3], const double box_max[3], double v_max, double v_stp, double lambda,
+                        int32_t *nv0, int32_t *nx, int32_t *nz)
+{
+    return quad_grid_query(box_min, box_max, v_max, v_stp, lambda, true, nv0, nx, nz);
+}
+
+// MCfieldFourierQuad.m:1-303, or with wl_re MCfieldFourierQuadOnWave.m:1-337.
+static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, const double *wl_im, double v_max,
+                            double v_stp, int64_t max_itr, double vsign, int32_t precision, double *u_re, double *u_im,
+                            scmc_stats *stats)
 {
     auto t0 = std::chrono::steady_clock::now();
+    const bool wave = wl_re != nullptr;
     if (!pr_in) return set_error(SCMC_E_ARG, "null problem");
     if (pr_in->dim != 3) return set_error(SCMC_E_ARG, "MCfieldFourierQuad is 3-D only");
     if (precision != SCMC_FP64 && precision != SCMC_MIXED) return set_error(SCMC_E_ARG, "invalid precision %d", precision);
@@ -96,7 +118,7 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
     QuadGeom g;
     double bmin[3], bmax[3];
     std::vector<double> gz;
-    if ((rc = quad_grid_setup(pr_in->box_min, pr_in->box_max, v_max, v_stp, pr_in->lambda, g, bmin, bmax, &gz)))
+    if ((rc = quad_grid_setup(pr_in->box_min, pr_in->box_max, v_max, v_stp, pr_in->lambda, wave, g, bmin, bmax, &gz)))
         return rc;
     if ((rc = ensure_device(nullptr))) return rc;
     const bool fp32 = precision == SCMC_MIXED;
@@ -128,8 +150,9 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
     {
         ProblemState ps;
         TraceChunk tc;
-        DevBuf dgz, grid, W, T1, H, zexp, u;
-        const int nl = pr.nl, nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
+        DevBuf dgz, grid, W, T1, H, zexp, u, dwl;
+        const int nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
+        const int nl = wave ? 1 : pr.nl;   // output planes
         const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
         const size_t ngrid = (size_t)nsl * nl * nx * nx;
         do {
@@ -137,6 +160,19 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
             if (!dgz.ensure(sizeof(double) * gz.size(), &rc) || !grid.ensure(cs * ngrid, &rc)) break;
             if (!hip_ok(hipMemcpy(dgz.p, gz.data(), sizeof(double) * gz.size(), hipMemcpyHostToDevice), "gz", &rc)) break;
             g.gz = (const double *)dgz.p;
+            if (wave) {
+                std::vector<double> hw((size_t)pr.nl * 2);
+                for (int l = 0; l < pr.nl; ++l) {
+                    hw[2 * (size_t)l] = wl_re[l];
+                    hw[2 * (size_t)l + 1] = wl_im ? wl_im[l] : 0.0;
+                }
+                if (!dwl.ensure(sizeof(double) * hw.size(), &rc) ||
+                    !hip_ok(hipMemcpy(dwl.p, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice), "wl", &rc))
+                    break;
+                g.wave = 1;
+                g.vsign = vsign;
+                g.wl = (const double2 *)dwl.p;
+            }
             launch_zero(grid.p, (cs * ngrid + 15) / 16 * 16 > grid.bytes ? grid.bytes : (cs * ngrid + 15) / 16 * 16, s);
             int64_t nstore = 0;
             double mean_len = 20;
@@ -156,12 +192,14 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
                 p0 += np;
             }
             if (rc) break;
-            // DFT matrix W[o][ix] = exp(+2 pi i ((o + nva_pad - m)(ix + nx_pad - m) mod n)/n)
-            const int n = g.nfft, m = (n - 1) / 2;
+            // DFT matrix of fliplr(flipud(fftshift(fft2(ifftshift(.))))) and the crop:
+            // W[o][ix] = exp(+2 pi i ((o + nva_pad - a)(ix + nx_pad - h) mod n)/n),
+            // h = floor(n/2) (the shift of fftshift/ifftshift), a = n - 1 - h
+            const int n = g.nfft, h = n / 2, am = n - 1 - h;
             std::vector<double> w((size_t)nv0 * nx * 2);
             for (int ix = 0; ix < nx; ++ix)
                 for (int o = 0; o < nv0; ++o) {
-                    long long f = o + g.nva_pad - m, q = ix + g.nx_pad - m;
+                    long long f = o + g.nva_pad - am, q = ix + g.nx_pad - h;
                     long long r = ((f * q) % n + n) % n;
                     double ang = 2 * M_PI * (double)r / n;
                     w[2 * ((size_t)ix * nv0 + o)] = cos(ang);
@@ -188,7 +226,8 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
                     double dir = vz < 0 ? 1 / fabs(vz) : 0, idir = vz > 0 ? 1 / fabs(vz) : 0;
                     size_t o = (size_t)i + (size_t)nv0 * j;
                     ze[2 * o] = -pr.sigt / 2 * (dir - idir);
-                    ze[2 * o + 1] = -2 * M_PI / pr.lambda * vz * vsign;
+                    // MCfieldFourierQuad.m:134 (-) / MCfieldFourierQuadOnWave.m:143 (+)
+                    ze[2 * o + 1] = (wave ? 2 : -2) * M_PI / pr.lambda * vz * vsign;
                     be[o] = exp(-pr.sigt / 2 * (-bmin[2] * dir + bmax[2] * idir)) *
                             std::exp(std::complex<double>(0, 2 * M_PI / pr.lambda *
                                                                  (-bmin[2] * (vz < 0) * vz + bmax[2] * (vz > 0) * vz)));
@@ -232,6 +271,20 @@ int scmc_mcfield_fourier_quad(const scmc_problem *pr_in, double v_max, double v_
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
     return rc;
+}
+
+int scmc_mcfield_fourier_quad(const scmc_problem *pr, double v_max, double v_stp, int64_t max_itr, double vsign,
+                              int32_t precision, double *u_re, double *u_im, scmc_stats *stats)
+{
+    return fourier_quad_run(pr, nullptr, nullptr, v_max, v_stp, max_itr, vsign, precision, u_re, u_im, stats);
+}
+
+int scmc_mcfield_fourier_quad_wave(const scmc_problem *pr, const double *wl_re, const double *wl_im, double v_max,
+                                   double v_stp, int64_t max_itr, double vsign, int32_t precision, double *u_re,
+                                   double *u_im, scmc_stats *stats)
+{
+    if (!wl_re) return set_error(SCMC_E_ARG, "Wl must have one weight per light");
+    return fourier_quad_run(pr, wl_re, wl_im, v_max, v_stp, max_itr, vsign, precision, u_re, u_im, stats);
 }
 
 } // extern "C"

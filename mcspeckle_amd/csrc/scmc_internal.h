@@ -64,9 +64,10 @@ struct CovEngine {
 
 // Field accumulation state across chunks.
 struct FieldEngine {
-    int nv = 0, nl = 0;
-    DevBuf acc, work, out;
-    int init(const ProblemState &ps, hipStream_t s);
+    int nv = 0, nl = 0, ncol = 0;
+    bool wave = false;   // lights combined with weights (MCfieldOnWave)
+    DevBuf acc, work, out, wl;
+    int init(const ProblemState &ps, hipStream_t s, const double *wl_re = nullptr, const double *wl_im = nullptr);
     int chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum);
     int finalize(const scmc_problem *pr, int64_t max_itr, double *u_re, double *u_im, hipStream_t s);
 };
@@ -90,7 +91,9 @@ struct Session {
     FieldEngine fe;
     scmc_stats st{};
     double mean_len = 20;
-    int init(const scmc_problem *pr, int kind, int precision);
+    // kind 0 MCcov, 1 MCfield; wl_re (kind 1 only) makes it MCfieldOnWave
+    int init(const scmc_problem *pr, int kind, int precision, const double *wl_re = nullptr,
+             const double *wl_im = nullptr);
     int run(int64_t first, int64_t n);
     int result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);
     void fill_stats(scmc_stats *out) const;

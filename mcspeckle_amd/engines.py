@@ -8,6 +8,11 @@ conventions, running on the MI355X through libscmc.so.
     u = MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp,
                            maxItr, lambda, doCBS, smpFlg, lmean0, vsign)
                                                       (code/MCfieldFourierQuad.m:1)
+    u = MCfieldOnWave(...MCfield's 14..., ampfunc0, lmean0, Wl)
+                                                      (code/MCfieldOnWave.m:1)
+    u = MCfieldFourierQuadOnWave(sigt, albedo, box_min, box_max, l, v_max,
+                                 v_stp, maxItr, lambda, doCBS, smpFlg, lmean0,
+                                 Wl, vsign)           (code/MCfieldFourierQuadOnWave.m:1)
 
 Differences from MATLAB are keyword-only: ``seed`` and ``path_offset``
 (Philox key and first path id; MATLAB uses the global ``rng``), ``precision``
@@ -227,14 +232,47 @@ def MCfield(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lamb
     return mcfield_spec(spec, int(maxItr), precision, stats)
 
 
-def quad_grid(box_min, box_max, v_max, v_stp, lambda_):
-    """Grid sizes (nv0, nx, nz) of MCfieldFourierQuad.m:80-121."""
+def _weights(Wl, nl):
+    w = np.asarray(Wl, dtype=complex).reshape(-1)
+    if w.size != nl:
+        raise ScmcArgumentError(f"Wl must have one weight per light ({nl}), got {w.size}")
+    return np.ascontiguousarray(w.real), np.ascontiguousarray(w.imag)
+
+
+def MCfieldOnWave(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg, sct_type,
+                  ampfunc, ampfunc0=None, lmean0=None, Wl=None, *, seed=0, path_offset=0, precision="fp64",
+                  stats=None):
+    """code/MCfieldOnWave.m:1-267 on the GPU: the field (nv,) of the
+    illumination wave sum_l Wl[l] * light l.  ``ampfunc0`` / ``lmean0`` may be
+    None (not given), which the reference's positional form cannot express
+    when Wl follows them."""
+    spec = make_problem(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS, smpFlg, sct_type,
+                        ampfunc, ampfunc0, lmean0=lmean0, seed=seed, path_offset=path_offset)
+    if not (np.isscalar(maxItr) and maxItr >= 1):
+        raise ScmcArgumentError("Invalid photons number")
+    if Wl is None:
+        raise ScmcArgumentError("Wl (one weight per light) is required")
+    wr, wi = _weights(Wl, spec.nl)
+    u_re, u_im = np.empty(spec.nv), np.empty(spec.nv)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(N.lib().scmc_mcfield_wave(C.byref(p), N.dptr(wr), N.dptr(wi), int(maxItr), _prec(precision),
+                                      N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return u_re + 1j * u_im
+
+
+def quad_grid(box_min, box_max, v_max, v_stp, lambda_, wave=False):
+    """Grid sizes (nv0, nx, nz) of MCfieldFourierQuad.m:80-121 (wave:
+    MCfieldFourierQuadOnWave.m:84-131)."""
     L = N.lib()
     bmin = np.ascontiguousarray(np.asarray(box_min, dtype=np.float64).reshape(3))
     bmax = np.ascontiguousarray(np.asarray(box_max, dtype=np.float64).reshape(3))
     nv0, nx, nz = C.c_int32(), C.c_int32(), C.c_int32()
-    N.check(L.scmc_quad_grid(N.dptr(bmin), N.dptr(bmax), float(v_max), float(v_stp), float(lambda_),
-                             C.byref(nv0), C.byref(nx), C.byref(nz)))
+    f = L.scmc_quad_grid_wave if wave else L.scmc_quad_grid
+    N.check(f(N.dptr(bmin), N.dptr(bmax), float(v_max), float(v_stp), float(lambda_), C.byref(nv0), C.byref(nx),
+              C.byref(nz)))
     return nv0.value, nx.value, nz.value
 
 
@@ -259,6 +297,31 @@ def MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp, maxItr, 
     if stats is not None:
         stats.update(st.as_dict())
     return (u_re + 1j * u_im).reshape((nv0, nv0, nl), order="F")
+
+
+def MCfieldFourierQuadOnWave(sigt, albedo, box_min, box_max, l, v_max, v_stp, maxItr, lambda_, doCBS, smpFlg,
+                             lmean0=None, Wl=None, vsign=-1, *, sct_type=1, ampfunc=0, seed=0, path_offset=0,
+                             precision="mixed", stats=None):
+    """code/MCfieldFourierQuadOnWave.m:1-337 on the GPU. Returns u (nv0 x nv0),
+    the field of the illumination wave sum_l Wl[l] * light l."""
+    spec = make_problem(sigt, albedo, box_min, box_max, l, np.zeros((3, 0)), 1, 1, lambda_, doCBS, smpFlg,
+                        sct_type, ampfunc, None, lmean0=lmean0, seed=seed, path_offset=path_offset,
+                        need_views=False)
+    if spec.dim != 3:
+        raise ScmcArgumentError("MCfieldFourierQuadOnWave is 3-D only")
+    if Wl is None:
+        raise ScmcArgumentError("Wl (one weight per light) is required")
+    wr, wi = _weights(Wl, spec.nl)
+    nv0, _, _ = quad_grid(spec.box_min, spec.box_max, v_max, v_stp, lambda_, wave=True)
+    u_re, u_im = np.empty(nv0 * nv0), np.empty(nv0 * nv0)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(N.lib().scmc_mcfield_fourier_quad_wave(C.byref(p), N.dptr(wr), N.dptr(wi), float(v_max), float(v_stp),
+                                                   int(maxItr), float(vsign), _prec(precision), N.dptr(u_re),
+                                                   N.dptr(u_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return (u_re + 1j * u_im).reshape((nv0, nv0), order="F")
 
 
 # ------------------------------------------------------------ path sets

@@ -85,6 +85,12 @@ def lib():
                                                C.c_double, C.c_int64, C.c_double, C.c_int, C.c_int, _dp,
                                                C.c_double, C.c_uint64, _dp, _dp]
         L.orc_mcfield_fourier_quad.restype = C.c_int64
+        L.orc_mcfield_wave.argtypes = [P, _dp, _dp, C.c_int64, _dp, _dp]
+        L.orc_quad_grid_setup_wave.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(OGrid)]
+        L.orc_mcfield_fourier_quad_wave.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                                    C.c_double, C.c_double, C.c_int64, C.c_double, C.c_int,
+                                                    C.c_int, _dp, C.c_double, C.c_uint64, _dp, _dp]
+        L.orc_mcfield_fourier_quad_wave.restype = C.c_int64
         L.orc_tabulated_scatter.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_inv_cdf.argtypes = [_dp, C.c_int, C.c_int, _dp]
         _lib = L
@@ -223,12 +229,46 @@ def mcfield_replay(spec, ps, max_itr):
     return (re + 1j * im).reshape((spec.nv, spec.nl), order="F")
 
 
-def quad_grid(box_min, box_max, v_max, v_stp, lam):
+def mcfield_wave(spec, wl, max_itr):
+    """MCfieldOnWave: the field (nv,) of the wave sum_l wl[l] * light l."""
+    L = lib()
+    p, keep = problem(spec)
+    wl = np.asarray(wl, dtype=complex).reshape(-1)
+    wr, wi = np.ascontiguousarray(wl.real), np.ascontiguousarray(wl.imag)
+    re, im = np.empty(spec.nv), np.empty(spec.nv)
+    L.orc_mcfield_wave(C.byref(p), _p(wr), _p(wi), int(max_itr), _p(re), _p(im))
+    return re + 1j * im
+
+
+def quad_grid(box_min, box_max, v_max, v_stp, lam, wave=False):
     g = OGrid()
     bmin = np.ascontiguousarray(box_min, dtype=np.float64)
     bmax = np.ascontiguousarray(box_max, dtype=np.float64)
-    lib().orc_quad_grid_setup(_p(bmin), _p(bmax), v_max, v_stp, lam, C.byref(g))
+    f = lib().orc_quad_grid_setup_wave if wave else lib().orc_quad_grid_setup
+    f(_p(bmin), _p(bmax), v_max, v_stp, lam, C.byref(g))
     return g
+
+
+def mcfield_fourier_quad_wave(sigt, albedo, box_min, box_max, l, wl, v_max, v_stp, max_itr, lam, do_cbs, smp_flg,
+                              lmean0=None, vsign=-1.0, seed=0):
+    """MCfieldFourierQuadOnWave: (nv0, nv0) field of the wave sum_l wl[l] * light l."""
+    L = lib()
+    g = quad_grid(box_min, box_max, v_max, v_stp, lam, wave=True)
+    l = np.asarray(l, dtype=np.float64).reshape(3, -1)
+    nl = l.shape[1]
+    lc = np.ascontiguousarray(l.T)
+    wl = np.asarray(wl, dtype=complex).reshape(-1)
+    wr, wi = np.ascontiguousarray(wl.real), np.ascontiguousarray(wl.imag)
+    bmin = np.ascontiguousarray(box_min, dtype=np.float64)
+    bmax = np.ascontiguousarray(box_max, dtype=np.float64)
+    lm = None if lmean0 is None else np.ascontiguousarray(lmean0, dtype=np.float64)
+    n = g.nv0 * g.nv0
+    re, im = np.empty(n), np.empty(n)
+    nstore = L.orc_mcfield_fourier_quad_wave(sigt, albedo, _p(bmin), _p(bmax), _p(lc), nl, _p(wr), _p(wi), v_max,
+                                             v_stp, int(max_itr), lam, int(do_cbs), int(smp_flg),
+                                             _p(lm) if lm is not None else None, float(vsign), int(seed), _p(re),
+                                             _p(im))
+    return (re + 1j * im).reshape((g.nv0, g.nv0), order="F"), nstore
 
 
 def mcfield_fourier_quad(sigt, albedo, box_min, box_max, l, v_max, v_stp, max_itr, lam, do_cbs, smp_flg,

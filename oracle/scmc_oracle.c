@@ -808,6 +808,70 @@ void orc_mcfield(const orc_problem *pr, int64_t max_itr, double *u_re, double *u
     free(u);
 }
 
+/* code/MCfieldOnWave.m:113-267 for one path: the lights are combined with
+ * the weights Wl before the outer product, so u is nv x 1.  The single-
+ * scattering term is e_v0 .* (af_ang_vl * (e_l0 .* Wl)) (MCfieldOnWave.m:214,
+ * written here as the sum over lights for every scattering type; the
+ * reference's isotropic branch reduces along the wrong dimension unless
+ * Nl = 1), e_l0_ms and e_l_ms are sums over lights (:177, :199). */
+static void mcfield_wave_path(const orc_problem *pr, const cplx *wl, const path_head *h, const vtx *vs,
+                              int64_t nvtx, nee_ws *ws, cplx *u)
+{
+    const int nv = pr->nv, nl = pr->nl;
+    nee_path(pr, h->x1, h->w1, h->w0p, ws);
+    cplx s0 = 0;   /* sum(e_l0 .* af_l .* Wl) */
+    for (int j = 0; j < nl; ++j) s0 += ws->e_l0_ms[j] * wl[j];
+    for (int64_t n = 1; n <= nvtx; ++n) {
+        const vtx *vv = &vs[n - 1];
+        cplx sl = 0;
+        if (n > 1) {
+            nee_vertex(pr, vv->x, vv->ow, ws);
+            if (pr->do_cbs)
+                for (int j = 0; j < nl; ++j) sl += ws->e_l_ms[j] * wl[j];
+        }
+        double amp = sqrt(vv->weight / h->px);
+        cplx ph = cexp(CMPLX(0, 2 * M_PI * vv->xi));
+        for (int i = 0; i < nv; ++i) {
+            cplx t;
+            if (n == 1) {
+                cplx acc = 0;
+                for (int j = 0; j < nl; ++j) acc += ws->af_vl[i + (size_t)nv * j] * (ws->e_l0[j] * wl[j]);
+                t = ws->e_v0[i] * acc;
+            } else {
+                t = ws->e_v_ms[i] * s0;
+                if (pr->do_cbs) t = 1 / sqrt(2.0) * (t + ws->e_v0_ms[i] * sl);
+            }
+            u[i] += (amp * t) * ph;
+        }
+    }
+}
+
+void orc_mcfield_wave(const orc_problem *pr, const double *wl_re, const double *wl_im, int64_t max_itr,
+                      double *u_re, double *u_im)
+{
+    double unmean[3], meanl[3];
+    mean_dir(pr, unmean, meanl);
+    cplx *wl = malloc(sizeof(cplx) * (size_t)pr->nl);
+    for (int j = 0; j < pr->nl; ++j) wl[j] = CMPLX(wl_re[j], wl_im ? wl_im[j] : 0.0);
+    cplx *u = calloc((size_t)pr->nv, sizeof(cplx));
+    nee_ws ws;
+    nee_alloc(&ws, pr->nv, pr->nl, pr->dim);
+    vtx *buf = NULL;
+    int64_t cap = 0;
+    path_head h;
+    for (int64_t i = 0; i < max_itr; ++i) {
+        int64_t n = walk(pr, (uint64_t)i, unmean, meanl, &h, &buf, &cap);
+        mcfield_wave_path(pr, wl, &h, buf, n, &ws, u);
+    }
+    free(buf);
+    nee_free(&ws);
+    /* MCfieldOnWave.m:265-267 */
+    double s = sqrt(1.0 / (double)max_itr * box_volume(pr) * pr->sigt);
+    split_out(u, (size_t)pr->nv, s, u_re, u_im);
+    free(u);
+    free(wl);
+}
+
 void orc_mcfield_replay(const orc_problem *pr, const orc_pathset *ps, int64_t max_itr, double *u_re, double *u_im)
 {
     size_t ne = (size_t)pr->nv * pr->nl;
@@ -895,15 +959,17 @@ static int colon_len(double a, double s, double b)
     return (int)floor(q + q * 1e-12 + 1e-10) + 1;
 }
 
-int orc_quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp,
-                        double lambda, orc_quad_grid *g)
+static int quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp,
+                           double lambda, int parity_fix, orc_quad_grid *g)
 {
-    /* MCfieldFourierQuad.m:80-121 */
+    /* MCfieldFourierQuad.m:80-121 (MCfieldFourierQuadOnWave.m:84-131) */
     g->nv0 = colon_len(-v_max, v_stp, v_max);
     g->nv = g->nv0;
     g->box_stp = lambda / (v_max * 2 + v_stp);
     double bw0 = box_max[0] - box_min[0];
     g->nx = (int)ceil(bw0 / g->box_stp / 2) * 2 + 1;
+    /* MCfieldFourierQuadOnWave.m:100-102: same parity as the view grid */
+    if (parity_fix && (g->nv - g->nx) % 2 != 0) g->nx += 1;
     double bwn = g->nx * g->box_stp;
     for (int j = 0; j < 3; ++j) { g->box_min[j] = box_min[j]; g->box_max[j] = box_max[j]; }
     for (int j = 0; j < 2; ++j) {
@@ -931,6 +997,18 @@ int orc_quad_grid_setup(const double *box_min, const double *box_max, double v_m
     return g->nv0;
 }
 
+int orc_quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp,
+                        double lambda, orc_quad_grid *g)
+{
+    return quad_grid_setup(box_min, box_max, v_max, v_stp, lambda, 0, g);
+}
+
+int orc_quad_grid_setup_wave(const double *box_min, const double *box_max, double v_max, double v_stp,
+                             double lambda, orc_quad_grid *g)
+{
+    return quad_grid_setup(box_min, box_max, v_max, v_stp, lambda, 1, g);
+}
+
 typedef struct { double z1; int64_t idx; } zkey;
 static int zkey_cmp(const void *a, const void *b)
 {
@@ -950,13 +1028,18 @@ static void dft1(const cplx *in, size_t stride, int n, cplx *out, const cplx *tw
     }
 }
 
-int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_min_in, const double *box_max_in,
-                                 const double *l, int nl, double v_max, double v_stp, int64_t max_itr,
-                                 double lambda, int do_cbs, int smp_flg, const double *lmean0,
-                                 double vsign, uint64_t seed, double *u_re, double *u_im)
+/* MCfieldFourierQuad.m:1-303, or with wl != NULL MCfieldFourierQuadOnWave.m:
+ * 1-337 (lights combined per vertex with the weights Wl, one output plane,
+ * the grid-parity fix of :100-102 and the vsign conventions of :143, :290,
+ * :302). */
+static int64_t fourier_quad(double sigt, double albedo, const double *box_min_in, const double *box_max_in,
+                            const double *l, int nl, const cplx *wl, double v_max, double v_stp, int64_t max_itr,
+                            double lambda, int do_cbs, int smp_flg, const double *lmean0, double vsign,
+                            uint64_t seed, double *u_re, double *u_im)
 {
     orc_quad_grid G;
-    orc_quad_grid_setup(box_min_in, box_max_in, v_max, v_stp, lambda, &G);
+    quad_grid_setup(box_min_in, box_max_in, v_max, v_stp, lambda, wl != NULL, &G);
+    const int nout = wl ? 1 : nl;
     const int nv0 = G.nv0, nx = G.nx, nfft = G.nfft;
     /* view grids MCfieldFourierQuad.m:108-113 */
     double *vz0 = malloc(sizeof(double) * (size_t)nv0 * nv0);
@@ -1016,11 +1099,11 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
     cplx *tw = malloc(sizeof(cplx) * (size_t)nfft);
     for (int k = 0; k < nfft; ++k) tw[k] = cexp(CMPLX(0, -2 * M_PI * k / nfft));
     const size_t nf2 = (size_t)nfft * nfft, nvv = (size_t)nv0 * nv0;
-    cplx *u = calloc(nvv * (size_t)nl, sizeof(cplx));
+    cplx *u = calloc(nvv * (size_t)nout, sizeof(cplx));
     cplx *uxy = malloc(sizeof(cplx) * nf2), *tmp = malloc(sizeof(cplx) * nf2);
     cplx *colo = malloc(sizeof(cplx) * (size_t)nfft);
     zkey *keys = malloc(sizeof(zkey) * (size_t)(nstore ? nstore : 1));
-    const int m = (nfft - 1) / 2;
+    const int hs = nfft / 2;   /* floor(n/2): the shift of fftshift / ifftshift */
 
     for (int tt = 0; tt < 2; ++tt) {
         if (tt == 1) {
@@ -1038,7 +1121,7 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
             zmax_i = c;
             if (hi <= lo) continue;
             double tz = (gz[j1] + gz[j1 + 1]) / 2;
-            for (int j2 = 0; j2 < nl; ++j2) {
+            for (int j2 = 0; j2 < nout; ++j2) {
                 const double *lj = &l[3 * j2];
                 memset(uxy, 0, sizeof(cplx) * nf2);
                 for (int64_t q = lo; q < hi; ++q) {
@@ -1050,18 +1133,28 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
                     if (ix > nx - 1) ix = nx - 1;
                     if (iy < 0) iy = 0;
                     if (iy > nx - 1) iy = nx - 1;
-                    cplx e1 = cexp(CMPLX(0, -kk * (-x1[2] + tz)));
                     double att = -sigt / 2 * ((xa[2] - G.box_min[2]) * l_diratt + (G.box_max[2] - xa[2]) * l_idiratt);
-                    cplx e2 = cexp(CMPLX(att, kk * dot3(lj, xa, 3)));
+                    cplx val;
+                    if (wl) {
+                        /* MCfieldFourierQuadOnWave.m:281-302 */
+                        cplx twl = 0;
+                        for (int j = 0; j < nl; ++j) twl += wl[j] * cexp(CMPLX(0, vsign * kk * dot3(&l[3 * j], xa, 3)));
+                        cplx e1 = cexp(CMPLX(0, vsign * kk * (-x1[2] + tz)));
+                        val = pxL[id] * e1 * twl * exp(att);
+                    } else {
+                        cplx e1 = cexp(CMPLX(0, -kk * (-x1[2] + tz)));
+                        cplx e2 = cexp(CMPLX(att, kk * dot3(lj, xa, 3)));
+                        val = pxL[id] * e1 * e2;
+                    }
                     /* padded placement: padarray by nx_pad on each side */
                     size_t pi = (size_t)(ix + G.nx_pad), pj = (size_t)(iy + G.nx_pad);
-                    uxy[pi + (size_t)nfft * pj] += pxL[id] * e1 * e2;
+                    uxy[pi + (size_t)nfft * pj] += val;
                 }
                 /* tfu = fliplr(flipud(fftshift(fft2(ifftshift(u_xy))))) */
-                /* ifftshift: x'[p] = x[(p + m) mod n] */
+                /* ifftshift: x'[p] = x[(p + floor(n/2)) mod n] */
                 for (int j = 0; j < nfft; ++j)
                     for (int i = 0; i < nfft; ++i)
-                        tmp[i + (size_t)nfft * j] = uxy[(size_t)((i + m) % nfft) + (size_t)nfft * ((j + m) % nfft)];
+                        tmp[i + (size_t)nfft * j] = uxy[(size_t)((i + hs) % nfft) + (size_t)nfft * ((j + hs) % nfft)];
                 /* DFT along dim 1 (columns), then dim 2 */
                 for (int j = 0; j < nfft; ++j) {
                     dft1(&tmp[(size_t)nfft * j], 1, nfft, colo, tw);
@@ -1071,18 +1164,19 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
                     dft1(&tmp[i], (size_t)nfft, nfft, colo, tw);
                     for (int j = 0; j < nfft; ++j) tmp[i + (size_t)nfft * j] = colo[j];
                 }
-                /* fftshift: y[i] = F[(i + m + 1) mod n]; flips: H[a] = y[n-1-a] */
+                /* fftshift: y[i] = F[(i - floor(n/2)) mod n]; flips: H[a] = y[n-1-a] */
                 for (int b = G.nva_pad; b < nfft - G.nva_pad; ++b)
                     for (int a = G.nva_pad; a < nfft - G.nva_pad; ++a) {
                         int ya = nfft - 1 - a, yb = nfft - 1 - b;
-                        int fa = (ya + m + 1) % nfft, fb = (yb + m + 1) % nfft;
+                        int fa = (ya - hs + nfft) % nfft, fb = (yb - hs + nfft) % nfft;
                         cplx val = tmp[(size_t)fa + (size_t)nfft * fb];
                         int oa = a - G.nva_pad, ob = b - G.nva_pad;
                         size_t o = (size_t)oa + (size_t)nv0 * ob;
-                        /* zexp0 = -2 pi i/lambda vz0 vsign - sigt/2 (diratt0 - idiratt0) */
+                        /* zexp0 = -+2 pi i/lambda vz0 vsign - sigt/2 (diratt0 - idiratt0)
+                         * (MCfieldFourierQuad.m:134; + in MCfieldFourierQuadOnWave.m:143) */
                         double vz = vz0[o];
                         double dir = (vz < 0) ? 1 / fabs(vz) : 0, idir = (vz > 0) ? 1 / fabs(vz) : 0;
-                        cplx zexp = CMPLX(-sigt / 2 * (dir - idir), -2 * M_PI / lambda * vz * vsign);
+                        cplx zexp = CMPLX(-sigt / 2 * (dir - idir), (wl ? 2 : -2) * M_PI / lambda * vz * vsign);
                         u[o + nvv * (size_t)j2] += val * cexp(zexp * tz);
                     }
             }
@@ -1094,15 +1188,38 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
         double dir = (vz < 0) ? 1 / fabs(vz) : 0, idir = (vz > 0) ? 1 / fabs(vz) : 0;
         cplx be = exp(-sigt / 2 * (-G.box_min[2] * dir + G.box_max[2] * idir)) *
                   cexp(CMPLX(0, 2 * M_PI / lambda * (-G.box_min[2] * (vz < 0) * vz + G.box_max[2] * (vz > 0) * vz)));
-        for (int j2 = 0; j2 < nl; ++j2) u[o + nvv * (size_t)j2] *= be;
+        for (int j2 = 0; j2 < nout; ++j2) u[o + nvv * (size_t)j2] *= be;
     }
     /* MCfieldFourierQuad.m:302-303 : maxItr is the stored vertex count */
     /* box_w(1:2) = box_w_n for both lateral axes (MCfieldFourierQuad.m:98) */
     double bwn = G.nx * G.box_stp;
     double V = bwn * bwn * (G.box_max[2] - G.box_min[2]);
     double s = nstore ? sqrt(1.0 / (double)nstore * V * sigt) : 0;
-    split_out(u, nvv * (size_t)nl, s, u_re, u_im);
+    split_out(u, nvv * (size_t)nout, s, u_re, u_im);
     free(vz0); free(gz); free(xL); free(x1L); free(pxL); free(tw); free(u); free(uxy); free(tmp);
     free(colo); free(keys);
     return nstore;
+}
+
+int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_min, const double *box_max,
+                                 const double *l, int nl, double v_max, double v_stp, int64_t max_itr,
+                                 double lambda, int do_cbs, int smp_flg, const double *lmean0,
+                                 double vsign, uint64_t seed, double *u_re, double *u_im)
+{
+    return fourier_quad(sigt, albedo, box_min, box_max, l, nl, NULL, v_max, v_stp, max_itr, lambda, do_cbs,
+                        smp_flg, lmean0, vsign, seed, u_re, u_im);
+}
+
+int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *box_min, const double *box_max,
+                                      const double *l, int nl, const double *wl_re, const double *wl_im,
+                                      double v_max, double v_stp, int64_t max_itr, double lambda, int do_cbs,
+                                      int smp_flg, const double *lmean0, double vsign, uint64_t seed,
+                                      double *u_re, double *u_im)
+{
+    cplx *wl = malloc(sizeof(cplx) * (size_t)nl);
+    for (int j = 0; j < nl; ++j) wl[j] = CMPLX(wl_re[j], wl_im ? wl_im[j] : 0.0);
+    int64_t r = fourier_quad(sigt, albedo, box_min, box_max, l, nl, wl, v_max, v_stp, max_itr, lambda, do_cbs,
+                             smp_flg, lmean0, vsign, seed, u_re, u_im);
+    free(wl);
+    return r;
 }

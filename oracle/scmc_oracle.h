@@ -27,6 +27,11 @@
  *                        icdf index scaling.
  *   fourier_quad       : direct DFT definition vs numpy.fft on the same
  *                        grid; scatter-add vs brute force.
+ *   near-field sources : point-source closed form of evalphaseatt; sources
+ *                        at distance R reproduce the far-field results.
+ *   mcfield_wave       : equals mcfield contracted with Wl on the same paths.
+ *   fourier_quad_wave  : numpy.fft restatement at odd and even transform
+ *                        sizes; linear in Wl.
  */
 #ifndef SCMC_ORACLE_H
 #define SCMC_ORACLE_H
@@ -130,6 +135,11 @@ void orc_mcfield_sums(const orc_problem *pr, uint64_t p0, int64_t np, int nthrea
 void orc_mcfield_replay(const orc_problem *pr, const orc_pathset *ps, int64_t max_itr,
                         double *u_re, double *u_im);
 
+/* code/MCfieldOnWave.m:1-267 : the field of the illumination wave
+ * sum_l Wl(l) * light l.  Out: u nv x 1 complex. */
+void orc_mcfield_wave(const orc_problem *pr, const double *wl_re, const double *wl_im, int64_t max_itr,
+                      double *u_re, double *u_im);
+
 /* code/MCfieldFourierQuad.m grid set-up (lines 80-138).  Returns the view
  * count nv0 and the lateral grid size nx; fills the enlarged box. */
 typedef struct {
@@ -140,6 +150,10 @@ typedef struct {
 } orc_quad_grid;
 int orc_quad_grid_setup(const double *box_min, const double *box_max, double v_max, double v_stp,
                         double lambda, orc_quad_grid *g);
+/* The same with MCfieldFourierQuadOnWave.m:100-102 (nx takes the parity of
+ * the view grid, so the transform size may be even). */
+int orc_quad_grid_setup_wave(const double *box_min, const double *box_max, double v_max, double v_stp,
+                             double lambda, orc_quad_grid *g);
 /* code/MCfieldFourierQuad.m:1-303 (isotropic only, 3-D).  l is 3 x nl,
  * lmean0 may be NULL (defaults to [0;0;1]).  Out: u nv0 x nv0 x nl complex,
  * column-major; returns the number of stored vertices (the normaliser). */
@@ -147,6 +161,14 @@ int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_m
                                  const double *l, int nl, double v_max, double v_stp, int64_t max_itr,
                                  double lambda, int do_cbs, int smp_flg, const double *lmean0,
                                  double vsign, uint64_t seed, double *u_re, double *u_im);
+
+/* code/MCfieldFourierQuadOnWave.m:1-337 : the lights are combined per vertex
+ * with the weights Wl (length nl, wl_im may be NULL).  Out: u nv0 x nv0. */
+int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *box_min, const double *box_max,
+                                      const double *l, int nl, const double *wl_re, const double *wl_im,
+                                      double v_max, double v_stp, int64_t max_itr, double lambda, int do_cbs,
+                                      int smp_flg, const double *lmean0, double vsign, uint64_t seed,
+                                      double *u_re, double *u_im);
 
 /* interface/scatter/tabulatedAmplitudeScatter.m:53-99 + coreFunctions/invCDF.m
  * Returns D (2 or 3) or a negative error code. Outputs are length n. */

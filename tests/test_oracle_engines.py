@@ -9,6 +9,8 @@
 * MCfield second moment: over the random phases, E[u u^H] equals the
   covariance of the same paths (MCfield.m:207-222 vs MCcov.m:232-264).
 * Invariants: Hermitian, positive semi-definite.
+* MCfieldOnWave: equals MCfield contracted with the weights Wl on the same
+  paths (every term of MCfieldOnWave.m is linear in the lights).
 * Near-field sources: placed at distance R along the far-field directions,
   the near-field covariance and field tend to the far-field ones (factor
   R^(dim-1) per near side, e^{ikR} per near side in the field).
@@ -291,3 +293,20 @@ def test_near_field_covariance_hermitian_psd(oracle):
         np.testing.assert_allclose(C, C.conj().T, atol=1e-13 * np.abs(C).max())
         ev = np.linalg.eigvalsh((C + C.conj().T) / 2)
         assert ev.min() > -1e-10 * ev.max()
+
+
+@pytest.mark.parametrize("kind,dim,cbs,near", [("hg", 3, True, False), ("tab", 3, False, False),
+                                               ("iso", 3, True, False), ("hg", 2, True, False),
+                                               ("hg", 3, True, True)])
+def test_mcfield_wave_is_mcfield_contracted_with_the_weights(oracle, kind, dim, cbs, near):
+    if near:
+        _, spec = near_far_pair(kind, dim, cbs, 5.0, "both", nv=4, nl=3)
+    else:
+        spec = small_spec(kind, dim, cbs, nv=4, nl=3, seed=12, tab=tab3d() if kind == "tab" else None)
+    rng = np.random.default_rng(3)
+    wl = rng.uniform(0.2, 1, spec.nl) * np.exp(2j * np.pi * rng.uniform(size=spec.nl))
+    n = 200
+    u = oracle.mcfield(spec, n)
+    uw = oracle.mcfield_wave(spec, wl, n)
+    want = u @ wl
+    np.testing.assert_allclose(uw, want, rtol=0, atol=1e-12 * np.abs(want).max())

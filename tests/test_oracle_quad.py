@@ -1,17 +1,22 @@
-"""Pins of the oracle's MCfieldFourierQuad restatement (CPU only).
+"""Pins of the oracle's MCfieldFourierQuad and MCfieldFourierQuadOnWave
+restatements (CPU only).
 
 A second restatement written with numpy's FFT routines (fftshift, ifftshift,
 fft2 and flips exactly as MCfieldFourierQuad.m:276-280 spells them) and the
 oracle's own walks (traced in the enlarged box) must reproduce the oracle's
-direct-DFT result; the grid sizes must match the C-ABI's scmc_quad_grid."""
+direct-DFT result, at odd and (OnWave only) even transform sizes; the wave
+variant must be linear in its weights; the grid sizes must match the C-ABI's
+scmc_quad_grid."""
 import numpy as np
 import pytest
 
 from mcspeckle_amd import engines as E
 
 
-def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, cbs, smp, vsign, seed):
-    g = oracle.quad_grid(np.array(bmin, float), np.array(bmax, float), v_max, v_stp, lam)
+def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, cbs, smp, vsign, seed, wl=None):
+    """MCfieldFourierQuad.m, or with wl MCfieldFourierQuadOnWave.m, in numpy."""
+    wave = wl is not None
+    g = oracle.quad_grid(np.array(bmin, float), np.array(bmax, float), v_max, v_stp, lam, wave=wave)
     nv0, nx = g.nv0, g.nx
     Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
     spec = E.make_problem(sigt, albedo, Bmin, Bmax, l, np.zeros((3, 0)), 1, 1, lam, cbs, smp, 1, 0,
@@ -38,11 +43,13 @@ def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, 
     lz = l[2, -1]
     l_diratt = 1 / abs(lz) if lz > 0 else 0
     l_idiratt = 1 / abs(lz) if lz < 0 else 0
-    zexp0 = -2j * np.pi / lam * vz * vsign - sigt / 2 * (diratt0 - idiratt0)
+    # MCfieldFourierQuad.m:134 / MCfieldFourierQuadOnWave.m:143
+    zexp0 = (2j if wave else -2j) * np.pi / lam * vz * vsign - sigt / 2 * (diratt0 - idiratt0)
     boxexp0 = np.exp(-sigt / 2 * (-Bmin[2] * diratt0 + Bmax[2] * idiratt0))
     boxexp0 = boxexp0 * np.exp(2j * np.pi / lam * (-Bmin[2] * (vz < 0) * vz + Bmax[2] * (vz > 0) * vz))
     nl = l.shape[1]
-    u = np.zeros((nv0, nv0, nl), complex)
+    nout = 1 if wave else nl
+    u = np.zeros((nv0, nv0, nout), complex)
     pad, crop = g.nx_pad, g.nva_pad
     for tt in range(2):
         if tt == 1:
@@ -59,10 +66,14 @@ def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, 
             tz = (gz[j1] + gz[j1 + 1]) / 2
             ix = np.clip(np.floor((xs[0, sel] - g.gx0) / g.box_stp + 0.5).astype(int), 0, nx - 1)
             iy = np.clip(np.floor((xs[1, sel] - g.gy0) / g.box_stp + 0.5).astype(int), 0, nx - 1)
-            for j2 in range(nl):
-                term = pw[sel] * np.exp(-2j * np.pi / lam * (-xs[2, sel] + tz)) * np.exp(
-                    2j * np.pi / lam * (l[:, j2] @ xl[:, sel])
-                    - sigt / 2 * ((xl[2, sel] - Bmin[2]) * l_diratt + (Bmax[2] - xl[2, sel]) * l_idiratt))
+            atl = np.exp(-sigt / 2 * ((xl[2, sel] - Bmin[2]) * l_diratt + (Bmax[2] - xl[2, sel]) * l_idiratt))
+            for j2 in range(nout):
+                if wave:
+                    twl = np.asarray(wl).reshape(-1) @ np.exp(vsign * 2j * np.pi / lam * (l.T @ xl[:, sel]))
+                    term = pw[sel] * np.exp(vsign * 2j * np.pi / lam * (-xs[2, sel] + tz)) * twl * atl
+                else:
+                    term = pw[sel] * np.exp(-2j * np.pi / lam * (-xs[2, sel] + tz)) * np.exp(
+                        2j * np.pi / lam * (l[:, j2] @ xl[:, sel])) * atl
                 uxy = np.zeros((nx, nx), complex)
                 np.add.at(uxy, (ix, iy), term)
                 uxy = np.pad(uxy, pad)
@@ -72,7 +83,8 @@ def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, 
                 u[:, :, j2] += tfu * np.exp(zexp0 * tz)
     u *= boxexp0[:, :, None]
     V = (nx * g.box_stp) ** 2 * (Bmax[2] - Bmin[2])
-    return u * np.sqrt(V * sigt / K), K
+    u = u * np.sqrt(V * sigt / K)
+    return (u[:, :, 0] if wave else u), K
 
 
 @pytest.mark.parametrize("cbs,vmax,vsign", [(False, 0.3, -1.0), (True, 0.3, -1.0), (True, 0.25, 1.0)])
@@ -95,3 +107,38 @@ def test_quad_grid_sizes_match(oracle, bw, vmax, vstp, lam):
     nv0, nx, nz = E.quad_grid(bmin, bmax, vmax, vstp, lam)
     assert (nv0, nx, nz) == (g.nv0, g.nx, g.nz)
     assert nv0 % 2 == 1 and nx % 2 == 1
+
+
+L2 = np.array([[0.05, -0.1, 0.02], [0.0, 0.08, -0.04],
+               [np.sqrt(1 - 0.05 ** 2), np.sqrt(1 - 0.1 ** 2 - 0.08 ** 2), np.sqrt(1 - 0.02 ** 2 - 0.04 ** 2)]])
+WL = np.array([0.7 * np.exp(0.3j), 0.4 * np.exp(-1.1j), 0.9 * np.exp(2.0j)])
+
+
+@pytest.mark.parametrize("cbs,vmax,vsign", [(False, 0.3, -1.0), (True, 0.325, -1.0), (True, 0.3, 1.0),
+                                            (False, 0.325, 1.0)])
+def test_oracle_quad_wave_matches_numpy_fft_restatement(oracle, cbs, vmax, vsign):
+    """v_max 0.325 / v_stp 0.05 gives 14 views per axis: the OnWave grid then
+    has an even lateral size and an even transform."""
+    args = dict(sigt=3.0, albedo=0.95, bmin=[-2.0, -2.0, 0.0], bmax=[2.0, 2.0, 0.6], l=L2, v_max=vmax, v_stp=0.05,
+                n_paths=120, lam=0.5, cbs=cbs, smp=2, vsign=vsign, seed=6)
+    g = oracle.quad_grid(np.array(args["bmin"]), np.array(args["bmax"]), vmax, 0.05, 0.5, wave=True)
+    assert (g.nv0 - g.nx) % 2 == 0
+    u_np, K = quad_numpy(oracle, wl=WL, **args)
+    u_or, K2 = oracle.mcfield_fourier_quad_wave(args["sigt"], args["albedo"], args["bmin"], args["bmax"], L2, WL,
+                                                vmax, 0.05, 120, 0.5, cbs, 2, None, vsign, 6)
+    assert K == K2
+    np.testing.assert_allclose(u_or, u_np, rtol=0, atol=1e-10 * np.abs(u_np).max())
+
+
+def test_oracle_quad_wave_even_transform_size(oracle):
+    g = oracle.quad_grid(np.array([-2.0, -2.0, 0.0]), np.array([2.0, 2.0, 0.6]), 0.325, 0.05, 0.5, wave=True)
+    assert g.nv0 == 14 and g.nx % 2 == 0 and g.nfft % 2 == 0
+
+
+def test_oracle_quad_wave_is_linear_in_the_weights(oracle):
+    args = (3.0, 0.95, [-2.0, -2.0, 0.0], [2.0, 2.0, 0.6], L2)
+    rest = (0.3, 0.05, 80, 0.5, True, 2, None, -1.0, 9)
+    parts = [oracle.mcfield_fourier_quad_wave(*args, np.eye(3)[j], *rest)[0] for j in range(3)]
+    whole = oracle.mcfield_fourier_quad_wave(*args, WL, *rest)[0]
+    want = sum(WL[j] * parts[j] for j in range(3))
+    np.testing.assert_allclose(whole, want, rtol=0, atol=1e-12 * np.abs(want).max())
