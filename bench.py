@@ -188,6 +188,58 @@ def quad_leg(n_paths, precision="mixed", seed=3):
             "ms_dft": st["ms_nee"], "precision": precision}
 
 
+def field_wave_leg(n_paths, precision="mixed"):
+    """MCfieldOnWave: the field's view grid (128 x 128 over +-10 degrees, slab
+    T=2, sigma_s = 10, sigma_a = 0.05, HG g = 0.8) under one illumination wave
+    of 64 plane waves (8 x 8 directions within +-5 degrees, unit-modulus
+    weights with seeded random phases), CBS on.  Entries are views (one
+    column); the same paths through MCfield would need 64 columns."""
+    from mcspeckle_amd import engines as E
+    th = np.deg2rad(np.linspace(-10, 10, 128))
+    X, Y = np.meshgrid(th, th)
+    v = np.vstack([np.sin(X.ravel()), np.sin(Y.ravel()), np.ones(X.size)])
+    v /= np.linalg.norm(v, axis=0)
+    tl = np.deg2rad(np.linspace(-5, 5, 8))
+    LX, LY = np.meshgrid(tl, tl)
+    l = np.vstack([np.sin(LX.ravel()), np.sin(LY.ravel()), np.ones(LX.size)])
+    l /= np.linalg.norm(l, axis=0)
+    wl = np.exp(2j * np.pi * np.random.default_rng(1).uniform(size=l.shape[1]))
+    sigt = 10.05
+    st = {}
+    t0 = time.perf_counter()
+    E.MCfieldOnWave(sigt, 10 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 2], l, v, 1, 1, n_paths, 1.0, True, 2, 3, 0.8,
+                    Wl=wl, seed=0, precision=precision, stats=st)
+    dt = time.perf_counter() - t0
+    return {"paths": n_paths, "views": v.shape[1], "lights": l.shape[1],
+            "value": n_paths * v.shape[1] / dt, "unit": "paths*entries/s",
+            "light_equivalent_value": n_paths * v.shape[1] * l.shape[1] / dt,
+            "seconds": dt, "ms_accum": st["ms_accum"], "vertices": st["vertices"], "precision": precision}
+
+
+def quad_wave_leg(n_paths, precision="mixed"):
+    """MCfieldFourierQuadOnWave: the Fourier leg's slab (box 10 x 10 x 1,
+    sigma_s = 8, sigma_a = 0.02, isotropic as in the reference) under a wave of
+    81 plane waves (9 x 9 sines within +-0.02, the layout of
+    test3DFourierSmpQuadOnWave.m:25-48, unit weights with seeded random
+    phases), 511 x 511 view grid."""
+    from mcspeckle_amd import engines as E
+    tl = np.linspace(-0.02, 0.02, 9)
+    LX, LY = np.meshgrid(tl, tl, indexing="ij")
+    l = np.vstack([LX.ravel(), LY.ravel(), np.sqrt(1 - LX.ravel() ** 2 - LY.ravel() ** 2)])
+    wl = np.exp(2j * np.pi * np.random.default_rng(2).uniform(size=l.shape[1]))
+    v_max, v_stp = 0.25, 0.5 / 510
+    lam = 10 * (2 * v_max + v_stp) / 511
+    sigt = 8.02
+    st = {}
+    t0 = time.perf_counter()
+    u = E.MCfieldFourierQuadOnWave(sigt, 8 / sigt, [-5, -5, 0], [5, 5, 1], l, v_max, v_stp, n_paths, lam, False, 2,
+                                   None, wl, -1, seed=0, precision=precision, stats=st)
+    dt = time.perf_counter() - t0
+    return {"paths": n_paths, "grid": list(u.shape), "lights": l.shape[1],
+            "value": n_paths * u.shape[0] * u.shape[1] / dt, "unit": "paths*entries/s", "seconds": dt,
+            "vertices": st["vertices"], "ms_scatter": st["ms_accum"], "ms_dft": st["ms_nee"], "precision": precision}
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of `mib` MiB through the HIP runtime, the
     reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -309,6 +361,8 @@ def main():
     if not args.no_legs:
         out["mcfield"] = field_leg(args.field_paths)
         out["mcfield_fourier_quad"] = quad_leg(args.quad_paths)
+        out["mcfield_on_wave"] = field_wave_leg(args.field_paths)
+        out["mcfield_fourier_quad_on_wave"] = quad_wave_leg(args.quad_paths)
 
     # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
     if not args.no_serve:

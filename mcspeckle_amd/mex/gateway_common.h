@@ -86,6 +86,19 @@ inline void read_phase(const mxArray *a, int type, scmc_phase &ph, std::vector<d
 
 // Philox key drawn from MATLAB's global stream, so rng(seed) in scmc.m:329
 // fixes the result and successive calls differ as they do with rand.
+// Wl of the illumination-wave variants: nl complex weights, any shape with
+// numel == nl (real or complex).
+inline void read_weights(const mxArray *a, int nl, std::vector<double> &re, std::vector<double> &im, const char *id)
+{
+    if ((int)mxGetNumberOfElements(a) != nl) mexErrMsgIdAndTxt(id, "Wl must have one weight per light");
+    const double *pr = mxGetPr(a), *pi = mxGetPi(a);
+    re.assign(pr, pr + nl);
+    if (pi)
+        im.assign(pi, pi + nl);
+    else
+        im.assign((size_t)nl, 0.0);
+}
+
 inline uint64_t seed_from_matlab()
 {
     mxArray *out = nullptr;

@@ -1,4 +1,5 @@
-"""MEX gateways (MCcov/MCfield/MCfieldFourierQuad) under a fake libmx.
+"""MEX gateways (MCcov/MCfield/MCfieldFourierQuad and the illumination-wave
+variants MCfieldOnWave/MCfieldFourierQuadOnWave) under a fake libmx.
 
 CPU: the argument contract of MCcov.m:45-70 / MCfield.m:41-66 (narginchk,
 'Invalid box size', 'Invalid light source input', 'Invalid view source
@@ -88,4 +89,49 @@ def test_quad_gateway_runs(mex):
     u0 = E.MCfieldFourierQuad(4.0, 0.97, [-2, -2, 0], [2, 2, 0.6], l, 0.3, 0.05, 200, 0.5, 0, 2, None, -1.0,
                               seed=key, precision="fp64")
     # the scatter uses FP64 atomics, so only the summation order differs
+    np.testing.assert_allclose(u, u0, rtol=0, atol=1e-12 * np.abs(u0).max())
+
+
+WL = np.array([0.8 * np.exp(0.4j), 0.5 * np.exp(-2.0j)])
+
+
+def test_wave_gateway_argument_contract(mex):
+    a = base_args() + [np.zeros((0, 0)), np.zeros((0, 0))]
+    _, err = mex.call("MCfieldOnWave", 1, *a)
+    assert err[0] == "MATLAB:narginchk:notEnoughInputs"
+    _, err = mex.call("MCfieldOnWave", 1, *(a + [WL[:1]]))
+    assert err == ("scmc:MCfieldOnWave", "Wl must have one weight per light")
+    b = a + [WL]
+    b[4] = np.zeros((2, 2))
+    assert mex.call("MCfieldOnWave", 1, *b)[1][1] == "Invalid light source input"
+    q = [4.0, 0.97, np.array([[-2.0], [-2.0], [0.0]]), np.array([[2.0], [2.0], [0.6]]), np.eye(3)[:, :2], 0.3, 0.05,
+         100, 0.5, 0, 2, np.zeros((0, 0))]
+    _, err = mex.call("MCfieldFourierQuadOnWave", 1, *q)
+    assert err[0] == "MATLAB:narginchk:notEnoughInputs"
+    _, err = mex.call("MCfieldFourierQuadOnWave", 1, *(q + [WL, -1.0, 7]))
+    assert err[0] == "MATLAB:narginchk:tooManyInputs"
+
+
+@pytest.mark.gpu
+def test_mcfield_wave_gateway_equals_engine(mex):
+    a = base_args() + [np.zeros((0, 0)), np.zeros((0, 0)), WL.reshape(1, -1)]
+    key = mex.seed_for_next_call(21)
+    (u,), err = mex.call("MCfieldOnWave", 1, *a)
+    assert err is None, err
+    u0 = E.MCfieldOnWave(*a[:14], None, None, WL, seed=key, precision="fp64")
+    assert u.shape == (3, 1)
+    np.testing.assert_array_equal(u[:, 0], u0)
+
+
+@pytest.mark.gpu
+def test_quad_wave_gateway_equals_engine(mex):
+    l = np.array([[0.0, 0.05], [0.0, -0.03], [1.0, np.sqrt(1 - 0.05 ** 2 - 0.03 ** 2)]])
+    key = mex.seed_for_next_call(13)
+    (u,), err = mex.call("MCfieldFourierQuadOnWave", 1, 4.0, 0.97, np.array([[-2.0], [-2.0], [0.0]]),
+                         np.array([[2.0], [2.0], [0.6]]), l, 0.325, 0.05, 200, 0.5, 1, 2, np.zeros((0, 0)),
+                         WL.reshape(-1, 1), 1.0)
+    assert err is None, err
+    u0 = E.MCfieldFourierQuadOnWave(4.0, 0.97, [-2, -2, 0], [2, 2, 0.6], l, 0.325, 0.05, 200, 0.5, 1, 2, None, WL,
+                                    1.0, seed=key, precision="fp64")
+    assert u.shape == (14, 14)
     np.testing.assert_allclose(u, u0, rtol=0, atol=1e-12 * np.abs(u0).max())
