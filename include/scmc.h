@@ -85,6 +85,16 @@ typedef struct {
     uint64_t path_offset;    /* first Philox path id                           */
 } scmc_problem;
 
+/* Voxel medium of MCCovHetro.m:1 (sigt, albedo grids over box_bin cells). */
+typedef struct {
+    int32_t nb[3];           /* cells per axis: round((box_max-box_min)./box_bin)  */
+    double box_bin[3];       /* cell size                                          */
+    const double *sigt;      /* prod(nb) extinction values, column-major (x first) */
+    const double *albedo;    /* prod(nb) albedos, or NULL: pr->albedo everywhere   */
+    int32_t has_lrad;        /* optional lrad argument (MCCovHetro.m:45)           */
+    double lrad;
+} scmc_medium;
+
 /* Per-solve statistics returned by the engines (may be passed as NULL). */
 typedef struct {
     int64_t paths;
@@ -158,6 +168,18 @@ int scmc_mcfield_fourier_quad_wave(const scmc_problem *pr, const double *wl_re, 
 /* Grid sizes of MCfieldFourierQuadOnWave.m:84-131. */
 int scmc_quad_grid_wave(const double box_min[3], const double box_max[3], double v_max, double v_stp,
                         double lambda, int32_t *nv0, int32_t *nx, int32_t *nz);
+
+/* [Ms,Mm,mean1] = MCCovHetro(sigt,albedo,box_min,box_max,box_bin,l,v,is_ff_l,
+ * is_ff_v,maxItr,lambda,doCBS,smpFlg,sct_type,ampfunc[,ampfunc0[,lrad]]) --
+ * code/MCCovHetro.m:1-276 (heterogeneous media: Woodcock tracking, cell-by-
+ * cell attenuation).  pr supplies the box, sources, flags, phase functions
+ * and seed (pr->sigt is not used; pr->albedo when med->albedo is NULL;
+ * has_lmean0 is ignored).  Ms, Mm are nv x nv x nl x nl, normalised as
+ * MCCovHetro.m:248-265 (Mm/2 with CBS, /maxItr); mean1 is nv x nl
+ * (MCCovHetro.m:271-276 with cs = 1).  Any output pointer may be NULL. */
+int scmc_mccov_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max_itr, int32_t precision,
+                     double *ms_re, double *ms_im, double *mm_re, double *mm_im,
+                     double *mean_re, double *mean_im, scmc_stats *stats);
 
 /* ---- sessions: accumulate path ranges, read results at any time -------- */
 /* A session owns the device accumulators of one MCcov (kind 0) or MCfield

@@ -217,14 +217,17 @@ double box_volume(const scmc_problem *pr)
 }
 
 // ------------------------------------------------------ traced chunk
-int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s)
+int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, const HetMedium *M)
 {
     int rc = SCMC_OK;
     np = n;
     if (!counts.ensure(sizeof(int32_t) * (size_t)n, &rc) || !offsets.ensure(sizeof(int64_t) * (size_t)(n + 1), &rc) ||
         !paths.ensure(sizeof(PathRec) * (size_t)n, &rc))
         return rc;
-    launch_trace_count(P, p0, n, (int32_t *)counts.p, s);
+    if (M)
+        launch_hetero_trace(P, *M, p0, n, (int32_t *)counts.p, nullptr, nullptr, nullptr, nullptr, false, s);
+    else
+        launch_trace_count(P, p0, n, (int32_t *)counts.p, s);
     launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, s);
     int64_t total = 0;
     if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)offsets.p + n, sizeof total, hipMemcpyDeviceToHost, s), "copy total",
@@ -235,8 +238,12 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s)
     if (!verts.ensure(sizeof(VertexRec) * (size_t)std::max<int64_t>(total, 1), &rc) ||
         !vpath.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), &rc))
         return rc;
-    launch_trace_emit(P, p0, n, (const int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
-                      (int32_t *)vpath.p, s);
+    if (M)
+        launch_hetero_trace(P, *M, p0, n, nullptr, (int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
+                            (int32_t *)vpath.p, true, s);
+    else
+        launch_trace_emit(P, p0, n, (const int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
+                          (int32_t *)vpath.p, s);
     return hip_ok(hipGetLastError(), "trace launch", &rc) ? SCMC_OK : rc;
 }
 
@@ -291,7 +298,10 @@ int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
     n = nv * nl;
     tile = mixed ? kTileMixed : kTileF64;
     if (!bound.ensure(16, &rc)) return rc;
-    (void)s;
+    if (ps.het) {
+        if (!mean.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
+        launch_zero(mean.p, sizeof(double2) * (size_t)n, s);
+    }
     return SCMC_OK;
 }
 
@@ -366,8 +376,16 @@ int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, doub
     (void)hipEventCreate(&e1);
     (void)hipEventCreate(&e2);
     (void)hipEventRecord(e0, s);
-    launch_nee_cov(P, np, nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p, (const int32_t *)tc.vpath.p,
-                   (const int64_t *)tc.offsets.p, f, mixed, s);
+    if (ps.het) {
+        if (!EV0.ensure(sizeof(double2) * (size_t)np * nv, &rc) || !UL0.ensure(sizeof(double2) * (size_t)np * nl, &rc))
+            return rc;
+        f.EV0 = (double2 *)EV0.p;
+        f.UL0 = (double2 *)UL0.p;
+    }
+    launch_nee_cov(P, ps.het ? &ps.hm : nullptr, np, nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
+                   (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, f, mixed, s);
+    if (ps.het)
+        launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p, s);
     (void)hipEventRecord(e1, s);
     const int ksm = pick_ksplit(k2, n, tile), kss = pick_ksplit(np, n, tile);
     if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), s))) return rc;
@@ -467,7 +485,9 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
     const size_t ne = (size_t)n * n;
     DevBuf out, af;
     if (!out.ensure(sizeof(double) * 2 * ne, &rc) || !af.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
-    const double scale = 1.0 / (double)max_itr * box_volume(pr) * pr->sigt;
+    // MCcov.m:293-299: x V sigt / maxItr; MCCovHetro.m:258-265: / maxItr only
+    // (its first-vertex density already carries sum(sigt) * cell volume)
+    const double scale = ps.het ? 1.0 / (double)max_itr : 1.0 / (double)max_itr * box_volume(pr) * pr->sigt;
     // far field: the single-scatter amplitude table is a per-row factor; near
     // field it is already inside the rows
     const bool far = ps.dp.is_ff_l && ps.dp.is_ff_v;
@@ -497,6 +517,27 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
     // MCcov.m:293-299: Mm/2 with CBS, then /maxItr*V*sigt; Ms /maxItr*V*sigt
     if ((rc = emit(acc_s, ks_s, scale, far ? (const double2 *)af.p : nullptr, Dst{ms_re, ms_im}))) return rc;
     if ((rc = emit(acc_m, ks_m, pr->do_cbs ? scale / 2 : scale, nullptr, Dst{mm_re, mm_im}))) return rc;
+    return SCMC_OK;
+}
+
+int CovEngine::finalize_mean(int64_t max_itr, double *re, double *im, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    if (!re) return SCMC_OK;
+    std::vector<double> h((size_t)n * 2);
+    if (!hip_ok(hipMemcpyAsync(h.data(), mean.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s), "copy mean", &rc) ||
+        !hip_ok(hipStreamSynchronize(s), "mean sync", &rc))
+        return rc;
+    const double sc = 1.0 / (double)max_itr;   // MCCovHetro.m:276 with cs = 1
+    for (int i = 0; i < n; ++i) {
+        if (im) {
+            re[i] = h[2 * (size_t)i] * sc;
+            im[i] = h[2 * (size_t)i + 1] * sc;
+        } else {
+            re[2 * i] = h[2 * (size_t)i] * sc;
+            re[2 * i + 1] = h[2 * (size_t)i + 1] * sc;
+        }
+    }
     return SCMC_OK;
 }
 
@@ -565,7 +606,8 @@ void ProblemCopy::assign(const scmc_problem *src)
     }
 }
 
-int Session::init(const scmc_problem *pr, int kind_, int precision, const double *wl_re, const double *wl_im)
+int Session::init(const scmc_problem *pr, int kind_, int precision, const double *wl_re, const double *wl_im,
+                  const scmc_medium *med)
 {
     int rc = validate_problem(pr, 1, true);
     if (rc) return rc;
@@ -577,6 +619,7 @@ int Session::init(const scmc_problem *pr, int kind_, int precision, const double
     pc.assign(pr);
     if (!hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
     if ((rc = ps.setup(&pc.pr, stream))) return rc;
+    if (med && (rc = ps.setup_medium(&pc.pr, med, stream))) return rc;
     memset(&st, 0, sizeof st);
     return kind == 0 ? ce.init(ps, mixed, stream) : fe.init(ps, stream, wl_re, wl_im);
 }
@@ -595,7 +638,7 @@ int Session::run(int64_t first, int64_t n)
         const int64_t np = kind == 0 ? chunk_paths(ce.n, rem, mean_len, mixed)
                                      : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
         auto a = std::chrono::steady_clock::now();
-        if ((rc = tc.run(ps.dp, (uint64_t)p0, np, stream))) return rc;
+        if ((rc = tc.run(ps.dp, (uint64_t)p0, np, stream, ps.het ? &ps.hm : nullptr))) return rc;
         if (!hip_ok(hipStreamSynchronize(stream), "trace", &rc)) return rc;
         st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         if (kind == 0)

@@ -1,0 +1,162 @@
+// hetero.hip -- walks and the mean field of code/MCCovHetro.m (voxel media).
+//
+// The walks fill the same path and vertex records as the homogeneous tracer,
+// so the NEE factors (with the cell-by-cell attenuation of hetero.h) and the
+// matrix-core covariance update are shared.  A path whose first vertex falls
+// outside the box (MCCovHetro.m:121-123, the iteration is spent) is stored as
+// one vertex of weight 0 at the box centre: every factor it feeds is zero and
+// the path-major layout (one first vertex per path) is kept.
+#include "hetero.h"
+#include "kernels.h"
+
+namespace scmc {
+
+template <int DIM>
+__global__ __launch_bounds__(256) void hetero_count_kernel(DevProblem P, HetMedium M, uint64_t p0, int64_t np,
+                                                           int32_t *counts)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint64_t path = P.path_offset + p0 + (uint64_t)i;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
+    double2 w0p;
+    if (!hetero_path_start<DIM>(P, M, path, x, px, w, w0p)) {
+        counts[i] = 1;
+        return;
+    }
+    double weight = med_albedo<DIM>(P, M, x);
+    uint32_t n = 0;
+    for (;;) {
+        ++n;
+        const double2 a = uniform_pair(P.seed, path, n, 0);
+        const double2 e = uniform_pair(P.seed, path, n, 3);
+        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y)) break;
+    }
+    counts[i] = (int32_t)n;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void hetero_emit_kernel(DevProblem P, HetMedium M, uint64_t p0, int64_t np,
+                                                          const int64_t *offsets, PathRec *paths, VertexRec *verts,
+                                                          int32_t *vpath)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint64_t path = P.path_offset + p0 + (uint64_t)i;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
+    double2 w0p;
+    const bool ok = hetero_path_start<DIM>(P, M, path, x, px, w, w0p);
+    int64_t o = offsets[i];
+    PathRec pr;
+    if (!ok) {
+        for (int j = 0; j < 3; ++j) {
+            pr.x1[j] = j < DIM ? 0.5 * (P.bmin[j] + P.bmax[j]) : 0.0;
+            pr.w1[j] = P.meanl[j];
+        }
+        pr.px = 1;
+        pr.pad = 0;
+        pr.w0p = make_double2(P.w0p_iso, 0.0);
+        paths[i] = pr;
+        VertexRec r;
+        for (int j = 0; j < 3; ++j) {
+            r.x[j] = pr.x1[j];
+            r.ow[j] = 0;
+        }
+        r.weight = 0;
+        r.xi = 0;
+        verts[o] = r;
+        vpath[o] = (int32_t)i;
+        return;
+    }
+    for (int j = 0; j < 3; ++j) {
+        pr.x1[j] = x[j];
+        pr.w1[j] = w[j];
+    }
+    pr.px = px;
+    pr.pad = 0;
+    pr.w0p = w0p;
+    paths[i] = pr;
+    double weight = med_albedo<DIM>(P, M, x);
+    const int64_t end = offsets[i + 1];
+    uint32_t n = 0;
+    for (;;) {
+        ++n;
+        const double2 a = uniform_pair(P.seed, path, n, 0);
+        const double2 e = uniform_pair(P.seed, path, n, 3);
+        if (o < end) {
+            VertexRec r;
+            for (int j = 0; j < 3; ++j) {
+                r.x[j] = x[j];
+                r.ow[j] = ow[j];
+            }
+            r.weight = weight;
+            r.xi = a.x;
+            verts[o] = r;
+            vpath[o] = (int32_t)i;
+            ++o;
+        }
+        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y)) break;
+    }
+}
+
+// MCCovHetro.m:152, 210-211: mean1(v,l) += W_1/px af_ang_vl(v,l) e_v0(v) u_l0(l)
+// with u_l0 the unattenuated light term (evalphaseatt with sigt = 0).  EV0
+// holds W_1/px * e_v0 per path, UL0 u_l0 per path (both written by the NEE
+// path kernel).  One thread per (v, l), paths summed in order.
+template <int DIM>
+__global__ __launch_bounds__(256) void hetero_mean_kernel(DevProblem P, int64_t np, const PathRec *paths,
+                                                          const double2 *EV0, const double2 *UL0, double2 *mean)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)P.nv * P.nl) return;
+    const int v = (int)(idx % P.nv), l = (int)(idx / P.nv);
+    const bool far = P.is_ff_l && P.is_ff_v;
+    double2 af = far ? eval_amp<DIM>(dot<DIM>(P.l + (size_t)l * DIM, P.v + (size_t)v * DIM), P.amp)
+                     : make_double2(0.0, 0.0);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t p = 0; p < np; ++p) {
+        if (!far) {
+            double rl[3], rv[3];
+            light_dir<DIM>(P, paths[p].x1, l, rl);
+            view_dir<DIM>(P, paths[p].x1, v, rv);
+            af = eval_amp<DIM>(dot<DIM>(rl, rv), P.amp);
+        }
+        const double2 t = cmul(af, cmul(EV0[p * P.nv + v], UL0[p * P.nl + l]));
+        acc.x += t.x;
+        acc.y += t.y;
+    }
+    mean[idx].x += acc.x;
+    mean[idx].y += acc.y;
+}
+
+void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
+                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit, hipStream_t s)
+{
+    if (np <= 0) return;
+    dim3 g((unsigned)((np + 255) / 256));
+    if (!emit) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(hetero_count_kernel<3>, g, dim3(256), 0, s, P, M, p0, np, counts);
+        else
+            hipLaunchKernelGGL(hetero_count_kernel<2>, g, dim3(256), 0, s, P, M, p0, np, counts);
+        return;
+    }
+    if (P.dim == 3)
+        hipLaunchKernelGGL(hetero_emit_kernel<3>, g, dim3(256), 0, s, P, M, p0, np, offsets, paths, verts, vpath);
+    else
+        hipLaunchKernelGGL(hetero_emit_kernel<2>, g, dim3(256), 0, s, P, M, p0, np, offsets, paths, verts, vpath);
+}
+
+void launch_hetero_mean(const DevProblem &P, int64_t np, const PathRec *paths, const double2 *EV0,
+                        const double2 *UL0, double2 *mean, hipStream_t s)
+{
+    const int64_t n = (int64_t)P.nv * P.nl;
+    if (np <= 0 || n <= 0) return;
+    dim3 g((unsigned)((n + 255) / 256));
+    if (P.dim == 3)
+        hipLaunchKernelGGL(hetero_mean_kernel<3>, g, dim3(256), 0, s, P, np, paths, EV0, UL0, mean);
+    else
+        hipLaunchKernelGGL(hetero_mean_kernel<2>, g, dim3(256), 0, s, P, np, paths, EV0, UL0, mean);
+}
+
+} // namespace scmc

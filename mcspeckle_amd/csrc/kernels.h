@@ -42,11 +42,24 @@ struct CovFactors {
     void *SA, *SB;             // single-scatter factors per path
     void *Gp, *Fp2;            // path-level e_l0_ms, e_v0_ms
     void *U, *ones;            // near field: single-scatter rows [np][nv*nl], ones[np]
+    double2 *EV0 = nullptr;    // voxel medium: W/px e_v0 per path (mean field)
+    double2 *UL0 = nullptr;    // voxel medium: unattenuated u_l0 per path
     float *maxG, *maxF2;       // path-level maxima (mixed scaling)
     unsigned *bound;           // [0] multi, [1] single: float bits of max |u|
 };
-void launch_nee_cov(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
-                    const int32_t *vpath, const int64_t *offsets, const CovFactors &f, bool mixed, hipStream_t s);
+struct HetMedium;   // hetero.h
+// M is null for a homogeneous medium.
+void launch_nee_cov(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
+                    const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, const CovFactors &f,
+                    bool mixed, hipStream_t s);
+
+// ------------------------------------------------------ voxel media
+// Walks of MCCovHetro.m: count pass (emit = false, fills counts) or emit pass.
+void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
+                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit, hipStream_t s);
+// mean1 += sum_p af_ang_vl EV0[p] UL0[p]^T (MCCovHetro.m:210-211).
+void launch_hetero_mean(const DevProblem &P, int64_t np, const PathRec *paths, const double2 *EV0,
+                        const double2 *UL0, double2 *mean, hipStream_t s);
 
 // ------------------------------------------------------------ HERK
 struct HerkProblem {

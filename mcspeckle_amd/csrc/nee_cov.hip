@@ -14,6 +14,7 @@
 // Evaluation is FP64 in both precision modes (it is <1% of the work at the
 // headline size); the mixed mode stores FP32 factors and records the largest
 // |u| bound of the chunk for the FP16 power-of-two scaling.
+#include "hetero.h"
 #include "kernels.h"
 
 namespace scmc {
@@ -48,34 +49,57 @@ __device__ __forceinline__ void atomic_max_pos(unsigned *p, float v)
 // bound is the product of the factor maxima (the amplitude table is applied
 // when finalising); near field the single-scatter rows carry the amplitude
 // and nee_single_near_kernel records their bound.
-template <int DIM, typename T>
-__global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, int64_t np, const PathRec *paths, T *Gp,
+// E(x, d): evalphaseatt (homogeneous) or evalphaseattHetro (voxel medium).
+template <int DIM, bool HET>
+__device__ __forceinline__ double2 nee_e(const DevProblem &P, const HetMedium &M, const double *x, const double *dcol,
+                                         double sign, int is_ff)
+{
+    if (HET) return phase_att_hetro<DIM>(P, M, x, dcol, sign, is_ff);
+    return phase_att64<DIM>(P, x, dcol, sign, is_ff);
+}
+
+// The single-scattering weight is the first vertex's W (albedo, or the
+// albedo of its cell in a voxel medium, MCCovHetro.m:164).  With a medium,
+// EV0 / UL0 receive W/px e_v0 and the unattenuated u_l0 for the mean field.
+template <int DIM, typename T, bool HET>
+__global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedium M, int64_t np, const PathRec *paths,
+                                                           const VertexRec *verts, const int64_t *offsets, T *Gp,
                                                            T *Fp2, T *SA, T *SB, float *maxG, float *maxF2,
-                                                           unsigned *bound)
+                                                           unsigned *bound, double2 *EV0, double2 *UL0)
 {
     __shared__ float sh[4];
     const int64_t p = blockIdx.x;
     if (p >= np) return;
     const PathRec pr = paths[p];
-    const double ssw = sqrt(P.albedo / pr.px);
+    const double W1 = verts[offsets[p]].weight;
+    const double ssw = sqrt(W1 / pr.px);
     float mG = 0.f, mF = 0.f, mSA = 0.f, mSB = 0.f;
     for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
         const double *lc = P.l + (size_t)i * DIM;
         double rl[3];
         light_dir<DIM>(P, pr.x1, i, rl);
-        double2 e = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
+        double2 e = nee_e<DIM, HET>(P, M, pr.x1, lc, 1.0, P.is_ff_l);
         double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, rl), P.amp), pr.w0p);
         double2 g = cmul(e, af);
         Gp[p * P.nl + i] = to_c<T>(g);
         SB[p * P.nl + i] = to_c<T>(e);
+        if (HET && UL0) {
+            DevProblem P0 = P;
+            P0.sigt = 0.0;   // MCCovHetro.m:152: evalphaseatt(x, l, is_ff_l, 0, ...)
+            UL0[p * P.nl + i] = phase_att64<DIM>(P0, pr.x1, lc, 1.0, P.is_ff_l);
+        }
         mG = fmaxf(mG, cmag(g));
         mSB = fmaxf(mSB, cmag(e));
     }
     for (int i = threadIdx.x; i < P.nv; i += blockDim.x) {
         const double *vc = P.v + (size_t)i * DIM;
-        double2 e = phase_att64<DIM>(P, pr.x1, vc, P.ff_sign, P.is_ff_v);
+        double2 e = nee_e<DIM, HET>(P, M, pr.x1, vc, P.ff_sign, P.is_ff_v);
         double2 se = make_double2(ssw * e.x, ssw * e.y);
         SA[p * P.nv + i] = to_c<T>(se);
+        if (HET && EV0) {
+            const double wp = W1 / pr.px;
+            EV0[p * P.nv + i] = make_double2(wp * e.x, wp * e.y);
+        }
         mSA = fmaxf(mSA, cmag(se));
         if (P.do_cbs) {
             double rv[3];
@@ -134,8 +158,9 @@ __global__ __launch_bounds__(256) void nee_single_near_kernel(DevProblem P, int6
 
 // MCcov.m:205-222 : vertex-level factors for pL > 1, written at the
 // multiple-scattering index m = vertex - path - 1 (path-major order).
-template <int DIM, typename T>
-__global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64_t nvert, const PathRec *paths,
+template <int DIM, typename T, bool HET>
+__global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, HetMedium M, int64_t nvert,
+                                                             const PathRec *paths,
                                                              const VertexRec *verts, const int32_t *vpath,
                                                              const int64_t *offsets, const T *Gp, const T *Fp2,
                                                              const float *maxG, const float *maxF2, T *A1, T *B1,
@@ -155,7 +180,7 @@ __global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64
         const double *vc = P.v + (size_t)i * DIM;
         double rv[3];
         view_dir<DIM>(P, pr.x1, i, rv);
-        double2 e = phase_att64<DIM>(P, r.x, vc, P.ff_sign, P.is_ff_v);
+        double2 e = nee_e<DIM, HET>(P, M, r.x, vc, P.ff_sign, P.is_ff_v);
         double2 af = eval_amp<DIM>(dot<DIM>(r.ow, rv), P.amp);
         double2 f = cmul(e, af);
         f.x *= sw;
@@ -170,7 +195,7 @@ __global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64
             const double *lc = P.l + (size_t)i * DIM;
             double rl[3];
             light_dir<DIM>(P, pr.x1, i, rl);
-            double2 e = phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l);
+            double2 e = nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l);
             double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
             double2 af = eval_amp<DIM>(dot<DIM>(mow, rl), P.amp);
             double2 g = cmul(e, af);
@@ -185,37 +210,48 @@ __global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, int64
     if (threadIdx.x == 0) atomic_max_pos(&bound[0], 2.f * (mA * maxG[p] + maxF2[p] * mB));
 }
 
-template <int DIM, typename T>
-static void nee_cov_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths,
+template <int DIM, typename T, bool HET>
+static void nee_cov_dispatch(const DevProblem &P, const HetMedium &M, int64_t np, int64_t nvert, const PathRec *paths,
                              const VertexRec *verts, const int32_t *vpath, const int64_t *offsets,
                              const CovFactors &f, hipStream_t s)
 {
     if (np > 0)
-        hipLaunchKernelGGL((nee_cov_path_kernel<DIM, T>), dim3((unsigned)np), dim3(128), 0, s, P, np, paths,
-                           (T *)f.Gp, (T *)f.Fp2, (T *)f.SA, (T *)f.SB, f.maxG, f.maxF2, f.bound);
+        hipLaunchKernelGGL((nee_cov_path_kernel<DIM, T, HET>), dim3((unsigned)np), dim3(128), 0, s, P, M, np, paths,
+                           verts, offsets, (T *)f.Gp, (T *)f.Fp2, (T *)f.SA, (T *)f.SB, f.maxG, f.maxF2, f.bound,
+                           f.EV0, f.UL0);
     if (np > 0 && !(P.is_ff_l && P.is_ff_v))
         hipLaunchKernelGGL((nee_single_near_kernel<DIM, T>), dim3((unsigned)np), dim3(256), 0, s, P, np, paths,
                            (const T *)f.SA, (const T *)f.SB, (T *)f.U, (T *)f.ones, f.bound);
     if (nvert > 0)
-        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T>), dim3((unsigned)nvert), dim3(128), 0, s, P, nvert, paths,
-                           verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1,
-                           (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound);
+        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET>), dim3((unsigned)nvert), dim3(128), 0, s, P, M, nvert,
+                           paths, verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2,
+                           (T *)f.A1, (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound);
 }
 
-void launch_nee_cov(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
-                    const int32_t *vpath, const int64_t *offsets, const CovFactors &f, bool mixed, hipStream_t s)
+void launch_nee_cov(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
+                    const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, const CovFactors &f,
+                    bool mixed, hipStream_t s)
 {
+    HetMedium none{};
+#define SCMC_NEE(D, T)                                                                                  \
+    do {                                                                                                \
+        if (M)                                                                                          \
+            nee_cov_dispatch<D, T, true>(P, *M, np, nvert, paths, verts, vpath, offsets, f, s);         \
+        else                                                                                            \
+            nee_cov_dispatch<D, T, false>(P, none, np, nvert, paths, verts, vpath, offsets, f, s);      \
+    } while (0)
     if (P.dim == 3) {
         if (mixed)
-            nee_cov_dispatch<3, float2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+            SCMC_NEE(3, float2);
         else
-            nee_cov_dispatch<3, double2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+            SCMC_NEE(3, double2);
     } else {
         if (mixed)
-            nee_cov_dispatch<2, float2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+            SCMC_NEE(2, float2);
         else
-            nee_cov_dispatch<2, double2>(P, np, nvert, paths, verts, vpath, offsets, f, s);
+            SCMC_NEE(2, double2);
     }
+#undef SCMC_NEE
 }
 
 // af_ang_vl(v,l) = evalampfunc_general(l(:,j)'*v) (MCcov.m:91-96), far field.

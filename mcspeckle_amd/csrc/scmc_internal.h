@@ -1,5 +1,6 @@
 // scmc_internal.h -- host-side state shared by the C-ABI entry points.
 #pragma once
+#include "hetero.h"
 #include "kernels.h"
 #include "scmc.h"
 
@@ -33,7 +34,12 @@ struct DevBuf {
 struct ProblemState {
     DevProblem dp;
     DevBuf dl, dv, tab_amp, tab_icdf, tab_amp0, tab_icdf0;
+    // voxel medium (MCCovHetro)
+    bool het = false;
+    HetMedium hm{};
+    DevBuf dsig, dalb, dcell_icdf;
     int setup(const scmc_problem *pr, hipStream_t s);
+    int setup_medium(const scmc_problem *pr, const scmc_medium *m, hipStream_t s);
     int upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevBuf &icdf, DevPhase &out, hipStream_t s);
 };
 
@@ -41,7 +47,7 @@ struct ProblemState {
 struct TraceChunk {
     int64_t np = 0, nvert = 0;
     DevBuf counts, offsets, paths, verts, vpath;
-    int run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s);
+    int run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, const HetMedium *M = nullptr);
     int load(const scmc_pathset *ps, hipStream_t s);
 };
 
@@ -55,11 +61,13 @@ struct CovEngine {
     int64_t herk_launches = 0;
     DevBuf acc_m, acc_s, bound, zbuf, order;
     DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2, U, ones;
+    DevBuf EV0, UL0, mean;   // voxel medium: mean field (MCCovHetro's mean1)
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
     int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee);
     int finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
                  double *mm_re, double *mm_im, hipStream_t s);
+    int finalize_mean(int64_t max_itr, double *re, double *im, hipStream_t s);
 };
 
 // Field accumulation state across chunks.
@@ -93,7 +101,7 @@ struct Session {
     double mean_len = 20;
     // kind 0 MCcov, 1 MCfield; wl_re (kind 1 only) makes it MCfieldOnWave
     int init(const scmc_problem *pr, int kind, int precision, const double *wl_re = nullptr,
-             const double *wl_im = nullptr);
+             const double *wl_im = nullptr, const scmc_medium *med = nullptr);
     int run(int64_t first, int64_t n);
     int result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);
     void fill_stats(scmc_stats *out) const;

@@ -8,6 +8,10 @@ conventions, running on the MI355X through libscmc.so.
     u = MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp,
                            maxItr, lambda, doCBS, smpFlg, lmean0, vsign)
                                                       (code/MCfieldFourierQuad.m:1)
+    [Ms, Mm, mean1] = MCCovHetro(sigt, albedo, box_min, box_max, box_bin, l, v,
+                                 is_ff_l, is_ff_v, maxItr, lambda, doCBS, smpFlg,
+                                 sct_type, ampfunc[, ampfunc0[, lrad]])
+                                                      (code/MCCovHetro.m:1)
     u = MCfieldOnWave(...MCfield's 14..., ampfunc0, lmean0, Wl)
                                                       (code/MCfieldOnWave.m:1)
     u = MCfieldFourierQuadOnWave(sigt, albedo, box_min, box_max, l, v_max,
@@ -261,6 +265,64 @@ def MCfieldOnWave(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr
     if stats is not None:
         stats.update(st.as_dict())
     return u_re + 1j * u_im
+
+
+def c_medium(spec, sigt, box_bin, albedo=None, lrad=None):
+    """scmc_medium for a sigt grid over box_bin cells (x fastest, MATLAB
+    order); albedo is a scalar (spec.albedo) or a grid of the same shape."""
+    dim = spec.dim
+    sig = np.asarray(sigt, dtype=np.float64)
+    bb = np.asarray(box_bin, dtype=np.float64).reshape(-1)
+    if bb.size != dim:
+        raise ScmcArgumentError("box_bin must have one size per dimension")
+    nb = np.rint((spec.box_max - spec.box_min) / bb).astype(int)
+    flat = np.ascontiguousarray(sig.reshape(-1, order="F"))
+    if flat.size != int(nb.prod()):
+        raise ScmcArgumentError(f"sigt has {flat.size} cells, (box_max-box_min)./box_bin gives {tuple(nb)}")
+    m = N.Medium()
+    for j in range(3):
+        m.nb[j] = int(nb[j]) if j < dim else 1
+        m.box_bin[j] = float(bb[j]) if j < dim else 1.0
+    m.sigt = N.dptr(flat)
+    keep = [flat]
+    alb = None if albedo is None else np.asarray(albedo, dtype=np.float64)
+    if alb is not None and alb.size > 1:
+        af = np.ascontiguousarray(alb.reshape(-1, order="F"))
+        if af.size != flat.size:
+            raise ScmcArgumentError("albedo grid must match sigt")
+        m.albedo = N.dptr(af)
+        keep.append(af)
+    else:
+        m.albedo = C.cast(None, C.POINTER(C.c_double))
+    m.has_lrad = 0 if lrad is None else 1
+    m.lrad = 0.0 if lrad is None else float(lrad)
+    return m, keep
+
+
+def MCCovHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg,
+               sct_type, ampfunc, ampfunc0=None, lrad=None, *, seed=0, path_offset=0, precision="fp64", stats=None):
+    """code/MCCovHetro.m:1-276 on the GPU: (Ms, Mm, mean1) for a voxel medium
+    (sigt and, optionally, albedo grids over box_bin cells)."""
+    sig = np.asarray(sigt, dtype=np.float64)
+    alb = np.asarray(albedo, dtype=np.float64)
+    a0 = float(alb.reshape(-1)[0]) if alb.size == 1 else 1.0
+    spec = make_problem(max(float(sig.max()), 1e-300), a0, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS,
+                        smpFlg, sct_type, ampfunc, ampfunc0, seed=seed, path_offset=path_offset)
+    if not (np.isscalar(maxItr) and maxItr >= 1):
+        raise ScmcArgumentError("Invalid photons number")
+    m, keep = c_medium(spec, sig, box_bin, alb if alb.size > 1 else None, lrad)
+    nv, nl = spec.nv, spec.nl
+    ne, nf = nv * nv * nl * nl, nv * nl
+    bufs = [np.empty(ne), np.empty(ne), np.empty(ne), np.empty(ne), np.empty(nf), np.empty(nf)]
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(N.lib().scmc_mccov_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision),
+                                     *[N.dptr(b) for b in bufs], C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    sh = (nv, nv, nl, nl)
+    return ((bufs[0] + 1j * bufs[1]).reshape(sh, order="F"), (bufs[2] + 1j * bufs[3]).reshape(sh, order="F"),
+            (bufs[4] + 1j * bufs[5]).reshape((nv, nl), order="F"))
 
 
 def quad_grid(box_min, box_max, v_max, v_stp, lambda_, wave=False):

@@ -319,3 +319,103 @@ def tabulated_scatter(theta, f):
     out["cs"] = cs.value
     out["evalAmp"] = out.pop("re") + 1j * out.pop("im")
     return out
+
+
+# ------------------------------------------------ heterogeneous media
+
+class OMedium(C.Structure):
+    _fields_ = [("nb", C.c_int * 3), ("bin", C.c_double * 3), ("sigt", _dp), ("albedo", _dp),
+                ("has_lrad", C.c_int), ("lrad", C.c_double)]
+
+
+def medium(spec, sigt, box_bin, albedo=None, lrad=None):
+    """orc_medium for sigt (an array of cells, x fastest like MATLAB) over box_bin cells."""
+    dim = spec.dim
+    sig = np.ascontiguousarray(np.asarray(sigt, dtype=np.float64).reshape(-1, order="F"))
+    bb = np.asarray(box_bin, dtype=np.float64).reshape(-1)
+    nb = np.rint((spec.box_max - spec.box_min) / bb).astype(int)
+    assert nb.prod() == sig.size, (nb, sig.size)
+    m = OMedium()
+    for j in range(dim):
+        m.nb[j] = int(nb[j])
+        m.bin[j] = float(bb[j])
+    for j in range(dim, 3):
+        m.nb[j] = 1
+        m.bin[j] = 1.0
+    m.sigt = _p(sig)
+    alb = None
+    if albedo is not None and np.size(albedo) > 1:
+        alb = np.ascontiguousarray(np.asarray(albedo, dtype=np.float64).reshape(-1, order="F"))
+        m.albedo = _p(alb)
+    else:
+        m.albedo = C.cast(None, _dp)
+    m.has_lrad = 0 if lrad is None else 1
+    m.lrad = 0.0 if lrad is None else float(lrad)
+    return m, (sig, alb)
+
+
+def _hetro_sigs(L):
+    MP, P = C.POINTER(OMedium), C.POINTER(OProblem)
+    L.orc_mccov_hetro.argtypes = [P, MP, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_mccov_hetro.restype = C.c_int64
+    L.orc_mccov_hetro_sums.argtypes = [P, MP, C.c_uint64, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_mccov_hetro_sums.restype = C.c_int64
+    L.orc_eval_phase_att_hetro.argtypes = [P, MP, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp, _dp]
+    L.orc_woodcock.argtypes = [P, MP, _dp, _dp, C.c_uint64]
+    L.orc_woodcock.restype = C.c_double
+    L.orc_hetro_icdf.argtypes = [_dp, C.c_int, C.c_int, C.POINTER(C.c_int32)]
+    L.orc_hetro_icdf.restype = C.c_int
+
+
+def mccov_hetro(spec, sigt, box_bin, max_itr, albedo=None, lrad=None, sums_from=None):
+    """MCCovHetro: (Ms, Mm, mean1, vertex count).  With sums_from=p0 the
+    unnormalised sums over path ids [p0, p0 + max_itr)."""
+    L = lib()
+    _hetro_sigs(L)
+    p, keep = problem(spec)
+    m, keep2 = medium(spec, sigt, box_bin, albedo, lrad)
+    nv, nl = spec.nv, spec.nl
+    ne, nf = nv * nv * nl * nl, nv * nl
+    a, b, c, d = np.empty(ne), np.empty(ne), np.empty(ne), np.empty(ne)
+    e, f = np.empty(nf), np.empty(nf)
+    if sums_from is None:
+        k = L.orc_mccov_hetro(C.byref(p), C.byref(m), int(max_itr), _p(a), _p(b), _p(c), _p(d), _p(e), _p(f))
+    else:
+        k = L.orc_mccov_hetro_sums(C.byref(p), C.byref(m), int(sums_from), int(max_itr), _p(a), _p(b), _p(c), _p(d),
+                                   _p(e), _p(f))
+    sh = (nv, nv, nl, nl)
+    return ((a + 1j * b).reshape(sh, order="F"), (c + 1j * d).reshape(sh, order="F"),
+            (e + 1j * f).reshape((nv, nl), order="F"), k)
+
+
+def phase_att_hetro(spec, sigt, box_bin, x, dirs, is_ff, sign=1.0, albedo=None):
+    L = lib()
+    _hetro_sigs(L)
+    p, keep = problem(spec)
+    m, keep2 = medium(spec, sigt, box_bin, albedo)
+    dirs = np.asarray(dirs, dtype=np.float64).reshape(spec.dim, -1)
+    dc = np.ascontiguousarray(dirs.T)
+    xx = np.ascontiguousarray(np.r_[np.asarray(x, dtype=np.float64).reshape(-1), np.zeros(3)][:3])
+    n = dirs.shape[1]
+    re, im = np.empty(n), np.empty(n)
+    L.orc_eval_phase_att_hetro(C.byref(p), C.byref(m), _p(xx), _p(dc), n, int(is_ff), float(sign), _p(re), _p(im))
+    return re + 1j * im
+
+
+def woodcock(spec, sigt, box_bin, x, w, paths):
+    L = lib()
+    _hetro_sigs(L)
+    p, keep = problem(spec)
+    m, keep2 = medium(spec, sigt, box_bin)
+    xx = np.ascontiguousarray(np.r_[np.asarray(x, float).reshape(-1), np.zeros(3)][:3])
+    ww = np.ascontiguousarray(np.r_[np.asarray(w, float).reshape(-1), np.zeros(3)][:3])
+    return np.array([L.orc_woodcock(C.byref(p), C.byref(m), _p(xx), _p(ww), int(q)) for q in paths])
+
+
+def hetro_icdf(sigt, n_out):
+    L = lib()
+    _hetro_sigs(L)
+    sig = np.ascontiguousarray(np.asarray(sigt, dtype=np.float64).reshape(-1, order="F"))
+    out = np.empty(n_out, dtype=np.int32)
+    L.orc_hetro_icdf(_p(sig), sig.size, n_out, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out

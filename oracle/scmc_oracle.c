@@ -434,10 +434,12 @@ typedef struct {
     cplx *e_v_ms, *e_l_ms;                           /* vertex level          */
     double *rv, *rl;                                 /* entrance directions  */
     double *tmp_re, *tmp_im;
+    const orc_medium *med;                           /* voxel medium or NULL */
 } nee_ws;
 
 static void nee_alloc(nee_ws *ws, int nv, int nl, int dim)
 {
+    ws->med = NULL;
     ws->nv = nv; ws->nl = nl; ws->dim = dim;
     ws->e_l0 = calloc((size_t)nl, sizeof(cplx));
     ws->e_l0_ms = calloc((size_t)nl, sizeof(cplx));
@@ -462,8 +464,11 @@ static void nee_free(nee_ws *ws)
 static void phase_att_c(const orc_problem *pr, const double *x, const double *d, int m, int is_ff,
                         double sign, nee_ws *ws, cplx *out)
 {
-    orc_eval_phase_att(x, d, m, pr->dim, is_ff, sign, pr->sigt, pr->lambda, pr->box_min, pr->box_max,
-                       ws->tmp_re, ws->tmp_im);
+    if (ws->med)
+        orc_eval_phase_att_hetro(pr, ws->med, x, d, m, is_ff, sign, ws->tmp_re, ws->tmp_im);
+    else
+        orc_eval_phase_att(x, d, m, pr->dim, is_ff, sign, pr->sigt, pr->lambda, pr->box_min, pr->box_max,
+                           ws->tmp_re, ws->tmp_im);
     for (int i = 0; i < m; ++i) out[i] = CMPLX(ws->tmp_re[i], ws->tmp_im[i]);
 }
 
@@ -546,11 +551,25 @@ static void outer(const cplx *a, const cplx *b, int n, cplx *m)
 /* Accumulate one path (head + vertices) into unnormalised Ms, Mm following
  * MCcov.m:181-264 term by term. */
 static void mccov_path(const orc_problem *pr, const path_head *h, const vtx *vs, int64_t nvtx,
-                       nee_ws *ws, mcov_ws *mw, cplx *Ms, cplx *Mm)
+                       nee_ws *ws, mcov_ws *mw, cplx *Ms, cplx *Mm, cplx *mean1)
 {
     const int nv = pr->nv, nl = pr->nl;
     const size_t nv2 = (size_t)nv * nv;
     nee_path(pr, h->x1, h->w1, h->w0p, ws);
+    if (mean1 && nvtx > 0) {
+        /* MCCovHetro.m:152, 210-211: mean1 += weight/px af_ang_vl .* (e_v0 * u_l0.'),
+         * u_l0 = evalphaseatt with sigt = 0 */
+        double *ure = malloc(sizeof(double) * (size_t)nl), *uim = malloc(sizeof(double) * (size_t)nl);
+        orc_eval_phase_att(h->x1, pr->l, nl, pr->dim, pr->is_ff_l, 1.0, 0.0, pr->lambda, pr->box_min, pr->box_max,
+                           ure, uim);
+        const double wp = vs[0].weight / h->px;
+        for (int j = 0; j < nl; ++j)
+            for (int i = 0; i < nv; ++i)
+                mean1[i + (size_t)nv * j] +=
+                    wp * ws->af_vl[i + (size_t)nv * j] * (ws->e_v0[i] * CMPLX(ure[j], uim[j]));
+        free(ure);
+        free(uim);
+    }
     /* MCcov.m:190 */
     outer(ws->e_v0, ws->e_v0, nv, mw->m00);
     for (int64_t n = 1; n <= nvtx; ++n) {
@@ -624,7 +643,7 @@ static void mccov_range(const orc_problem *pr, uint64_t p0, int64_t np, cplx *Ms
     path_head h;
     for (int64_t i = 0; i < np; ++i) {
         int64_t n = walk(pr, p0 + (uint64_t)i, unmean, meanl, &h, &buf, &cap);
-        mccov_path(pr, &h, buf, n, &ws, &mw, Ms, Mm);
+        mccov_path(pr, &h, buf, n, &ws, &mw, Ms, Mm, NULL);
     }
     free(buf);
     free(mw.m00); free(mw.mnn); free(mw.mn0); free(mw.m0n);
@@ -717,7 +736,7 @@ void orc_mccov_replay(const orc_problem *pr, const orc_pathset *ps, int64_t max_
     path_head h;
     for (int64_t i = 0; i < ps->n_paths; ++i) {
         load_path(ps, i, &h, &buf, &cap, &n);
-        mccov_path(pr, &h, buf, n, &ws, &mw, Ms, Mm);
+        mccov_path(pr, &h, buf, n, &ws, &mw, Ms, Mm, NULL);
     }
     mccov_normalise(pr, max_itr, Ms, Mm, ne, ms_re, ms_im, mm_re, mm_im);
     free(buf);
@@ -1222,4 +1241,445 @@ int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *
                              smp_flg, lmean0, vsign, seed, u_re, u_im);
     free(wl);
     return r;
+}
+
+/* ================================================== heterogeneous media */
+/* code/MCCovHetro.m:1-276 with coreFunctions/{findBin,smpWoodcock,
+ * evalphaseattHetro,expSmpXHetro,cubeProj,invCDFvec,smpicdf}.m.
+ *
+ * Random numbers (same Philox keys as the homogeneous walk, so a uniform
+ * one-voxel medium walks MCcov's paths exactly):
+ *   vertex 0: g0 = (u_z, u_x), g1 = (u_y, d0), g2 = (d1, d2), g3 = (d3, a)
+ *             a picks the voxel (smpFlg 1); smpFlg 2 uses (g0.y, g1.x) for the
+ *             plane patch and (g0.x, g3.y) for the first Woodcock trial;
+ *             Woodcock trials k >= 1 at vertex 0 use group 16 + k; the lrad
+ *             retries of smpFlg 1 use groups 1024 + 2k, 1025 + 2k.
+ *   vertex n: g0 = (xi, first Woodcock distance), g1 = (u_roulette, d0),
+ *             g2 = (d1, d2), g3 = (d3, first Woodcock acceptance); trials
+ *             k >= 1 use group 16 + k. */
+
+static int med_nvox(const orc_medium *m, int dim)
+{
+    int n = 1;
+    for (int j = 0; j < dim; ++j) n *= m->nb[j];
+    return n;
+}
+
+/* findBin.m:1-21 (0-based index; the cell of a point on the lower face is
+ * clamped to the first cell, MATLAB would index 0) */
+static int find_bin(const orc_problem *pr, const orc_medium *m, const double *x, double *bmn, double *bmx)
+{
+    int idx = 0, stride = 1;
+    for (int j = 0; j < pr->dim; ++j) {
+        double q = ceil((x[j] - pr->box_min[j]) / m->bin[j]);
+        int ii = (int)q;
+        if (ii < 1) ii = 1;
+        if (ii > m->nb[j]) ii = m->nb[j];
+        idx += (ii - 1) * stride;
+        stride *= m->nb[j];
+        if (bmn) {
+            bmn[j] = pr->box_min[j] + m->bin[j] * (ii - 1);
+            bmx[j] = bmn[j] + m->bin[j];
+        }
+    }
+    return idx;
+}
+
+static double med_albedo(const orc_problem *pr, const orc_medium *m, const double *x)
+{
+    if (!m->albedo) return pr->albedo;
+    return m->albedo[find_bin(pr, m, x, NULL, NULL)];
+}
+
+static double med_sigmax(const orc_problem *pr, const orc_medium *m)
+{
+    double s = 0;
+    for (int i = 0; i < med_nvox(m, pr->dim); ++i) s = fmax(s, m->sigt[i]);
+    return s;
+}
+
+/* evalphaseattHetro.m:29-48: optical depth from x along -v (towards the
+ * source), cell by cell, capped at the distance nd. */
+static double march_depth(const orc_problem *pr, const orc_medium *m, const double *x, const double *v, double nd,
+                          int cap)
+{
+    const int dim = pr->dim;
+    double mv[3] = {-v[0], -v[1], -v[2]};
+    double nx[3] = {x[0], x[1], x[2]};
+    double bdv = 0, wbdv = 0;
+    for (int it = 0; it < cap; ++it) {
+        double bmn[3], bmx[3];
+        int b = find_bin(pr, m, nx, bmn, bmx);
+        double dist = orc_cube_dist(nx, bmn, bmx, mv, dim);
+        double ep = 0;
+        for (int j = 0; j < dim; ++j) ep = fmax(ep, fabs(bmx[j] - bmn[j]));
+        ep = 0.00010000000000000002 * ep;   /* 0.1^4 as computed by pow */
+        for (int j = 0; j < dim; ++j) nx[j] = nx[j] + (dist + ep) * mv[j];
+        bdv = bdv + dist;
+        wbdv = wbdv + dist * m->sigt[b];
+        if (bdv >= nd) {
+            double md = bdv - nd;
+            bdv = nd;
+            wbdv = wbdv - md * m->sigt[b];
+            break;
+        }
+        if (outside(nx, pr->box_min, pr->box_max, dim)) break;
+    }
+    return wbdv;
+}
+
+static int march_cap(const orc_medium *m, int dim)
+{
+    return 4 * (m->nb[0] + m->nb[1] + (dim == 3 ? m->nb[2] : 0)) + 16;
+}
+
+/* evalphaseattHetro.m:1-51: march from x towards the source through the
+ * cells (cubeDist per cell, stepping 1e-4 cell sizes past each face), the
+ * optical depth sum d*sigt(cell) capped at the source distance nd. */
+void orc_eval_phase_att_hetro(const orc_problem *pr, const orc_medium *m, const double *x, const double *d,
+                              int nd_count, int is_ff, double sign, double *out_re, double *out_im)
+{
+    const int dim = pr->dim;
+    const double k = M_PI * 2.0 / pr->lambda;
+    const int cap = march_cap(m, dim);
+    for (int i = 0; i < nd_count; ++i) {
+        double v[3] = {0, 0, 0};
+        for (int j = 0; j < dim; ++j) v[j] = sign * d[i * dim + j];
+        double pv, rv, nd;
+        if (is_ff) {
+            pv = dot3(x, v, dim);
+            rv = 1.0;
+            nd = 1e12;
+        } else {
+            double dd[3] = {0, 0, 0}, s = 0;
+            for (int j = 0; j < dim; ++j) { dd[j] = x[j] - v[j]; s += dd[j] * dd[j]; }
+            nd = sqrt(s);
+            for (int j = 0; j < dim; ++j) v[j] = dd[j] / nd;
+            pv = nd;
+            rv = 1.0 / pow(nd + 0.01 * pr->lambda, (dim - 1) / 2.0);
+        }
+        double wbdv = march_depth(pr, m, x, v, nd, cap);
+        double mag = exp(-0.5 * wbdv) * rv;
+        double ph = k * pv;
+        out_re[i] = mag * cos(ph);
+        out_im[i] = mag * sin(ph);
+    }
+}
+
+/* smpWoodcock.m:1-15 (delta tracking with sigt_max) */
+static double woodcock(const orc_problem *pr, const orc_medium *m, double smax, const double *x, const double *w,
+                       uint64_t path, uint32_t vertex, double u_dist0, double u_acc0)
+{
+    double d = 0;
+    for (int it = 0; it < 1000000; ++it) {
+        double ud = u_dist0, ua = u_acc0;
+        if (it > 0) {
+            double g[2];
+            orc_uniform_pair(pr->seed, path, vertex, 16u + (uint32_t)it, g);
+            ud = g[0];
+            ua = g[1];
+        }
+        d = d - log(1 - ud) / smax;
+        double nx[3] = {0, 0, 0};
+        for (int j = 0; j < pr->dim; ++j) nx[j] = x[j] + d * w[j];
+        if (outside(nx, pr->box_min, pr->box_max, pr->dim)) break;
+        int b = find_bin(pr, m, nx, NULL, NULL);
+        if (ua <= m->sigt[b] / smax) break;
+    }
+    return d;
+}
+
+double orc_woodcock(const orc_problem *pr, const orc_medium *m, const double *x, const double *w, uint64_t path)
+{
+    double g0[2], g3[2];
+    orc_uniform_pair(pr->seed, path, 1, 0, g0);
+    orc_uniform_pair(pr->seed, path, 1, 3, g3);
+    return woodcock(pr, m, med_sigmax(pr, m), x, w, path, 1, g0[1], g3[1]);
+}
+
+/* invCDFvec.m:1-8 with MATLAB's hist(y, N): N equal bins over [min, max],
+ * values on an inner edge counted in the upper bin, then the cumulative
+ * counts.  icdf has n_out entries. */
+int orc_hetro_icdf(const double *sigt, int nvox, int n_out, int32_t *icdf)
+{
+    double tot = 0;
+    for (int i = 0; i < nvox; ++i) tot += sigt[i];
+    if (!(tot > 0)) return -1;
+    double *cdf = malloc(sizeof(double) * (size_t)nvox);
+    double c = 0;
+    for (int i = 0; i < nvox; ++i) {
+        c += sigt[i] / tot;   /* cumsum(sigt(:)'/sum(sigt(:))) */
+        cdf[i] = c;
+    }
+    double lo = cdf[0], hi = cdf[0];
+    for (int i = 0; i < nvox; ++i) { lo = fmin(lo, cdf[i]); hi = fmax(hi, cdf[i]); }
+    int32_t *cnt = calloc((size_t)n_out, sizeof(int32_t));
+    for (int i = 0; i < nvox; ++i) {
+        int b;
+        if (hi == lo) {
+            b = n_out / 2;   /* hist of a constant puts everything in the middle bin */
+        } else {
+            double bw = (hi - lo) / n_out;
+            b = 0;
+            /* largest b with lo + b*bw <= cdf (edges: lo + bw*(1:N-1)) */
+            double q = floor((cdf[i] - lo) / bw);
+            b = (int)q;
+            if (b > n_out - 1) b = n_out - 1;
+            if (b < 0) b = 0;
+            while (b > 0 && cdf[i] < lo + bw * b) --b;
+            while (b < n_out - 1 && cdf[i] >= lo + bw * (b + 1)) ++b;
+        }
+        cnt[b] += 1;
+    }
+    int32_t run = 0;
+    for (int b = 0; b < n_out; ++b) { run += cnt[b]; icdf[b] = run; }
+    free(cnt);
+    free(cdf);
+    return 0;
+}
+
+/* an orthonormal basis of the complement of lmean (the columns of
+ * null(lmean') up to an in-plane rotation) */
+static void null_basis(const double *lm, int dim, double u[3][3])
+{
+    if (dim == 2) {
+        u[0][0] = -lm[1];
+        u[1][0] = lm[0];
+        return;
+    }
+    /* the axis least aligned with lmean */
+    int a = 0;
+    for (int j = 1; j < 3; ++j)
+        if (fabs(lm[j]) < fabs(lm[a])) a = j;
+    double e[3] = {0, 0, 0};
+    e[a] = 1;
+    double c1[3] = {lm[1] * e[2] - lm[2] * e[1], lm[2] * e[0] - lm[0] * e[2], lm[0] * e[1] - lm[1] * e[0]};
+    double n1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
+    for (int j = 0; j < 3; ++j) c1[j] /= n1;
+    double c2[3] = {lm[1] * c1[2] - lm[2] * c1[1], lm[2] * c1[0] - lm[0] * c1[2], lm[0] * c1[1] - lm[1] * c1[0]};
+    for (int j = 0; j < 3; ++j) {
+        u[j][0] = c1[j];
+        u[j][1] = c2[j];
+    }
+}
+
+/* cubeProj.m:1-20 : move x along v to just inside the box; 0 when the
+ * point is already past the entry (the reference returns a scalar there) */
+static int cube_proj(const double *x, const double *bmin, const double *bmax, const double *v, int dim, double *xd)
+{
+    const double thr = 1.0000000000000004e-07;
+    double d = -INFINITY;
+    for (int j = 0; j < dim; ++j) {
+        double ad = fmax(fabs(v[j]), thr);
+        double bd = (v[j] <= 0) ? (x[j] - bmax[j]) / ad : (-x[j] + bmin[j]) / ad;
+        d = fmax(d, bd);
+    }
+    if (!(d >= 0)) return 0;
+    double ep = 0;
+    for (int j = 0; j < dim; ++j) ep = fmax(ep, fabs(bmax[j] - bmin[j]));
+    ep = 0.00010000000000000002 * ep;
+    for (int j = 0; j < dim; ++j) xd[j] = x[j] + (d + ep) * v[j];
+    return 1;
+}
+
+/* One path of MCCovHetro.m:101-244.  Returns the vertex count, 0 when the
+ * first vertex falls outside the box (the iteration is spent, MCCovHetro.m:121-123). */
+static int64_t walk_hetro(const orc_problem *pr, const orc_medium *m, const int32_t *icdf, int n_icdf, double V,
+                          double smax, uint64_t path, const double *meanl, path_head *h, vtx **buf, int64_t *cap)
+{
+    const int dim = pr->dim;
+    const double kill_thr = 0.2;
+    double g0[2], g1[2], g2[2], g3[2];
+    orc_uniform_pair(pr->seed, path, 0, 0, g0);
+    orc_uniform_pair(pr->seed, path, 0, 1, g1);
+    orc_uniform_pair(pr->seed, path, 0, 2, g2);
+    orc_uniform_pair(pr->seed, path, 0, 3, g3);
+    double x[3] = {0, 0, 0}, px;
+    if (pr->smp_flg == 1) {
+        /* MCCovHetro.m:103-114 : cell from the quantised icdf (smpicdf.m), uniform in the cell */
+        int i = (int)ceil(g3[1] * n_icdf);
+        if (i < 1) i = 1;
+        /* a table entry of 0 (only from hist of a constant cdf, one cell) is
+         * not a valid MATLAB index: take the first cell */
+        int b = icdf[i - 1] - 1;
+        if (b < 0) b = 0;
+        int rem = b;
+        double bmn[3] = {0, 0, 0};
+        for (int j = 0; j < dim; ++j) {
+            int ii = rem % m->nb[j];
+            rem /= m->nb[j];
+            bmn[j] = pr->box_min[j] + m->bin[j] * ii;
+        }
+        double r[3] = {g0[1], dim == 3 ? g1[0] : g0[0], g0[0]};
+        for (int j = 0; j < dim; ++j) x[j] = r[j] * m->bin[j] + bmn[j];
+        px = 1 / V;
+        if (m->has_lrad) {
+            /* MCCovHetro.m:45-48, 107-113 (one light) */
+            double ls[3];
+            for (int j = 0; j < dim; ++j) ls[j] = (pr->box_min[j] + pr->box_max[j]) / 2;
+            ls[dim - 1] = (pr->l[dim - 1] > 0) ? pr->box_min[dim - 1] : pr->box_max[dim - 1];
+            for (int it = 0;; ++it) {
+                double lx = dot3(pr->l, x, dim), r2 = 0;
+                for (int j = 0; j < dim; ++j) {
+                    double q = x[j] - lx * pr->l[j] - ls[j];
+                    r2 += q * q;
+                }
+                if (!(sqrt(r2) > m->lrad) || it > 100000) break;
+                double a[2], c[2];
+                orc_uniform_pair(pr->seed, path, 0, 1024u + 2u * (uint32_t)it, a);
+                orc_uniform_pair(pr->seed, path, 0, 1025u + 2u * (uint32_t)it, c);
+                double rr[3] = {a[0], a[1], c[0]};
+                for (int j = 0; j < dim; ++j) x[j] = rr[j] * (pr->box_max[j] - pr->box_min[j]) + pr->box_min[j];
+                px = 1;
+            }
+        }
+    } else {
+        /* expSmpXHetro.m:1-31 */
+        double lrad = 0;
+        if (m->has_lrad) {
+            lrad = m->lrad;
+        } else {
+            for (int j = 0; j < dim; ++j) lrad += (pr->box_max[j] - pr->box_min[j]) * (pr->box_max[j] - pr->box_min[j]);
+            lrad = sqrt(lrad);
+        }
+        double u[3][3] = {{0}};
+        null_basis(meanl, dim, u);
+        double c = 0;
+        for (int j = 0; j < dim; ++j) c = fmax(c, fmax(fabs(pr->box_max[j]), fabs(pr->box_min[j])));
+        c = c * dim;
+        double r[2] = {(g0[1] - 0.5) * lrad, (g1[0] - 0.5) * lrad};
+        double x0[3] = {0, 0, 0};
+        for (int j = 0; j < dim; ++j) {
+            double s = 0;
+            for (int q = 0; q < dim - 1; ++q) s += u[j][q] * r[q];
+            x0[j] = s - meanl[j] * c;
+        }
+        int ok = cube_proj(x0, pr->box_min, pr->box_max, meanl, dim, x);
+        if (!ok || outside(x, pr->box_min, pr->box_max, dim)) return 0;
+        double d = woodcock(pr, m, smax, x, meanl, path, 0, g0[0], g3[1]);
+        for (int j = 0; j < dim; ++j) x[j] = x[j] + d * meanl[j];
+        if (outside(x, pr->box_min, pr->box_max, dim)) return 0;
+        /* px = abs(evalphaseattHetro(x, lmean, 1, sigt, 1, ...))^2 / lrad^(dim-1);
+         * the modulus of the unit phase is taken as exactly 1 */
+        double mag = exp(-0.5 * march_depth(pr, m, x, meanl, 1e12, march_cap(m, dim)));
+        px = mag * mag / pow(lrad, dim - 1);
+    }
+    if (outside(x, pr->box_min, pr->box_max, dim)) return 0;
+    double du[4] = {g1[1], g2[0], g2[1], g3[0]};
+    double w[3] = {0, 0, 0};
+    cplx w0p;
+    if (!pr->has_amp0) {
+        iso_dir(du, dim, w);
+        w0p = 1.0 / sqrt(pow(2.0, dim - 1) * M_PI);
+    } else {
+        orc_phase p0 = pr->amp0;
+        p0.type = pr->amp.type;
+        orc_sample_dir(meanl, &p0, dim, du, w);
+        w0p = eval_amp_c(dot3(meanl, w, dim), &p0, dim);
+    }
+    memcpy(h->x1, x, sizeof x);
+    h->px = px;
+    memcpy(h->w1, w, sizeof w);
+    h->w0p = w0p;
+
+    double weight = med_albedo(pr, m, x);   /* MCCovHetro.m:164 */
+    double ow[3] = {0, 0, 0};
+    int64_t n = 0;
+    for (;;) {
+        ++n;
+        double a[2], b[2], c[2], e[2];
+        orc_uniform_pair(pr->seed, path, (uint32_t)n, 0, a);
+        orc_uniform_pair(pr->seed, path, (uint32_t)n, 3, e);
+        if (n > *cap) {
+            *cap = *cap ? *cap * 2 : 64;
+            *buf = (vtx *)realloc(*buf, sizeof(vtx) * (size_t)*cap);
+        }
+        vtx *vv = &(*buf)[n - 1];
+        memcpy(vv->x, x, sizeof x);
+        memcpy(vv->ow, ow, sizeof ow);
+        vv->weight = weight;
+        vv->xi = a[0];
+        /* MCCovHetro.m:228-233 */
+        double d = woodcock(pr, m, smax, x, w, path, (uint32_t)n, a[1], e[1]);
+        for (int j = 0; j < dim; ++j) x[j] = x[j] + d * w[j];
+        if (outside(x, pr->box_min, pr->box_max, dim)) break;
+        orc_uniform_pair(pr->seed, path, (uint32_t)n, 1, b);
+        /* MCCovHetro.m:234-241 */
+        if (weight < kill_thr) {
+            if (b[0] > med_albedo(pr, m, x)) break;
+        } else {
+            weight = weight * med_albedo(pr, m, x);
+        }
+        orc_uniform_pair(pr->seed, path, (uint32_t)n, 2, c);
+        double u4[4] = {b[1], c[0], c[1], e[0]};
+        memcpy(ow, w, sizeof w);
+        orc_sample_dir(ow, &pr->amp, dim, u4, w);
+    }
+    return n;
+}
+
+/* The per-medium constants of MCCovHetro.m:64-66 and the path ids
+ * [path_offset, path_offset + max_itr). */
+int64_t orc_mccov_hetro_sums(const orc_problem *pr, const orc_medium *m, uint64_t p0, int64_t np,
+                             double *ms_re, double *ms_im, double *mm_re, double *mm_im, double *mean_re,
+                             double *mean_im)
+{
+    const int nvox = med_nvox(m, pr->dim);
+    const int n_icdf = nvox * 100;
+    int32_t *icdf = malloc(sizeof(int32_t) * (size_t)n_icdf);
+    double tot = 0;
+    for (int i = 0; i < nvox; ++i) tot += m->sigt[i];
+    double vol = 1;
+    for (int j = 0; j < pr->dim; ++j) vol *= m->bin[j];
+    const double V = tot * vol;   /* V = sum(sigt(:))*prod(box_bin) */
+    if (orc_hetro_icdf(m->sigt, nvox, n_icdf, icdf)) {
+        free(icdf);
+        return -1;
+    }
+    const double smax = med_sigmax(pr, m);
+    double unmean[3], meanl[3];
+    orc_problem q = *pr;
+    q.has_lmean0 = 0;   /* MCCovHetro.m:40-41 */
+    mean_dir(&q, unmean, meanl);
+    const int nv = pr->nv, nl = pr->nl;
+    const size_t ne = (size_t)nv * nv * nl * nl, nf = (size_t)nv * nl;
+    cplx *Ms = calloc(ne, sizeof(cplx)), *Mm = calloc(ne, sizeof(cplx)), *mean1 = calloc(nf, sizeof(cplx));
+    nee_ws ws;
+    nee_alloc(&ws, nv, nl, pr->dim);
+    ws.med = m;
+    mcov_ws mw = {calloc((size_t)nv * nv, sizeof(cplx)), calloc((size_t)nv * nv, sizeof(cplx)),
+                  calloc((size_t)nv * nv, sizeof(cplx)), calloc((size_t)nv * nv, sizeof(cplx))};
+    vtx *buf = NULL;
+    int64_t cap = 0, nvert = 0;
+    path_head h;
+    for (int64_t i = 0; i < np; ++i) {
+        int64_t n = walk_hetro(pr, m, icdf, n_icdf, V, smax, p0 + (uint64_t)i, meanl, &h, &buf, &cap);
+        if (n == 0) continue;
+        nvert += n;
+        mccov_path(pr, &h, buf, n, &ws, &mw, Ms, Mm, mean1);
+    }
+    split_out(Ms, ne, 1.0, ms_re, ms_im);
+    split_out(Mm, ne, 1.0, mm_re, mm_im);
+    split_out(mean1, nf, 1.0, mean_re, mean_im);
+    free(buf); free(icdf); free(Ms); free(Mm); free(mean1);
+    free(mw.m00); free(mw.mnn); free(mw.mn0); free(mw.m0n);
+    nee_free(&ws);
+    return nvert;
+}
+
+/* MCCovHetro.m:248-276 normalisation: Mm/2 with CBS, /maxItr; mean1 /maxItr
+ * (/sqrt(cs) with cs = 1, see the header). */
+int64_t orc_mccov_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_itr, double *ms_re, double *ms_im,
+                        double *mm_re, double *mm_im, double *mean_re, double *mean_im)
+{
+    int64_t nvert = orc_mccov_hetro_sums(pr, m, 0, max_itr, ms_re, ms_im, mm_re, mm_im, mean_re, mean_im);
+    if (nvert < 0) return nvert;
+    const size_t ne = (size_t)pr->nv * pr->nv * pr->nl * pr->nl, nf = (size_t)pr->nv * pr->nl;
+    const double s = 1.0 / (double)max_itr, sm = (pr->do_cbs ? 0.5 : 1.0) * s;
+    for (size_t i = 0; i < ne; ++i) {
+        ms_re[i] *= s; ms_im[i] *= s;
+        mm_re[i] *= sm; mm_im[i] *= sm;
+    }
+    for (size_t i = 0; i < nf; ++i) { mean_re[i] *= s; mean_im[i] *= s; }
+    return nvert;
 }

@@ -32,6 +32,11 @@
  *   mcfield_wave       : equals mcfield contracted with Wl on the same paths.
  *   fourier_quad_wave  : numpy.fft restatement at odd and even transform
  *                        sizes; linear in Wl.
+ *   mccov_hetro        : a uniform one-cell medium walks MCcov's paths and
+ *                        reproduces MCcov; Woodcock distances follow the
+ *                        exact transmittance of a layered medium (KS);
+ *                        single-scatter closed form of a two-layer slab;
+ *                        the ray march equals a direct sum over layers.
  */
 #ifndef SCMC_ORACLE_H
 #define SCMC_ORACLE_H
@@ -51,6 +56,16 @@ typedef struct {
     int n_icdf;            /* tabulated: length of sampleIcdf            */
     const double *icdf;    /* sampleIcdf (angles)                        */
 } orc_phase;
+
+/* Voxel medium of MCCovHetro.m:1 (sigt / albedo grids over box_bin cells). */
+typedef struct {
+    int nb[3];                /* cells per axis, round((box_max-box_min)./box_bin) */
+    double bin[3];            /* box_bin                                          */
+    const double *sigt;       /* prod(nb) extinction values, column-major         */
+    const double *albedo;     /* prod(nb) values, or NULL for the scalar albedo   */
+    int has_lrad;             /* lrad given (MCCovHetro.m:45, expSmpXHetro.m:4)   */
+    double lrad;
+} orc_medium;
 
 /* Problem descriptor: the positional arguments of MCcov.m:1 / MCfield.m:1. */
 typedef struct {
@@ -169,6 +184,28 @@ int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *
                                       double v_max, double v_stp, int64_t max_itr, double lambda, int do_cbs,
                                       int smp_flg, const double *lmean0, double vsign, uint64_t seed,
                                       double *u_re, double *u_im);
+
+/* code/MCCovHetro.m:1-276 (heterogeneous media).  pr supplies everything but
+ * sigt/albedo grids (pr->albedo is used when m->albedo is NULL; pr->sigt is
+ * ignored).  Outputs Ms, Mm (nv x nv x nl x nl) and mean1 (nv x nl); returns
+ * the vertex count, or -1 for an all-zero sigt grid.  mean1's normaliser
+ * sqrt(cs) is taken as 1: the reference reads ampfunc.pdf, a field the
+ * tabulatedAmplitudeScatter struct does not have, whose evalAmp is already
+ * normalised by cs. */
+int64_t orc_mccov_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_itr, double *ms_re, double *ms_im,
+                        double *mm_re, double *mm_im, double *mean_re, double *mean_im);
+/* unnormalised sums over path ids [p0, p0 + np) */
+int64_t orc_mccov_hetro_sums(const orc_problem *pr, const orc_medium *m, uint64_t p0, int64_t np,
+                             double *ms_re, double *ms_im, double *mm_re, double *mm_im, double *mean_re,
+                             double *mean_im);
+/* evalphaseattHetro.m:1-51 for n directions / positions (sign * d[:, i]). */
+void orc_eval_phase_att_hetro(const orc_problem *pr, const orc_medium *m, const double *x, const double *d, int n,
+                              int is_ff, double sign, double *out_re, double *out_im);
+/* smpWoodcock.m:1-15 from x along w with the Philox stream of vertex 1 of
+ * the given path (the test hook of the sampler). */
+double orc_woodcock(const orc_problem *pr, const orc_medium *m, const double *x, const double *w, uint64_t path);
+/* invCDFvec(cumsum(sigt(:)'/sum(sigt(:))), n_out) : icdf has n_out entries. */
+int orc_hetro_icdf(const double *sigt, int nvox, int n_out, int32_t *icdf);
 
 /* interface/scatter/tabulatedAmplitudeScatter.m:53-99 + coreFunctions/invCDF.m
  * Returns D (2 or 3) or a negative error code. Outputs are length n. */

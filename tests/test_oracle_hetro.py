@@ -1,0 +1,160 @@
+"""Pins of the oracle's MCCovHetro restatement (heterogeneous media, CPU).
+
+* A uniform one-cell medium walks exactly MCcov's paths (the Philox slots
+  line up, Woodcock accepts its first trial, the march is one cubeDist), so
+  Ms and Mm equal MCcov's to rounding.
+* evalphaseattHetro's march through layers equals the sum over layers of
+  sigma * path length, minus the 1e-4-cell-size step past every crossed face
+  (evalphaseattHetro.m:33-36 with cubeDist.m:13-16) counted in the next cell.
+* smpWoodcock's distances follow the exact transmittance of a layered
+  medium: P(d > s) = exp(-tau(s)) (Kolmogorov-Smirnov on the escape-
+  conditioned part, and the escape fraction).
+* invCDFvec's table equals numpy's histogram + cumsum (MATLAB's hist bins).
+* Two-layer single scattering: E[Ms(v,v,l,l)] equals the closed-form depth
+  integral (expSmpXHetro sampling), within 3.5 standard errors.
+* A uniform multi-cell medium agrees with MCcov statistically (smpFlg 2).
+"""
+import numpy as np
+import pytest
+from scipy import stats
+
+from conftest import cov_matrix, cs_scale, far_dirs
+from mcspeckle_amd.engines import make_problem
+from test_oracle_engines import near_far_pair, small_spec, tab3d
+
+
+@pytest.mark.parametrize("kind,dim,cbs,near", [("hg", 3, True, False), ("iso", 2, True, False),
+                                               ("tab", 3, False, False), ("hg", 3, True, True)])
+def test_one_cell_medium_walks_mccov_paths(oracle, kind, dim, cbs, near):
+    if near:
+        _, spec = near_far_pair(kind, dim, cbs, 5.0, "both", nv=3, nl=2)
+    else:
+        spec = small_spec(kind, dim, cbs, nv=3, nl=2, seed=5, smp=1, tab=tab3d() if kind == "tab" else None)
+    cell = np.full((1,) * dim, spec.sigt)
+    Ms, Mm, _, K = oracle.mccov_hetro(spec, cell, spec.box_max - spec.box_min, 300)
+    Ms0, Mm0 = oracle.mccov(spec, 300)
+    for A, B in ((Ms, Ms0), (Mm, Mm0)):
+        C, R = cov_matrix(A), cov_matrix(B)
+        assert np.max(np.abs(C - R) / cs_scale(R)) < 1e-12
+
+
+def layered(sig, T=1.0, W=200.0):
+    """3-D slab of len(sig) layers along z, lateral +-W/2."""
+    L = len(sig)
+    bmin, bmax = np.array([-W / 2, -W / 2, 0.0]), np.array([W / 2, W / 2, T])
+    return bmin, bmax, np.array([W, W, T / L]), np.asarray(sig, float).reshape(1, 1, L)
+
+
+def test_ray_march_is_the_sum_over_layers(oracle):
+    sig = [1.5, 0.4, 3.0, 0.0, 2.2]
+    bmin, bmax, bb, grid = layered(sig, T=2.0, W=50.0)
+    spec = make_problem(1.0, 0.9, bmin, bmax, [[0], [0], [1]], [[0], [0], [1]], 1, 1, 1.0, False, 2, 1, 0)
+    h = 2.0 / len(sig)
+    ep = 1e-4 * 50.0                       # 1e-4 x the largest cell dimension
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        x = np.array([rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(0.01, 1.99)])
+        th, ph = rng.uniform(0, 0.6), rng.uniform(0, 2 * np.pi)
+        d = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])   # light going up
+        e = oracle.phase_att_hetro(spec, grid, bb, x, d, 1)[0]
+        # march downwards (-d) from x to z = 0
+        k0 = int(x[2] // h)
+        tau = sig[k0] * (x[2] - k0 * h) / d[2]
+        for k in range(k0 - 1, -1, -1):
+            tau += sig[k] * (h / d[2] - ep)
+        assert abs(np.abs(e) - np.exp(-tau / 2)) < 1e-12
+        assert abs(np.angle(e) - np.angle(np.exp(2j * np.pi * (x @ d)))) < 1e-9
+
+
+def test_woodcock_follows_the_transmittance(oracle):
+    sig = [0.5, 4.0, 1.0]
+    bmin, bmax, bb, grid = layered(sig, T=3.0, W=100.0)
+    spec = make_problem(1.0, 0.9, bmin, bmax, [[0], [0], [1]], [[0], [0], [1]], 1, 1, 1.0, False, 2, 1, 0, seed=17)
+    x = np.array([0.3, -0.2, 0.2])
+    w = np.array([0.3, 0.1, np.sqrt(1 - 0.1)])
+    d = oracle.woodcock(spec, grid, bb, x, w, range(20000))
+    zexit = (3.0 - x[2]) / w[2]
+
+    def tau(s):
+        z = x[2] + s * w[2]
+        t = 0.0
+        for k, sk in enumerate(sig):
+            lo, hi = k * 1.0, (k + 1) * 1.0
+            a, b = max(lo, x[2]), min(hi, z)
+            if b > a:
+                t += sk * (b - a) / w[2]
+        return t
+
+    esc = d > zexit
+    assert abs(esc.mean() - np.exp(-tau(zexit))) < 4 * np.sqrt(np.exp(-tau(zexit)) / d.size)
+    inside = np.sort(d[~esc])
+    cdf = lambda s: (1 - np.exp(-np.vectorize(tau)(s))) / (1 - np.exp(-tau(zexit)))
+    assert stats.kstest(inside, cdf).pvalue > 1e-3
+
+
+def test_icdf_table_is_matlab_hist(oracle):
+    rng = np.random.default_rng(2)
+    sig = rng.uniform(0, 3, 24)
+    sig[[3, 7]] = 0
+    n = sig.size * 100
+    got = oracle.hetro_icdf(sig, n)
+    cdf = np.cumsum(sig / sig.sum())
+    want = np.cumsum(np.histogram(cdf, n)[0])
+    np.testing.assert_array_equal(got, want)
+
+
+def test_two_layer_single_scatter_known_answer(oracle):
+    """E[Ms(v,v,l,l)] = A |a(l.v)|^2 albedo int sigma(z) exp(-tau_l(z) - tau_v(z)) dz
+    with the march's step loss ep * sigma_next per crossed face."""
+    s0, s1, T, W = 2.0, 0.7, 1.0, 200.0
+    bmin, bmax, bb, grid = layered([s0, s1], T=T, W=W)
+    rng = np.random.default_rng(6)
+    l, v = far_dirs(2, 15, rng), far_dirs(2, 15, rng)
+    alb = 0.8
+    spec = make_problem(1.0, alb, bmin, bmax, l, v, 1, 1, 1.0, False, 2, 1, 0, seed=8)
+    B, n = 16, 1500
+    est = np.array([oracle.mccov_hetro(spec, grid, bb, n, sums_from=b * n)[0] / n for b in range(B)])
+    mean, se = est.mean(0), est.std(0, ddof=1) / np.sqrt(B)
+    h, ep = T / 2, 1e-4 * W
+    a2 = 1 / (4 * np.pi)
+    for i in range(2):
+        for j in range(2):
+            lz, vz = l[2, j], v[2, i]
+            q0 = s0 * (1 / lz - 1 / vz)
+            c0 = s0 * h / vz + s1 * ((T - h) / vz - ep)
+            I0 = s0 * np.exp(-c0) * (1 - np.exp(-q0 * h)) / q0
+            q1 = s1 * (1 / lz - 1 / vz)
+            c1 = -s1 * h / lz + s0 * (h / lz - ep) + s1 * T / vz
+            I1 = s1 * np.exp(-c1) * (np.exp(-q1 * h) - np.exp(-q1 * T)) / q1
+            want = W * W * a2 * alb * (I0 + I1)
+            got = mean[i, i, j, j]
+            assert abs(got.imag) < 1e-9 * abs(want)
+            assert abs(got.real - want) < 3.5 * se[i, i, j, j].real, (got, want, se[i, i, j, j])
+
+
+def test_uniform_cells_agree_with_mccov_statistically(oracle):
+    rng = np.random.default_rng(3)
+    l, v = far_dirs(2, 20, rng), far_dirs(2, 20, rng)
+    sigt = 3.0
+    bmin, bmax = np.array([-0.75, -0.75, 0.0]), np.array([0.75, 0.75, 1.0])
+    spec = make_problem(sigt, 0.9, bmin, bmax, l, v, 1, 1, 1.0, True, 2, 3, 0.6, seed=2)
+    grid = np.full((3, 3, 2), sigt)
+    bb = np.array([0.5, 0.5, 0.5])
+    B, n = 12, 1500
+    V = np.prod(bmax - bmin)
+    h = []
+    for b in range(B):
+        Ms, Mm, _, _ = oracle.mccov_hetro(spec, grid, bb, n, sums_from=b * n)
+        h.append(cov_matrix(Ms + Mm / 2) / n)
+    spec2 = make_problem(sigt, 0.9, bmin, bmax, l, v, 1, 1, 1.0, True, 2, 3, 0.6, seed=9)
+    o = []
+    for b in range(B):
+        Ms, Mm = oracle.mccov_sums(spec2, b * n, n, 1)
+        o.append(cov_matrix(Ms + Mm / 2) * V * sigt / n)
+    h, o = np.array(h), np.array(o)
+    z = []
+    for part in (np.real, np.imag):
+        se = np.sqrt(part(h).std(0, ddof=1) ** 2 / B + part(o).std(0, ddof=1) ** 2 / B + (1e-9 * cs_scale(o.mean(0))) ** 2)
+        z.append((np.abs(part(h).mean(0) - part(o).mean(0)) / se).ravel())
+    z = np.concatenate(z)
+    assert np.mean(z < 3) >= 0.95 and z.max() < 6
