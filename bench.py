@@ -240,6 +240,33 @@ def quad_wave_leg(n_paths, precision="mixed"):
             "vertices": st["vertices"], "ms_scatter": st["ms_accum"], "ms_dft": st["ms_nee"], "precision": precision}
 
 
+def hetro_leg(n_paths, precision="mixed", seed=4):
+    """MCCovHetro: a voxel medium of 8 x 8 x 4 cells over x, y in +-10, z in
+    [0, 2] with sigma ~ U[2, 10] per cell (seeded), albedo 0.95, HG g = 0.9,
+    32 lights x 32 views on a 15 degree cone (a 1024^2 covariance), CBS on,
+    plane-patch first vertices (smpFlg 2)."""
+    from mcspeckle_amd import engines as E
+    rng = np.random.default_rng(seed)
+    grid = rng.uniform(2.0, 10.0, (8, 8, 4))
+
+    def cone(n):
+        ct = rng.uniform(np.cos(np.deg2rad(15)), 1.0, n)
+        st = np.sqrt(1 - ct ** 2)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        return np.vstack([st * np.cos(ph), st * np.sin(ph), ct])
+
+    l, v = cone(32), cone(32)
+    st = {}
+    t0 = time.perf_counter()
+    E.MCCovHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], l, v, 1, 1, n_paths, 1.0, True, 2, 3, 0.9,
+                 seed=0, precision=precision, stats=st)
+    dt = time.perf_counter() - t0
+    N = 32 * 32
+    return {"paths": n_paths, "n": N, "cells": list(grid.shape), "value": n_paths * N * N / dt,
+            "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_trace": st["ms_trace"],
+            "ms_nee": st["ms_nee"], "ms_accum": st["ms_accum"], "precision": precision}
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of `mib` MiB through the HIP runtime, the
     reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -275,6 +302,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--field-paths", type=int, default=1000000)
     ap.add_argument("--quad-paths", type=int, default=5000000)
+    ap.add_argument("--hetro-paths", type=int, default=200000)
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     args = ap.parse_args()
@@ -363,6 +391,7 @@ def main():
         out["mcfield_fourier_quad"] = quad_leg(args.quad_paths)
         out["mcfield_on_wave"] = field_wave_leg(args.field_paths)
         out["mcfield_fourier_quad_on_wave"] = quad_wave_leg(args.quad_paths)
+        out["mccov_hetro"] = hetro_leg(args.hetro_paths)
 
     # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
     if not args.no_serve:

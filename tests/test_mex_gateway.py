@@ -135,3 +135,39 @@ def test_quad_wave_gateway_equals_engine(mex):
                                     1.0, seed=key, precision="fp64")
     assert u.shape == (14, 14)
     np.testing.assert_allclose(u, u0, rtol=0, atol=1e-12 * np.abs(u0).max())
+
+
+def hetro_args():
+    grid = np.random.default_rng(1).uniform(0.5, 3.0, (2, 3, 2))
+    l = np.array([[0.0, 0.1], [0.0, 0.0], [1.0, np.sqrt(1 - 0.01)]])
+    v = np.array([[0.05, -0.05, 0.0], [0.0, 0.02, 0.1], [0.9987, 0.9985, 0.995]])
+    return [grid, 0.9, np.array([[-2.0], [-3.0], [0.0]]), np.array([[2.0], [3.0], [1.0]]),
+            np.array([[2.0], [2.0], [0.5]]), l, v, 1, 1, 300, 1.0, 1, 2, 3, 0.7]
+
+
+def test_hetro_gateway_argument_contract(mex):
+    a = hetro_args()
+    a[0] = a[0].reshape(2, 6, order="F")
+    _, err = mex.call("MCCovHetro", 3, *a[:14])
+    assert err[0] == "MATLAB:narginchk:notEnoughInputs"
+    _, err = mex.call("MCCovHetro", 3, *(a + [np.zeros((0, 0)), 1.0, 5]))
+    assert err[0] == "MATLAB:narginchk:tooManyInputs"
+    b = list(a)
+    b[0] = np.ones((2, 2))
+    assert mex.call("MCCovHetro", 3, *b)[1] == ("scmc:MCCovHetro", "sigt does not match the cell grid")
+
+
+@pytest.mark.gpu
+def test_hetro_gateway_equals_engine(mex):
+    a = hetro_args()
+    g = list(a)
+    g[0] = a[0].reshape(2, 6, order="F")   # the fake libmx holds 2-D arrays; numel and order are kept
+    key = mex.seed_for_next_call(31)
+    out, err = mex.call("MCCovHetro", 3, *g)
+    assert err is None, err
+    Ms, Mm, m1 = out
+    Ms0, Mm0, m10 = E.MCCovHetro(*a[:15], seed=key, precision="fp64")
+    assert Ms.shape == (3, 3, 2, 2) and m1.shape == (3, 2)
+    np.testing.assert_array_equal(Ms, Ms0)
+    np.testing.assert_array_equal(Mm, Mm0)
+    np.testing.assert_array_equal(m1, m10)
