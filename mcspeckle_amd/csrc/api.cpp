@@ -297,7 +297,6 @@ int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
     nl = ps.dp.nl;
     n = nv * nl;
     tile = mixed ? kTileMixed : kTileF64;
-    if (!bound.ensure(16, &rc)) return rc;
     if (ps.het) {
         if (!mean.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
         launch_zero(mean.p, sizeof(double2) * (size_t)n, s);
@@ -336,146 +335,203 @@ int CovEngine::ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s)
     return SCMC_OK;
 }
 
-int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee)
+// Producer half of a chunk (stream sp): NEE factors of the traced chunk into
+// the current slot and, in mixed precision, the FP16 hi/lo row expansion.
+// The slot's previous update must be finished before its buffers are
+// overwritten: its events are harvested first (host wait, at most two chunks
+// behind).
+int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
 {
     int rc = SCMC_OK;
+    CovSlot &S = slot[cur];
+    if (S.used && (rc = harvest(S))) return rc;
     const DevProblem &P = ps.dp;
     const int64_t np = tc.np, nvert = tc.nvert;
     const int64_t k2 = nvert - np;
     const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
     const bool cbs = P.do_cbs != 0;
-    if (!A1.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nv, &rc) || !B1.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nl, &rc) ||
-        !SA.ensure(cs * (size_t)np * nv, &rc) || !SB.ensure(cs * (size_t)np * nl, &rc) ||
-        !Gp.ensure(cs * (size_t)np * nl, &rc) || !maxG.ensure(sizeof(float) * (size_t)np, &rc) ||
-        !maxF2.ensure(sizeof(float) * (size_t)np, &rc))
+    const size_t kk = (size_t)std::max<int64_t>(k2, 1);
+    if (!S.A1.ensure(cs * kk * nv, &rc) || !S.B1.ensure(cs * kk * nl, &rc) || !S.SA.ensure(cs * (size_t)np * nv, &rc) ||
+        !S.SB.ensure(cs * (size_t)np * nl, &rc) || !S.Gp.ensure(cs * (size_t)np * nl, &rc) ||
+        !S.maxG.ensure(sizeof(float) * (size_t)np, &rc) || !S.maxF2.ensure(sizeof(float) * (size_t)np, &rc) ||
+        !S.bound.ensure(16, &rc))
         return rc;
     if (cbs)
-        if (!A2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nv, &rc) ||
-            !B2.ensure(cs * (size_t)std::max<int64_t>(k2, 1) * nl, &rc) || !Fp2.ensure(cs * (size_t)np * nv, &rc))
+        if (!S.A2.ensure(cs * kk * nv, &rc) || !S.B2.ensure(cs * kk * nl, &rc) || !S.Fp2.ensure(cs * (size_t)np * nv, &rc))
             return rc;
     const bool near = !(P.is_ff_l && P.is_ff_v);
     if (near)
-        if (!U.ensure(cs * (size_t)np * n, &rc) || !ones.ensure(cs * (size_t)np, &rc)) return rc;
-    launch_zero(bound.p, 16, s);
+        if (!S.U.ensure(cs * (size_t)np * n, &rc) || !S.ones.ensure(cs * (size_t)np, &rc)) return rc;
+    if (!S.ev[0])
+        for (auto &e : S.ev)
+            if (!hip_ok(hipEventCreate(&e), "event", &rc)) return rc;
+    if (!S.bound_host && !hip_ok(hipHostMalloc((void **)&S.bound_host, sizeof(unsigned)), "pinned bound", &rc))
+        return rc;
+    launch_zero(S.bound.p, 16, sp);
     CovFactors f;
-    f.A1 = A1.p;
-    f.B1 = B1.p;
-    f.A2 = cbs ? A2.p : nullptr;
-    f.B2 = cbs ? B2.p : nullptr;
-    f.SA = SA.p;
-    f.SB = SB.p;
-    f.Gp = Gp.p;
-    f.Fp2 = cbs ? Fp2.p : nullptr;
-    f.U = near ? U.p : nullptr;
-    f.ones = near ? ones.p : nullptr;
-    f.maxG = (float *)maxG.p;
-    f.maxF2 = (float *)maxF2.p;
-    f.bound = (unsigned *)bound.p;
-    hipEvent_t e0, e1, e2;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventCreate(&e2);
-    (void)hipEventRecord(e0, s);
+    f.A1 = S.A1.p;
+    f.B1 = S.B1.p;
+    f.A2 = cbs ? S.A2.p : nullptr;
+    f.B2 = cbs ? S.B2.p : nullptr;
+    f.SA = S.SA.p;
+    f.SB = S.SB.p;
+    f.Gp = S.Gp.p;
+    f.Fp2 = cbs ? S.Fp2.p : nullptr;
+    f.U = near ? S.U.p : nullptr;
+    f.ones = near ? S.ones.p : nullptr;
+    f.maxG = (float *)S.maxG.p;
+    f.maxF2 = (float *)S.maxF2.p;
+    f.bound = (unsigned *)S.bound.p;
     if (ps.het) {
-        if (!EV0.ensure(sizeof(double2) * (size_t)np * nv, &rc) || !UL0.ensure(sizeof(double2) * (size_t)np * nl, &rc))
+        if (!S.EV0.ensure(sizeof(double2) * (size_t)np * nv, &rc) || !S.UL0.ensure(sizeof(double2) * (size_t)np * nl, &rc))
             return rc;
-        f.EV0 = (double2 *)EV0.p;
-        f.UL0 = (double2 *)UL0.p;
+        f.EV0 = (double2 *)S.EV0.p;
+        f.UL0 = (double2 *)S.UL0.p;
     }
+    (void)hipEventRecord(S.ev[0], sp);
     launch_nee_cov(P, ps.het ? &ps.hm : nullptr, np, nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
-                   (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, f, mixed, s);
+                   (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, f, mixed, sp);
     if (ps.het)
-        launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p, s);
-    (void)hipEventRecord(e1, s);
-    const int ksm = pick_ksplit(k2, n, tile), kss = pick_ksplit(np, n, tile);
-    if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), s))) return rc;
-    if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), s))) return rc;
-    HerkProblem hp;
-    hp.nv = nv;
-    hp.nl = nl;
-    hp.n = n;
-    // multiple scattering
-    hp.k = k2;
-    hp.A1 = A1.p;
-    hp.B1 = B1.p;
-    hp.A2 = cbs ? A2.p : nullptr;
-    hp.B2 = cbs ? B2.p : nullptr;
-    hp.acc = (double2 *)acc_m.p;
-    hp.bound = (const unsigned *)bound.p;
-    hp.ksplit = ksm;
-    if (mixed && !zbuf.ensure(mixed_zbuf_bytes(std::max<int64_t>(k2, np), n), &rc)) return rc;
-    hp.zbuf = zbuf.p;
+        launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p, sp);
+    (void)hipEventRecord(S.ev[1], sp);
+    // the two rank-K problems of the chunk: multiple scattering, and single
+    // scattering as Kronecker factors (far field) or whole rows with a unit
+    // second factor (near field)
+    HerkProblem &hm = S.hp[0], &hs = S.hp[1];
+    hm = HerkProblem();
+    hm.nv = nv;
+    hm.nl = nl;
+    hm.n = n;
+    hm.k = k2;
+    hm.A1 = S.A1.p;
+    hm.B1 = S.B1.p;
+    hm.A2 = cbs ? S.A2.p : nullptr;
+    hm.B2 = cbs ? S.B2.p : nullptr;
+    hm.bound = (const unsigned *)S.bound.p;
+    hs = HerkProblem();
+    hs.nv = near ? n : nv;
+    hs.nl = near ? 1 : nl;
+    hs.n = n;
+    hs.k = np;
+    hs.A1 = near ? S.U.p : S.SA.p;
+    hs.B1 = near ? S.ones.p : S.SB.p;
+    hs.bound = (const unsigned *)S.bound.p + 1;
+    if (mixed) {
+        if (!S.zbuf_m.ensure(mixed_zbuf_bytes(std::max<int64_t>(k2, 1), n), &rc) ||
+            !S.zbuf_s.ensure(mixed_zbuf_bytes(std::max<int64_t>(np, 1), n), &rc))
+            return rc;
+        hm.zbuf = S.zbuf_m.p;
+        hs.zbuf = S.zbuf_s.p;
+        launch_expand_mixed(hm, sp);
+        launch_expand_mixed(hs, sp);
+    }
+    (void)hipEventRecord(S.ev[2], sp);
+    S.np = np;
+    S.k2 = k2;
+    return hip_ok(hipGetLastError(), "covariance factors", &rc) ? SCMC_OK : rc;
+}
+
+// Consumer half (stream sc): waits for the slot's factors, runs the
+// matrix-core updates into the accumulators (only ever written on sc, so
+// chunks accumulate in order), then moves to the other slot.
+int CovEngine::consume(hipStream_t sc)
+{
+    int rc = SCMC_OK;
+    CovSlot &S = slot[cur];
+    if (!hip_ok(hipStreamWaitEvent(sc, S.ev[2], 0), "wait factors", &rc)) return rc;
+    const int ksm = pick_ksplit(S.k2, n, tile), kss = pick_ksplit(S.np, n, tile);
+    if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), sc))) return rc;
+    if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), sc))) return rc;
     if (mixed && !order.p) {
         std::vector<int> o = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
         if (!order.ensure(sizeof(int) * o.size(), &rc)) return rc;
         if (!hip_ok(hipMemcpy(order.p, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice), "tile order", &rc))
             return rc;
     }
-    hp.order = order.p;
-    hipEvent_t h0, hm, h1;
-    (void)hipEventCreate(&h0);
-    (void)hipEventCreate(&hm);
-    (void)hipEventCreate(&h1);
-    (void)hipEventRecord(h0, s);
-    if (mixed) {
-        hp.ev_mid = hm;
-        launch_herk_mixed(hp, s);
-        hp.ev_mid = nullptr;
-    } else {
-        launch_herk_fp64(hp, s);
-        (void)hipEventRecord(hm, s);
-    }
-    (void)hipEventRecord(h1, s);
-    // single scattering: Kronecker factors (far field) or whole rows with a
-    // unit second factor (near field)
-    hp.k = np;
-    hp.A1 = near ? U.p : SA.p;
-    hp.B1 = near ? ones.p : SB.p;
-    if (near) {
-        hp.nv = n;
-        hp.nl = 1;
-    }
-    hp.A2 = nullptr;
-    hp.B2 = nullptr;
-    hp.acc = (double2 *)acc_s.p;
-    hp.bound = (const unsigned *)bound.p + 1;
-    hp.ksplit = kss;
+    HerkProblem &hm = S.hp[0], &hs = S.hp[1];
+    hm.acc = (double2 *)acc_m.p;
+    hm.ksplit = ksm;
+    hm.order = order.p;
+    hs.acc = (double2 *)acc_s.p;
+    hs.ksplit = kss;
+    hs.order = order.p;
+    (void)hipEventRecord(S.ev[3], sc);
     if (mixed)
-        launch_herk_mixed(hp, s);
+        launch_herk_mixed_update(hm, sc);
     else
-        launch_herk_fp64(hp, s);
-    (void)hipEventRecord(e2, s);
-    if (!hip_ok(hipGetLastError(), "covariance launch", &rc)) return rc;
-    if (!hip_ok(hipEventSynchronize(e2), "covariance sync", &rc)) return rc;
-    float t1 = 0, t2 = 0, th = 0, tx = 0;
-    (void)hipEventElapsedTime(&t1, e0, e1);
-    (void)hipEventElapsedTime(&t2, e1, e2);
-    (void)hipEventElapsedTime(&tx, h0, hm);
-    (void)hipEventElapsedTime(&th, mixed ? hm : h0, h1);
-    *ms_nee += t1;
-    *ms_accum += t2;
-    if (k2 > 0) {
+        launch_herk_fp64(hm, sc);
+    (void)hipEventRecord(S.ev[4], sc);
+    if (mixed)
+        launch_herk_mixed_update(hs, sc);
+    else
+        launch_herk_fp64(hs, sc);
+    // the scale bound goes to pinned memory on the stream: a blocking copy on
+    // the null stream would wait for the update in flight on the other stream
+    if (mixed && !hip_ok(hipMemcpyAsync(S.bound_host, S.bound.p, sizeof(unsigned), hipMemcpyDeviceToHost, sc),
+                         "copy bound", &rc))
+        return rc;
+    (void)hipEventRecord(S.ev[5], sc);
+    if (!hip_ok(hipGetLastError(), "covariance update", &rc)) return rc;
+    S.used = true;
+    cur ^= 1;
+    return SCMC_OK;
+}
+
+// Wait for a slot's events and add its timings to the statistics.
+int CovEngine::harvest(CovSlot &S)
+{
+    int rc = SCMC_OK;
+    if (!S.used) return SCMC_OK;
+    if (!hip_ok(hipEventSynchronize(S.ev[5]), "covariance sync", &rc)) return rc;
+    float tn = 0, tx = 0, th = 0, ts = 0;
+    (void)hipEventElapsedTime(&tn, S.ev[0], S.ev[1]);
+    (void)hipEventElapsedTime(&tx, S.ev[1], S.ev[2]);
+    (void)hipEventElapsedTime(&th, S.ev[3], S.ev[4]);
+    (void)hipEventElapsedTime(&ts, S.ev[4], S.ev[5]);
+    ms_nee += tn;
+    ms_accum += tx + th + ts;
+    if (S.k2 > 0) {
         ms_herk += th;
         if (mixed) ms_expand += tx;
         herk_launches += 1;
-        herk_flops += 8.0 * (double)n * (double)n * (double)k2;
+        herk_flops += 8.0 * (double)n * (double)n * (double)S.k2;
     }
-    (void)hipEventDestroy(h0);
-    (void)hipEventDestroy(hm);
-    (void)hipEventDestroy(h1);
     if (mixed) {
-        unsigned b = 0;
-        (void)hipMemcpy(&b, bound.p, sizeof b, hipMemcpyDeviceToHost);
+        const unsigned b = *S.bound_host;
         float bf;
         memcpy(&bf, &b, sizeof bf);
         int ex = 0;
         if (bf > 0) frexpf(bf, &ex);
         last_scale_log2 = 15 - ex;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
+    S.used = false;
     return SCMC_OK;
+}
+
+int CovEngine::drain()
+{
+    int rc = SCMC_OK;
+    for (auto &S : slot)
+        if ((rc = harvest(S))) return rc;
+    return SCMC_OK;
+}
+
+CovEngine::~CovEngine()
+{
+    for (auto &S : slot) {
+        for (auto &e : S.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (S.bound_host) (void)hipHostFree(S.bound_host);
+    }
+}
+
+// One chunk on one stream (path-set replay).
+int CovEngine::chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s)
+{
+    int rc = produce(ps, tc, s);
+    if (!rc) rc = consume(s);
+    if (!rc) rc = drain();
+    return rc;
 }
 
 int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
@@ -618,6 +674,7 @@ int Session::init(const scmc_problem *pr, int kind_, int precision, const double
     mixed = precision == SCMC_MIXED;
     pc.assign(pr);
     if (!hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
+    if (kind == 0 && !hip_ok(hipStreamCreate(&stream2), "stream", &rc)) return rc;
     if ((rc = ps.setup(&pc.pr, stream))) return rc;
     if (med && (rc = ps.setup_medium(&pc.pr, med, stream))) return rc;
     memset(&st, 0, sizeof st);
@@ -626,6 +683,9 @@ int Session::init(const scmc_problem *pr, int kind_, int precision, const double
 
 Session::~Session()
 {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -638,19 +698,28 @@ int Session::run(int64_t first, int64_t n)
         const int64_t np = kind == 0 ? chunk_paths(ce.n, rem, mean_len, mixed)
                                      : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
         auto a = std::chrono::steady_clock::now();
-        if ((rc = tc.run(ps.dp, (uint64_t)p0, np, stream, ps.het ? &ps.hm : nullptr))) return rc;
-        if (!hip_ok(hipStreamSynchronize(stream), "trace", &rc)) return rc;
+        hipStream_t ts = kind == 0 ? stream2 : stream;
+        if ((rc = tc.run(ps.dp, (uint64_t)p0, np, ts, ps.het ? &ps.hm : nullptr))) return rc;
+        if (!hip_ok(hipStreamSynchronize(ts), "trace", &rc)) return rc;
         st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-        if (kind == 0)
-            rc = ce.chunk(ps, tc, stream, &st.ms_accum, &st.ms_nee);
-        else
+        if (kind == 0) {
+            // producer on stream2 (trace above, factors, expansion), matrix-core
+            // update on stream: the next chunk is prepared while this one updates
+            if ((rc = ce.produce(ps, tc, stream2)) || (rc = ce.consume(stream))) return rc;
+        } else {
             rc = fe.chunk(ps, tc, mixed, stream, &st.ms_accum);
+        }
         if (rc) return rc;
         st.paths += np;
         st.vertices += tc.nvert;
         st.chunks += 1;
         mean_len = (double)st.vertices / (double)st.paths;
         p0 += np;
+    }
+    if (kind == 0) {
+        if ((rc = ce.drain())) return rc;
+        st.ms_accum = ce.ms_accum;
+        st.ms_nee = ce.ms_nee;
     }
     return SCMC_OK;
 }
@@ -843,9 +912,8 @@ int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t 
         ProblemState ps;
         TraceChunk tc;
         CovEngine ce;
-        double a = 0, b = 0;
         if (!(rc = ps.setup(pr, s)) && !(rc = ce.init(ps, precision == SCMC_MIXED, s)) && !(rc = tc.load(pset, s)) &&
-            !(rc = ce.chunk(ps, tc, s, &a, &b)))
+            !(rc = ce.chunk(ps, tc, s)))
             rc = ce.finalize(ps, pr, max_itr, ms_re, ms_im, mm_re, mm_im, s);
     }
     (void)hipStreamDestroy(s);

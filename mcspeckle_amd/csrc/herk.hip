@@ -93,51 +93,55 @@ __host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k +
 
 // Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
 // for K-step ks (vertices 8ks..8ks+7) and row i = v + nv*l, scaled by the
-// chunk's power of two.  One thread per (row, K-step).
+// chunk's power of two.  A workgroup covers 256 rows x kExpandSteps K-steps;
+// one thread per row, looping over the K-steps.
+constexpr int kExpandSteps = 16;
 template <bool CBS>
 __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
                                                            const float2 *__restrict__ B1, const float2 *__restrict__ A2,
                                                            const float2 *__restrict__ B2, const unsigned *__restrict__ bound,
                                                            uint4 *__restrict__ Z)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int npad = mixed::pad_rows(n);
     const int64_t nks = mixed::pad_ksteps(k);
-    if (idx >= nks * npad) return;
-    const int i = (int)(idx % npad);
-    const int64_t ks = idx / npad;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npad) return;
     const bool ok = i < n;
     const int v = ok ? i % nv : 0, l = ok ? i / nv : 0;
     float b = bound ? __uint_as_float(*bound) : 0.f;
     int ex = 0;
     if (b > 0.f) frexpf(b, &ex);
     const float scale = ldexpf(1.f, 15 - ex);
-    half8 h[2], lo[2];
+    const int64_t ks0 = (int64_t)blockIdx.y * kExpandSteps;
+    const int64_t ks1 = ks0 + kExpandSteps < nks ? ks0 + kExpandSteps : nks;
+    for (int64_t ks = ks0; ks < ks1; ++ks) {
+        half8 h[2], lo[2];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int64_t kk = ks * 8 + q;
-        float2 u = make_float2(0.f, 0.f);
-        if (ok && kk < k) {
-            u = fcmul(A1[kk * nv + v], B1[kk * nl + l]);
-            if (CBS) {
-                float2 u2 = fcmul(A2[kk * nv + v], B2[kk * nl + l]);
-                u.x += u2.x;
-                u.y += u2.y;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t kk = ks * 8 + q;
+            float2 u = make_float2(0.f, 0.f);
+            if (ok && kk < k) {
+                u = fcmul(A1[kk * nv + v], B1[kk * nl + l]);
+                if (CBS) {
+                    float2 u2 = fcmul(A2[kk * nv + v], B2[kk * nl + l]);
+                    u.x += u2.x;
+                    u.y += u2.y;
+                }
+                u.x *= scale;
+                u.y *= scale;
             }
-            u.x *= scale;
-            u.y *= scale;
+            _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
+            h[q >> 2][(q & 3) * 2] = hr;
+            h[q >> 2][(q & 3) * 2 + 1] = hi_;
+            lo[q >> 2][(q & 3) * 2] = (_Float16)(u.x - (float)hr);
+            lo[q >> 2][(q & 3) * 2 + 1] = (_Float16)(u.y - (float)hi_);
         }
-        _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
-        h[q >> 2][(q & 3) * 2] = hr;
-        h[q >> 2][(q & 3) * 2 + 1] = hi_;
-        lo[q >> 2][(q & 3) * 2] = (_Float16)(u.x - (float)hr);
-        lo[q >> 2][(q & 3) * 2 + 1] = (_Float16)(u.y - (float)hi_);
+        uint4 *d = Z + ((size_t)ks * npad + i) * 4;
+        d[0] = __builtin_bit_cast(uint4, h[0]);
+        d[1] = __builtin_bit_cast(uint4, h[1]);
+        d[2] = __builtin_bit_cast(uint4, lo[0]);
+        d[3] = __builtin_bit_cast(uint4, lo[1]);
     }
-    uint4 *d = Z + (ks * npad + i) * 4;
-    d[0] = __builtin_bit_cast(uint4, h[0]);
-    d[1] = __builtin_bit_cast(uint4, h[1]);
-    d[2] = __builtin_bit_cast(uint4, lo[0]);
-    d[3] = __builtin_bit_cast(uint4, lo[1]);
 }
 
 // The Hermitian update over pre-expanded rows: a streaming GEMM.  The rows
@@ -417,12 +421,11 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
 
 static int ntiles(int n, int tm) { return (n + tm - 1) / tm; }
 
-void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
+void launch_expand_mixed(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
     const int64_t nks = mixed::pad_ksteps(hp.k);
-    const int64_t nthr = nks * mixed::pad_rows(hp.n);
-    dim3 ge((unsigned)((nthr + 255) / 256));
+    dim3 ge((unsigned)((mixed::pad_rows(hp.n) + 255) / 256), (unsigned)((nks + kExpandSteps - 1) / kExpandSteps));
     if (hp.A2)
         hipLaunchKernelGGL(expand_mixed_kernel<true>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
                            (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,
@@ -431,7 +434,12 @@ void launch_herk_mixed(const HerkProblem &hp, hipStream_t s)
         hipLaunchKernelGGL(expand_mixed_kernel<false>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
                            (const float2 *)hp.A1, (const float2 *)hp.B1, nullptr, nullptr, hp.bound,
                            (uint4 *)hp.zbuf);
-    if (hp.ev_mid) (void)hipEventRecord(hp.ev_mid, s);
+}
+
+void launch_herk_mixed_update(const HerkProblem &hp, hipStream_t s)
+{
+    if (hp.k <= 0) return;
+    const int64_t nks = mixed::pad_ksteps(hp.k);
     const int nt = ntiles(hp.n, kTileMixed);
     const int ntile = nt * (nt + 1) / 2;
     const unsigned grid = (unsigned)(ntile * hp.ksplit);

@@ -71,7 +71,6 @@ struct HerkProblem {
     int ksplit;
     void *zbuf = nullptr;          // mixed: expanded FP16 hi/lo rows (mixed_zbuf_bytes)
     const void *order = nullptr;   // mixed: int2 tile order (blocked_tile_order)
-    hipEvent_t ev_mid = nullptr;   // mixed: recorded between expansion and update
 };
 // (ti, tj) pairs of the lower-triangle tiles in the blocked order used by
 // the mixed update, flattened.
@@ -81,7 +80,10 @@ size_t mixed_zbuf_bytes(int64_t k, int n);
 // Tile edge of the mixed and FP64 kernels (the finaliser needs it).
 constexpr int kTileMixed = 128;
 constexpr int kTileF64 = 64;
-void launch_herk_mixed(const HerkProblem &hp, hipStream_t s);
+// Mixed precision in two steps: FP16 hi/lo row expansion into hp.zbuf, then
+// the matrix-core update from hp.zbuf into hp.acc.
+void launch_expand_mixed(const HerkProblem &hp, hipStream_t s);
+void launch_herk_mixed_update(const HerkProblem &hp, hipStream_t s);
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the

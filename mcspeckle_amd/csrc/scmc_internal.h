@@ -51,20 +51,37 @@ struct TraceChunk {
     int load(const scmc_pathset *ps, hipStream_t s);
 };
 
+// Factor buffers of one in-flight chunk (two alternate, so the factors of
+// the next chunk are built while the current one updates the accumulators).
+struct CovSlot {
+    DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2, U, ones, EV0, UL0, bound, zbuf_m, zbuf_s;
+    hipEvent_t ev[6] = {};   // factors start / NEE done / expansion done; update start / multi done / single done
+    HerkProblem hp[2];       // multiple and single scattering
+    bool used = false;       // events recorded and not yet harvested
+    int64_t np = 0, k2 = 0;
+    unsigned *bound_host = nullptr;   // pinned copy of bound[0], written before ev[5]
+};
+
 // Covariance accumulation state across chunks.
 struct CovEngine {
     bool mixed = false;
     int nv = 0, nl = 0, n = 0, tile = 0;
     int ks_m = 0, ks_s = 0;
     int last_scale_log2 = 0;
-    double ms_herk = 0, herk_flops = 0, ms_expand = 0;
+    double ms_herk = 0, herk_flops = 0, ms_expand = 0, ms_accum = 0, ms_nee = 0;
     int64_t herk_launches = 0;
-    DevBuf acc_m, acc_s, bound, zbuf, order;
-    DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2, U, ones;
-    DevBuf EV0, UL0, mean;   // voxel medium: mean field (MCCovHetro's mean1)
+    DevBuf acc_m, acc_s, order;
+    DevBuf mean;   // voxel medium: mean field (MCCovHetro's mean1)
+    CovSlot slot[2];
+    int cur = 0;
+    ~CovEngine();
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
-    int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s, double *ms_accum, double *ms_nee);
+    int produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp);
+    int consume(hipStream_t sc);
+    int harvest(CovSlot &S);
+    int drain();
+    int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s);
     int finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
                  double *mm_re, double *mm_im, hipStream_t s);
     int finalize_mean(int64_t max_itr, double *re, double *im, hipStream_t s);
@@ -92,7 +109,7 @@ struct Session {
     int kind = 0;   // 0 MCcov, 1 MCfield
     bool mixed = false;
     ProblemCopy pc;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: covariance producer
     ProblemState ps;
     TraceChunk tc;
     CovEngine ce;
