@@ -394,6 +394,7 @@ int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
     if (ps.het)
         launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p, sp);
     (void)hipEventRecord(S.ev[1], sp);
+    nee_done = S.ev[1];
     // the two rank-K problems of the chunk: multiple scattering, and single
     // scattering as Kronecker factors (far field) or whole rows with a unit
     // second factor (near field)
@@ -699,13 +700,17 @@ int Session::run(int64_t first, int64_t n)
                                      : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
         auto a = std::chrono::steady_clock::now();
         hipStream_t ts = kind == 0 ? stream2 : stream;
+        // the trace overwrites the path records the previous chunk's factors read
+        if (kind == 0 && ce.nee_done && !hip_ok(hipStreamWaitEvent(ts, ce.nee_done, 0), "wait factors", &rc)) return rc;
         if ((rc = tc.run(ps.dp, (uint64_t)p0, np, ts, ps.het ? &ps.hm : nullptr))) return rc;
         if (!hip_ok(hipStreamSynchronize(ts), "trace", &rc)) return rc;
         st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         if (kind == 0) {
-            // producer on stream2 (trace above, factors, expansion), matrix-core
-            // update on stream: the next chunk is prepared while this one updates
-            if ((rc = ce.produce(ps, tc, stream2)) || (rc = ce.consume(stream))) return rc;
+            // the tracer (a narrow kernel: one thread per path) runs on stream2
+            // beside the matrix-core update of the previous chunk; the factors,
+            // the expansion and the update fill the whole GPU and follow each
+            // other on stream
+            if ((rc = ce.produce(ps, tc, stream)) || (rc = ce.consume(stream))) return rc;
         } else {
             rc = fe.chunk(ps, tc, mixed, stream, &st.ms_accum);
         }

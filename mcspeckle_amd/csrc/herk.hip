@@ -91,10 +91,11 @@ __host__ __device__ __forceinline__ int pad_rows(int n) { return (n + TM - 1) / 
 __host__ __device__ __forceinline__ int64_t pad_ksteps(int64_t k) { return ((k + 7) / 8 + 3) / 4 * 4; }
 } // namespace mixed
 
-// Expansion: Z[ks][i] = 64 bytes = hi (8 vertices x (re,im) FP16) | lo (same),
-// for K-step ks (vertices 8ks..8ks+7) and row i = v + nv*l, scaled by the
-// chunk's power of two.  A workgroup covers 256 rows x kExpandSteps K-steps;
-// one thread per row, looping over the K-steps.
+// Expansion: for K-step ks (vertices 8ks..8ks+7) and row i = v + nv*l, plane
+// 0 holds hi and plane 1 lo, 32 bytes each (8 vertices x (re, im) FP16), at
+// Z[ks][plane][i], scaled by the chunk's power of two.  A workgroup covers
+// 128 rows x kExpandSteps K-steps; two threads per row, each forming four
+// vertices, so every store instruction of a wave writes 1 KB contiguously.
 constexpr int kExpandSteps = 16;
 template <bool CBS>
 __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
 {
     const int npad = mixed::pad_rows(n);
     const int64_t nks = mixed::pad_ksteps(k);
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.x * 128 + (threadIdx.x >> 1), hf = threadIdx.x & 1;
     if (i >= npad) return;
     const bool ok = i < n;
     const int v = ok ? i % nv : 0, l = ok ? i / nv : 0;
@@ -115,10 +116,10 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
     const int64_t ks0 = (int64_t)blockIdx.y * kExpandSteps;
     const int64_t ks1 = ks0 + kExpandSteps < nks ? ks0 + kExpandSteps : nks;
     for (int64_t ks = ks0; ks < ks1; ++ks) {
-        half8 h[2], lo[2];
+        half8 h, lo;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int64_t kk = ks * 8 + q;
+        for (int q = 0; q < 4; ++q) {
+            const int64_t kk = ks * 8 + hf * 4 + q;
             float2 u = make_float2(0.f, 0.f);
             if (ok && kk < k) {
                 u = fcmul(A1[kk * nv + v], B1[kk * nl + l]);
@@ -130,17 +131,15 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
                 u.x *= scale;
                 u.y *= scale;
             }
-            _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
-            h[q >> 2][(q & 3) * 2] = hr;
-            h[q >> 2][(q & 3) * 2 + 1] = hi_;
-            lo[q >> 2][(q & 3) * 2] = (_Float16)(u.x - (float)hr);
-            lo[q >> 2][(q & 3) * 2 + 1] = (_Float16)(u.y - (float)hi_);
+            const _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
+            h[2 * q] = hr;
+            h[2 * q + 1] = hi_;
+            lo[2 * q] = (_Float16)(u.x - (float)hr);
+            lo[2 * q + 1] = (_Float16)(u.y - (float)hi_);
         }
-        uint4 *d = Z + ((size_t)ks * npad + i) * 4;
-        d[0] = __builtin_bit_cast(uint4, h[0]);
-        d[1] = __builtin_bit_cast(uint4, h[1]);
-        d[2] = __builtin_bit_cast(uint4, lo[0]);
-        d[3] = __builtin_bit_cast(uint4, lo[1]);
+        uint4 *d = Z + ((size_t)ks * 2 * npad + i) * 2 + hf;
+        d[0] = __builtin_bit_cast(uint4, h);
+        d[(size_t)npad * 2] = __builtin_bit_cast(uint4, lo);
     }
 }
 
@@ -184,11 +183,11 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
     // Loading role: wave w moves tile T = w/2 (rows of ti, then columns of
     // tj; a diagonal tile loads the same rows twice), plane P = w%2, four
     // chunks of 32 rows x 32 B per K-step.  Lane l covers row l/2, 16 B
-    // half l%2: the chunk lands contiguously at M0 + 16 l, which is exactly
-    // the [row][32 B] layout of the stage.
+    // half l%2: each chunk is 1 KB contiguous in the plane and lands at
+    // M0 + 16 l, which is exactly the [row][32 B] layout of the stage.
     const int T = wave >> 1, P = wave & 1;
-    const uint4 *src = Z + (size_t)((T ? tj : ti) * TM + (lane >> 1)) * 4 + P * 2 + (lane & 1);
-    const size_t kstride = (size_t)npad * 4;   // uint4 per K-step
+    const uint4 *src = Z + ((size_t)P * npad + (size_t)((T ? tj : ti) * TM) + (lane >> 1)) * 2 + (lane & 1);
+    const size_t kstride = (size_t)npad * 4;   // uint4 per K-step (two planes)
     const int ldst = loff(T, P, 0, 0);
     typedef __attribute__((address_space(3))) void *lds_ptr;
 #define SCMC_ISSUE(kstep, buf)                                                                           \
@@ -196,9 +195,9 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
         const uint4 *g = src + (size_t)(kstep) * kstride;                                                \
         unsigned char *d = lds + (buf) * STAGEB + ldst;                                                  \
         __builtin_amdgcn_global_load_lds((const void *)g, (lds_ptr)d, 16, 0, 0);                         \
-        __builtin_amdgcn_global_load_lds((const void *)(g + 128), (lds_ptr)(d + 1024), 16, 0, 0);       \
-        __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_ptr)(d + 2048), 16, 0, 0);       \
-        __builtin_amdgcn_global_load_lds((const void *)(g + 384), (lds_ptr)(d + 3072), 16, 0, 0);       \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 64), (lds_ptr)(d + 1024), 16, 0, 0);        \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 128), (lds_ptr)(d + 2048), 16, 0, 0);       \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 192), (lds_ptr)(d + 3072), 16, 0, 0);       \
     } while (0)
 
     const int wr = wave >> 1, wc = wave & 1;
@@ -425,7 +424,7 @@ void launch_expand_mixed(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
     const int64_t nks = mixed::pad_ksteps(hp.k);
-    dim3 ge((unsigned)((mixed::pad_rows(hp.n) + 255) / 256), (unsigned)((nks + kExpandSteps - 1) / kExpandSteps));
+    dim3 ge((unsigned)(mixed::pad_rows(hp.n) / 128), (unsigned)((nks + kExpandSteps - 1) / kExpandSteps));
     if (hp.A2)
         hipLaunchKernelGGL(expand_mixed_kernel<true>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
                            (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,

@@ -17,6 +17,8 @@
 #include "hetero.h"
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace scmc {
 
 template <typename T> __device__ __forceinline__ T to_c(double2 v);
@@ -157,57 +159,76 @@ __global__ __launch_bounds__(256) void nee_single_near_kernel(DevProblem P, int6
 }
 
 // MCcov.m:205-222 : vertex-level factors for pL > 1, written at the
-// multiple-scattering index m = vertex - path - 1 (path-major order).
+// multiple-scattering index m = vertex - path - 1 (path-major order).  A
+// workgroup takes vpb consecutive vertices and spreads their nv view items
+// and nl light items over its threads; the per-vertex maxima meet in LDS and
+// the workgroup adds one value to the chunk bound.
+constexpr int kNeeVertexThreads = 256, kNeeMaxVpb = 64;
 template <int DIM, typename T, bool HET>
-__global__ __launch_bounds__(128) void nee_cov_vertex_kernel(DevProblem P, HetMedium M, int64_t nvert,
-                                                             const PathRec *paths,
-                                                             const VertexRec *verts, const int32_t *vpath,
-                                                             const int64_t *offsets, const T *Gp, const T *Fp2,
-                                                             const float *maxG, const float *maxF2, T *A1, T *B1,
-                                                             T *A2, T *B2, unsigned *bound)
+__global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
+    DevProblem P, HetMedium M, int64_t nvert, int vpb, const PathRec *paths, const VertexRec *verts,
+    const int32_t *vpath, const int64_t *offsets, const T *Gp, const T *Fp2, const float *maxG, const float *maxF2,
+    T *A1, T *B1, T *A2, T *B2, unsigned *bound)
 {
-    __shared__ float sh[4];
-    const int64_t vi = blockIdx.x;
-    if (vi >= nvert) return;
-    const int p = vpath[vi];
-    if (vi == offsets[p]) return;   // first vertex: single scattering
-    const int64_t m = vi - p - 1;
-    const VertexRec r = verts[vi];
-    const PathRec pr = paths[p];
-    const double sw = sqrt(r.weight / pr.px);
-    float mA = 0.f, mB = 0.f;
-    for (int i = threadIdx.x; i < P.nv; i += blockDim.x) {
-        const double *vc = P.v + (size_t)i * DIM;
-        double rv[3];
-        view_dir<DIM>(P, pr.x1, i, rv);
-        double2 e = nee_e<DIM, HET>(P, M, r.x, vc, P.ff_sign, P.is_ff_v);
-        double2 af = eval_amp<DIM>(dot<DIM>(r.ow, rv), P.amp);
-        double2 f = cmul(e, af);
-        f.x *= sw;
-        f.y *= sw;
-        A1[m * P.nv + i] = to_c<T>(f);
-        mA = fmaxf(mA, cmag(f));
-        if (P.do_cbs) A2[m * P.nv + i] = Fp2[(int64_t)p * P.nv + i];
-    }
-    for (int i = threadIdx.x; i < P.nl; i += blockDim.x) {
-        B1[m * P.nl + i] = Gp[(int64_t)p * P.nl + i];
-        if (P.do_cbs) {
-            const double *lc = P.l + (size_t)i * DIM;
-            double rl[3];
-            light_dir<DIM>(P, pr.x1, i, rl);
-            double2 e = nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l);
-            double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
-            double2 af = eval_amp<DIM>(dot<DIM>(mow, rl), P.amp);
-            double2 g = cmul(e, af);
-            g.x *= sw;
-            g.y *= sw;
-            B2[m * P.nl + i] = to_c<T>(g);
-            mB = fmaxf(mB, cmag(g));
+    __shared__ unsigned mAv[kNeeMaxVpb], mBv[kNeeMaxVpb];
+    __shared__ float sh[kNeeVertexThreads / 64];
+    const int64_t v0 = (int64_t)blockIdx.x * vpb;
+    for (int j = threadIdx.x; j < vpb; j += blockDim.x) mAv[j] = mBv[j] = 0u;
+    __syncthreads();
+    const int w = P.nv + P.nl;
+    for (int j = threadIdx.x; j < vpb * w; j += blockDim.x) {
+        const int vl = j / w, i = j - vl * w;
+        const int64_t vi = v0 + vl;
+        if (vi >= nvert) continue;
+        const int p = vpath[vi];
+        if (vi == offsets[p]) continue;   // first vertex: single scattering
+        const int64_t m = vi - p - 1;
+        const VertexRec r = verts[vi];
+        const PathRec pr = paths[p];
+        const double sw = sqrt(r.weight / pr.px);
+        if (i < P.nv) {
+            const double *vc = P.v + (size_t)i * DIM;
+            double rv[3];
+            view_dir<DIM>(P, pr.x1, i, rv);
+            double2 e = nee_e<DIM, HET>(P, M, r.x, vc, P.ff_sign, P.is_ff_v);
+            double2 af = eval_amp<DIM>(dot<DIM>(r.ow, rv), P.amp);
+            double2 f = cmul(e, af);
+            f.x *= sw;
+            f.y *= sw;
+            A1[m * P.nv + i] = to_c<T>(f);
+            const float mg = cmag(f);
+            if (mg > 0.f) atomicMax(&mAv[vl], __float_as_uint(mg));
+            if (P.do_cbs) A2[m * P.nv + i] = Fp2[(int64_t)p * P.nv + i];
+        } else {
+            const int li = i - P.nv;
+            B1[m * P.nl + li] = Gp[(int64_t)p * P.nl + li];
+            if (P.do_cbs) {
+                const double *lc = P.l + (size_t)li * DIM;
+                double rl[3];
+                light_dir<DIM>(P, pr.x1, li, rl);
+                double2 e = nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l);
+                double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
+                double2 af = eval_amp<DIM>(dot<DIM>(mow, rl), P.amp);
+                double2 g = cmul(e, af);
+                g.x *= sw;
+                g.y *= sw;
+                B2[m * P.nl + li] = to_c<T>(g);
+                const float mg = cmag(g);
+                if (mg > 0.f) atomicMax(&mBv[vl], __float_as_uint(mg));
+            }
         }
     }
-    mA = block_max(mA, sh);
-    mB = block_max(mB, sh);
-    if (threadIdx.x == 0) atomic_max_pos(&bound[0], 2.f * (mA * maxG[p] + maxF2[p] * mB));
+    __syncthreads();
+    float bv = 0.f;
+    for (int j = threadIdx.x; j < vpb; j += blockDim.x) {
+        const int64_t vi = v0 + j;
+        if (vi >= nvert) continue;
+        const int p = vpath[vi];
+        if (vi == offsets[p]) continue;
+        bv = fmaxf(bv, 2.f * (__uint_as_float(mAv[j]) * maxG[p] + maxF2[p] * __uint_as_float(mBv[j])));
+    }
+    bv = block_max(bv, sh);
+    if (threadIdx.x == 0) atomic_max_pos(&bound[0], bv);
 }
 
 template <int DIM, typename T, bool HET>
@@ -222,10 +243,14 @@ static void nee_cov_dispatch(const DevProblem &P, const HetMedium &M, int64_t np
     if (np > 0 && !(P.is_ff_l && P.is_ff_v))
         hipLaunchKernelGGL((nee_single_near_kernel<DIM, T>), dim3((unsigned)np), dim3(256), 0, s, P, np, paths,
                            (const T *)f.SA, (const T *)f.SB, (T *)f.U, (T *)f.ones, f.bound);
-    if (nvert > 0)
-        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET>), dim3((unsigned)nvert), dim3(128), 0, s, P, M, nvert,
-                           paths, verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2,
-                           (T *)f.A1, (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound);
+    if (nvert > 0) {
+        // about four items per thread
+        const int vpb = std::max(1, std::min(kNeeMaxVpb, 4 * kNeeVertexThreads / (P.nv + P.nl)));
+        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET>), dim3((unsigned)((nvert + vpb - 1) / vpb)),
+                           dim3(kNeeVertexThreads), 0, s, P, M, nvert, vpb, paths, verts, vpath, offsets,
+                           (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1, (T *)f.B1, (T *)f.A2,
+                           (T *)f.B2, f.bound);
+    }
 }
 
 void launch_nee_cov(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,

@@ -74,6 +74,7 @@ struct CovEngine {
     DevBuf mean;   // voxel medium: mean field (MCCovHetro's mean1)
     CovSlot slot[2];
     int cur = 0;
+    hipEvent_t nee_done = nullptr;   // the last produced chunk has read its path records
     ~CovEngine();
     int init(const ProblemState &ps, bool mixed, hipStream_t s);
     int ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s);
@@ -109,7 +110,7 @@ struct Session {
     int kind = 0;   // 0 MCcov, 1 MCfield
     bool mixed = false;
     ProblemCopy pc;
-    hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: covariance producer
+    hipStream_t stream = nullptr, stream2 = nullptr;   // stream2: covariance tracer
     ProblemState ps;
     TraceChunk tc;
     CovEngine ce;
