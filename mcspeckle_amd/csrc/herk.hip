@@ -143,6 +143,69 @@ __global__ __launch_bounds__(256) void expand_mixed_kernel(int nv, int nl, int n
     }
 }
 
+// The same expansion, one K-step per workgroup over all rows: the eight
+// vertices' factor vectors are staged in LDS once, then the K-step's block of
+// both planes is written front to back.  For factor vectors that fit in
+// kExpandStepLds bytes (launch_expand_mixed picks the kernel).
+constexpr int kExpandStepLds = 64 * 1024;
+template <bool CBS>
+__global__ __launch_bounds__(256) void expand_step_kernel(int nv, int nl, int n, int64_t k, const float2 *__restrict__ A1,
+                                                          const float2 *__restrict__ B1, const float2 *__restrict__ A2,
+                                                          const float2 *__restrict__ B2, const unsigned *__restrict__ bound,
+                                                          uint4 *__restrict__ Z)
+{
+    extern __shared__ float2 fs[];
+    float2 *a1 = fs, *b1 = a1 + 8 * nv, *a2 = b1 + 8 * nl, *b2 = a2 + 8 * nv;
+    const int npad = mixed::pad_rows(n);
+    const int64_t ks = blockIdx.x;
+    for (int j = threadIdx.x; j < 8 * nv; j += blockDim.x) {
+        const int q = j / nv, v = j - q * nv;
+        const int64_t kk = ks * 8 + q;
+        a1[j] = kk < k ? A1[kk * nv + v] : make_float2(0.f, 0.f);
+        if (CBS) a2[j] = kk < k ? A2[kk * nv + v] : make_float2(0.f, 0.f);
+    }
+    for (int j = threadIdx.x; j < 8 * nl; j += blockDim.x) {
+        const int q = j / nl, l = j - q * nl;
+        const int64_t kk = ks * 8 + q;
+        b1[j] = kk < k ? B1[kk * nl + l] : make_float2(0.f, 0.f);
+        if (CBS) b2[j] = kk < k ? B2[kk * nl + l] : make_float2(0.f, 0.f);
+    }
+    float b = bound ? __uint_as_float(*bound) : 0.f;
+    int ex = 0;
+    if (b > 0.f) frexpf(b, &ex);
+    const float scale = ldexpf(1.f, 15 - ex);
+    __syncthreads();
+    const int hf = threadIdx.x & 1;
+    uint4 *zk = Z + (size_t)ks * 4 * npad + hf;
+    for (int i = threadIdx.x >> 1; i < npad; i += blockDim.x >> 1) {
+        half8 h, lo;
+        const bool ok = i < n;
+        const int l = ok ? i / nv : 0, v = ok ? i - l * nv : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int qq = hf * 4 + q;
+            float2 u = make_float2(0.f, 0.f);
+            if (ok) {
+                u = fcmul(a1[qq * nv + v], b1[qq * nl + l]);
+                if (CBS) {
+                    float2 u2 = fcmul(a2[qq * nv + v], b2[qq * nl + l]);
+                    u.x += u2.x;
+                    u.y += u2.y;
+                }
+                u.x *= scale;
+                u.y *= scale;
+            }
+            const _Float16 hr = (_Float16)u.x, hi_ = (_Float16)u.y;
+            h[2 * q] = hr;
+            h[2 * q + 1] = hi_;
+            lo[2 * q] = (_Float16)(u.x - (float)hr);
+            lo[2 * q + 1] = (_Float16)(u.y - (float)hi_);
+        }
+        zk[(size_t)i * 2] = __builtin_bit_cast(uint4, h);
+        zk[((size_t)npad + i) * 2] = __builtin_bit_cast(uint4, lo);
+    }
+}
+
 // The Hermitian update over pre-expanded rows: a streaming GEMM.  The rows
 // of each K-step go from HBM straight into LDS (global_load_lds_dwordx4, no
 // register staging), NBUF - 1 K-steps ahead of the matrix cores; each wave
@@ -424,6 +487,18 @@ void launch_expand_mixed(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
     const int64_t nks = mixed::pad_ksteps(hp.k);
+    const size_t lds = sizeof(float2) * 8 * (size_t)(hp.nv + hp.nl) * (hp.A2 ? 2 : 1);
+    if (lds <= (size_t)kExpandStepLds) {
+        if (hp.A2)
+            hipLaunchKernelGGL(expand_step_kernel<true>, dim3((unsigned)nks), dim3(256), lds, s, hp.nv, hp.nl, hp.n,
+                               hp.k, (const float2 *)hp.A1, (const float2 *)hp.B1, (const float2 *)hp.A2,
+                               (const float2 *)hp.B2, hp.bound, (uint4 *)hp.zbuf);
+        else
+            hipLaunchKernelGGL(expand_step_kernel<false>, dim3((unsigned)nks), dim3(256), lds, s, hp.nv, hp.nl, hp.n,
+                               hp.k, (const float2 *)hp.A1, (const float2 *)hp.B1, nullptr, nullptr, hp.bound,
+                               (uint4 *)hp.zbuf);
+        return;
+    }
     dim3 ge((unsigned)(mixed::pad_rows(hp.n) / 128), (unsigned)((nks + kExpandSteps - 1) / kExpandSteps));
     if (hp.A2)
         hipLaunchKernelGGL(expand_mixed_kernel<true>, ge, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
