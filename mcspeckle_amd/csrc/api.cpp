@@ -696,8 +696,10 @@ int Session::run(int64_t first, int64_t n)
     if (first < 0 || n < 0) return set_error(SCMC_E_ARG, "invalid path range");
     for (int64_t p0 = first; p0 < first + n;) {
         const int64_t rem = first + n - p0;
-        const int64_t np = kind == 0 ? chunk_paths(ce.n, rem, mean_len, mixed)
-                                     : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
+        const int64_t cap = kind == 0 ? chunk_paths(ce.n, rem, mean_len, mixed)
+                                      : field_chunk_paths(pc.pr.nv, pc.pr.nl, rem, mean_len);
+        // equal chunks rather than full ones and a short tail
+        const int64_t nch = (rem + cap - 1) / cap, np = (rem + nch - 1) / nch;
         auto a = std::chrono::steady_clock::now();
         hipStream_t ts = kind == 0 ? stream2 : stream;
         // the trace overwrites the path records the previous chunk's factors read
