@@ -168,13 +168,16 @@ struct FEntry {
     double x[3];
     double ad[3];     // amplitude direction: ow (vertex) or -w1 (reversed)
     double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
+    float dlo[3];     // x - box_min and box_max - x, rounded to FP32 (mixed
+    float dhi[3];     // attenuation: the distance to the face a view leaves by)
     int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
     int p;            // path of the entry
 };
 
 template <int DIM>
-__device__ __forceinline__ void load_entry(int64_t e, int64_t nvert, const PathRec *paths, const VertexRec *verts,
-                                           const int32_t *vpath, const int64_t *offsets, FEntry &f)
+__device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64_t nvert, const PathRec *paths,
+                                           const VertexRec *verts, const int32_t *vpath, const int64_t *offsets,
+                                           FEntry &f)
 {
     if (e < nvert) {
         const VertexRec r = verts[e];
@@ -196,6 +199,10 @@ __device__ __forceinline__ void load_entry(int64_t e, int64_t nvert, const PathR
         f.p = (int)(e - nvert);
         double den = pr.w0p.x * pr.w0p.x + pr.w0p.y * pr.w0p.y;
         f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
+    }
+    for (int j = 0; j < 3; ++j) {
+        f.dlo[j] = j < DIM ? (float)(f.x[j] - P.bmin[j]) : 0.f;
+        f.dhi[j] = j < DIM ? (float)(P.bmax[j] - f.x[j]) : 0.f;
     }
 }
 
@@ -224,8 +231,9 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
 }
 
 // WAVE: one output column; coef/coefR hold the light-summed coefficients and
-// the first-vertex entry contracts af_ang_vl(v, :) with cl0[p, :].
-template <int DIM, bool MIXED, bool WAVE>
+// the first-vertex entry contracts af_ang_vl(v, :) with cl0[p, :].  NLQ:
+// light columns accumulated per pass (1 for one column, else kFieldNL).
+template <int DIM, bool MIXED, bool WAVE, int NLQ>
 __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
@@ -254,23 +262,24 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
         inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
         up[j] = nd > 0;
     }
-    double2 afv[kFieldNL];
-    for (int q = 0; q < kFieldNL; ++q)
+    double2 afv[NLQ];
+    for (int q = 0; q < NLQ; ++q)
         afv[q] = (!WAVE && vok && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)] : make_double2(0.0, 0.0);
     const float g = (float)P.amp.g;
     const float hg_k = (float)sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
     const float g2p1 = 1.f + g * g, g2 = 2.f * g;
     const float hs = (float)(P.sigt * 0.5);
 
-    double2 acc[kFieldNL];
-    for (int q = 0; q < kFieldNL; ++q) acc[q] = make_double2(0.0, 0.0);
+    double2 acc[NLQ];
+    for (int q = 0; q < NLQ; ++q) acc[q] = make_double2(0.0, 0.0);
 
     for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
         const int cnt = (int)((e1 - eb) < kFieldBatch ? (e1 - eb) : kFieldBatch);
         __syncthreads();
-        if (threadIdx.x < cnt) load_entry<DIM>(eb + threadIdx.x, nvert, paths, verts, vpath, offsets, se[threadIdx.x]);
-        for (int t = threadIdx.x; t < cnt * kFieldNL; t += blockDim.x) {
-            const int ei = t / kFieldNL, q = t % kFieldNL;
+        if (threadIdx.x < cnt)
+            load_entry<DIM>(P, eb + threadIdx.x, nvert, paths, verts, vpath, offsets, se[threadIdx.x]);
+        for (int t = threadIdx.x; t < cnt * NLQ; t += blockDim.x) {
+            const int ei = t / NLQ, q = t % NLQ;
             const int64_t e = eb + ei;
             double2 c = make_double2(0.0, 0.0);
             if (q < nlp)
@@ -296,20 +305,17 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                 continue;
             }
             double2 ev;
-            float2 am = make_float2(1.f, 0.f);
             if (MIXED) {
                 double pv = dot<DIM>(f.x, mv);
                 double cyc = pv * P.inv_lambda;
                 float fr = (float)(cyc - rint(cyc));
                 float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
                 float best = INFINITY;
-                for (int j = 0; j < DIM; ++j) {
-                    float dist = (float)(up[j] ? (P.bmax[j] - f.x[j]) : (f.x[j] - P.bmin[j]));
-                    best = fminf(best, dist * inv[j]);
-                }
+                for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? f.dhi[j] : f.dlo[j]) * inv[j]);
                 float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
-                ev = make_double2(mag * cs, mag * sn);
+                float2 evf = make_float2(mag * cs, mag * sn);
                 if (f.kind != 0) {
+                    float2 am;
                     if (P.amp.type == 2) {
                         am = amp_tab<DIM>(dot<DIM>(f.ad, vd), P.amp);
                     } else {
@@ -317,18 +323,18 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                                   (DIM == 3 ? (float)f.ad[2] * vf[2] : 0.f);
                         am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
                     }
+                    evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
                 }
+                ev = make_double2(evf.x, evf.y);
             } else {
                 ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
                 if (f.kind != 0) {
                     double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)v * DIM), P.amp);
-                    am = make_float2(0.f, 0.f);
                     ev = cmul(ev, a);
                 }
             }
-            if (MIXED && f.kind != 0) ev = cmul(ev, make_double2(am.x, am.y));
             if (f.kind == 2) ev = cmul(ev, f.div);
-            for (int q = 0; q < kFieldNL; ++q) {
+            for (int q = 0; q < NLQ; ++q) {
                 double2 t = (f.kind == 0) ? cmul(ev, afv[q]) : ev;
                 double2 c = sc[ei][q];
                 acc[q].x += t.x * c.x - t.y * c.y;
@@ -522,14 +528,18 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
                 SCMC_FIELD_NEAR((field_near_kernel<DIM, false>));
         } else if (mixed) {
             if (wave)
-                SCMC_FIELD((field_kernel<DIM, true, true>));
+                SCMC_FIELD((field_kernel<DIM, true, true, 1>));
+            else if (ncol == 1)
+                SCMC_FIELD((field_kernel<DIM, true, false, 1>));
             else
-                SCMC_FIELD((field_kernel<DIM, true, false>));
+                SCMC_FIELD((field_kernel<DIM, true, false, kFieldNL>));
         } else {
             if (wave)
-                SCMC_FIELD((field_kernel<DIM, false, true>));
+                SCMC_FIELD((field_kernel<DIM, false, true, 1>));
+            else if (ncol == 1)
+                SCMC_FIELD((field_kernel<DIM, false, false, 1>));
             else
-                SCMC_FIELD((field_kernel<DIM, false, false>));
+                SCMC_FIELD((field_kernel<DIM, false, false, kFieldNL>));
         }
 #undef SCMC_FIELD
 #undef SCMC_FIELD_NEAR
