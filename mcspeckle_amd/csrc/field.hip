@@ -170,6 +170,7 @@ struct FEntry {
     double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
     float dlo[3];     // x - box_min and box_max - x, rounded to FP32 (mixed
     float dhi[3];     // attenuation: the distance to the face a view leaves by)
+    float adf[3];     // ad rounded to FP32 (mixed HG amplitude)
     int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
     int p;            // path of the entry
 };
@@ -203,6 +204,7 @@ __device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64
     for (int j = 0; j < 3; ++j) {
         f.dlo[j] = j < DIM ? (float)(f.x[j] - P.bmin[j]) : 0.f;
         f.dhi[j] = j < DIM ? (float)(P.bmax[j] - f.x[j]) : 0.f;
+        f.adf[j] = (float)f.ad[j];
     }
 }
 
@@ -290,7 +292,9 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
         if (!vok) continue;
         for (int ei = 0; ei < cnt; ++ei) {
             const FEntry &f = se[ei];
-            if (WAVE && f.kind == 0) {
+            // every lane reads the same entry: branch on its kind as a scalar
+            const int kind = __builtin_amdgcn_readfirstlane(f.kind);
+            if (WAVE && kind == 0) {
                 // sum_l af_ang_vl(v, l) cl0[p, l], then the first-vertex E(x1, v)
                 double2 a = make_double2(0.0, 0.0);
                 for (int l = 0; l < P.nl; ++l) {
@@ -314,13 +318,12 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                 for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? f.dhi[j] : f.dlo[j]) * inv[j]);
                 float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
                 float2 evf = make_float2(mag * cs, mag * sn);
-                if (f.kind != 0) {
+                if (kind != 0) {
                     float2 am;
                     if (P.amp.type == 2) {
                         am = amp_tab<DIM>(dot<DIM>(f.ad, vd), P.amp);
                     } else {
-                        float c = (float)f.ad[0] * vf[0] + (float)f.ad[1] * vf[1] +
-                                  (DIM == 3 ? (float)f.ad[2] * vf[2] : 0.f);
+                        float c = f.adf[0] * vf[0] + f.adf[1] * vf[1] + (DIM == 3 ? f.adf[2] * vf[2] : 0.f);
                         am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
                     }
                     evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
@@ -328,17 +331,24 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                 ev = make_double2(evf.x, evf.y);
             } else {
                 ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
-                if (f.kind != 0) {
+                if (kind != 0) {
                     double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)v * DIM), P.amp);
                     ev = cmul(ev, a);
                 }
             }
-            if (f.kind == 2) ev = cmul(ev, f.div);
-            for (int q = 0; q < NLQ; ++q) {
-                double2 t = (f.kind == 0) ? cmul(ev, afv[q]) : ev;
-                double2 c = sc[ei][q];
-                acc[q].x += t.x * c.x - t.y * c.y;
-                acc[q].y += t.x * c.y + t.y * c.x;
+            if (kind == 2) ev = cmul(ev, f.div);
+            if (kind == 0) {
+                for (int q = 0; q < NLQ; ++q) {
+                    const double2 t = cmul(ev, afv[q]), c = sc[ei][q];
+                    acc[q].x += t.x * c.x - t.y * c.y;
+                    acc[q].y += t.x * c.y + t.y * c.x;
+                }
+            } else {
+                for (int q = 0; q < NLQ; ++q) {
+                    const double2 c = sc[ei][q];
+                    acc[q].x += ev.x * c.x - ev.y * c.y;
+                    acc[q].y += ev.x * c.y + ev.y * c.x;
+                }
             }
         }
     }
