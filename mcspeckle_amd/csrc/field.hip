@@ -232,8 +232,26 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
     return make_float2((float)a.x, (float)a.y);
 }
 
+// Far-field E(x, v) of the mixed variant: FP64 x.v and range reduction to a
+// fraction of a cycle, hardware sin/cos (input in revolutions), FP32
+// attenuation over the distance to the face the view leaves by (dlo/dhi: the
+// entry's x - box_min and box_max - x in FP32; inv, up: per view).
+template <int DIM>
+__device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double *x, const double *mv, const float *dlo,
+                                              const float *dhi, const bool *up, const float *inv, float hs)
+{
+    const double pv = dot<DIM>(x, mv);
+    const double cyc = pv * P.inv_lambda;
+    const float fr = (float)(cyc - rint(cyc));
+    const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
+    float best = INFINITY;
+    for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? dhi[j] : dlo[j]) * inv[j]);
+    const float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
+    return make_float2(mag * cs, mag * sn);
+}
+
 // WAVE: one output column; coef/coefR hold the light-summed coefficients and
-// the first-vertex entry contracts af_ang_vl(v, :) with cl0[p, :].  NLQ:
+// the first-vertex entries are left to field_wave_first_kernel.  NLQ:
 // light columns accumulated per pass (1 for one column, else kFieldNL).
 template <int DIM, bool MIXED, bool WAVE, int NLQ>
 __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
@@ -294,30 +312,11 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
             const FEntry &f = se[ei];
             // every lane reads the same entry: branch on its kind as a scalar
             const int kind = __builtin_amdgcn_readfirstlane(f.kind);
-            if (WAVE && kind == 0) {
-                // sum_l af_ang_vl(v, l) cl0[p, l], then the first-vertex E(x1, v)
-                double2 a = make_double2(0.0, 0.0);
-                for (int l = 0; l < P.nl; ++l) {
-                    double2 t = cmul(aftab[(size_t)v + (size_t)P.nv * l], cl0[(size_t)f.p * P.nl + l]);
-                    a.x += t.x;
-                    a.y += t.y;
-                }
-                double2 ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
-                double2 t = cmul(ev, a);
-                acc[0].x += t.x;
-                acc[0].y += t.y;
-                continue;
-            }
+            // wave first vertices: field_wave_first_kernel
+            if (WAVE && kind == 0) continue;
             double2 ev;
             if (MIXED) {
-                double pv = dot<DIM>(f.x, mv);
-                double cyc = pv * P.inv_lambda;
-                float fr = (float)(cyc - rint(cyc));
-                float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
-                float best = INFINITY;
-                for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? f.dhi[j] : f.dlo[j]) * inv[j]);
-                float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
-                float2 evf = make_float2(mag * cs, mag * sn);
+                float2 evf = e_far_mixed<DIM>(P, f.x, mv, f.dlo, f.dhi, up, inv, hs);
                 if (kind != 0) {
                     float2 am;
                     if (P.amp.type == 2) {
@@ -481,6 +480,100 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
         for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
 }
 
+// MCfieldOnWave first vertices, far field (MCfieldOnWave.m:143-148, 214):
+//   sum_p E(x1_p, v) sum_l af_ang_vl(v, l) cl0[p, l] = sum_l af_ang_vl(v, l) H[v, l],
+//   H[v, l] = sum_p E(x1_p, v) cl0[p, l],
+// so the table is applied once per (view, light) instead of once per (path,
+// view, light).  A workgroup holds 256 views x kWaveLG lights of H over one
+// range of paths, in FP64; the ranges' partials are reduced in a fixed order.
+constexpr int kWaveLG = 16;        // lights per workgroup
+constexpr int kWavePB = 32;        // paths per LDS batch
+constexpr int kWaveMaxSplit = 16;  // path ranges
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProblem P, int64_t np, int nsplit,
+                                                                       const PathRec *paths, const double2 *cl0,
+                                                                       double2 *hpart)
+{
+    __shared__ double sx[kWavePB][3];
+    __shared__ float sdlo[kWavePB][3], sdhi[kWavePB][3];
+    __shared__ double2 scl[kWavePB][kWaveLG];
+    const int v = blockIdx.x * kFieldViews + threadIdx.x;
+    const bool vok = v < P.nv;
+    double mv[3] = {0, 0, 0};
+    float inv[3] = {0, 0, 0};
+    bool up[3] = {false, false, false};
+    for (int j = 0; j < DIM && vok; ++j) {
+        const double nd = P.v[(size_t)v * DIM + j];
+        mv[j] = -nd;
+        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
+        up[j] = nd > 0;
+    }
+    const float hs = (float)(P.sigt * 0.5);
+    const int l0 = blockIdx.y * kWaveLG;
+    const int nlg = (P.nl - l0) < kWaveLG ? (P.nl - l0) : kWaveLG;
+    const int split = blockIdx.z;
+    const int64_t p0 = np * split / nsplit, p1 = np * (split + 1) / nsplit;
+    double2 acc[kWaveLG];
+#pragma unroll
+    for (int q = 0; q < kWaveLG; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int64_t pb = p0; pb < p1; pb += kWavePB) {
+        const int cnt = (int)((p1 - pb) < kWavePB ? (p1 - pb) : kWavePB);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt * 3; t += blockDim.x) {
+            const int i = t / 3, j = t % 3;
+            const double xj = paths[pb + i].x1[j];
+            sx[i][j] = xj;
+            sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
+            sdhi[i][j] = j < DIM ? (float)(P.bmax[j] - xj) : 0.f;
+        }
+        for (int t = threadIdx.x; t < cnt * kWaveLG; t += blockDim.x) {
+            const int i = t / kWaveLG, q = t % kWaveLG;
+            scl[i][q] = q < nlg ? cl0[(size_t)(pb + i) * P.nl + l0 + q] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (!vok) continue;
+        for (int i = 0; i < cnt; ++i) {
+            const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
+            double2 ev;
+            if (MIXED) {
+                const float2 e = e_far_mixed<DIM>(P, x, mv, sdlo[i], sdhi[i], up, inv, hs);
+                ev = make_double2(e.x, e.y);
+            } else {
+                ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+            }
+#pragma unroll
+            for (int q = 0; q < kWaveLG; ++q) {
+                const double2 c = scl[i][q];
+                acc[q].x += ev.x * c.x - ev.y * c.y;
+                acc[q].y += ev.x * c.y + ev.y * c.x;
+            }
+        }
+    }
+    if (vok)
+        for (int q = 0; q < nlg; ++q) hpart[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+}
+
+__global__ void field_wave_first_reduce_kernel(int nv, int nl, int nsplit, const double2 *aftab, const double2 *hpart,
+                                               double2 *acc)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int l = 0; l < nl; ++l) {
+        double2 h = make_double2(0.0, 0.0);
+        for (int k = 0; k < nsplit; ++k) {
+            const double2 t = hpart[((size_t)k * nl + l) * nv + v];
+            h.x += t.x;
+            h.y += t.y;
+        }
+        const double2 t = cmul(aftab[(size_t)v + (size_t)nv * l], h);
+        s.x += t.x;
+        s.y += t.y;
+    }
+    acc[v].x += s.x;
+    acc[v].y += s.y;
+}
+
 __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, double2 *acc)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,7 +594,8 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
 {
     const bool wave = wl != nullptr;
     const int ncol = wave ? 1 : P.nl;
-    // work layout: coef[nvert*ncol] | coefR[np*ncol] | aftab[nv*nl] | cl0[np*nl] (wave) | part[nsplit*nv*ncol]
+    // work layout: coef[nvert*ncol] | coefR[np*ncol] | aftab[nv*nl] | cl0[np*nl] (wave) | part[64*nv*ncol]
+    //              | hpart[kWaveMaxSplit*nl*nv] (wave)
     double2 *coef = work;
     double2 *coefR = coef + (size_t)nvert * ncol;
     double2 *aftab = coefR + (size_t)np * ncol;
@@ -556,13 +650,31 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     }
     const int64_t n = (int64_t)P.nv * ncol;
     hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nsplit, n, acc);
+    if (wave && !near && np > 0) {
+        double2 *hpart = part + (size_t)64 * P.nv * ncol;
+        const int ng = (P.nl + kWaveLG - 1) / kWaveLG;
+        int nsf = (2048 + vt * ng - 1) / (vt * ng);
+        const int64_t maxsf = (np + kWavePB - 1) / kWavePB;
+        if (nsf > maxsf) nsf = (int)maxsf;
+        if (nsf > kWaveMaxSplit) nsf = kWaveMaxSplit;
+        if (nsf < 1) nsf = 1;
+        dim3 gf((unsigned)vt, (unsigned)ng, (unsigned)nsf);
+        if (mixed)
+            hipLaunchKernelGGL((field_wave_first_kernel<DIM, true>), gf, dim3(kFieldViews), 0, s, P, np, nsf, paths,
+                               cl0, hpart);
+        else
+            hipLaunchKernelGGL((field_wave_first_kernel<DIM, false>), gf, dim3(kFieldViews), 0, s, P, np, nsf, paths,
+                               cl0, hpart);
+        hipLaunchKernelGGL(field_wave_first_reduce_kernel, dim3((unsigned)((P.nv + 255) / 256)), dim3(256), 0, s, P.nv,
+                           P.nl, nsf, aftab, hpart, acc);
+    }
 }
 
 size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave)
 {
     const int ncol = wave ? 1 : nl;
     return (size_t)nvert * ncol + (size_t)np * ncol + (size_t)nv * nl + (wave ? (size_t)np * nl : 0) +
-           (size_t)64 * nv * ncol;
+           (size_t)64 * nv * ncol + (wave ? (size_t)kWaveMaxSplit * nv * nl : 0);
 }
 
 void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
