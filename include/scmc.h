@@ -181,6 +181,18 @@ int scmc_mccov_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max
                      double *ms_re, double *ms_im, double *mm_re, double *mm_im,
                      double *mean_re, double *mean_im, scmc_stats *stats);
 
+/* u = MCsampleHetro(sigt,albedo,box_min,box_max,box_bin,l,v,is_ff_l,is_ff_v,
+ * maxItr,lambda,doCBS,smpFlg,sct_type,ampfunc[,ampfunc0[,lmean0]]) --
+ * code/MCsampleHetro.m:1-227 (the field sample of a voxel medium: the walks
+ * of MCCovHetro, E by the cell-by-cell march, u*sqrt(1/maxItr),
+ * MCsampleHetro.m:222).  pr as for scmc_mccov_hetro, except that has_lmean0
+ * / lmean0 are honoured (MCsampleHetro.m:35-40); med->has_lrad must be 0
+ * (MCsampleHetro has no lrad argument).  Output nv x nl.  The debug pairs
+ * (dbgpair, corrM) are not provided.  Both precision modes evaluate E in
+ * FP64. */
+int scmc_mcfield_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max_itr, int32_t precision,
+                       double *u_re, double *u_im, scmc_stats *stats);
+
 /* ---- sessions: accumulate path ranges, read results at any time -------- */
 /* A session owns the device accumulators of one MCcov (kind 0) or MCfield
  * (kind 1) problem.  scmc_session_run adds paths [first, first + n) (ids

@@ -24,6 +24,7 @@
 // in a fixed order (deterministic).  The mixed variant forms the phase k x.v
 // and its range reduction in FP64 and evaluates sin/cos (hardware, input in
 // revolutions), the attenuation exponential and the amplitude in FP32.
+#include "hetero.h"
 #include "kernels.h"
 
 namespace scmc {
@@ -33,8 +34,9 @@ constexpr int kFieldBatch = 64;    // entries per LDS batch
 constexpr int kFieldNL = 4;        // lights per launch pass
 
 // Per-path / per-vertex coefficients.  Block per path, lanes over lights.
-template <int DIM>
-__global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np, const PathRec *paths,
+// HET: voxel medium (MCsampleHetro.m:141-176, evalphaseattHetro).
+template <int DIM, bool HET>
+__global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium M, int64_t np, const PathRec *paths,
                                                         const VertexRec *verts, const int64_t *offsets,
                                                         double2 *coef, double2 *coefR)
 {
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np
         const double *lc = P.l + (size_t)l * DIM;
         double rl[3];
         light_dir<DIM>(P, pr.x1, l, rl);   // MCfield.m:134-137 (near field: from x1)
-        double2 el0 = phase_att64<DIM>(P, pr.x1, lc, 1.0, P.is_ff_l);
+        double2 el0 = nee_e<DIM, HET>(P, M, pr.x1, lc, 1.0, P.is_ff_l);
         double2 af = cdiv(eval_amp<DIM>(dot<DIM>(pr.w1, rl), P.amp), pr.w0p);
         double2 el0ms = cmul(el0, af);
         double2 R = make_double2(0.0, 0.0);
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, int64_t np
                     t.x *= rs2;
                     t.y *= rs2;
                     double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
-                    double2 el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l),
+                    double2 el = cmul(nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l),
                                       eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
                     double2 q = cmul(el, cn);
                     R.x += q.x * rs2;
@@ -253,8 +255,8 @@ __device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double 
 // WAVE: one output column; coef/coefR hold the light-summed coefficients and
 // the first-vertex entries are left to field_wave_first_kernel.  NLQ:
 // light columns accumulated per pass (1 for one column, else kFieldNL).
-template <int DIM, bool MIXED, bool WAVE, int NLQ>
-__global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
+template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false>
+__global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
                                                             const int64_t *offsets, const double2 *coef,
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, int64_
                 }
                 ev = make_double2(evf.x, evf.y);
             } else {
-                ev = phase_att64<DIM>(P, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+                ev = nee_e<DIM, HET>(P, M, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
                 if (kind != 0) {
                     double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)v * DIM), P.amp);
                     ev = cmul(ev, a);
@@ -369,8 +371,9 @@ struct NEntry {
     int p;
 };
 
-template <int DIM, bool WAVE>
-__global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, int64_t np, int64_t nvert, int64_t nent,
+template <int DIM, bool WAVE, bool HET = false>
+__global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert,
+                                                                 int64_t nent,
                                                                  int nsplit, int l0, int nlp, const PathRec *paths,
                                                                  const VertexRec *verts, const int32_t *vpath,
                                                                  const int64_t *offsets, const double2 *coef,
@@ -442,7 +445,7 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, i
             double rv[3];
             view_dir<DIM>(P, x1, v, rv);
             double xx[3] = {f.x[0], f.x[1], f.x[2]};
-            double2 ev = phase_att64<DIM>(P, xx, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+            double2 ev = nee_e<DIM, HET>(P, M, xx, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
             if (WAVE && f.kind == 0) {
                 // sum_l a(rl_l . rv) cl0[p, l] (MCfieldOnWave.m:143-148, 214)
                 double2 a = make_double2(0.0, 0.0);
@@ -588,11 +591,15 @@ __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, 
 }
 
 template <int DIM>
-static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths,
+static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np, int64_t nvert, const PathRec *paths,
                            const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work,
                            double2 *acc, bool mixed, const double2 *wl, hipStream_t s)
 {
-    const bool wave = wl != nullptr;
+    // a voxel medium (MCsampleHetro) has no wave variant and marches in FP64
+    const bool het = Mp != nullptr;
+    HetMedium M{};
+    if (het) M = *Mp;
+    const bool wave = wl != nullptr && !het;
     const int ncol = wave ? 1 : P.nl;
     // work layout: coef[nvert*ncol] | coefR[np*ncol] | aftab[nv*nl] | cl0[np*nl] (wave) | part[64*nv*ncol]
     //              | hpart[kWaveMaxSplit*nl*nv] (wave)
@@ -604,9 +611,12 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     if (wave)
         hipLaunchKernelGGL(field_wave_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts,
                            offsets, wl, coef, coefR, cl0);
+    else if (het)
+        hipLaunchKernelGGL((field_coef_kernel<DIM, true>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
+                           offsets, coef, coefR);
     else
-        hipLaunchKernelGGL(field_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts, offsets,
-                           coef, coefR);
+        hipLaunchKernelGGL((field_coef_kernel<DIM, false>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
+                           offsets, coef, coefR);
     const bool near = !(P.is_ff_l && P.is_ff_v);
     if (!near) launch_af_table(P, aftab, s);
     const int64_t nent = nvert + (P.do_cbs ? np : 0);
@@ -619,13 +629,20 @@ static void field_dispatch(const DevProblem &P, int64_t np, int64_t nvert, const
     for (int l0 = 0; l0 < ncol; l0 += kFieldNL) {
         const int nlp = (ncol - l0) < kFieldNL ? (ncol - l0) : kFieldNL;
         dim3 g((unsigned)vt, (unsigned)nsplit);
-#define SCMC_FIELD(KERNEL)                                                                                      \
-    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
+#define SCMC_FIELD(KERNEL)                                                                                         \
+    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, M, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
                        vpath, offsets, coef, coefR, aftab, cl0, part)
-#define SCMC_FIELD_NEAR(KERNEL)                                                                                 \
-    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
+#define SCMC_FIELD_NEAR(KERNEL)                                                                                    \
+    hipLaunchKernelGGL((KERNEL), g, dim3(kFieldViews), 0, s, P, M, np, nvert, nent, nsplit, l0, nlp, paths, verts, \
                        vpath, offsets, coef, coefR, cl0, part)
-        if (near) {
+        if (het) {
+            if (near)
+                SCMC_FIELD_NEAR((field_near_kernel<DIM, false, true>));
+            else if (ncol == 1)
+                SCMC_FIELD((field_kernel<DIM, false, false, 1, true>));
+            else
+                SCMC_FIELD((field_kernel<DIM, false, false, kFieldNL, true>));
+        } else if (near) {
             if (wave)
                 SCMC_FIELD_NEAR((field_near_kernel<DIM, true>));
             else
@@ -677,14 +694,14 @@ size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave)
            (size_t)64 * nv * ncol + (wave ? (size_t)kWaveMaxSplit * nv * nl : 0);
 }
 
-void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
-                  const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc, bool mixed,
-                  const double2 *wl, hipStream_t s)
+void launch_field(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
+                  const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc,
+                  bool mixed, const double2 *wl, hipStream_t s)
 {
     if (P.dim == 3)
-        field_dispatch<3>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
+        field_dispatch<3>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
     else
-        field_dispatch<2>(P, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
+        field_dispatch<2>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
 }
 
 } // namespace scmc

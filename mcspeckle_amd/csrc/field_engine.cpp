@@ -23,6 +23,7 @@ int FieldEngine::init(const ProblemState &ps, hipStream_t s, const double *wl_re
     nv = ps.dp.nv;
     nl = ps.dp.nl;
     wave = wl_re != nullptr;
+    het = ps.het;
     ncol = wave ? 1 : nl;
     if (!acc.ensure(sizeof(double2) * (size_t)nv * ncol, &rc)) return rc;
     launch_zero(acc.p, sizeof(double2) * (size_t)nv * ncol, s);
@@ -50,7 +51,8 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
-    launch_field(ps.dp, tc.np, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
+    launch_field(ps.dp, ps.het ? &ps.hm : nullptr, tc.np, tc.nvert, (const PathRec *)tc.paths.p,
+                 (const VertexRec *)tc.verts.p,
                  (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, (double2 *)work.p, (double2 *)acc.p, mixed,
                  wave ? (const double2 *)wl.p : nullptr, s);
     (void)hipEventRecord(e1, s);
@@ -72,8 +74,9 @@ int FieldEngine::finalize(const scmc_problem *pr, int64_t max_itr, double *u_re,
     if (!hip_ok(hipMemcpyAsync(h.data(), acc.p, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy field", &rc) ||
         !hip_ok(hipStreamSynchronize(s), "field sync", &rc))
         return rc;
-    // MCfield.m:253-254 (MCfieldOnWave.m:265-267)
-    const double sc = sqrt(1.0 / (double)max_itr * box_volume(pr) * pr->sigt);
+    // MCfield.m:253-254 (MCfieldOnWave.m:265-267); a voxel medium carries the
+    // volume in px and scales by sqrt(1/maxItr) (MCsampleHetro.m:222)
+    const double sc = het ? sqrt(1.0 / (double)max_itr) : sqrt(1.0 / (double)max_itr * box_volume(pr) * pr->sigt);
     for (size_t i = 0; i < ne; ++i) {
         if (u_im) {
             u_re[i] = h[2 * i] * sc;

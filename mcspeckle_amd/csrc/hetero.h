@@ -146,6 +146,15 @@ __device__ __forceinline__ double2 phase_att_hetro(const DevProblem &P, const He
 
 // smpWoodcock.m:1-15 from x along w; trial 0 uses (ud0, ua0), trial k >= 1
 // the pair of group 16 + k at this vertex.
+// E(x, d): evalphaseatt (homogeneous) or evalphaseattHetro (voxel medium).
+template <int DIM, bool HET>
+__device__ __forceinline__ double2 nee_e(const DevProblem &P, const HetMedium &M, const double *x, const double *dcol,
+                                         double sign, int is_ff)
+{
+    if (HET) return phase_att_hetro<DIM>(P, M, x, dcol, sign, is_ff);
+    return phase_att64<DIM>(P, x, dcol, sign, is_ff);
+}
+
 template <int DIM>
 __device__ __forceinline__ double woodcock(const DevProblem &P, const HetMedium &M, const double *x, const double *w,
                                            uint64_t path, uint32_t vertex, double ud0, double ua0)

@@ -181,3 +181,28 @@ extern "C" int scmc_mccov_hetro(const scmc_problem *pr, const scmc_medium *med, 
     }
     return SCMC_OK;
 }
+
+// MCsampleHetro.m:1-227: the field sample u (Nv x Nl) of a voxel medium.  The
+// walks are MCCovHetro's; lmean0 is honoured (MCsampleHetro.m:35-40); there is
+// no lrad argument (expSmpXHetro with lrad = [], MCsampleHetro.m:97).
+extern "C" int scmc_mcfield_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max_itr, int32_t precision,
+                                  double *u_re, double *u_im, scmc_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!pr) return set_error(SCMC_E_ARG, "null problem");
+    if (med && med->has_lrad) return set_error(SCMC_E_ARG, "MCsampleHetro has no lrad argument");
+    scmc_problem q = *pr;
+    if (!(q.sigt > 0)) q.sigt = 1.0;
+    if (med && med->albedo) q.albedo = 1.0;
+    int rc = validate_problem(&q, max_itr, true);
+    if (rc) return rc;
+    Session s;
+    if ((rc = s.init(&q, 1, precision, nullptr, nullptr, med))) return rc;
+    if ((rc = s.run(0, max_itr))) return rc;
+    if ((rc = s.result(max_itr, u_re, u_im, nullptr, nullptr))) return rc;
+    if (stats) {
+        s.fill_stats(stats);
+        stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return SCMC_OK;
+}

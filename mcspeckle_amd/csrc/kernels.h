@@ -98,10 +98,12 @@ void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s);
 // u(v,l) += sum over vertices of the MCfield.m:207-222 contributions,
 // FP64 accumulators acc[nv*nl] (column-major, v fastest).  With wl (device,
 // nl weights) the lights form one wave (MCfieldOnWave.m) and acc has nv
-// entries.  work holds field_work_elems(...) double2.
-void launch_field(const DevProblem &P, int64_t np, int64_t nvert, const PathRec *paths, const VertexRec *verts,
-                  const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc, bool mixed,
-                  const double2 *wl, hipStream_t s);
+// entries.  With M (a voxel medium, MCsampleHetro.m) E is the ray march of
+// evalphaseattHetro, in FP64 in both precision modes.  work holds
+// field_work_elems(...) double2.
+void launch_field(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
+                  const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc,
+                  bool mixed, const double2 *wl, hipStream_t s);
 size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave);
 
 // Device-wide helpers.

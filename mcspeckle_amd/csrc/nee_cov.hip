@@ -51,14 +51,6 @@ __device__ __forceinline__ void atomic_max_pos(unsigned *p, float v)
 // bound is the product of the factor maxima (the amplitude table is applied
 // when finalising); near field the single-scatter rows carry the amplitude
 // and nee_single_near_kernel records their bound.
-// E(x, d): evalphaseatt (homogeneous) or evalphaseattHetro (voxel medium).
-template <int DIM, bool HET>
-__device__ __forceinline__ double2 nee_e(const DevProblem &P, const HetMedium &M, const double *x, const double *dcol,
-                                         double sign, int is_ff)
-{
-    if (HET) return phase_att_hetro<DIM>(P, M, x, dcol, sign, is_ff);
-    return phase_att64<DIM>(P, x, dcol, sign, is_ff);
-}
 
 // The single-scattering weight is the first vertex's W (albedo, or the
 // albedo of its cell in a voxel medium, MCCovHetro.m:164).  With a medium,

@@ -92,6 +92,7 @@ struct CovEngine {
 struct FieldEngine {
     int nv = 0, nl = 0, ncol = 0;
     bool wave = false;   // lights combined with weights (MCfieldOnWave)
+    bool het = false;    // voxel medium (MCsampleHetro)
     DevBuf acc, work, out, wl;
     int init(const ProblemState &ps, hipStream_t s, const double *wl_re = nullptr, const double *wl_im = nullptr);
     int chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum);

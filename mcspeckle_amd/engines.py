@@ -325,6 +325,31 @@ def MCCovHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_v, 
             (bufs[4] + 1j * bufs[5]).reshape((nv, nl), order="F"))
 
 
+def MCsampleHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg,
+                  sct_type, ampfunc, ampfunc0=None, lmean0=None, *, seed=0, path_offset=0, precision="fp64",
+                  stats=None):
+    """code/MCsampleHetro.m:1-227 on the GPU: the field sample u (Nv x Nl) of
+    a voxel medium (sigt and, optionally, albedo grids over box_bin cells).
+    The debug pairs (dbgpair, corrM) of the reference are not provided."""
+    sig = np.asarray(sigt, dtype=np.float64)
+    alb = np.asarray(albedo, dtype=np.float64)
+    a0 = float(alb.reshape(-1)[0]) if alb.size == 1 else 1.0
+    spec = make_problem(max(float(sig.max()), 1e-300), a0, box_min, box_max, l, v, is_ff_l, is_ff_v, lambda_, doCBS,
+                        smpFlg, sct_type, ampfunc, ampfunc0, lmean0=lmean0, seed=seed, path_offset=path_offset)
+    if not (np.isscalar(maxItr) and maxItr >= 1):
+        raise ScmcArgumentError("Invalid photons number")
+    m, keep = c_medium(spec, sig, box_bin, alb if alb.size > 1 else None)
+    nv, nl = spec.nv, spec.nl
+    re, im = np.empty(nv * nl), np.empty(nv * nl)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(N.lib().scmc_mcfield_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision), N.dptr(re), N.dptr(im),
+                                       C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return (re + 1j * im).reshape((nv, nl), order="F")
+
+
 def quad_grid(box_min, box_max, v_max, v_stp, lambda_, wave=False):
     """Grid sizes (nv0, nx, nz) of MCfieldFourierQuad.m:80-121 (wave:
     MCfieldFourierQuadOnWave.m:84-131)."""

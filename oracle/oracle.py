@@ -365,6 +365,10 @@ def _hetro_sigs(L):
     L.orc_woodcock.restype = C.c_double
     L.orc_hetro_icdf.argtypes = [_dp, C.c_int, C.c_int, C.POINTER(C.c_int32)]
     L.orc_hetro_icdf.restype = C.c_int
+    L.orc_mcfield_hetro.argtypes = [P, MP, C.c_int64, _dp, _dp]
+    L.orc_mcfield_hetro.restype = C.c_int64
+    L.orc_mcfield_hetro_sums.argtypes = [P, MP, C.c_uint64, C.c_int64, _dp, _dp]
+    L.orc_mcfield_hetro_sums.restype = C.c_int64
 
 
 def mccov_hetro(spec, sigt, box_bin, max_itr, albedo=None, lrad=None, sums_from=None):
@@ -386,6 +390,24 @@ def mccov_hetro(spec, sigt, box_bin, max_itr, albedo=None, lrad=None, sums_from=
     sh = (nv, nv, nl, nl)
     return ((a + 1j * b).reshape(sh, order="F"), (c + 1j * d).reshape(sh, order="F"),
             (e + 1j * f).reshape((nv, nl), order="F"), k)
+
+
+def mcfield_hetro(spec, sigt, box_bin, max_itr, albedo=None, sums_from=None):
+    """MCsampleHetro: (u nv x nl, vertex count).  With sums_from=p0 the
+    unnormalised sum over path ids [p0, p0 + max_itr)."""
+    L = lib()
+    _hetro_sigs(L)
+    p, keep = problem(spec)
+    m, keep2 = medium(spec, sigt, box_bin, albedo)
+    nv, nl = spec.nv, spec.nl
+    re, im = np.empty(nv * nl), np.empty(nv * nl)
+    if sums_from is None:
+        k = L.orc_mcfield_hetro(C.byref(p), C.byref(m), int(max_itr), _p(re), _p(im))
+    else:
+        k = L.orc_mcfield_hetro_sums(C.byref(p), C.byref(m), int(sums_from), int(max_itr), _p(re), _p(im))
+    if k < 0:
+        raise ValueError("invalid medium")
+    return (re + 1j * im).reshape((nv, nl), order="F"), k
 
 
 def phase_att_hetro(spec, sigt, box_bin, x, dirs, is_ff, sign=1.0, albedo=None):

@@ -1683,3 +1683,64 @@ int64_t orc_mccov_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_
     for (size_t i = 0; i < nf; ++i) { mean_re[i] *= s; mean_im[i] *= s; }
     return nvert;
 }
+
+
+/* ------------------------------------------------------- MCsampleHetro */
+
+/* code/MCsampleHetro.m:82-216 over paths [p0, p0 + np): MCCovHetro's walks
+ * (walk_hetro; lmean0 honoured, MCsampleHetro.m:35-40; no lrad), each path's
+ * field terms as MCfield.m:207-222 with E from the cell-by-cell march
+ * (MCsampleHetro.m:141-186).  Unnormalised u (nv x nl); returns the vertex
+ * count, or -1 for an invalid medium. */
+int64_t orc_mcfield_hetro_sums(const orc_problem *pr, const orc_medium *m, uint64_t p0, int64_t np, double *u_re,
+                               double *u_im)
+{
+    if (m->has_lrad) return -1;
+    const int nvox = med_nvox(m, pr->dim);
+    const int n_icdf = nvox * 100;
+    int32_t *icdf = malloc(sizeof(int32_t) * (size_t)n_icdf);
+    double tot = 0;
+    for (int i = 0; i < nvox; ++i) tot += m->sigt[i];
+    double vol = 1;
+    for (int j = 0; j < pr->dim; ++j) vol *= m->bin[j];
+    const double V = tot * vol;   /* MCsampleHetro.m:60 */
+    if (orc_hetro_icdf(m->sigt, nvox, n_icdf, icdf)) {
+        free(icdf);
+        return -1;
+    }
+    const double smax = med_sigmax(pr, m);
+    double unmean[3], meanl[3];
+    mean_dir(pr, unmean, meanl);
+    const size_t ne = (size_t)pr->nv * pr->nl;
+    cplx *u = calloc(ne, sizeof(cplx));
+    nee_ws ws;
+    nee_alloc(&ws, pr->nv, pr->nl, pr->dim);
+    ws.med = m;
+    vtx *buf = NULL;
+    int64_t cap = 0, nvert = 0;
+    path_head h;
+    for (int64_t i = 0; i < np; ++i) {
+        int64_t n = walk_hetro(pr, m, icdf, n_icdf, V, smax, p0 + (uint64_t)i, meanl, &h, &buf, &cap);
+        if (n == 0) continue;   /* MCsampleHetro.m:100-102 */
+        nvert += n;
+        mcfield_path(pr, &h, buf, n, &ws, u);
+    }
+    split_out(u, ne, 1.0, u_re, u_im);
+    free(buf); free(icdf); free(u);
+    nee_free(&ws);
+    return nvert;
+}
+
+/* MCsampleHetro.m:222: u*sqrt(1/maxItr) (the volume is inside px). */
+int64_t orc_mcfield_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_itr, double *u_re, double *u_im)
+{
+    int64_t nvert = orc_mcfield_hetro_sums(pr, m, 0, max_itr, u_re, u_im);
+    if (nvert < 0) return nvert;
+    const size_t ne = (size_t)pr->nv * pr->nl;
+    const double s = sqrt(1.0 / (double)max_itr);
+    for (size_t i = 0; i < ne; ++i) {
+        u_re[i] *= s;
+        u_im[i] *= s;
+    }
+    return nvert;
+}

@@ -37,6 +37,10 @@
  *                        exact transmittance of a layered medium (KS);
  *                        single-scatter closed form of a two-layer slab;
  *                        the ray march equals a direct sum over layers.
+ *   mcfield_hetro      : a uniform one-cell medium reproduces MCfield on the
+ *                        same paths (far and near field, CBS, lmean0);
+ *                        E[|u|^2] of a two-layer slab equals the trace of
+ *                        MCCovHetro's Ms + Mm/2 (statistical).
  */
 #ifndef SCMC_ORACLE_H
 #define SCMC_ORACLE_H
@@ -198,6 +202,11 @@ int64_t orc_mccov_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_
 int64_t orc_mccov_hetro_sums(const orc_problem *pr, const orc_medium *m, uint64_t p0, int64_t np,
                              double *ms_re, double *ms_im, double *mm_re, double *mm_im, double *mean_re,
                              double *mean_im);
+/* code/MCsampleHetro.m:1-227 (field sample of a voxel medium); m->has_lrad
+ * must be 0.  Returns the vertex count or -1. */
+int64_t orc_mcfield_hetro(const orc_problem *pr, const orc_medium *m, int64_t max_itr, double *u_re, double *u_im);
+int64_t orc_mcfield_hetro_sums(const orc_problem *pr, const orc_medium *m, uint64_t p0, int64_t np, double *u_re,
+                               double *u_im);
 /* evalphaseattHetro.m:1-51 for n directions / positions (sign * d[:, i]). */
 void orc_eval_phase_att_hetro(const orc_problem *pr, const orc_medium *m, const double *x, const double *d, int n,
                               int is_ff, double sign, double *out_re, double *out_im);

@@ -158,3 +158,54 @@ def test_uniform_cells_agree_with_mccov_statistically(oracle):
         z.append((np.abs(part(h).mean(0) - part(o).mean(0)) / se).ravel())
     z = np.concatenate(z)
     assert np.mean(z < 3) >= 0.95 and z.max() < 6
+
+
+# ---------------------------------------------------------------- MCsampleHetro
+
+def _lm0_spec(seed=5):
+    rng = np.random.default_rng(seed)
+    l, v = far_dirs(2, 25, rng), far_dirs(3, 25, rng)
+    return make_problem(2.5, 0.9, [-3, -4, 0], [3, 4, 1.2], l, v, 1, 1, 0.8, True, 1, 3, 0.7, 0.4,
+                        lmean0=[0.2, -0.1, 1.0], seed=seed)
+
+
+@pytest.mark.parametrize("kind,dim,cbs,near,lm0", [("hg", 3, True, False, False), ("iso", 2, True, False, False),
+                                                   ("tab", 3, False, False, False), ("hg", 3, True, True, False),
+                                                   ("hg", 3, True, False, True)])
+def test_one_cell_medium_sample_is_mcfield(oracle, kind, dim, cbs, near, lm0):
+    """MCsampleHetro on a uniform one-cell medium walks MCfield's paths (the
+    walks line up as for MCCovHetro) and sqrt(1/maxItr) with px = 1/(sigma V)
+    equals MCfield's sqrt(V sigma / maxItr): u agrees to rounding.  lm0: an
+    lmean0 that steers the first direction (ampfunc0) of both."""
+    if lm0:
+        spec = _lm0_spec()
+    elif near:
+        _, spec = near_far_pair(kind, dim, cbs, 5.0, "both", nv=3, nl=2)
+    else:
+        spec = small_spec(kind, dim, cbs, nv=3, nl=2, seed=5, smp=1, tab=tab3d() if kind == "tab" else None)
+    cell = np.full((1,) * dim, spec.sigt)
+    u, K = oracle.mcfield_hetro(spec, cell, spec.box_max - spec.box_min, 300)
+    u0 = oracle.mcfield(spec, 300)
+    assert K > 0
+    assert np.max(np.abs(u - u0)) <= 1e-12 * np.max(np.abs(u0))
+
+
+def test_sample_power_is_the_covariance_diagonal(oracle):
+    """On the same walks, |u(v,l)|^2 - [Ms + Mm/2](v,v,l,l) is the sum of the
+    cross terms of different vertices, which carry independent uniform phases
+    (MCsampleHetro.m:207): zero in the mean over batches of paths."""
+    bmin, bmax, bb, sig = layered([1.0, 4.0], T=1.0, W=6.0)
+    rng = np.random.default_rng(8)
+    l, v = far_dirs(2, 15, rng), far_dirs(2, 15, rng)
+    spec = make_problem(2.5, 0.95, bmin, bmax, l, v, 1, 1, 0.8, True, 2, 3, 0.6, seed=4)
+    B, n = 30, 300
+    d = []
+    for b in range(B):
+        u, _ = oracle.mcfield_hetro(spec, sig, bb, n, sums_from=b * n)
+        Ms, Mm, _, _ = oracle.mccov_hetro(spec, sig, bb, n, sums_from=b * n)
+        diag = np.array([[Ms[i, i, j, j] + 0.5 * Mm[i, i, j, j] for j in range(2)] for i in range(2)])
+        assert np.all(np.abs(diag.imag) <= 1e-9 * np.abs(diag.real))
+        d.append((np.abs(u) ** 2 - diag.real) / n)
+    d = np.array(d)
+    z = d.mean(0) / (d.std(0, ddof=1) / np.sqrt(B))
+    assert np.all(np.abs(z) < 4), z
