@@ -2,7 +2,7 @@
 // box_min,box_max,box_bin,l,v,is_ff_l,is_ff_v,maxItr,lambda,doCBS,smpFlg,
 // sct_type,ampfunc[,ampfunc0[,lrad]]) (code/MCCovHetro.m:1-37).  sigt is the
 // cell grid (numel = prod(round((box_max-box_min)./box_bin))); albedo is a
-// scalar or a grid of the same size (MCCovHetro.m:28-30).
+// scalar or a grid of the same size (MCCovHetro.m:27-29).
 #include "gateway_common.h"
 
 using namespace scmc_mex;
@@ -17,30 +17,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     scmc_problem &p = A.pr;
     const mxArray *sig = prhs[0], *alb = prhs[1];
     p.dim = read_box(prhs[2], prhs[3], p.box_min, p.box_max, id);
-    if ((int)mxGetNumberOfElements(prhs[4]) != p.dim) mexErrMsgIdAndTxt(id, "Invalid box_bin size");
     scmc_medium med;
-    memset(&med, 0, sizeof med);
-    const double *bb = mxGetPr(prhs[4]);
-    int64_t ncell = 1;
-    for (int j = 0; j < 3; ++j) {
-        med.nb[j] = 1;
-        med.box_bin[j] = 1.0;
-    }
-    for (int j = 0; j < p.dim; ++j) {
-        med.box_bin[j] = bb[j];
-        med.nb[j] = (int32_t)std::lround((p.box_max[j] - p.box_min[j]) / bb[j]);
-        ncell *= med.nb[j];
-    }
-    if ((int64_t)mxGetNumberOfElements(sig) != ncell) mexErrMsgIdAndTxt(id, "sigt does not match the cell grid");
-    med.sigt = mxGetPr(sig);
-    p.sigt = 1.0;
-    if (mxGetNumberOfElements(alb) == 1) {
-        p.albedo = mxGetPr(alb)[0];
-    } else {
-        if ((int64_t)mxGetNumberOfElements(alb) != ncell) mexErrMsgIdAndTxt(id, "albedo does not match the cell grid");
-        med.albedo = mxGetPr(alb);
-        p.albedo = 1.0;
-    }
+    read_medium(sig, alb, prhs[4], p, med, id);
     // MCCovHetro.m:19-24: 1 x N rows are angles
     p.nl = read_sources(prhs[5], p.dim, A.l, id, "Invalid light source input");
     p.nv = read_sources(prhs[6], p.dim, A.v, id, "Invalid view source input");

@@ -127,4 +127,36 @@ inline void check_rc(int rc, const char *id)
     if (rc != SCMC_OK) mexErrMsgIdAndTxt(id, "%s", scmc_last_error());
 }
 
+// The voxel medium of MCCovHetro.m / MCsampleHetro.m: box_bin (dim sizes),
+// the sigt grid (numel = prod(round((box_max-box_min)./box_bin)), MATLAB
+// order) and albedo as a scalar (into p.albedo) or a grid of the same size
+// (MCCovHetro.m:27-29, MCsampleHetro.m:25-27).  p.dim and the box are read.
+inline void read_medium(const mxArray *sig, const mxArray *alb, const mxArray *bin, scmc_problem &p,
+                        scmc_medium &med, const char *id)
+{
+    if ((int)mxGetNumberOfElements(bin) != p.dim) mexErrMsgIdAndTxt(id, "Invalid box_bin size");
+    memset(&med, 0, sizeof med);
+    const double *bb = mxGetPr(bin);
+    int64_t ncell = 1;
+    for (int j = 0; j < 3; ++j) {
+        med.nb[j] = 1;
+        med.box_bin[j] = 1.0;
+    }
+    for (int j = 0; j < p.dim; ++j) {
+        med.box_bin[j] = bb[j];
+        med.nb[j] = (int32_t)std::lround((p.box_max[j] - p.box_min[j]) / bb[j]);
+        ncell *= med.nb[j];
+    }
+    if ((int64_t)mxGetNumberOfElements(sig) != ncell) mexErrMsgIdAndTxt(id, "sigt does not match the cell grid");
+    med.sigt = mxGetPr(sig);
+    p.sigt = 1.0;
+    if (mxGetNumberOfElements(alb) == 1) {
+        p.albedo = mxGetPr(alb)[0];
+    } else {
+        if ((int64_t)mxGetNumberOfElements(alb) != ncell) mexErrMsgIdAndTxt(id, "albedo does not match the cell grid");
+        med.albedo = mxGetPr(alb);
+        p.albedo = 1.0;
+    }
+}
+
 } // namespace scmc_mex

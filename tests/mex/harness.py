@@ -12,7 +12,7 @@ _dp = C.POINTER(C.c_double)
 
 class Mex:
     def __init__(self):
-        if not os.path.exists(os.path.join(B, "MCCovHetro.mexa64")):
+        if not os.path.exists(os.path.join(B, "MCsampleHetro.mexa64")):
             subprocess.run(["make", "-s", "-C", HERE], check=True)
         self.mx = C.CDLL(os.path.join(B, "libfakemx.so"), mode=C.RTLD_GLOBAL)
         mx = self.mx
@@ -38,7 +38,7 @@ class Mex:
         mx.mxDestroyArray.argtypes = [C.c_void_p]
         self.gw = {}
         for name in ("MCcov", "MCfield", "MCfieldFourierQuad", "MCfieldOnWave", "MCfieldFourierQuadOnWave",
-                     "MCCovHetro"):
+                     "MCCovHetro", "MCsampleHetro"):
             lib = C.CDLL(os.path.join(B, name + ".mexa64"))
             self.gw[name] = (lib, C.cast(lib.mexFunction, C.c_void_p))
 

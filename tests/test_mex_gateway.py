@@ -171,3 +171,32 @@ def test_hetro_gateway_equals_engine(mex):
     np.testing.assert_array_equal(Ms, Ms0)
     np.testing.assert_array_equal(Mm, Mm0)
     np.testing.assert_array_equal(m1, m10)
+
+
+def test_sample_hetro_gateway_argument_contract(mex):
+    a = hetro_args()
+    a[0] = a[0].reshape(2, 6, order="F")
+    _, err = mex.call("MCsampleHetro", 1, *a[:14])
+    assert err[0] == "MATLAB:narginchk:notEnoughInputs"
+    _, err = mex.call("MCsampleHetro", 1, *(a + [np.zeros((0, 0)), np.zeros((0, 0)), np.zeros((0, 0)), 5]))
+    assert err[0] == "MATLAB:narginchk:tooManyInputs"
+    _, err = mex.call("MCsampleHetro", 2, *a)
+    assert err == ("scmc:MCsampleHetro", "dbgpair / corrM are not provided by this build")
+    b = list(a)
+    b[0] = np.ones((2, 2))
+    assert mex.call("MCsampleHetro", 1, *b)[1] == ("scmc:MCsampleHetro", "sigt does not match the cell grid")
+
+
+@pytest.mark.gpu
+def test_sample_hetro_gateway_equals_engine(mex):
+    a = hetro_args()
+    g = list(a)
+    g[0] = a[0].reshape(2, 6, order="F")   # the fake libmx holds 2-D arrays; numel and order are kept
+    lm0 = np.array([[0.05], [0.0], [1.0]])
+    key = mex.seed_for_next_call(37)
+    out, err = mex.call("MCsampleHetro", 1, *(g + [np.zeros((0, 0)), lm0]))
+    assert err is None, err
+    (u,) = out
+    u0 = E.MCsampleHetro(*a[:15], None, lm0.ravel(), seed=key, precision="fp64")
+    assert u.shape == (3, 2)
+    np.testing.assert_array_equal(u, u0)
