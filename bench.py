@@ -267,6 +267,27 @@ def hetro_leg(n_paths, precision="mixed", seed=4):
             "ms_nee": st["ms_nee"], "ms_accum": st["ms_accum"], "precision": precision}
 
 
+def sample_hetro_leg(n_paths, precision="mixed", seed=4):
+    """MCsampleHetro: the same 8 x 8 x 4-cell medium as the MCCovHetro leg,
+    one light at normal incidence, 64 x 64 far-field views on +-10 degrees
+    (the MCfield speckle-image convention), CBS on, smpFlg 2."""
+    from mcspeckle_amd import engines as E
+    rng = np.random.default_rng(seed)
+    grid = rng.uniform(2.0, 10.0, (8, 8, 4))
+    t = np.deg2rad(np.linspace(-10, 10, 64))
+    tx, ty = np.meshgrid(t, t, indexing="ij")
+    v = np.vstack([np.sin(tx).ravel(), np.sin(ty).ravel(), np.ones(tx.size)])
+    v /= np.linalg.norm(v, axis=0)
+    st = {}
+    t0 = time.perf_counter()
+    E.MCsampleHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], [[0.0], [0.0], [1.0]], v, 1, 1, n_paths,
+                    1.0, True, 2, 3, 0.9, seed=0, precision=precision, stats=st)
+    dt = time.perf_counter() - t0
+    return {"paths": n_paths, "views": int(v.shape[1]), "cells": list(grid.shape), "value": n_paths * v.shape[1] / dt,
+            "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_trace": st["ms_trace"],
+            "ms_accum": st["ms_accum"], "precision": precision}
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of `mib` MiB through the HIP runtime, the
     reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -392,6 +413,7 @@ def main():
         out["mcfield_on_wave"] = field_wave_leg(args.field_paths)
         out["mcfield_fourier_quad_on_wave"] = quad_wave_leg(args.quad_paths)
         out["mccov_hetro"] = hetro_leg(args.hetro_paths)
+        out["mcsample_hetro"] = sample_hetro_leg(args.hetro_paths)
 
     # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
     if not args.no_serve:
