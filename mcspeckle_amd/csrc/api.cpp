@@ -298,7 +298,9 @@ int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
     n = nv * nl;
     tile = mixed ? kTileMixed : kTileF64;
     if (ps.het) {
-        if (!mean.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
+        if (!mean.ensure(sizeof(double2) * (size_t)n, &rc) ||
+            !mean_part.ensure(sizeof(double2) * (size_t)n * kHetMeanSplits, &rc))
+            return rc;
         launch_zero(mean.p, sizeof(double2) * (size_t)n, s);
     }
     return SCMC_OK;
@@ -392,7 +394,8 @@ int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
     launch_nee_cov(P, ps.het ? &ps.hm : nullptr, np, nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
                    (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, f, mixed, sp);
     if (ps.het)
-        launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p, sp);
+        launch_hetero_mean(P, np, (const PathRec *)tc.paths.p, f.EV0, f.UL0, (double2 *)mean.p,
+                           (double2 *)mean_part.p, sp);
     (void)hipEventRecord(S.ev[1], sp);
     nee_done = S.ev[1];
     // the two rank-K problems of the chunk: multiple scattering, and single

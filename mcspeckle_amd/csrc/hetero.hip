@@ -9,6 +9,8 @@
 #include "hetero.h"
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace scmc {
 
 template <int DIM>
@@ -102,19 +104,24 @@ __global__ __launch_bounds__(256) void hetero_emit_kernel(DevProblem P, HetMediu
 // MCCovHetro.m:152, 210-211: mean1(v,l) += W_1/px af_ang_vl(v,l) e_v0(v) u_l0(l)
 // with u_l0 the unattenuated light term (evalphaseatt with sigt = 0).  EV0
 // holds W_1/px * e_v0 per path, UL0 u_l0 per path (both written by the NEE
-// path kernel).  One thread per (v, l), paths summed in order.
+// path kernel).  Thread per (v, l), workgroup column per range of paths
+// (summed in order); the ranges' partials are added in order by
+// hetero_mean_reduce_kernel.
 template <int DIM>
-__global__ __launch_bounds__(256) void hetero_mean_kernel(DevProblem P, int64_t np, const PathRec *paths,
-                                                          const double2 *EV0, const double2 *UL0, double2 *mean)
+__global__ __launch_bounds__(256) void hetero_mean_kernel(DevProblem P, int64_t np, int nsplit, const PathRec *paths,
+                                                          const double2 *EV0, const double2 *UL0, double2 *part)
 {
+    const int64_t n = (int64_t)P.nv * P.nl;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)P.nv * P.nl) return;
+    if (idx >= n) return;
     const int v = (int)(idx % P.nv), l = (int)(idx / P.nv);
+    const int split = blockIdx.y;
+    const int64_t p0 = np * split / nsplit, p1 = np * (split + 1) / nsplit;
     const bool far = P.is_ff_l && P.is_ff_v;
     double2 af = far ? eval_amp<DIM>(dot<DIM>(P.l + (size_t)l * DIM, P.v + (size_t)v * DIM), P.amp)
                      : make_double2(0.0, 0.0);
     double2 acc = make_double2(0.0, 0.0);
-    for (int64_t p = 0; p < np; ++p) {
+    for (int64_t p = p0; p < p1; ++p) {
         if (!far) {
             double rl[3], rv[3];
             light_dir<DIM>(P, paths[p].x1, l, rl);
@@ -125,8 +132,20 @@ __global__ __launch_bounds__(256) void hetero_mean_kernel(DevProblem P, int64_t 
         acc.x += t.x;
         acc.y += t.y;
     }
-    mean[idx].x += acc.x;
-    mean[idx].y += acc.y;
+    part[(size_t)split * n + idx] = acc;
+}
+
+__global__ void hetero_mean_reduce_kernel(int64_t n, int nsplit, const double2 *part, double2 *mean)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    double2 s = mean[idx];
+    for (int k = 0; k < nsplit; ++k) {
+        const double2 t = part[(size_t)k * n + idx];
+        s.x += t.x;
+        s.y += t.y;
+    }
+    mean[idx] = s;
 }
 
 void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
@@ -148,15 +167,18 @@ void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, i
 }
 
 void launch_hetero_mean(const DevProblem &P, int64_t np, const PathRec *paths, const double2 *EV0,
-                        const double2 *UL0, double2 *mean, hipStream_t s)
+                        const double2 *UL0, double2 *mean, double2 *part, hipStream_t s)
 {
     const int64_t n = (int64_t)P.nv * P.nl;
     if (np <= 0 || n <= 0) return;
-    dim3 g((unsigned)((n + 255) / 256));
+    int nsplit = (int)std::min<int64_t>(kHetMeanSplits, std::max<int64_t>(1, np / 128));
+    dim3 g((unsigned)((n + 255) / 256), (unsigned)nsplit);
     if (P.dim == 3)
-        hipLaunchKernelGGL(hetero_mean_kernel<3>, g, dim3(256), 0, s, P, np, paths, EV0, UL0, mean);
+        hipLaunchKernelGGL(hetero_mean_kernel<3>, g, dim3(256), 0, s, P, np, nsplit, paths, EV0, UL0, part);
     else
-        hipLaunchKernelGGL(hetero_mean_kernel<2>, g, dim3(256), 0, s, P, np, paths, EV0, UL0, mean);
+        hipLaunchKernelGGL(hetero_mean_kernel<2>, g, dim3(256), 0, s, P, np, nsplit, paths, EV0, UL0, part);
+    hipLaunchKernelGGL(hetero_mean_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, nsplit, part,
+                       mean);
 }
 
 } // namespace scmc

@@ -57,9 +57,11 @@ void launch_nee_cov(const DevProblem &P, const HetMedium *M, int64_t np, int64_t
 // Walks of MCCovHetro.m: count pass (emit = false, fills counts) or emit pass.
 void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
                          int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit, hipStream_t s);
-// mean1 += sum_p af_ang_vl EV0[p] UL0[p]^T (MCCovHetro.m:210-211).
+// mean1 += sum_p af_ang_vl EV0[p] UL0[p]^T (MCCovHetro.m:210-211); part holds
+// kHetMeanSplits * nv * nl double2.
+constexpr int kHetMeanSplits = 64;
 void launch_hetero_mean(const DevProblem &P, int64_t np, const PathRec *paths, const double2 *EV0,
-                        const double2 *UL0, double2 *mean, hipStream_t s);
+                        const double2 *UL0, double2 *mean, double2 *part, hipStream_t s);
 
 // ------------------------------------------------------------ HERK
 struct HerkProblem {
