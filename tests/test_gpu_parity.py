@@ -156,6 +156,36 @@ def test_gpu_headline_size_mixed_vs_fp64(oracle):
     assert err <= 1e-4
 
 
+def test_gpu_headline_size_multi_chunk_pipeline():
+    """12 000 paths at 4096^2 run as several chunks in one call (the tracer of
+    chunk k+1 beside the update of chunk k, the two buffer slots reused):
+    mixed vs FP64 on the same paths <= 1e-4, and FP64 in one call vs the same
+    paths in four session slices <= 1e-12."""
+    rng = np.random.default_rng(2024)
+    l, v = cone_dirs(64, 15, rng), cone_dirs(64, 15, rng)
+    spec = E.make_problem(20.0, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.9, seed=7)
+    n = 12000
+    say("headline multi-chunk fp64")
+    with E.Session(spec, "cov", "fp64") as s:
+        s.run(0, n)
+        Msd, Mmd = s.result(n)
+        assert s.stats()["chunks"] >= 3
+    say("headline multi-chunk mixed")
+    with E.Session(spec, "cov", "mixed") as s:
+        s.run(0, n)
+        Msm, Mmm = s.result(n)
+        assert s.stats()["chunks"] >= 2
+    err = cs_err(Msm + Mmm, Msd + Mmd)
+    say("  mixed cs err", err)
+    assert err <= 1e-4
+    say("headline fp64 in slices")
+    with E.Session(spec, "cov", "fp64") as s:
+        for a in range(0, n, 3000):
+            s.run(a, 3000)
+        Ms1, Mm1 = s.result(n)
+    assert cs_err(Ms1, Msd) <= 1e-12 and cs_err(Mm1, Mmd) <= 1e-12
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_agrees_with_oracle_within_3_standard_errors(oracle, precision):
     """Independent seeds: per covariance entry (real and imaginary parts of the
@@ -273,3 +303,39 @@ def test_gpu_near_field_tile_edges(oracle, precision, tol):
     Ms, Mm = E.mccov_replay(spec, ps, n, precision)
     assert cs_err(Ms, Ms0) <= tol
     assert cs_err(Mm, Mm0) <= tol
+
+
+# ------------------------------------------------ sessions and tiny runs
+@pytest.mark.parametrize("kind", ["cov", "field"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_gpu_session_slices_equal_one_run(kind, precision, tol):
+    """A solve resumed in slices of paths (sessions: run(first, n) adds paths
+    [first, first + n)) equals the same paths in one call; the slices cut
+    through chunks at other places, so FP64 agrees to rounding."""
+    spec = build("hg", 3, True, 2, None, nv=6, nl=4, seed=3)
+    say("session slices", kind, precision)
+    n, cuts = 2500, [0, 700, 701, 1900, 2500]
+    with E.Session(spec, kind, precision) as s:
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            s.run(a, b - a)
+        out = s.result(n)
+    if kind == "cov":
+        Ms0, Mm0 = E.mccov_spec(spec, n, precision)
+        assert cs_err(out[0], Ms0) <= tol and cs_err(out[1], Mm0) <= tol
+    else:
+        u0 = E.mcfield_spec(spec, n, precision)
+        assert field_err(out, u0) <= (1e-12 if precision == "fp64" else 2e-5)
+
+
+@pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases()[:3])
+def test_gpu_single_path_runs(oracle, kind, dim, cbs, smp, amp0):
+    """maxItr = 1 (MCcov.m:293-299 and MCfield.m:253-254 normalise by it)."""
+    spec = build(kind, dim, cbs, smp, amp0)
+    say("single path", kind, dim, cbs, smp, amp0)
+    Ms0, Mm0 = oracle.mccov(spec, 1)
+    Ms, Mm = E.mccov_spec(spec, 1, "fp64")
+    scale = max(np.abs(Ms0).max(), np.abs(Mm0).max(), 1e-300)
+    assert np.abs(Ms - Ms0).max() <= 1e-10 * scale and np.abs(Mm - Mm0).max() <= 1e-10 * scale
+    u0 = oracle.mcfield(spec, 1)
+    u = E.mcfield_spec(spec, 1, "fp64")
+    assert np.abs(u - u0).max() <= 1e-10 * max(np.abs(u0).max(), 1e-300)
