@@ -255,7 +255,9 @@ __device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double 
 // WAVE: one output column; coef/coefR hold the light-summed coefficients and
 // the first-vertex entries are left to field_wave_first_kernel.  NLQ:
 // light columns accumulated per pass (1 for one column, else kFieldNL).
-template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false>
+// VPT views per lane (v, v + 256, ...): the entry reads from LDS and the
+// per-entry scalar work are shared between them.
+template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false, int VPT = 1>
 __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
@@ -266,34 +268,38 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     const int ncol = WAVE ? 1 : P.nl;   // columns of coef / coefR / part
     __shared__ FEntry se[kFieldBatch];
     __shared__ double2 sc[kFieldBatch][kFieldNL];
-    const int v = blockIdx.x * kFieldViews + threadIdx.x;
-    const bool vok = v < P.nv;
     const int split = blockIdx.y;
     const int64_t e0 = nent * split / nsplit, e1 = nent * (split + 1) / nsplit;
 
     // per-view constants
-    double vd[3] = {0, 0, 0};
-    if (vok)
-        for (int j = 0; j < DIM; ++j) vd[j] = P.v[(size_t)v * DIM + j];
-    double mv[3] = {-vd[0], -vd[1], -vd[2]};   // ff_sign * v (far field)
-    float vf[3] = {(float)vd[0], (float)vd[1], (float)vd[2]};
-    float inv[3] = {0, 0, 0};
-    bool up[3] = {false, false, false};   // nd_j > 0 -> distance to the max face
-    for (int j = 0; j < DIM; ++j) {
-        double nd = -mv[j];
-        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
-        up[j] = nd > 0;
+    int vv[VPT];
+    bool vok[VPT];
+    double vd[VPT][3], mv[VPT][3];
+    float vf[VPT][3], inv[VPT][3];
+    bool up[VPT][3];   // nd_j > 0 -> distance to the max face
+    double2 afv[VPT][NLQ], acc[VPT][NLQ];
+#pragma unroll
+    for (int t = 0; t < VPT; ++t) {
+        const int v = (blockIdx.x * VPT + t) * kFieldViews + threadIdx.x;
+        vv[t] = v;
+        vok[t] = v < P.nv;
+        for (int j = 0; j < 3; ++j) vd[t][j] = (vok[t] && j < DIM) ? P.v[(size_t)v * DIM + j] : 0.0;
+        for (int j = 0; j < 3; ++j) {
+            mv[t][j] = -vd[t][j];   // ff_sign * v (far field)
+            vf[t][j] = (float)vd[t][j];
+            inv[t][j] = (float)(1.0 / fmax(fabs(vd[t][j]), 1.0000000000000004e-07));
+            up[t][j] = vd[t][j] > 0;
+        }
+        for (int q = 0; q < NLQ; ++q) {
+            afv[t][q] = (!WAVE && vok[t] && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)]
+                                                      : make_double2(0.0, 0.0);
+            acc[t][q] = make_double2(0.0, 0.0);
+        }
     }
-    double2 afv[NLQ];
-    for (int q = 0; q < NLQ; ++q)
-        afv[q] = (!WAVE && vok && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)] : make_double2(0.0, 0.0);
     const float g = (float)P.amp.g;
     const float hg_k = (float)sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
     const float g2p1 = 1.f + g * g, g2 = 2.f * g;
     const float hs = (float)(P.sigt * 0.5);
-
-    double2 acc[NLQ];
-    for (int q = 0; q < NLQ; ++q) acc[q] = make_double2(0.0, 0.0);
 
     for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
         const int cnt = (int)((e1 - eb) < kFieldBatch ? (e1 - eb) : kFieldBatch);
@@ -309,52 +315,66 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             sc[ei][q] = c;
         }
         __syncthreads();
-        if (!vok) continue;
+        if (!vok[0]) continue;
         for (int ei = 0; ei < cnt; ++ei) {
             const FEntry &f = se[ei];
             // every lane reads the same entry: branch on its kind as a scalar
             const int kind = __builtin_amdgcn_readfirstlane(f.kind);
             // wave first vertices: field_wave_first_kernel
             if (WAVE && kind == 0) continue;
-            double2 ev;
-            if (MIXED) {
-                float2 evf = e_far_mixed<DIM>(P, f.x, mv, f.dlo, f.dhi, up, inv, hs);
-                if (kind != 0) {
-                    float2 am;
-                    if (P.amp.type == 2) {
-                        am = amp_tab<DIM>(dot<DIM>(f.ad, vd), P.amp);
-                    } else {
-                        float c = f.adf[0] * vf[0] + f.adf[1] * vf[1] + (DIM == 3 ? f.adf[2] * vf[2] : 0.f);
-                        am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
-                    }
-                    evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
-                }
-                ev = make_double2(evf.x, evf.y);
-            } else {
-                ev = nee_e<DIM, HET>(P, M, f.x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
-                if (kind != 0) {
-                    double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)v * DIM), P.amp);
-                    ev = cmul(ev, a);
-                }
+            const double x[3] = {f.x[0], f.x[1], f.x[2]};
+            float dlo[3], dhi[3], adf[3];
+            for (int j = 0; j < 3; ++j) {
+                dlo[j] = f.dlo[j];
+                dhi[j] = f.dhi[j];
+                adf[j] = f.adf[j];
             }
-            if (kind == 2) ev = cmul(ev, f.div);
-            if (kind == 0) {
-                for (int q = 0; q < NLQ; ++q) {
-                    const double2 t = cmul(ev, afv[q]), c = sc[ei][q];
-                    acc[q].x += t.x * c.x - t.y * c.y;
-                    acc[q].y += t.x * c.y + t.y * c.x;
+            double2 sce[NLQ];
+            for (int q = 0; q < NLQ; ++q) sce[q] = sc[ei][q];
+#pragma unroll
+            for (int t = 0; t < VPT; ++t) {
+                double2 ev;
+                if (MIXED) {
+                    float2 evf = e_far_mixed<DIM>(P, x, mv[t], dlo, dhi, up[t], inv[t], hs);
+                    if (kind != 0) {
+                        float2 am;
+                        if (P.amp.type == 2) {
+                            am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
+                        } else {
+                            float c = adf[0] * vf[t][0] + adf[1] * vf[t][1] + (DIM == 3 ? adf[2] * vf[t][2] : 0.f);
+                            am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
+                        }
+                        evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
+                    }
+                    ev = make_double2(evf.x, evf.y);
+                } else {
+                    ev = nee_e<DIM, HET>(P, M, x, P.v + (size_t)vv[t] * DIM, P.ff_sign, P.is_ff_v);
+                    if (kind != 0) {
+                        double2 a = eval_amp<DIM>(dot<DIM>(f.ad, P.v + (size_t)vv[t] * DIM), P.amp);
+                        ev = cmul(ev, a);
+                    }
                 }
-            } else {
-                for (int q = 0; q < NLQ; ++q) {
-                    const double2 c = sc[ei][q];
-                    acc[q].x += ev.x * c.x - ev.y * c.y;
-                    acc[q].y += ev.x * c.y + ev.y * c.x;
+                if (kind == 2) ev = cmul(ev, f.div);
+                if (kind == 0) {
+                    for (int q = 0; q < NLQ; ++q) {
+                        const double2 tt = cmul(ev, afv[t][q]), c = sce[q];
+                        acc[t][q].x += tt.x * c.x - tt.y * c.y;
+                        acc[t][q].y += tt.x * c.y + tt.y * c.x;
+                    }
+                } else {
+                    for (int q = 0; q < NLQ; ++q) {
+                        const double2 c = sce[q];
+                        acc[t][q].x += ev.x * c.x - ev.y * c.y;
+                        acc[t][q].y += ev.x * c.y + ev.y * c.x;
+                    }
                 }
             }
         }
     }
-    if (vok)
-        for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
+#pragma unroll
+    for (int t = 0; t < VPT; ++t)
+        if (vok[t])
+            for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + vv[t]] = acc[t][q];
 }
 
 // Near-field sources (either side): the entrance directions rl/rv depend on
@@ -621,7 +641,11 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
     if (!near) launch_af_table(P, aftab, s);
     const int64_t nent = nvert + (P.do_cbs ? np : 0);
     const int vt = (P.nv + kFieldViews - 1) / kFieldViews;
-    int nsplit = (2048 + vt - 1) / vt;
+    // the mixed one-column kernel takes two views per lane: its workgroups
+    // cover 512 views, so size the entry splits on that
+    const int vpt = (mixed && ncol == 1 && !het && !near) ? 2 : 1;
+    const int vtw = (P.nv + vpt * kFieldViews - 1) / (vpt * kFieldViews);
+    int nsplit = (2048 + vtw - 1) / vtw;
     const int64_t maxsplit = (nent + kFieldBatch - 1) / kFieldBatch;
     if (nsplit > maxsplit) nsplit = (int)maxsplit;
     if (nsplit > 64) nsplit = 64;
@@ -648,10 +672,15 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
             else
                 SCMC_FIELD_NEAR((field_near_kernel<DIM, false>));
         } else if (mixed) {
+            dim3 g2((unsigned)((P.nv + 2 * kFieldViews - 1) / (2 * kFieldViews)), (unsigned)nsplit);
             if (wave)
-                SCMC_FIELD((field_kernel<DIM, true, true, 1>));
+                hipLaunchKernelGGL((field_kernel<DIM, true, true, 1, false, 2>), g2, dim3(kFieldViews), 0, s, P, M, np,
+                                   nvert, nent, nsplit, l0, nlp, paths, verts, vpath, offsets, coef, coefR, aftab, cl0,
+                                   part);
             else if (ncol == 1)
-                SCMC_FIELD((field_kernel<DIM, true, false, 1>));
+                hipLaunchKernelGGL((field_kernel<DIM, true, false, 1, false, 2>), g2, dim3(kFieldViews), 0, s, P, M,
+                                   np, nvert, nent, nsplit, l0, nlp, paths, verts, vpath, offsets, coef, coefR, aftab,
+                                   cl0, part);
             else
                 SCMC_FIELD((field_kernel<DIM, true, false, kFieldNL>));
         } else {
