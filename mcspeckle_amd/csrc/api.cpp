@@ -51,6 +51,9 @@ void DevBuf::release()
 bool DevBuf::ensure(size_t b, int *rc)
 {
     if (b <= bytes && p) return true;
+    // hipFree synchronises the device, so a buffer that kernels queued on
+    // another stream still read (the path records of the previous chunk) is
+    // only released once they have finished
     release();
     size_t want = b ? b : 16;
     want = (want + 255) & ~(size_t)255;
@@ -122,6 +125,7 @@ int ProblemState::setup(const scmc_problem *pr, hipStream_t s)
     const int dim = pr->dim;
     DevProblem &P = dp;
     memset(&P, 0, sizeof P);
+    het = false;   // setup_medium sets it (a reused state may have held a medium)
     P.dim = dim;
     P.sigt = pr->sigt;
     P.albedo = pr->albedo;
@@ -289,14 +293,24 @@ int TraceChunk::load(const scmc_pathset *ps, hipStream_t s)
 }
 
 // ---------------------------------------------------- covariance engine
+// (Re)start a solve.  A reused engine keeps its device buffers (they only
+// grow) and resets everything that belongs to the solve: accumulator copies
+// (zeroed on first use), the tile order when N changes, statistics.
 int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
 {
     int rc = SCMC_OK;
+    if ((rc = drain())) return rc;
     mixed = mixed_;
     nv = ps.dp.nv;
     nl = ps.dp.nl;
     n = nv * nl;
     tile = mixed ? kTileMixed : kTileF64;
+    ks_m = ks_s = 0;
+    last_scale_log2 = 0;
+    ms_herk = herk_flops = ms_expand = ms_accum = ms_nee = 0;
+    herk_launches = 0;
+    nee_done = nullptr;
+    if (order_n != n) order.release();
     if (ps.het) {
         if (!mean.ensure(sizeof(double2) * (size_t)n, &rc) ||
             !mean_part.ensure(sizeof(double2) * (size_t)n * kHetMeanSplits, &rc))
@@ -323,9 +337,16 @@ int CovEngine::ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s)
 {
     int rc = SCMC_OK;
     if (acc.p && ks <= ks_have) return SCMC_OK;
+    size_t one = sizeof(double2) * (size_t)n * n;
+    if (ks_have == 0) {
+        // first use in this solve: reuse the buffer when it is large enough
+        if (!acc.ensure(one * ks, &rc)) return rc;
+        launch_zero(acc.p, one * ks, s);
+        ks_have = ks;
+        return SCMC_OK;
+    }
     // grow: allocate a larger set of copies, copy the existing sums over
     DevBuf nb;
-    size_t one = sizeof(double2) * (size_t)n * n;
     if (!nb.ensure(one * ks, &rc)) return rc;
     launch_zero(nb.p, one * ks, s);
     if (acc.p && ks_have > 0)
@@ -447,6 +468,7 @@ int CovEngine::consume(hipStream_t sc)
     if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), sc))) return rc;
     if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), sc))) return rc;
     if (mixed && !order.p) {
+        order_n = n;
         std::vector<int> o = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
         if (!order.ensure(sizeof(int) * o.size(), &rc)) return rc;
         if (!hip_ok(hipMemcpy(order.p, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice), "tile order", &rc))
@@ -677,8 +699,10 @@ int Session::init(const scmc_problem *pr, int kind_, int precision, const double
     kind = kind_;
     mixed = precision == SCMC_MIXED;
     pc.assign(pr);
-    if (!hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
-    if (kind == 0 && !hip_ok(hipStreamCreate(&stream2), "stream", &rc)) return rc;
+    // a reused session keeps its streams and device buffers
+    if (!stream && !hip_ok(hipStreamCreate(&stream), "stream", &rc)) return rc;
+    if (kind == 0 && !stream2 && !hip_ok(hipStreamCreate(&stream2), "stream", &rc)) return rc;
+    mean_len = 20;
     if ((rc = ps.setup(&pc.pr, stream))) return rc;
     if (med && (rc = ps.setup_medium(&pc.pr, med, stream))) return rc;
     memset(&st, 0, sizeof st);

@@ -4,9 +4,9 @@
 // scmc_submit_mccov.  The server keeps `max_active` slots; a slot takes the
 // first pending request whose device footprint fits the remaining HBM budget
 // (FIFO with backfill), so requests are admitted as slots and memory free up.
-// Each admitted request owns a session (its own HIP stream and accumulators)
-// and advances in slices of paths, so the kernels of all admitted requests
-// overlap on the GPU; a cancel takes effect at the next slice boundary and
+// Each admitted request runs in its slot's session (its own HIP streams and
+// accumulators, reused from request to request) and advances in slices of
+// paths, so the kernels of all admitted requests overlap on the GPU; a cancel takes effect at the next slice boundary and
 // frees the slot.  Results are normalised exactly as scmc_mccov and depend
 // only on the request (Philox paths, fixed slice schedule), not on the mix.
 #include "scmc_internal.h"
@@ -78,9 +78,8 @@ int64_t request_footprint(int nv, int nl, int precision, bool near)
     return (int64_t)(acc + chunk + 64.0 * 1024 * 1024);
 }
 
-static void run_request(Server *S, Request *r)
+static void run_request(Server *S, Request *r, Session &sess)
 {
-    Session sess;
     int rc = sess.init(&r->pc.pr, 0, r->precision);
     const int64_t slice = S->slice > 0 ? S->slice : std::max<int64_t>(1000, r->max_itr / 8);
     for (int64_t p0 = 0; !rc && p0 < r->max_itr; p0 += slice) {
@@ -103,8 +102,14 @@ static void run_request(Server *S, Request *r)
     if (rc) r->err = scmc_last_error();
 }
 
+// A slot keeps one session between requests: its streams and device buffers
+// are reused (they only grow), so admitting a request allocates nothing in
+// the steady state and finishing one frees nothing -- hipFree synchronises
+// the whole device and would stall the other slots.  A session that served a
+// request larger than a 1/(2 max_active) share of the budget is released.
 static void slot_loop(Server *S)
 {
+    std::unique_ptr<Session> sess;
     for (;;) {
         Request *r = nullptr;
         {
@@ -127,7 +132,9 @@ static void slot_loop(Server *S)
             r->state = 1;
             r->t_start = clk::now();
         }
-        run_request(S, r);
+        if (!sess) sess.reset(new Session());
+        run_request(S, r, *sess);
+        if (r->rc || r->footprint * 2 * std::max(1, S->cfg.max_active) > S->budget) sess.reset();
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->used -= r->footprint;
