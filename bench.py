@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--quad-paths", type=int, default=5000000)
     ap.add_argument("--hetro-paths", type=int, default=200000)
     ap.add_argument("--no-legs", action="store_true")
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated subset of the legs: mcfield, quad, wave, quad_wave, hetro, sample_hetro")
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe_mib:
@@ -408,12 +410,19 @@ def main():
 
     # ---- the field and Fourier-field configurations
     if not args.no_legs:
-        out["mcfield"] = field_leg(args.field_paths)
-        out["mcfield_fourier_quad"] = quad_leg(args.quad_paths)
-        out["mcfield_on_wave"] = field_wave_leg(args.field_paths)
-        out["mcfield_fourier_quad_on_wave"] = quad_wave_leg(args.quad_paths)
-        out["mccov_hetro"] = hetro_leg(args.hetro_paths)
-        out["mcsample_hetro"] = sample_hetro_leg(args.hetro_paths)
+        legs = {"mcfield": ("mcfield", lambda: field_leg(args.field_paths)),
+                "quad": ("mcfield_fourier_quad", lambda: quad_leg(args.quad_paths)),
+                "wave": ("mcfield_on_wave", lambda: field_wave_leg(args.field_paths)),
+                "quad_wave": ("mcfield_fourier_quad_on_wave", lambda: quad_wave_leg(args.quad_paths)),
+                "hetro": ("mccov_hetro", lambda: hetro_leg(args.hetro_paths)),
+                "sample_hetro": ("mcsample_hetro", lambda: sample_hetro_leg(args.hetro_paths))}
+        pick = list(legs) if args.legs == "all" else [x.strip() for x in args.legs.split(",") if x.strip()]
+        for name in pick:
+            if name not in legs:
+                print(f"unknown leg {name!r} (known: {', '.join(legs)})", file=sys.stderr)
+                sys.exit(2)
+            key, fn = legs[name]
+            out[key] = fn()
 
     # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
     if not args.no_serve:
