@@ -10,8 +10,9 @@ namespace scmc {
 int64_t field_chunk_paths(int nv, int nl, int64_t remaining, double mean_len)
 {
     (void)nv;
-    (void)nl;
-    int64_t target_vertices = 1 << 21;
+    // vertices per chunk: enough paths to keep the tracer's waves busy, with
+    // the per-vertex coefficients (nl columns of 16 B) near 1 GB
+    int64_t target_vertices = std::max<int64_t>(1 << 21, std::min<int64_t>(1 << 23, (1LL << 26) / std::max(nl, 1)));
     int64_t p = (int64_t)(target_vertices / std::max(mean_len, 1.0));
     p = std::max<int64_t>(256, std::min<int64_t>(p, 1 << 20));
     return std::min(p, remaining);
