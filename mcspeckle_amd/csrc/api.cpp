@@ -201,15 +201,18 @@ int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevB
         a[2 * i + 1] = ph.amp_im ? ph.amp_im[i] : 0.0;
     }
     if (!amp.ensure(a.size() * sizeof(double), &rc) || !icdf.ensure(sizeof(double) * ph.n_icdf, &rc)) return rc;
-    if (!hip_ok(hipMemcpy(amp.p, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice), "copy evalAmp", &rc))
-        return rc;
-    if (!hip_ok(hipMemcpy(icdf.p, ph.icdf, sizeof(double) * ph.n_icdf, hipMemcpyHostToDevice), "copy sampleIcdf", &rc))
+    // on the session's stream (a null-stream copy would wait for every other
+    // session's work); a is local, so wait for the copies here
+    if (!hip_ok(hipMemcpyAsync(amp.p, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy evalAmp",
+                &rc) ||
+        !hip_ok(hipMemcpyAsync(icdf.p, ph.icdf, sizeof(double) * ph.n_icdf, hipMemcpyHostToDevice, s),
+                "copy sampleIcdf", &rc) ||
+        !hip_ok(hipStreamSynchronize(s), "table sync", &rc))
         return rc;
     out.n_amp = ph.n_amp;
     out.amp = (const double2 *)amp.p;
     out.n_icdf = ph.n_icdf;
     out.icdf = (const double *)icdf.p;
-    (void)s;
     return SCMC_OK;
 }
 
@@ -468,10 +471,14 @@ int CovEngine::consume(hipStream_t sc)
     if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), sc))) return rc;
     if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), sc))) return rc;
     if (mixed && !order.p) {
+        // kept in order_host until the next N: the copy runs on the solve's
+        // stream (a null-stream copy would wait for every other session)
         order_n = n;
-        std::vector<int> o = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
-        if (!order.ensure(sizeof(int) * o.size(), &rc)) return rc;
-        if (!hip_ok(hipMemcpy(order.p, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice), "tile order", &rc))
+        order_host = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
+        if (!order.ensure(sizeof(int) * order_host.size(), &rc)) return rc;
+        if (!hip_ok(hipMemcpyAsync(order.p, order_host.data(), sizeof(int) * order_host.size(), hipMemcpyHostToDevice,
+                                   sc),
+                    "tile order", &rc))
             return rc;
     }
     HerkProblem &hm = S.hp[0], &hs = S.hp[1];
