@@ -252,6 +252,20 @@ __device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double 
     return make_float2(mag * cs, mag * sn);
 }
 
+// The same for a voxel medium (far field): the optical depth by the FP64
+// march of evalphaseattHetro, the phase reduced in FP64, sin/cos/exp in FP32.
+template <int DIM>
+__device__ __forceinline__ float2 e_far_mixed_het(const DevProblem &P, const HetMedium &M, const double *x,
+                                                  const double *mv)
+{
+    const double cyc = dot<DIM>(x, mv) * P.inv_lambda;
+    const float fr = (float)(cyc - rint(cyc));
+    const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
+    const float depth = (float)march_depth<DIM>(P, M, x, mv, 1e12);
+    const float mag = __builtin_amdgcn_exp2f(-0.5f * 1.4426950408889634f * depth);
+    return make_float2(mag * cs, mag * sn);
+}
+
 // WAVE: one output column; coef/coefR hold the light-summed coefficients and
 // the first-vertex entries are left to field_wave_first_kernel.  NLQ:
 // light columns accumulated per pass (1 for one column, else kFieldNL).
@@ -335,7 +349,8 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             for (int t = 0; t < VPT; ++t) {
                 double2 ev;
                 if (MIXED) {
-                    float2 evf = e_far_mixed<DIM>(P, x, mv[t], dlo, dhi, up[t], inv[t], hs);
+                    float2 evf = HET ? e_far_mixed_het<DIM>(P, M, x, mv[t])
+                                     : e_far_mixed<DIM>(P, x, mv[t], dlo, dhi, up[t], inv[t], hs);
                     if (kind != 0) {
                         float2 am;
                         if (P.amp.type == 2) {
@@ -662,6 +677,10 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
         if (het) {
             if (near)
                 SCMC_FIELD_NEAR((field_near_kernel<DIM, false, true>));
+            else if (mixed && ncol == 1)
+                SCMC_FIELD((field_kernel<DIM, true, false, 1, true>));
+            else if (mixed)
+                SCMC_FIELD((field_kernel<DIM, true, false, kFieldNL, true>));
             else if (ncol == 1)
                 SCMC_FIELD((field_kernel<DIM, false, false, 1, true>));
             else

@@ -36,6 +36,7 @@ struct HetMedium {
     double pushc;           // max(abs([box_max; box_min])) * dim
     double u[3][2];         // orthonormal basis of the complement of meanl
     int cap;                // bound on cells crossed by one ray
+    double inv_bin[3];      // 1 / bin (the ray march locates cells by multiplication)
 };
 
 #pragma clang fp contract(off)
@@ -79,20 +80,40 @@ __device__ __forceinline__ bool outside_box(const double *x, const DevProblem &P
 
 // evalphaseattHetro.m:29-48: optical depth from x along -v, cell by cell,
 // stepping 1e-4 cell sizes past each face (cubeDist.m:13-16), capped at nd.
+// The cell of a point and the distance to its faces use the reciprocals of
+// the cell size and of the clamped direction components, computed once (the
+// reference divides per cell; the two differ in the last bit only).
 template <int DIM>
 __device__ __forceinline__ double march_depth(const DevProblem &P, const HetMedium &M, const double *x,
                                               const double *v, double nd)
 {
+    const double thr = 1.0000000000000004e-07;   // cubeDist.m:5, 0.1^7 as computed by pow
     double mv[3] = {-v[0], -v[1], -v[2]};
     double nx[3] = {x[0], x[1], x[2]};
+    double inv_ad[3] = {0, 0, 0};
+    bool neg[3] = {false, false, false};
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+        inv_ad[j] = 1.0 / fmax(fabs(mv[j]), thr);
+        neg[j] = mv[j] <= 0;
+    }
     double bdv = 0, wbdv = 0;
     for (int it = 0; it < M.cap; ++it) {
-        double bmn[3], bmx[3];
-        const int b = find_bin<DIM>(P, M, nx, bmn, bmx);
-        const double dist = cube_dist<DIM>(nx, bmn, bmx, mv);
-        double ep = 0;
+        int b = 0, stride = 1;
+        double dist = INFINITY, ep = 0;
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) ep = fmax(ep, fabs(bmx[j] - bmn[j]));
+        for (int j = 0; j < DIM; ++j) {
+            // findBin.m (0-based, a point on a lower face clamped to the first cell)
+            int ii = (int)ceil((nx[j] - P.bmin[j]) * M.inv_bin[j]);
+            ii = ii < 1 ? 1 : (ii > M.nb[j] ? M.nb[j] : ii);
+            b += (ii - 1) * stride;
+            stride *= M.nb[j];
+            const double bmn = P.bmin[j] + M.bin[j] * (ii - 1), bmx = bmn + M.bin[j];
+            // cubeDist.m
+            const double bd = neg[j] ? (nx[j] - bmn) * inv_ad[j] : (-nx[j] + bmx) * inv_ad[j];
+            dist = bd < dist ? bd : dist;
+            ep = fmax(ep, fabs(bmx - bmn));
+        }
         ep = 0.00010000000000000002 * ep;   // 0.1^4 as computed by pow
 #pragma unroll
         for (int j = 0; j < DIM; ++j) nx[j] = nx[j] + (dist + ep) * mv[j];

@@ -89,6 +89,7 @@ int ProblemState::setup_medium(const scmc_problem *pr, const scmc_medium *m, hip
     for (int j = 0; j < 3; ++j) {
         M.nb[j] = 1;
         M.bin[j] = 1.0;
+        M.inv_bin[j] = 1.0;
     }
     for (int j = 0; j < dim; ++j) {
         if (!(m->box_bin[j] > 0)) return set_error(SCMC_E_ARG, "box_bin must be positive");
@@ -99,6 +100,7 @@ int ProblemState::setup_medium(const scmc_problem *pr, const scmc_medium *m, hip
                              m->nb[j], j + 1, nb);
         M.nb[j] = nb;
         M.bin[j] = m->box_bin[j];
+        M.inv_bin[j] = 1.0 / m->box_bin[j];
         nvox *= nb;
     }
     std::vector<double> sig(m->sigt, m->sigt + nvox);

@@ -101,7 +101,8 @@ void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s);
 // FP64 accumulators acc[nv*nl] (column-major, v fastest).  With wl (device,
 // nl weights) the lights form one wave (MCfieldOnWave.m) and acc has nv
 // entries.  With M (a voxel medium, MCsampleHetro.m) E is the ray march of
-// evalphaseattHetro, in FP64 in both precision modes.  work holds
+// evalphaseattHetro (FP64 march; the mixed mode evaluates sin/cos/exp in
+// FP32 in the far field).  work holds
 // field_work_elems(...) double2.
 void launch_field(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
                   const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc,

@@ -4,9 +4,11 @@ Philox walks.
 Tolerances (Cauchy-Schwarz metric for Ms, Mm; max |d| / max |ref| for mean1
 and for the field u): FP64 <= 1e-10 (walks identical, transcendental ulps
 aside), mixed precision <= 1e-4 for the covariance; mean1 is accumulated
-from FP64 factors in both modes (<= 1e-10), and so is MCsampleHetro's u
-(E by the FP64 march in both modes, <= 1e-10).  A uniform one-cell medium
-equals the GPU MCcov / MCfield to 1e-12.
+from FP64 factors in both modes (<= 1e-10).  MCsampleHetro's u: FP64 <=
+1e-10; mixed <= 2e-5 in the far field (FP64 march and phase reduction,
+FP32 sin/cos/exp, as MCfield's mixed mode), FP64 in the near field
+(<= 1e-10).  A uniform one-cell medium equals the GPU MCcov / MCfield to
+1e-12.
 """
 import sys
 
@@ -154,11 +156,11 @@ def test_gpu_sample_hetro_matches_oracle(oracle, kind, dim, cbs, smp, near, albg
     n = 800
     u0, K = oracle.mcfield_hetro(spec, grid, bb, n, albedo=alb)
     assert K > 0
-    for precision in ("fp64", "mixed"):
+    for precision, tol in (("fp64", 1e-10), ("mixed", 1e-10 if near != "far" else 2e-5)):
         u = run_gpu_sample(spec, grid, bb, n, precision, albedo=alb)
         err = np.abs(u - u0).max() / np.abs(u0).max()
         say("  ", precision, err)
-        assert err <= 1e-10
+        assert err <= tol
 
 
 def test_gpu_sample_hetro_lmean0(oracle):
