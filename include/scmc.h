@@ -110,6 +110,10 @@ typedef struct {
     int64_t herk_launches;   /* number of those launches                        */
     double herk_flops;       /* their algorithmic flops, 8 N^2 K                */
     double ms_expand;        /* mixed: FP16 hi/lo row expansion before them     */
+    int64_t nonfinite;       /* vertices (covariance) or paths (field) whose
+                                factors came out NaN/Inf (e.g. a NaN path weight
+                                in a replayed set); the results then hold NaN,
+                                as the reference's would                        */
 } scmc_stats;
 
 /* Replayable path set (vertex-major), the format of the oracle's sampler and
@@ -215,12 +219,13 @@ int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int
                double *x1, double *px, double *w1, double *w0p_re, double *w0p_im,
                double *x, double *ow, double *weight, double *xi);
 /* MCcov contributions (MCcov.m:181-264) of a given path set, normalised by
- * max_itr; the GPU covariance kernels with the tracer bypassed. */
+ * max_itr; the GPU covariance kernels with the tracer bypassed.  stats (may
+ * be NULL) receives the path / vertex counts and the non-finite count. */
 int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t max_itr, int32_t precision,
-                      double *ms_re, double *ms_im, double *mm_re, double *mm_im);
+                      double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats);
 /* MCfield contributions (MCfield.m:194-222) of a given path set. */
 int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t max_itr, int32_t precision,
-                        double *u_re, double *u_im);
+                        double *u_re, double *u_im, scmc_stats *stats);
 
 /* ---- serving: device-resident continuous batching ---------------------- */
 

@@ -65,7 +65,7 @@ class Stats(C.Structure):
         ("ms_trace", C.c_double), ("ms_nee", C.c_double), ("ms_accum", C.c_double),
         ("ms_total", C.c_double), ("fp16_scale_log2", C.c_double),
         ("ms_herk", C.c_double), ("herk_launches", C.c_int64), ("herk_flops", C.c_double),
-        ("ms_expand", C.c_double),
+        ("ms_expand", C.c_double), ("nonfinite", C.c_int64),
     ]
 
     def as_dict(self):
@@ -126,8 +126,8 @@ def lib():
         "scmc_mccov_hetro": ([P, C.POINTER(Medium), C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, S], C.c_int),
         "scmc_mcfield_hetro": ([P, C.POINTER(Medium), C.c_int64, C.c_int32, _dp, _dp, S], C.c_int),
         "scmc_trace": ([P, C.c_int64, _i64p, _i64p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
-        "scmc_mccov_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp, _dp, _dp], C.c_int),
-        "scmc_mcfield_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp], C.c_int),
+        "scmc_mccov_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, S], C.c_int),
+        "scmc_mcfield_replay": ([P, C.POINTER(PathSet), C.c_int64, C.c_int32, _dp, _dp, S], C.c_int),
         "scmc_session_create": ([P, C.c_int32, C.c_int32, _i64p], C.c_int),
         "scmc_session_run": ([C.c_int64, C.c_int64, C.c_int64], C.c_int),
         "scmc_session_result": ([C.c_int64, C.c_int64, _dp, _dp, _dp, _dp], C.c_int),

@@ -314,6 +314,8 @@ int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
     herk_launches = 0;
     nee_done = nullptr;
     if (order_n != n) order.release();
+    if (!nonfinite.ensure(sizeof(unsigned long long), &rc)) return rc;
+    launch_zero(nonfinite.p, sizeof(unsigned long long), s);
     if (ps.het) {
         if (!mean.ensure(sizeof(double2) * (size_t)n, &rc) ||
             !mean_part.ensure(sizeof(double2) * (size_t)n * kHetMeanSplits, &rc))
@@ -408,6 +410,7 @@ int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
     f.maxG = (float *)S.maxG.p;
     f.maxF2 = (float *)S.maxF2.p;
     f.bound = (unsigned *)S.bound.p;
+    f.nonfinite = (unsigned long long *)nonfinite.p;
     if (ps.het) {
         if (!S.EV0.ensure(sizeof(double2) * (size_t)np * nv, &rc) || !S.UL0.ensure(sizeof(double2) * (size_t)np * nl, &rc))
             return rc;
@@ -538,6 +541,19 @@ int CovEngine::harvest(CovSlot &S)
         last_scale_log2 = 15 - ex;
     }
     S.used = false;
+    return SCMC_OK;
+}
+
+// The solve's count of vertices with NaN/Inf factors (after the stream's work).
+int CovEngine::read_nonfinite(int64_t *out, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    unsigned long long h = 0;
+    if (nonfinite.p &&
+        (!hip_ok(hipMemcpyAsync(&h, nonfinite.p, sizeof h, hipMemcpyDeviceToHost, s), "copy nonfinite", &rc) ||
+         !hip_ok(hipStreamSynchronize(s), "nonfinite sync", &rc)))
+        return rc;
+    *out = (int64_t)h;
     return SCMC_OK;
 }
 
@@ -761,8 +777,9 @@ int Session::run(int64_t first, int64_t n)
         if ((rc = ce.drain())) return rc;
         st.ms_accum = ce.ms_accum;
         st.ms_nee = ce.ms_nee;
+        return ce.read_nonfinite(&st.nonfinite, stream);
     }
-    return SCMC_OK;
+    return fe.read_nonfinite(&st.nonfinite, stream);
 }
 
 int Session::result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im)
@@ -941,7 +958,7 @@ int scmc_trace(const scmc_problem *pr, int64_t n_paths, int64_t *n_vertices, int
 }
 
 int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t max_itr, int32_t precision,
-                      double *ms_re, double *ms_im, double *mm_re, double *mm_im)
+                      double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
 {
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
@@ -956,13 +973,20 @@ int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t 
         if (!(rc = ps.setup(pr, s)) && !(rc = ce.init(ps, precision == SCMC_MIXED, s)) && !(rc = tc.load(pset, s)) &&
             !(rc = ce.chunk(ps, tc, s)))
             rc = ce.finalize(ps, pr, max_itr, ms_re, ms_im, mm_re, mm_im, s);
+        if (!rc && stats) {
+            memset(stats, 0, sizeof *stats);
+            stats->paths = tc.np;
+            stats->vertices = tc.nvert;
+            stats->chunks = 1;
+            rc = ce.read_nonfinite(&stats->nonfinite, s);
+        }
     }
     (void)hipStreamDestroy(s);
     return rc;
 }
 
 int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_t max_itr, int32_t precision,
-                        double *u_re, double *u_im)
+                        double *u_re, double *u_im, scmc_stats *stats)
 {
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
@@ -978,6 +1002,14 @@ int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *pset, int64_
         if (!(rc = ps.setup(pr, s)) && !(rc = fe.init(ps, s)) && !(rc = tc.load(pset, s)) &&
             !(rc = fe.chunk(ps, tc, precision == SCMC_MIXED, s, &a)))
             rc = fe.finalize(pr, max_itr, u_re, u_im, s);
+        if (!rc && stats) {
+            memset(stats, 0, sizeof *stats);
+            stats->paths = tc.np;
+            stats->vertices = tc.nvert;
+            stats->chunks = 1;
+            stats->ms_accum = a;
+            rc = fe.read_nonfinite(&stats->nonfinite, s);
+        }
     }
     (void)hipStreamDestroy(s);
     return rc;

@@ -35,13 +35,15 @@ constexpr int kFieldNL = 4;        // lights per launch pass
 
 // Per-path / per-vertex coefficients.  Block per path, lanes over lights.
 // HET: voxel medium (MCsampleHetro.m:141-176, evalphaseattHetro).
+// nonfinite += paths with a NaN/Inf coefficient (path weights, positions).
 template <int DIM, bool HET>
 __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium M, int64_t np, const PathRec *paths,
                                                         const VertexRec *verts, const int64_t *offsets,
-                                                        double2 *coef, double2 *coefR)
+                                                        double2 *coef, double2 *coefR, unsigned long long *nonfinite)
 {
     const int64_t p = blockIdx.x;
     if (p >= np) return;
+    bool bad = false;
     const PathRec pr = paths[p];
     const int64_t o = offsets[p], e = offsets[p + 1];
     const double rs2 = 1 / sqrt(2.0);
@@ -76,9 +78,12 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium 
                 }
             }
             coef[vi * P.nl + l] = t;
+            bad |= !(isfinite(t.x) && isfinite(t.y));
         }
         if (P.do_cbs) coefR[p * P.nl + l] = R;
+        bad |= !(isfinite(R.x) && isfinite(R.y));
     }
+    if (__syncthreads_or(bad) && threadIdx.x == 0 && nonfinite) atomicAdd(nonfinite, 1ULL);
 }
 
 __device__ __forceinline__ double2 wave_sum(double2 v)
@@ -101,10 +106,11 @@ template <int DIM>
 __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64_t np, const PathRec *paths,
                                                              const VertexRec *verts, const int64_t *offsets,
                                                              const double2 *wl, double2 *coef, double2 *coefR,
-                                                             double2 *cl0)
+                                                             double2 *cl0, unsigned long long *nonfinite)
 {
     const int64_t p = blockIdx.x;
     if (p >= np) return;
+    bool bad = false;
     const PathRec pr = paths[p];
     const int64_t o = offsets[p], e = offsets[p + 1];
     const double rs2 = 1 / sqrt(2.0);
@@ -128,6 +134,7 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
         s0.x += t.x;
         s0.y += t.y;
         cl0[p * P.nl + l] = cmul(cmul(el0, wl[l]), c1);
+        bad |= !(isfinite(cl0[p * P.nl + l].x) && isfinite(cl0[p * P.nl + l].y));
     }
     s0 = wave_sum(s0);
     for (int64_t vi = o + lane; vi < e; vi += 64) {
@@ -140,6 +147,7 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
             }
         }
         coef[vi] = t;
+        bad |= !(isfinite(t.x) && isfinite(t.y));
     }
     if (P.do_cbs) {
         double2 R = make_double2(0.0, 0.0);
@@ -163,7 +171,9 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
             R.y += q.y * rs2;
         }
         if (lane == 0) coefR[p] = R;
+        bad |= !(isfinite(R.x) && isfinite(R.y));
     }
+    if (__syncthreads_or(bad) && threadIdx.x == 0 && nonfinite) atomicAdd(nonfinite, 1ULL);
 }
 
 struct FEntry {
@@ -628,7 +638,7 @@ __global__ void field_reduce_kernel(const double2 *part, int nsplit, int64_t n, 
 template <int DIM>
 static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np, int64_t nvert, const PathRec *paths,
                            const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work,
-                           double2 *acc, bool mixed, const double2 *wl, hipStream_t s)
+                           double2 *acc, bool mixed, const double2 *wl, unsigned long long *nonfinite, hipStream_t s)
 {
     // a voxel medium (MCsampleHetro) has no wave variant and marches in FP64
     const bool het = Mp != nullptr;
@@ -645,13 +655,13 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
     double2 *part = cl0 + (wave ? (size_t)np * P.nl : 0);
     if (wave)
         hipLaunchKernelGGL(field_wave_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts,
-                           offsets, wl, coef, coefR, cl0);
+                           offsets, wl, coef, coefR, cl0, nonfinite);
     else if (het)
         hipLaunchKernelGGL((field_coef_kernel<DIM, true>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
-                           offsets, coef, coefR);
+                           offsets, coef, coefR, nonfinite);
     else
         hipLaunchKernelGGL((field_coef_kernel<DIM, false>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
-                           offsets, coef, coefR);
+                           offsets, coef, coefR, nonfinite);
     const bool near = !(P.is_ff_l && P.is_ff_v);
     if (!near) launch_af_table(P, aftab, s);
     const int64_t nent = nvert + (P.do_cbs ? np : 0);
@@ -744,12 +754,12 @@ size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave)
 
 void launch_field(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
                   const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc,
-                  bool mixed, const double2 *wl, hipStream_t s)
+                  bool mixed, const double2 *wl, unsigned long long *nonfinite, hipStream_t s)
 {
     if (P.dim == 3)
-        field_dispatch<3>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
+        field_dispatch<3>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, nonfinite, s);
     else
-        field_dispatch<2>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, s);
+        field_dispatch<2>(P, M, np, nvert, paths, verts, vpath, offsets, work, acc, mixed, wl, nonfinite, s);
 }
 
 } // namespace scmc

@@ -26,8 +26,10 @@ int FieldEngine::init(const ProblemState &ps, hipStream_t s, const double *wl_re
     wave = wl_re != nullptr;
     het = ps.het;
     ncol = wave ? 1 : nl;
-    if (!acc.ensure(sizeof(double2) * (size_t)nv * ncol, &rc)) return rc;
+    if (!acc.ensure(sizeof(double2) * (size_t)nv * ncol, &rc) || !nonfinite.ensure(sizeof(unsigned long long), &rc))
+        return rc;
     launch_zero(acc.p, sizeof(double2) * (size_t)nv * ncol, s);
+    launch_zero(nonfinite.p, sizeof(unsigned long long), s);
     if (wave) {
         // the light weights Wl of MCfieldOnWave.m (nl complex)
         std::vector<double> h((size_t)nl * 2);
@@ -55,7 +57,7 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
     launch_field(ps.dp, ps.het ? &ps.hm : nullptr, tc.np, tc.nvert, (const PathRec *)tc.paths.p,
                  (const VertexRec *)tc.verts.p,
                  (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, (double2 *)work.p, (double2 *)acc.p, mixed,
-                 wave ? (const double2 *)wl.p : nullptr, s);
+                 wave ? (const double2 *)wl.p : nullptr, (unsigned long long *)nonfinite.p, s);
     (void)hipEventRecord(e1, s);
     if (!hip_ok(hipGetLastError(), "field launch", &rc)) return rc;
     if (!hip_ok(hipEventSynchronize(e1), "field sync", &rc)) return rc;
@@ -64,6 +66,18 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
     *ms_accum += t;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return SCMC_OK;
+}
+
+int FieldEngine::read_nonfinite(int64_t *out, hipStream_t s)
+{
+    int rc = SCMC_OK;
+    unsigned long long h = 0;
+    if (nonfinite.p &&
+        (!hip_ok(hipMemcpyAsync(&h, nonfinite.p, sizeof h, hipMemcpyDeviceToHost, s), "copy nonfinite", &rc) ||
+         !hip_ok(hipStreamSynchronize(s), "nonfinite sync", &rc)))
+        return rc;
+    *out = (int64_t)h;
     return SCMC_OK;
 }
 

@@ -46,6 +46,7 @@ struct CovFactors {
     double2 *UL0 = nullptr;    // voxel medium: unattenuated u_l0 per path
     float *maxG, *maxF2;       // path-level maxima (mixed scaling)
     unsigned *bound;           // [0] multi, [1] single: float bits of max |u|
+    unsigned long long *nonfinite = nullptr;   // += vertices whose factors are NaN/Inf
 };
 struct HetMedium;   // hetero.h
 // M is null for a homogeneous medium.
@@ -104,9 +105,10 @@ void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s);
 // evalphaseattHetro (FP64 march; the mixed mode evaluates sin/cos/exp in
 // FP32 in the far field).  work holds
 // field_work_elems(...) double2.
+// nonfinite (may be null) += paths whose coefficients are NaN/Inf.
 void launch_field(const DevProblem &P, const HetMedium *M, int64_t np, int64_t nvert, const PathRec *paths,
                   const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, double2 *work, double2 *acc,
-                  bool mixed, const double2 *wl, hipStream_t s);
+                  bool mixed, const double2 *wl, unsigned long long *nonfinite, hipStream_t s);
 size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave);
 
 // Device-wide helpers.

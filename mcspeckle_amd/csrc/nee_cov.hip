@@ -26,6 +26,7 @@ template <> __device__ __forceinline__ float2 to_c<float2>(double2 v) { return m
 template <> __device__ __forceinline__ double2 to_c<double2>(double2 v) { return v; }
 
 __device__ __forceinline__ float cmag(double2 v) { return (float)fmax(fabs(v.x), fabs(v.y)); }
+__device__ __forceinline__ bool cfinite(double2 v) { return isfinite(v.x) && isfinite(v.y); }
 
 __device__ __forceinline__ float block_max(float v, float *sh)
 {
@@ -59,11 +60,13 @@ template <int DIM, typename T, bool HET>
 __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedium M, int64_t np, const PathRec *paths,
                                                            const VertexRec *verts, const int64_t *offsets, T *Gp,
                                                            T *Fp2, T *SA, T *SB, float *maxG, float *maxF2,
-                                                           unsigned *bound, double2 *EV0, double2 *UL0)
+                                                           unsigned *bound, double2 *EV0, double2 *UL0,
+                                                           unsigned long long *nonfinite)
 {
     __shared__ float sh[4];
     const int64_t p = blockIdx.x;
     if (p >= np) return;
+    bool bad = false;   // a NaN/Inf factor (path weight, position, direction)
     const PathRec pr = paths[p];
     const double W1 = verts[offsets[p]].weight;
     const double ssw = sqrt(W1 / pr.px);
@@ -77,6 +80,7 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedi
         double2 g = cmul(e, af);
         Gp[p * P.nl + i] = to_c<T>(g);
         SB[p * P.nl + i] = to_c<T>(e);
+        bad |= !cfinite(g) || !cfinite(e);
         if (HET && UL0) {
             DevProblem P0 = P;
             P0.sigt = 0.0;   // MCCovHetro.m:152: evalphaseatt(x, l, is_ff_l, 0, ...)
@@ -90,6 +94,7 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedi
         double2 e = nee_e<DIM, HET>(P, M, pr.x1, vc, P.ff_sign, P.is_ff_v);
         double2 se = make_double2(ssw * e.x, ssw * e.y);
         SA[p * P.nv + i] = to_c<T>(se);
+        bad |= !cfinite(se);
         if (HET && EV0) {
             const double wp = W1 / pr.px;
             EV0[p * P.nv + i] = make_double2(wp * e.x, wp * e.y);
@@ -103,6 +108,7 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedi
             double2 f = cmul(e, af);
             Fp2[p * P.nv + i] = to_c<T>(f);
             mF = fmaxf(mF, cmag(f));
+            bad |= !cfinite(f);
         }
     }
     mG = block_max(mG, sh);
@@ -114,6 +120,7 @@ __global__ __launch_bounds__(128) void nee_cov_path_kernel(DevProblem P, HetMedi
         maxF2[p] = mF;
         if (P.is_ff_l && P.is_ff_v) atomic_max_pos(&bound[1], 2.f * mSA * mSB);
     }
+    if (__syncthreads_or(bad) && threadIdx.x == 0 && nonfinite) atomicAdd(nonfinite, 1ULL);
 }
 
 // Near-field single scattering (MCcov.m:138-142, 234-241): the amplitude
@@ -160,12 +167,12 @@ template <int DIM, typename T, bool HET>
 __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
     DevProblem P, HetMedium M, int64_t nvert, int vpb, const PathRec *paths, const VertexRec *verts,
     const int32_t *vpath, const int64_t *offsets, const T *Gp, const T *Fp2, const float *maxG, const float *maxF2,
-    T *A1, T *B1, T *A2, T *B2, unsigned *bound)
+    T *A1, T *B1, T *A2, T *B2, unsigned *bound, unsigned long long *nonfinite)
 {
-    __shared__ unsigned mAv[kNeeMaxVpb], mBv[kNeeMaxVpb];
+    __shared__ unsigned mAv[kNeeMaxVpb], mBv[kNeeMaxVpb], badv[kNeeMaxVpb];
     __shared__ float sh[kNeeVertexThreads / 64];
     const int64_t v0 = (int64_t)blockIdx.x * vpb;
-    for (int j = threadIdx.x; j < vpb; j += blockDim.x) mAv[j] = mBv[j] = 0u;
+    for (int j = threadIdx.x; j < vpb; j += blockDim.x) mAv[j] = mBv[j] = badv[j] = 0u;
     __syncthreads();
     const int w = P.nv + P.nl;
     for (int j = threadIdx.x; j < vpb * w; j += blockDim.x) {
@@ -190,6 +197,7 @@ __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
             A1[m * P.nv + i] = to_c<T>(f);
             const float mg = cmag(f);
             if (mg > 0.f) atomicMax(&mAv[vl], __float_as_uint(mg));
+            if (!cfinite(f)) atomicOr(&badv[vl], 1u);
             if (P.do_cbs) A2[m * P.nv + i] = Fp2[(int64_t)p * P.nv + i];
         } else {
             const int li = i - P.nv;
@@ -207,6 +215,7 @@ __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
                 B2[m * P.nl + li] = to_c<T>(g);
                 const float mg = cmag(g);
                 if (mg > 0.f) atomicMax(&mBv[vl], __float_as_uint(mg));
+                if (!cfinite(g)) atomicOr(&badv[vl], 1u);
             }
         }
     }
@@ -218,6 +227,7 @@ __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
         const int p = vpath[vi];
         if (vi == offsets[p]) continue;
         bv = fmaxf(bv, 2.f * (__uint_as_float(mAv[j]) * maxG[p] + maxF2[p] * __uint_as_float(mBv[j])));
+        if (badv[j] && nonfinite) atomicAdd(nonfinite, 1ULL);
     }
     bv = block_max(bv, sh);
     if (threadIdx.x == 0) atomic_max_pos(&bound[0], bv);
@@ -231,7 +241,7 @@ static void nee_cov_dispatch(const DevProblem &P, const HetMedium &M, int64_t np
     if (np > 0)
         hipLaunchKernelGGL((nee_cov_path_kernel<DIM, T, HET>), dim3((unsigned)np), dim3(128), 0, s, P, M, np, paths,
                            verts, offsets, (T *)f.Gp, (T *)f.Fp2, (T *)f.SA, (T *)f.SB, f.maxG, f.maxF2, f.bound,
-                           f.EV0, f.UL0);
+                           f.EV0, f.UL0, f.nonfinite);
     if (np > 0 && !(P.is_ff_l && P.is_ff_v))
         hipLaunchKernelGGL((nee_single_near_kernel<DIM, T>), dim3((unsigned)np), dim3(256), 0, s, P, np, paths,
                            (const T *)f.SA, (const T *)f.SB, (T *)f.U, (T *)f.ones, f.bound);
@@ -241,7 +251,7 @@ static void nee_cov_dispatch(const DevProblem &P, const HetMedium &M, int64_t np
         hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET>), dim3((unsigned)((nvert + vpb - 1) / vpb)),
                            dim3(kNeeVertexThreads), 0, s, P, M, nvert, vpb, paths, verts, vpath, offsets,
                            (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1, (T *)f.B1, (T *)f.A2,
-                           (T *)f.B2, f.bound);
+                           (T *)f.B2, f.bound, f.nonfinite);
     }
 }
 

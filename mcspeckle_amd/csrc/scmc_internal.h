@@ -74,6 +74,7 @@ struct CovEngine {
     int order_n = 0;               // N the tile order was built for
     std::vector<int> order_host;   // its host copy (source of the async upload)
     DevBuf mean, mean_part;   // voxel medium: mean field (MCCovHetro's mean1), per-range partials
+    DevBuf nonfinite;         // vertices with NaN/Inf factors in this solve (device counter)
     CovSlot slot[2];
     int cur = 0;
     hipEvent_t nee_done = nullptr;   // the last produced chunk has read its path records
@@ -88,6 +89,7 @@ struct CovEngine {
     int finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
                  double *mm_re, double *mm_im, hipStream_t s);
     int finalize_mean(int64_t max_itr, double *re, double *im, hipStream_t s);
+    int read_nonfinite(int64_t *out, hipStream_t s);
 };
 
 // Field accumulation state across chunks.
@@ -96,9 +98,11 @@ struct FieldEngine {
     bool wave = false;   // lights combined with weights (MCfieldOnWave)
     bool het = false;    // voxel medium (MCsampleHetro)
     DevBuf acc, work, out, wl;
+    DevBuf nonfinite;    // paths with NaN/Inf coefficients in this solve (device counter)
     int init(const ProblemState &ps, hipStream_t s, const double *wl_re = nullptr, const double *wl_im = nullptr);
     int chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipStream_t s, double *ms_accum);
     int finalize(const scmc_problem *pr, int64_t max_itr, double *u_re, double *u_im, hipStream_t s);
+    int read_nonfinite(int64_t *out, hipStream_t s);
 };
 
 // Deep copy of a problem description (the session owns its arrays).

@@ -134,9 +134,11 @@ __global__ void zero_kernel(uint4 *p, size_t n16)
 
 // Clears with a fill kernel rather than hipMemsetAsync so the same code is
 // safe inside captured graphs.  bytes must be a multiple of 16.
+// Zeroes `bytes` rounded up to whole 16-byte words: the buffers are device
+// allocations padded to 256 bytes (DevBuf), so the rounding stays inside them.
 void launch_zero(void *p, size_t bytes, hipStream_t s)
 {
-    size_t n16 = bytes / 16;
+    size_t n16 = (bytes + 15) / 16;
     if (!n16) return;
     size_t blocks = (n16 + 255) / 256;
     if (blocks > 4096) blocks = 4096;

@@ -443,27 +443,33 @@ def c_pathset(ps):
     return c
 
 
-def mccov_replay(spec, ps, max_itr, precision="fp64"):
+def mccov_replay(spec, ps, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
     ne = (nv * nl) ** 2
     ms_re, ms_im, mm_re, mm_im = (np.empty(ne) for _ in range(4))
     p = c_problem(spec)
     cps = c_pathset(ps)
+    st = N.Stats()
     N.check(L.scmc_mccov_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(ms_re),
-                                N.dptr(ms_im), N.dptr(mm_re), N.dptr(mm_im)))
+                                N.dptr(ms_im), N.dptr(mm_re), N.dptr(mm_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
     shape = (nv, nv, nl, nl)
     return ((ms_re + 1j * ms_im).reshape(shape, order="F"), (mm_re + 1j * mm_im).reshape(shape, order="F"))
 
 
-def mcfield_replay(spec, ps, max_itr, precision="fp64"):
+def mcfield_replay(spec, ps, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
     u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
     p = c_problem(spec)
     cps = c_pathset(ps)
+    st = N.Stats()
     N.check(L.scmc_mcfield_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(u_re),
-                                  N.dptr(u_im)))
+                                  N.dptr(u_im), C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
     return (u_re + 1j * u_im).reshape((nv, nl), order="F")
 
 

@@ -339,3 +339,34 @@ def test_gpu_single_path_runs(oracle, kind, dim, cbs, smp, amp0):
     u0 = oracle.mcfield(spec, 1)
     u = E.mcfield_spec(spec, 1, "fp64")
     assert np.abs(u - u0).max() <= 1e-10 * max(np.abs(u0).max(), 1e-300)
+
+
+# ------------------------------------------------------ non-finite inputs
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_nonfinite_path_weights_are_flagged(oracle, precision):
+    """A NaN path weight injected into a replayed path set is detected: the
+    solve reports the vertices / paths whose factors came out non-finite
+    (scmc_stats.nonfinite) and, as the reference would, the result holds
+    NaN; the clean set reports 0."""
+    spec = build("hg", 3, True, 2, None)
+    n = 200
+    ps = oracle.trace(spec, n)
+    st = {}
+    Ms, Mm = E.mccov_replay(spec, ps, n, precision, stats=st)
+    assert st["nonfinite"] == 0 and np.isfinite(Ms).all() and np.isfinite(Mm).all()
+    u = E.mcfield_replay(spec, ps, n, precision, stats=st)
+    assert st["nonfinite"] == 0 and np.isfinite(u).all()
+    off = ps["offset"]
+    p = int(np.argmax(np.diff(off) >= 3))          # a path with >= 3 vertices
+    bad = dict(ps)
+    bad["weight"] = ps["weight"].copy()
+    bad["weight"][off[p] + 1] = np.nan            # its second vertex
+    say("nonfinite", precision, "path", p)
+    st = {}
+    Ms, Mm = E.mccov_replay(spec, bad, n, precision, stats=st)
+    assert st["nonfinite"] == 1
+    assert not np.isfinite(Mm).all()
+    st = {}
+    u = E.mcfield_replay(spec, bad, n, precision, stats=st)
+    assert st["nonfinite"] == 1
+    assert not np.isfinite(u).all()
