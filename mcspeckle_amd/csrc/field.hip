@@ -220,20 +220,6 @@ __device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64
     }
 }
 
-// HG / isotropic / tabulated amplitude in FP32 (mixed variant).
-template <int DIM>
-__device__ __forceinline__ float2 amp_f32(float c, const DevPhase &ph, float hg_k, float g2p1, float g2)
-{
-    if (ph.type == 1) return make_float2((float)ph.iso, 0.f);
-    if (ph.type == 3) {
-        float t = fmaxf(g2p1 - g2 * c, 1e-30f);   // 1 + g^2 - 2 g c
-        float a = (DIM == 3) ? hg_k * __builtin_amdgcn_exp2f(-0.75f * __builtin_amdgcn_logf(t))
-                             : hg_k * __builtin_amdgcn_rsqf(t);
-        return make_float2(a, 0.f);
-    }
-    return make_float2(0.f, 0.f);   // tabulated: see amp_tab (FP64 bin index)
-}
-
 // Tabulated amplitude with the nearest-bin index formed in FP64, so the bin
 // matches evalampfunc_amplitude.m:8 exactly (an FP32 acos flips bins near
 // their edges).
@@ -248,17 +234,22 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
 // fraction of a cycle, hardware sin/cos (input in revolutions), FP32
 // attenuation over the distance to the face the view leaves by (dlo/dhi: the
 // entry's x - box_min and box_max - x in FP32; inv, up: per view).
+// The view's direction v enters as x.v times nilam = -1/lambda: the far-field
+// phase is k x.(-v), and negating either factor is exact, so the cycle count
+// is bit-identical to (x.(-v))/lambda.  lamp: log2 of a real amplitude factor
+// folded into the same exp2 (0 for none), so an isotropic or HG amplitude
+// costs no extra exponential and no complex multiply.
 template <int DIM>
-__device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double *x, const double *mv, const float *dlo,
-                                              const float *dhi, const bool *up, const float *inv, float hs)
+__device__ __forceinline__ float2 e_far_mixed(const double *x, const double *v, double nilam, const float *dlo,
+                                              const float *dhi, const bool *up, const float *inv, float hs,
+                                              float lamp = 0.f)
 {
-    const double pv = dot<DIM>(x, mv);
-    const double cyc = pv * P.inv_lambda;
+    const double cyc = dot<DIM>(x, v) * nilam;
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
     float best = INFINITY;
     for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? dhi[j] : dlo[j]) * inv[j]);
-    const float mag = __builtin_amdgcn_exp2f(-hs * 1.4426950408889634f * best);
+    const float mag = __builtin_amdgcn_exp2f(fmaf(-hs * 1.4426950408889634f, best, lamp));
     return make_float2(mag * cs, mag * sn);
 }
 
@@ -266,13 +257,13 @@ __device__ __forceinline__ float2 e_far_mixed(const DevProblem &P, const double 
 // march of evalphaseattHetro, the phase reduced in FP64, sin/cos/exp in FP32.
 template <int DIM>
 __device__ __forceinline__ float2 e_far_mixed_het(const DevProblem &P, const HetMedium &M, const double *x,
-                                                  const double *mv)
+                                                  const double *mv, float lamp = 0.f)
 {
     const double cyc = dot<DIM>(x, mv) * P.inv_lambda;
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
     const float depth = (float)march_depth<DIM>(P, M, x, mv, 1e12);
-    const float mag = __builtin_amdgcn_exp2f(-0.5f * 1.4426950408889634f * depth);
+    const float mag = __builtin_amdgcn_exp2f(fmaf(-0.5f * 1.4426950408889634f, depth, lamp));
     return make_float2(mag * cs, mag * sn);
 }
 
@@ -298,7 +289,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     // per-view constants
     int vv[VPT];
     bool vok[VPT];
-    double vd[VPT][3], mv[VPT][3];
+    double vd[VPT][3], mv[VPT][3];   // mv = -v: the far-field march direction (HET)
     float vf[VPT][3], inv[VPT][3];
     bool up[VPT][3];   // nd_j > 0 -> distance to the max face
     double2 afv[VPT][NLQ], acc[VPT][NLQ];
@@ -309,7 +300,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
         vok[t] = v < P.nv;
         for (int j = 0; j < 3; ++j) vd[t][j] = (vok[t] && j < DIM) ? P.v[(size_t)v * DIM + j] : 0.0;
         for (int j = 0; j < 3; ++j) {
-            mv[t][j] = -vd[t][j];   // ff_sign * v (far field)
+            mv[t][j] = -vd[t][j];
             vf[t][j] = (float)vd[t][j];
             inv[t][j] = (float)(1.0 / fmax(fabs(vd[t][j]), 1.0000000000000004e-07));
             up[t][j] = vd[t][j] > 0;
@@ -321,9 +312,21 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
         }
     }
     const float g = (float)P.amp.g;
-    const float hg_k = (float)sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
     const float g2p1 = 1.f + g * g, g2 = 2.f * g;
     const float hs = (float)(P.sigt * 0.5);
+    // FP32 amplitude in the log2 domain (evalampfunc_general.m:5-18,
+    // evaluateHG.m:25-39): isotropic a = iso; HG a = hg_k t^mu with
+    // t = 1 + g^2 - 2 g c, mu = -3/4 (3-D) or -1/2 (2-D).  Tabulated
+    // amplitudes are complex (amp_tab).  Types other than 1 and 2 are HG,
+    // as in eval_amp.
+    const bool tab = P.amp.type == 2, hg = P.amp.type != 1 && !tab;
+    // uniform: kept in a scalar register
+    const float lk = __builtin_bit_cast(
+        float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                   int, hg ? (float)log2(sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi)))
+                           : (float)log2(P.amp.iso))));
+    const double nilam = -P.inv_lambda;
+    const float mu = (DIM == 3) ? -0.75f : -0.5f;
 
     for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
         const int cnt = (int)((e1 - eb) < kFieldBatch ? (e1 - eb) : kFieldBatch);
@@ -359,16 +362,23 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             for (int t = 0; t < VPT; ++t) {
                 double2 ev;
                 if (MIXED) {
-                    float2 evf = HET ? e_far_mixed_het<DIM>(P, M, x, mv[t])
-                                     : e_far_mixed<DIM>(P, x, mv[t], dlo, dhi, up[t], inv[t], hs);
-                    if (kind != 0) {
-                        float2 am;
-                        if (P.amp.type == 2) {
-                            am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
-                        } else {
+                    // isotropic / HG: a real amplitude, log2 a = lk + mu log2(1 + g^2 - 2 g c)
+                    float lamp = 0.f;
+                    if (kind != 0 && !tab) {
+                        lamp = lk;
+                        if (hg) {
                             float c = adf[0] * vf[t][0] + adf[1] * vf[t][1] + (DIM == 3 ? adf[2] * vf[t][2] : 0.f);
-                            am = amp_f32<DIM>(c, P.amp, hg_k, g2p1, g2);
+                            lamp = fmaf(mu, __builtin_amdgcn_logf(fmaxf(g2p1 - g2 * c, 1e-30f)), lk);
                         }
+                    }
+                    float2 evf;
+                    if (HET) {
+                        evf = e_far_mixed_het<DIM>(P, M, x, mv[t], lamp);
+                    } else {
+                        evf = e_far_mixed<DIM>(x, vd[t], nilam, dlo, dhi, up[t], inv[t], hs, lamp);
+                    }
+                    if (kind != 0 && tab) {
+                        const float2 am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
                         evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
                     }
                     ev = make_double2(evf.x, evf.y);
@@ -399,7 +409,9 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
 #pragma unroll
     for (int t = 0; t < VPT; ++t)
         if (vok[t])
-            for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + vv[t]] = acc[t][q];
+#pragma unroll
+            for (int q = 0; q < NLQ; ++q)   // static indices keep acc in registers
+                if (q < nlp) part[((size_t)split * ncol + l0 + q) * P.nv + vv[t]] = acc[t][q];
 }
 
 // Near-field sources (either side): the entrance directions rl/rv depend on
@@ -525,7 +537,9 @@ __global__ __launch_bounds__(kFieldViews) void field_near_kernel(DevProblem P, H
         }
     }
     if (vok)
-        for (int q = 0; q < nlp; ++q) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
+#pragma unroll
+        for (int q = 0; q < kFieldNL; ++q)   // static indices keep acc in registers
+            if (q < nlp) part[((size_t)split * ncol + l0 + q) * P.nv + v] = acc[q];
 }
 
 // MCfieldOnWave first vertices, far field (MCfieldOnWave.m:143-148, 214):
@@ -547,12 +561,13 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
     __shared__ double2 scl[kWavePB][kWaveLG];
     const int v = blockIdx.x * kFieldViews + threadIdx.x;
     const bool vok = v < P.nv;
-    double mv[3] = {0, 0, 0};
+    double vd[3] = {0, 0, 0};
     float inv[3] = {0, 0, 0};
     bool up[3] = {false, false, false};
+    const double nilam = -P.inv_lambda;
     for (int j = 0; j < DIM && vok; ++j) {
         const double nd = P.v[(size_t)v * DIM + j];
-        mv[j] = -nd;
+        vd[j] = nd;
         inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
         up[j] = nd > 0;
     }
@@ -584,7 +599,7 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
             double2 ev;
             if (MIXED) {
-                const float2 e = e_far_mixed<DIM>(P, x, mv, sdlo[i], sdhi[i], up, inv, hs);
+                const float2 e = e_far_mixed<DIM>(x, vd, nilam, sdlo[i], sdhi[i], up, inv, hs);
                 ev = make_double2(e.x, e.y);
             } else {
                 ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
@@ -598,7 +613,9 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
         }
     }
     if (vok)
-        for (int q = 0; q < nlg; ++q) hpart[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
+#pragma unroll
+        for (int q = 0; q < kWaveLG; ++q)   // static indices keep acc in registers
+            if (q < nlg) hpart[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
 }
 
 __global__ void field_wave_first_reduce_kernel(int nv, int nl, int nsplit, const double2 *aftab, const double2 *hpart,

@@ -301,6 +301,12 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
         accR[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(X##al##rb, X##bh1, accR[rb][1], 0, 0, 0);      \
         accI[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(rl, X##bh1, accI[rb][1], 0, 0, 0);            \
     } while (0)
+    // s_barrier orders execution only: the compiler may move LDS reads across
+    // it.  The empty asm with a memory clobber (no instruction) keeps every
+    // LDS read of a stage after the barrier that publishes that stage (its
+    // global_load_lds writes come from all four waves).  No s_waitcnt is
+    // added, so the loads of later stages stay in flight.
+#define SCMC_COMPILER_FENCE() asm volatile("" ::: "memory")
     // nstage is even.  Every load stays inside the item: stages past the
     // last reload the last one into a buffer that is never read again.
     const int last = nstage - 1;
@@ -311,6 +317,7 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
         SCMC_ISSUE(sb + (3 < last ? 3 : last), 3);
         __builtin_amdgcn_s_waitcnt(0xF7C);   // vmcnt(12): stage 0
         __builtin_amdgcn_s_barrier();
+        SCMC_COMPILER_FENCE();
         SCMC_READ(F, 0);
     }
     // K-step st with fragments X: publish stage st + 1 (its loads were issued
@@ -322,6 +329,7 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
     do {                                                                                                 \
         __builtin_amdgcn_s_waitcnt(0xF78); /* vmcnt(8): stage st+1 landed */                            \
         __builtin_amdgcn_s_barrier();                                                                    \
+        SCMC_COMPILER_FENCE();                                                                           \
         SCMC_ISSUE(sb + ((st) + 4 < last ? (st) + 4 : last), ((st) + 4) % NBUF);                         \
         SCMC_MFMA_ROW(X, 0);                                                                             \
         SCMC_READ(Y, ((st) + 1) % NBUF);                                                                 \
@@ -332,6 +340,7 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_t
         SCMC_STEP(G, F, st + 1);
     }
 #undef SCMC_STEP
+#undef SCMC_COMPILER_FENCE
 #undef SCMC_MFMA_ROW
 #undef SCMC_READ
 #undef SCMC_ISSUE
