@@ -105,10 +105,13 @@ typedef struct {
     double ms_accum;         /* covariance update / field accumulation         */
     double ms_total;         /* wall time of the call                          */
     double fp16_scale_log2;  /* mixed: last chunk's power-of-two scale         */
-    double ms_herk;          /* multiple-scattering covariance update kernels,
-                                HIP-event time on the solve stream             */
+    double ms_herk;          /* covariance update launches, HIP-event time on
+                                the solve stream: mixed, one launch per chunk
+                                for multiple and single scattering; FP64, the
+                                multiple-scattering launches                   */
     int64_t herk_launches;   /* number of those launches                        */
-    double herk_flops;       /* their algorithmic flops, 8 N^2 K                */
+    double herk_flops;       /* their algorithmic flops, 8 N^2 K (K: the rank-1
+                                terms those launches add)                      */
     double ms_expand;        /* mixed: FP16 hi/lo row expansion before them     */
     int64_t nonfinite;       /* vertices (covariance) or paths (field) whose
                                 factors came out NaN/Inf (e.g. a NaN path weight

@@ -491,16 +491,15 @@ int CovEngine::consume(hipStream_t sc)
     hs.acc = (double2 *)acc_s.p;
     hs.ksplit = kss;
     hs.order = order.p;
+    // mixed: one launch for both problems (ev[3]..ev[4]); FP64: the
+    // multiple-scattering launch between ev[3] and ev[4], then the single
     (void)hipEventRecord(S.ev[3], sc);
     if (mixed)
-        launch_herk_mixed_update(hm, sc);
+        launch_herk_mixed_update(hm, &hs, sc);
     else
         launch_herk_fp64(hm, sc);
     (void)hipEventRecord(S.ev[4], sc);
-    if (mixed)
-        launch_herk_mixed_update(hs, sc);
-    else
-        launch_herk_fp64(hs, sc);
+    if (!mixed) launch_herk_fp64(hs, sc);
     // the scale bound goes to pinned memory on the stream: a blocking copy on
     // the null stream would wait for the update in flight on the other stream
     if (mixed && !hip_ok(hipMemcpyAsync(S.bound_host, S.bound.p, sizeof(unsigned), hipMemcpyDeviceToHost, sc),
@@ -526,11 +525,13 @@ int CovEngine::harvest(CovSlot &S)
     (void)hipEventElapsedTime(&ts, S.ev[4], S.ev[5]);
     ms_nee += tn;
     ms_accum += tx + th + ts;
-    if (S.k2 > 0) {
+    // the timed launch's rank-1 terms: both problems in mixed mode
+    const int64_t kt = S.k2 + (mixed ? S.np : 0);
+    if (kt > 0) {
         ms_herk += th;
         if (mixed) ms_expand += tx;
         herk_launches += 1;
-        herk_flops += 8.0 * (double)n * (double)n * (double)S.k2;
+        herk_flops += 8.0 * (double)n * (double)n * (double)kt;
     }
     if (mixed) {
         const unsigned b = *S.bound_host;

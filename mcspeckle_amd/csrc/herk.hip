@@ -212,15 +212,35 @@ __global__ __launch_bounds__(256) void expand_step_kernel(int nv, int nl, int n,
 // reads its 64-row A and 64-column B fragments one K-step ahead and issues
 // 24 MFMAs per K-step (Re and Im, hi*hi + hi*lo + lo*hi).  Each K-step costs
 // one workgroup barrier.
-__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, int64_t nks_total, const uint4 *__restrict__ Z,
-                                                            double2 *__restrict__ acc, const unsigned *__restrict__ bound,
-                                                            int ksplit, const int2 *__restrict__ order, int ntile)
+//
+// One launch carries the two rank-K problems of a chunk (multiple and single
+// scattering, same N and tile order): work items [0, u0.nitems) update u0,
+// the rest u1.  The short single-scattering items come last and fill the
+// workgroup slots the long ones leave free at the end of the grid.
+struct MixedUpdate {
+    int64_t nks;               // K-steps of the expanded rows (pad_ksteps)
+    const uint4 *Z;            // expanded rows
+    double2 *acc;              // [ksplit][n][n] accumulator copies
+    const unsigned *bound;     // float bits of max |u| (scale), may be null
+    int ksplit;
+    int nitems;                // ntile * ksplit, 0 for an empty problem
+};
+
+__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, MixedUpdate u0, MixedUpdate u1,
+                                                            const int2 *__restrict__ order, int ntile)
 {
     using namespace mixed;
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool second = (int)blockIdx.x >= u0.nitems;   // uniform
+    const MixedUpdate &U = second ? u1 : u0;
+    const int64_t nks_total = U.nks;
+    const uint4 *__restrict__ Z = U.Z;
+    double2 *__restrict__ acc = U.acc;
+    const unsigned *__restrict__ bound = U.bound;
+    const int ksplit = U.ksplit;
     int s, t;
-    item_decode(blockIdx.x, ksplit, ntile, s, t);
+    item_decode((int)blockIdx.x - (second ? u0.nitems : 0), ksplit, ntile, s, t);
     const int ti = order[t].x, tj = order[t].y;
     const int npad = pad_rows(n);
     const int64_t npair = nks_total / 2;   // K-steps come in pairs (padded to 4)
@@ -519,15 +539,26 @@ void launch_expand_mixed(const HerkProblem &hp, hipStream_t s)
                            (uint4 *)hp.zbuf);
 }
 
-void launch_herk_mixed_update(const HerkProblem &hp, hipStream_t s)
+void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStream_t s)
 {
-    if (hp.k <= 0) return;
-    const int64_t nks = mixed::pad_ksteps(hp.k);
-    const int nt = ntiles(hp.n, kTileMixed);
+    const int nt = ntiles(a.n, kTileMixed);
     const int ntile = nt * (nt + 1) / 2;
-    const unsigned grid = (unsigned)(ntile * hp.ksplit);
-    hipLaunchKernelGGL(herk_mixed_kernel, dim3(grid), dim3(256), 0, s, hp.n, nks, (const uint4 *)hp.zbuf, hp.acc,
-                       hp.bound, hp.ksplit, (const int2 *)hp.order, ntile);
+    auto desc = [&](const HerkProblem *hp) {
+        MixedUpdate u{};
+        if (hp && hp->k > 0) {
+            u.nks = mixed::pad_ksteps(hp->k);
+            u.Z = (const uint4 *)hp->zbuf;
+            u.acc = hp->acc;
+            u.bound = hp->bound;
+            u.ksplit = hp->ksplit;
+            u.nitems = ntile * hp->ksplit;
+        }
+        return u;
+    };
+    const MixedUpdate u0 = desc(&a), u1 = desc(b);
+    const unsigned grid = (unsigned)(u0.nitems + u1.nitems);
+    if (grid == 0) return;
+    hipLaunchKernelGGL(herk_mixed_kernel, dim3(grid), dim3(256), 0, s, a.n, u0, u1, (const int2 *)a.order, ntile);
 }
 
 // Lower-triangle tiles (ti >= tj) enumerated by 8x8 super-blocks.

@@ -84,9 +84,11 @@ size_t mixed_zbuf_bytes(int64_t k, int n);
 constexpr int kTileMixed = 128;
 constexpr int kTileF64 = 64;
 // Mixed precision in two steps: FP16 hi/lo row expansion into hp.zbuf, then
-// the matrix-core update from hp.zbuf into hp.acc.
+// the matrix-core update from hp.zbuf into hp.acc.  The update launch takes
+// a second problem of the same N and tile order (b, may be null), whose work
+// items follow a's in the same grid.
 void launch_expand_mixed(const HerkProblem &hp, hipStream_t s);
-void launch_herk_mixed_update(const HerkProblem &hp, hipStream_t s);
+void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStream_t s);
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the
