@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -16,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace scmc {
@@ -41,21 +44,94 @@ bool hip_ok(hipError_t e, const char *what, int *rc)
 }
 
 // ------------------------------------------------------------ buffers
+// hipFree synchronises the whole device: it returns once every kernel queued
+// so far, on any stream, has finished.  That is what makes releasing a
+// buffer safe while kernels queued on another stream may still read it (the
+// path records of the previous chunk).  While a server runs, the frees go to
+// a reaper thread instead, so a slot thread that grows a buffer does not
+// wait for the other slots' work; the block is still freed only after that
+// device-wide wait.
+namespace {
+struct Reaper {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<void *> q;
+    bool stop = false;
+    std::thread th;
+};
+std::mutex g_reaper_mu;
+Reaper *g_reaper = nullptr;   // guarded by g_reaper_mu
+} // namespace
+
+static void device_free(void *p)
+{
+    {
+        std::lock_guard<std::mutex> g(g_reaper_mu);
+        if (g_reaper) {
+            {
+                std::lock_guard<std::mutex> l(g_reaper->mu);
+                g_reaper->q.push_back(p);
+            }
+            g_reaper->cv.notify_one();
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+void reaper_start()
+{
+    std::lock_guard<std::mutex> g(g_reaper_mu);
+    if (g_reaper) return;
+    Reaper *R = new Reaper();
+    R->th = std::thread([R] {
+        (void)ensure_device(nullptr);
+        std::unique_lock<std::mutex> l(R->mu);
+        for (;;) {
+            R->cv.wait(l, [R] { return R->stop || !R->q.empty(); });
+            if (R->q.empty()) return;   // stop requested and drained
+            void *p = R->q.front();
+            R->q.pop_front();
+            l.unlock();
+            (void)hipFree(p);
+            l.lock();
+        }
+    });
+    g_reaper = R;
+}
+
+void reaper_stop()
+{
+    Reaper *R;
+    {
+        std::lock_guard<std::mutex> g(g_reaper_mu);
+        R = g_reaper;
+        g_reaper = nullptr;   // later frees are synchronous again
+    }
+    if (!R) return;
+    {
+        std::lock_guard<std::mutex> l(R->mu);
+        R->stop = true;
+    }
+    R->cv.notify_all();
+    R->th.join();   // frees what was queued
+    delete R;
+}
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) device_free(p);
     p = nullptr;
     bytes = 0;
 }
 bool DevBuf::ensure(size_t b, int *rc)
 {
     if (b <= bytes && p) return true;
-    // hipFree synchronises the device, so a buffer that kernels queued on
-    // another stream still read (the path records of the previous chunk) is
-    // only released once they have finished
     release();
-    size_t want = b ? b : 16;
+    // 1/8 headroom: the next chunk, a few percent larger, fits without a
+    // reallocation (and its device-wide wait)
+    size_t want = b ? b + b / 8 : 16;
     want = (want + 255) & ~(size_t)255;
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
@@ -589,7 +665,7 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
 {
     int rc = SCMC_OK;
     const size_t ne = (size_t)n * n;
-    DevBuf out, af;
+    DevBuf &out = fin_out, &af = fin_af;
     if (!out.ensure(sizeof(double) * 2 * ne, &rc) || !af.ensure(sizeof(double2) * (size_t)n, &rc)) return rc;
     // MCcov.m:293-299: x V sigt / maxItr; MCCovHetro.m:258-265: / maxItr only
     // (its first-vertex density already carries sum(sigt) * cell volume)
@@ -741,7 +817,7 @@ Session::~Session()
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-int Session::run(int64_t first, int64_t n)
+int Session::run(int64_t first, int64_t n, bool settle)
 {
     int rc = SCMC_OK;
     if (first < 0 || n < 0) return set_error(SCMC_E_ARG, "invalid path range");
@@ -774,6 +850,7 @@ int Session::run(int64_t first, int64_t n)
         mean_len = (double)st.vertices / (double)st.paths;
         p0 += np;
     }
+    if (!settle) return SCMC_OK;
     if (kind == 0) {
         if ((rc = ce.drain())) return rc;
         st.ms_accum = ce.ms_accum;

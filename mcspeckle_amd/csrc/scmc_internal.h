@@ -30,6 +30,13 @@ struct DevBuf {
     bool ensure(size_t b, int *rc);
 };
 
+// While started, DevBuf frees are handed to a background thread (hipFree's
+// device-wide wait moves off the calling thread); reaper_stop frees what is
+// queued and makes frees synchronous again.  The server brackets its slot
+// threads with them.
+void reaper_start();
+void reaper_stop();
+
 // Device copy of the problem description (directions, tables, constants).
 struct ProblemState {
     DevProblem dp;
@@ -75,6 +82,8 @@ struct CovEngine {
     std::vector<int> order_host;   // its host copy (source of the async upload)
     DevBuf mean, mean_part;   // voxel medium: mean field (MCCovHetro's mean1), per-range partials
     DevBuf nonfinite;         // vertices with NaN/Inf factors in this solve (device counter)
+    DevBuf fin_out, fin_af;   // finalize: output staging and amplitude table (kept: hipFree
+                              // synchronises the device, which would stall other sessions)
     CovSlot slot[2];
     int cur = 0;
     hipEvent_t nee_done = nullptr;   // the last produced chunk has read its path records
@@ -127,7 +136,9 @@ struct Session {
     // kind 0 MCcov, 1 MCfield; wl_re (kind 1 only) makes it MCfieldOnWave
     int init(const scmc_problem *pr, int kind, int precision, const double *wl_re = nullptr,
              const double *wl_im = nullptr, const scmc_medium *med = nullptr);
-    int run(int64_t first, int64_t n);
+    // settle = false leaves the last chunks in flight (no device sync, the
+    // statistics lag behind); a later run or result() continues the stream
+    int run(int64_t first, int64_t n, bool settle = true);
     int result(int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);
     void fill_stats(scmc_stats *out) const;
     ~Session();

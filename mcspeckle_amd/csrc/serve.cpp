@@ -87,7 +87,10 @@ static void run_request(Server *S, Request *r, Session &sess)
             rc = set_error(SCMC_E_CANCELLED, "request %lld cancelled", (long long)r->id);
             break;
         }
-        rc = sess.run(p0, std::min(slice, r->max_itr - p0));
+        // only the last slice waits for the device: the chunks of one slice
+        // stay in flight while the next one is traced
+        const int64_t len = std::min(slice, r->max_itr - p0);
+        rc = sess.run(p0, len, p0 + len >= r->max_itr);
     }
     if (!rc) {
         const size_t ne = (size_t)r->pc.pr.nv * r->pc.pr.nv * r->pc.pr.nl * r->pc.pr.nl;
@@ -198,6 +201,7 @@ int scmc_server_start(const scmc_server_config *cfg)
     S->budget = S->cfg.hbm_budget_gb > 0 ? (int64_t)(S->cfg.hbm_budget_gb * 1e9) : (int64_t)(0.8 * (double)freeb);
     S->slice = S->cfg.paths_per_slice;
     Server *raw = S.get();
+    reaper_start();
     for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);
     g_srv = std::move(S);
     return SCMC_OK;
@@ -217,7 +221,8 @@ int scmc_server_stop(void)
         for (auto &kv : S->all) kv.second->cancel = true;
     }
     S->cv_work.notify_all();
-    for (auto &t : S->slots) t.join();
+    for (auto &t : S->slots) t.join();   // their sessions' buffers go to the reaper
+    reaper_stop();
     return SCMC_OK;
 }
 
