@@ -305,13 +305,14 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
     int rc = SCMC_OK;
     np = n;
     if (!counts.ensure(sizeof(int32_t) * (size_t)n, &rc) || !offsets.ensure(sizeof(int64_t) * (size_t)(n + 1), &rc) ||
-        !paths.ensure(sizeof(PathRec) * (size_t)n, &rc))
+        !paths.ensure(sizeof(PathRec) * (size_t)n, &rc) ||
+        !tsum.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(scan_tiles(n), 1), &rc))
         return rc;
     if (M)
         launch_hetero_trace(P, *M, p0, n, (int32_t *)counts.p, nullptr, nullptr, nullptr, nullptr, false, s);
     else
         launch_trace_count(P, p0, n, (int32_t *)counts.p, s);
-    launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, s);
+    launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, (int64_t *)tsum.p, s);
     int64_t total = 0;
     if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)offsets.p + n, sizeof total, hipMemcpyDeviceToHost, s), "copy total",
                 &rc))

@@ -27,7 +27,10 @@ struct PathRec {
 // ---------------------------------------------------------------- tracer
 void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, hipStream_t s);
 // offsets[np + 1] = exclusive scan of counts (offsets[np] = total).
-void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, hipStream_t s);
+// Exclusive scan of counts[np] into offsets[np + 1]; tsum: scratch of
+// scan_tiles(np) int64 (at least one).
+int64_t scan_tiles(int64_t np);
+void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, int64_t *tsum, hipStream_t s);
 void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64_t *offsets, PathRec *paths,
                        VertexRec *verts, int32_t *vpath, hipStream_t s);
 

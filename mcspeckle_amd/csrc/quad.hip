@@ -19,11 +19,21 @@
 #include "kernels.h"
 #include "quad.h"
 
+#include <type_traits>
+
 namespace scmc {
 
 template <typename T> struct cx;
 template <> struct cx<double> { typedef double2 t; };
 template <> struct cx<float> { typedef float2 t; };
+
+// (cos, sin) of 2 pi cyc: FP64 reduction to a fraction of a cycle, then the
+// hardware FP32 sin/cos (whose input is in revolutions)
+__device__ __forceinline__ float2 cis_cycles(double cyc)
+{
+    const float fr = (float)(cyc - rint(cyc));
+    return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
+}
 
 __device__ __forceinline__ void atomic_add_c(double2 *p, double re, double im)
 {
@@ -61,6 +71,47 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
     const bool first = (e == offsets[p]);
     double amp = sqrt(r.weight / pr.px);
     if (P.do_cbs && !first) amp *= 1 / sqrt(2.0);
+    if constexpr (std::is_same<T, float>::value) {
+        // FP32 grid: every phase counted in cycles and reduced in FP64, then
+        // hardware sin/cos/exp2 in FP32 (the FP64 grid keeps libm FP64)
+        const float2 ex = cis_cycles(r.xi);   // e^{2 pi i xi}
+        const float pre = (float)amp * ex.x, pim = (float)amp * ex.y;
+        for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
+            const double *xs = pass == 0 ? pr.x1 : r.x;
+            const double *xl = pass == 0 ? r.x : pr.x1;
+            const int j = slab_of(xs[2], g);
+            if (j < 0) continue;
+            long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
+            ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
+            iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
+            const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
+            const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
+            const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
+            if (g.wave) {
+                float twr = 0.f, twi = 0.f;
+                for (int l = 0; l < P.nl; ++l) {
+                    const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+                    const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
+                    twr += wr * c2.x - wi * c2.y;
+                    twi += wr * c2.y + wi * c2.x;
+                }
+                const float2 e1 = cis_cycles(g.vsign * (tz - xs[2]) * P.inv_lambda);
+                const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
+                atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], (ar * twr - ai * twi) * m,
+                             (ar * twi + ai * twr) * m);
+                continue;
+            }
+            const float2 e1 = cis_cycles(-(tz - xs[2]) * P.inv_lambda);
+            const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
+            for (int l = 0; l < P.nl; ++l) {
+                const float2 c2 = cis_cycles(dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+                const float br = m * c2.x, bi = m * c2.y;
+                atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], ar * br - ai * bi,
+                             ar * bi + ai * br);
+            }
+        }
+        return;
+    }
     double sx, cxv;
     sincos(2 * kPi * r.xi, &sx, &cxv);
     const double pre = amp * cxv, pim = amp * sx;

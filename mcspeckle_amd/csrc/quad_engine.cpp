@@ -177,7 +177,10 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             int64_t nstore = 0;
             double mean_len = 20;
             for (int64_t p0 = 0; p0 < max_itr;) {
-                int64_t np = std::min<int64_t>(max_itr - p0, std::max<int64_t>(256, (int64_t)(4.0e6 / mean_len)));
+                // ~3.2e7 vertices (2 GB of records) per chunk: a trace pass takes
+                // about the longest walk's step count times the step latency
+                // whatever the chunk size, so few large chunks amortise it
+                int64_t np = std::min<int64_t>(max_itr - p0, std::max<int64_t>(256, (int64_t)(3.2e7 / mean_len)));
                 auto a = std::chrono::steady_clock::now();
                 if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
                 launch_quad_scatter(ps.dp, g, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,

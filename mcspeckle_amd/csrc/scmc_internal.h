@@ -53,7 +53,7 @@ struct ProblemState {
 // One traced chunk of paths: offsets, path records and vertex records.
 struct TraceChunk {
     int64_t np = 0, nvert = 0;
-    DevBuf counts, offsets, paths, verts, vpath;
+    DevBuf counts, offsets, paths, verts, vpath, tsum;   // tsum: scan scratch
     int run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, const HetMedium *M = nullptr);
     int load(const scmc_pathset *ps, hipStream_t s);
 };

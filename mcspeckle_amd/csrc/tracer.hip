@@ -73,31 +73,85 @@ __global__ __launch_bounds__(256) void trace_emit_kernel(DevProblem P, uint64_t 
     }
 }
 
-// Single-workgroup exclusive scan of int32 counts into int64 offsets.  The
-// chunk sizes here are at most a few hundred thousand paths.
-__global__ __launch_bounds__(1024) void scan_kernel(const int32_t *counts, int64_t np, int64_t *offsets)
+// Exclusive scan of the int32 vertex counts into int64 offsets (offsets[np]
+// = total), in three launches: per-tile sums, a scan of the tile sums, and a
+// rescan of each tile from its base.  A tile is 256 threads x 8 consecutive
+// counts, so every load and store is contiguous across the lanes.
+constexpr int kScanThreads = 256, kScanPer = 8, kScanTile = kScanThreads * kScanPer;
+
+// Exclusive scan of one value per thread across the workgroup; returns the
+// thread's prefix and sets *total to the workgroup sum.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *sh, int64_t *total)
 {
-    __shared__ int64_t part[1024];
     const int t = threadIdx.x;
-    const int64_t per = (np + 1023) / 1024;
-    const int64_t a = t * per, b = (a + per < np) ? a + per : np;
-    int64_t s = 0;
-    for (int64_t i = a; i < b; ++i) s += counts[i];
-    part[t] = s;
+    sh[t] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = (t >= off) ? part[t - off] : 0;
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int64_t u = t >= off ? sh[t - off] : 0;
         __syncthreads();
-        part[t] += v;
+        sh[t] += u;
         __syncthreads();
     }
-    int64_t run = part[t] - s;   // exclusive prefix of this thread's slice
-    for (int64_t i = a; i < b; ++i) {
-        offsets[i] = run;
-        run += counts[i];
-    }
-    if (t == 1023) offsets[np] = part[1023];
+    const int64_t incl = sh[t];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
 }
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sum_kernel(const int32_t *counts, int64_t np, int64_t *tsum)
+{
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    int64_t v = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j)
+        if (base + j < np) v += counts[base + j];
+    int64_t total;
+    (void)block_exclusive_scan(v, sh, &total);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+// In place: tsum[b] becomes the sum of the tiles before b; offsets[np] the total.
+__global__ __launch_bounds__(kScanThreads) void scan_tile_base_kernel(int64_t *tsum, int64_t nt, int64_t np,
+                                                                      int64_t *offsets)
+{
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t per = (nt + kScanThreads - 1) / kScanThreads;
+    const int64_t a = (int64_t)threadIdx.x * per, b = a + per < nt ? a + per : nt;
+    int64_t v = 0;
+    for (int64_t i = a; i < b; ++i) v += tsum[i];
+    int64_t total;
+    int64_t run = block_exclusive_scan(v, sh, &total);
+    for (int64_t i = a; i < b; ++i) {
+        const int64_t c = tsum[i];
+        tsum[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 0) offsets[np] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_apply_kernel(const int32_t *counts, int64_t np,
+                                                                       const int64_t *tbase, int64_t *offsets)
+{
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    int32_t c[kScanPer];
+    int64_t v = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        c[j] = base + j < np ? counts[base + j] : 0;
+        v += c[j];
+    }
+    int64_t total;
+    int64_t run = tbase[blockIdx.x] + block_exclusive_scan(v, sh, &total);
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        if (base + j < np) offsets[base + j] = run;
+        run += c[j];
+    }
+}
+
+int64_t scan_tiles(int64_t np) { return (np + kScanTile - 1) / kScanTile; }
 
 void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, hipStream_t s)
 {
@@ -109,9 +163,14 @@ void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *c
         hipLaunchKernelGGL(trace_count_kernel<2>, g, dim3(256), 0, s, P, p0, np, counts);
 }
 
-void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, hipStream_t s)
+void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, int64_t *tsum, hipStream_t s)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, np, offsets);
+    const int64_t nt = scan_tiles(np);
+    if (nt > 0) hipLaunchKernelGGL(scan_tile_sum_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, s, counts, np, tsum);
+    hipLaunchKernelGGL(scan_tile_base_kernel, dim3(1), dim3(kScanThreads), 0, s, tsum, nt, np, offsets);
+    if (nt > 0)
+        hipLaunchKernelGGL(scan_tile_apply_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, s, counts, np,
+                           (const int64_t *)tsum, offsets);
 }
 
 void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64_t *offsets, PathRec *paths,

@@ -68,6 +68,18 @@ def test_gpu_trace_reproduces_oracle_walks(oracle, kind, dim, cbs, smp, amp0):
     np.testing.assert_allclose(g["w0p_re"], o["w0p_re"], rtol=1e-12)
 
 
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 6145, 120001])
+def test_gpu_trace_offsets_across_scan_tiles(oracle, n):
+    """The vertex offsets (a tiled exclusive scan of the per-path counts,
+    2048 counts per tile) at path counts on and beside the tile edges."""
+    spec = build("hg", 3, True, 2, None)
+    say("trace offsets", n)
+    g = E.trace(spec, n)
+    o = oracle.trace(spec, n)
+    assert g["offset"][0] == 0
+    assert np.array_equal(g["offset"], o["offset"])
+
+
 @pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
 def test_gpu_mccov_replay(oracle, kind, dim, cbs, smp, amp0, precision, tol):
