@@ -68,16 +68,22 @@ def test_gpu_trace_reproduces_oracle_walks(oracle, kind, dim, cbs, smp, amp0):
     np.testing.assert_allclose(g["w0p_re"], o["w0p_re"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 6145, 120001])
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 6145, 120001, 700001])
 def test_gpu_trace_offsets_across_scan_tiles(oracle, n):
     """The vertex offsets (a tiled exclusive scan of the per-path counts,
-    2048 counts per tile) at path counts on and beside the tile edges."""
+    2048 counts per tile) at path counts on and beside the tile edges, and
+    the walks themselves at a chunk size of the Fourier field's order."""
     spec = build("hg", 3, True, 2, None)
     say("trace offsets", n)
     g = E.trace(spec, n)
     o = oracle.trace(spec, n)
     assert g["offset"][0] == 0
     assert np.array_equal(g["offset"], o["offset"])
+    if n > 100000:
+        np.testing.assert_array_equal(g["weight"], o["weight"])
+        np.testing.assert_array_equal(g["xi"], o["xi"])
+        np.testing.assert_allclose(g["x"], o["x"], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(g["px"], o["px"], rtol=1e-12)
 
 
 @pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
