@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -901,6 +902,18 @@ int scmc_session_create(const scmc_problem *pr, int32_t kind, int32_t precision,
     std::unique_ptr<Session> s(new Session());
     int rc = s->init(pr, kind, precision);
     if (rc) return rc;
+    // sessions the caller never destroys are released before the HIP runtime
+    // shuts down at exit (init has initialised HIP; atexit runs LIFO)
+    static std::once_flag at_exit;
+    std::call_once(at_exit, [] {
+        std::atexit([] {
+            std::map<int64_t, std::unique_ptr<Session>> left;
+            {
+                std::lock_guard<std::mutex> g(g_sess_mu);
+                left.swap(g_sessions);
+            }
+        });
+    });
     std::lock_guard<std::mutex> g(g_sess_mu);
     *session = g_next_session++;
     g_sessions[*session] = std::move(s);

@@ -11,6 +11,8 @@
 // only on the request (Philox paths, fixed slice schedule), not on the mix.
 #include "scmc_internal.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -53,6 +55,25 @@ struct Server {
     std::vector<std::thread> slots;
     int64_t next_id = 1;
     bool stop = false;
+    // A server still running when the process exits (no scmc_server_stop)
+    // is stopped here: joinable threads would otherwise end the process in
+    // std::terminate.  Running requests are cancelled at their next slice.
+    ~Server()
+    {
+        bool joined = false;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+            for (auto &kv : all) kv.second->cancel = true;
+        }
+        cv_work.notify_all();
+        for (auto &t : slots)
+            if (t.joinable()) {
+                t.join();
+                joined = true;
+            }
+        if (joined) reaper_stop();
+    }
 };
 
 static std::mutex g_srv_mu;
@@ -200,6 +221,11 @@ int scmc_server_start(const scmc_server_config *cfg)
     (void)hipMemGetInfo(&freeb, &total);
     S->budget = S->cfg.hbm_budget_gb > 0 ? (int64_t)(S->cfg.hbm_budget_gb * 1e9) : (int64_t)(0.8 * (double)freeb);
     S->slice = S->cfg.paths_per_slice;
+    // Stop a server the caller leaves running before the HIP runtime shuts
+    // down at exit: HIP registers its teardown when it initialises (above),
+    // and atexit handlers run last-registered first.
+    static std::once_flag at_exit;
+    std::call_once(at_exit, [] { std::atexit([] { (void)scmc_server_stop(); }); });
     Server *raw = S.get();
     reaper_start();
     for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);

@@ -88,3 +88,28 @@ def test_cancel_frees_the_slot():
         assert ei.value.code == N.SCMC_E_CANCELLED
         Ms, Mm, st = S.wait(small)
         assert np.all(np.isfinite(Ms + Mm))
+
+
+def test_server_left_running_at_exit_ends_cleanly():
+    """A process that exits with the server still running (no stop, a request
+    in flight) ends normally: the server's destructor cancels the request and
+    joins its threads."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "from conftest import cone_dirs\n"
+        "from mcspeckle_amd import engines as E\n"
+        "rng = np.random.default_rng(3)\n"
+        "spec = E.make_problem(5.0, 0.99, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(8, 15, rng), cone_dirs(8, 15, rng),"
+        " 1, 1, 1.0, True, 2, 3, 0.8, seed=1)\n"
+        "S = E.Server(max_active=2)\n"
+        "S.submit(spec, 200000, 'mixed')\n"
+        "print('submitted', flush=True)\n"
+    ) % (root, os.path.join(root, "tests"))
+    say("server exit without stop")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=180)
+    assert "submitted" in r.stdout, r.stderr[-2000:]
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
