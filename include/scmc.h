@@ -237,6 +237,9 @@ typedef struct {
     int32_t max_active;      /* max requests in flight                          */
     double hbm_budget_gb;    /* admission budget for request buffers            */
     int64_t paths_per_slice; /* paths per request per scheduling round (0=auto) */
+    int32_t shared_launches; /* 1: the mixed updates of small requests (N <= 1024)
+                                running at the same time share launches;
+                                0 (default): each request launches its own     */
 } scmc_server_config;
 
 int scmc_server_start(const scmc_server_config *cfg);

@@ -119,12 +119,63 @@ void reaper_stop()
     delete R;
 }
 
+thread_local BlockPool *tls_block_pool = nullptr;
+
+void *BlockPool::take(size_t want, size_t *got)
+{
+    // smallest ready block of at least want bytes, at most twice that
+    size_t best = ready.size();
+    for (size_t i = 0; i < ready.size(); ++i)
+        if (ready[i].bytes >= want && ready[i].bytes <= 2 * want &&
+            (best == ready.size() || ready[i].bytes < ready[best].bytes))
+            best = i;
+    if (best == ready.size()) return nullptr;
+    void *q = ready[best].p;
+    *got = ready[best].bytes;
+    ready.erase(ready.begin() + (long)best);
+    return q;
+}
+
+void BlockPool::settle()
+{
+    ready.insert(ready.end(), retired.begin(), retired.end());
+    retired.clear();
+    size_t total = 0;
+    for (const Blk &b : ready) total += b.bytes;
+    // over the cap: give the largest blocks back
+    std::sort(ready.begin(), ready.end(), [](const Blk &a, const Blk &b) { return a.bytes < b.bytes; });
+    while (total > cap && !ready.empty()) {
+        total -= ready.back().bytes;
+        device_free(ready.back().p);
+        ready.pop_back();
+    }
+}
+
+void BlockPool::trim(bool now)
+{
+    for (const std::vector<Blk> *v : {&ready, &retired})
+        for (const Blk &b : *v) {
+            if (now)
+                (void)hipFree(b.p);
+            else
+                device_free(b.p);
+        }
+    ready.clear();
+    retired.clear();
+}
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release()
 {
-    if (p) device_free(p);
+    if (p) {
+        if (pool)
+            pool->put(p, bytes);
+        else
+            device_free(p);
+    }
     p = nullptr;
     bytes = 0;
+    pool = nullptr;
 }
 bool DevBuf::ensure(size_t b, int *rc)
 {
@@ -134,13 +185,30 @@ bool DevBuf::ensure(size_t b, int *rc)
     // reallocation (and its device-wide wait)
     size_t want = b ? b + b / 8 : 16;
     want = (want + 255) & ~(size_t)255;
+    BlockPool *bp = tls_block_pool;
+    if (bp) {
+        size_t got = 0;
+        if (void *q = bp->take(want, &got)) {
+            p = q;
+            bytes = got;
+            pool = bp;
+            return true;
+        }
+    }
     hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess && bp) {
+        // kept blocks may be what the driver is missing
+        (void)hipGetLastError();
+        bp->trim(true);
+        e = hipMalloc(&p, want);
+    }
     if (e != hipSuccess) {
         p = nullptr;
         *rc = set_error(SCMC_E_NOMEM, "device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
         return false;
     }
     bytes = want;
+    pool = bp;
     return true;
 }
 
@@ -543,6 +611,80 @@ int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
 // Consumer half (stream sc): waits for the slot's factors, runs the
 // matrix-core updates into the accumulators (only ever written on sc, so
 // chunks accumulate in order), then moves to the other slot.
+// ------------------------------------------------------ shared updates
+int UpdateBatcher::init()
+{
+    int rc = SCMC_OK;
+    if (!stream && !hip_ok(hipStreamCreate(&stream), "batcher stream", &rc)) return rc;
+    return SCMC_OK;
+}
+
+UpdateBatcher::~UpdateBatcher()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int UpdateBatcher::run(Job &j)
+{
+    int rc = SCMC_OK;
+    std::unique_lock<std::mutex> lk(mu);
+    pending.push_back(&j);
+    while (!j.launched) {
+        // retire finished groups (non-blocking queries)
+        for (size_t i = 0; i < inflight.size();) {
+            if (hipEventQuery(inflight[i]) == hipSuccess) {
+                spare.push_back(inflight[i]);
+                inflight.erase(inflight.begin() + (long)i);
+            } else {
+                ++i;
+            }
+        }
+        if (launching || (int)inflight.size() >= max_inflight) {
+            cv.wait_for(lk, std::chrono::microseconds(100));
+            continue;
+        }
+        // launch everything pending as one group (more than kMaxUpdateGroup
+        // problems go out as several launches in the same stream order)
+        launching = true;
+        std::vector<Job *> take;
+        take.swap(pending);
+        hipEvent_t end = nullptr;
+        if (!spare.empty()) {
+            end = spare.back();
+            spare.pop_back();
+        }
+        lk.unlock();
+        std::vector<const HerkProblem *> list;
+        bool ok = end || hip_ok(hipEventCreateWithFlags(&end, hipEventDisableTiming), "batcher event", &rc);
+        for (Job *q : take) {
+            ok = ok && hip_ok(hipStreamWaitEvent(stream, q->ready, 0), "batcher wait", &rc);
+            if (ok) (void)hipEventRecord(q->t0, stream);
+            list.push_back(q->hm);
+            list.push_back(q->hs);
+        }
+        if (ok) {
+            launch_herk_mixed_group(list.data(), (int)list.size(), stream);
+            for (Job *q : take) (void)hipEventRecord(q->t1, stream);
+            (void)hipEventRecord(end, stream);
+            ok = hip_ok(hipGetLastError(), "shared update", &rc);
+        }
+        lk.lock();
+        launching = false;
+        if (end) inflight.push_back(end);
+        groups += 1;
+        jobs += (int64_t)take.size();
+        // a failed launch still releases the waiters: their sessions then
+        // report the error from their own next HIP call or this one
+        for (Job *q : take) q->launched = true;
+        cv.notify_all();
+        if (!ok) return rc;
+    }
+    return rc;
+}
+
 int CovEngine::consume(hipStream_t sc)
 {
     int rc = SCMC_OK;
@@ -571,13 +713,30 @@ int CovEngine::consume(hipStream_t sc)
     hs.order = order.p;
     // mixed: one launch for both problems (ev[3]..ev[4]); FP64: the
     // multiple-scattering launch between ev[3] and ev[4], then the single
-    (void)hipEventRecord(S.ev[3], sc);
-    if (mixed)
-        launch_herk_mixed_update(hm, &hs, sc);
-    else
-        launch_herk_fp64(hm, sc);
-    (void)hipEventRecord(S.ev[4], sc);
-    if (!mixed) launch_herk_fp64(hs, sc);
+    if (mixed && batcher && n <= kSharedUpdateMaxN) {
+        // serving: one launch shared with the other small requests' chunks
+        // (a larger problem fills the GPU alone and launches on its own
+        // stream, so small updates do not queue behind it)
+        // the accumulators were zeroed / grown and the tile order uploaded on
+        // sc after ev[2]: the shared stream waits for all of it
+        (void)hipEventRecord(S.ev[6], sc);
+        UpdateBatcher::Job j;
+        j.hm = &hm;
+        j.hs = &hs;
+        j.ready = S.ev[6];
+        j.t0 = S.ev[3];
+        j.t1 = S.ev[4];
+        if ((rc = batcher->run(j))) return rc;
+        if (!hip_ok(hipStreamWaitEvent(sc, S.ev[4], 0), "wait shared update", &rc)) return rc;
+    } else {
+        (void)hipEventRecord(S.ev[3], sc);
+        if (mixed)
+            launch_herk_mixed_update(hm, &hs, sc);
+        else
+            launch_herk_fp64(hm, sc);
+        (void)hipEventRecord(S.ev[4], sc);
+        if (!mixed) launch_herk_fp64(hs, sc);
+    }
     // the scale bound goes to pinned memory on the stream: a blocking copy on
     // the null stream would wait for the update in flight on the other stream
     if (mixed && !hip_ok(hipMemcpyAsync(S.bound_host, S.bound.p, sizeof(unsigned), hipMemcpyDeviceToHost, sc),

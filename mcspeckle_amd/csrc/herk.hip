@@ -24,6 +24,8 @@
 // the upper triangle by Hermitian symmetry.
 #include "kernels.h"
 
+#include <initializer_list>
+
 #include <algorithm>
 #include <vector>
 
@@ -213,34 +215,56 @@ __global__ __launch_bounds__(256) void expand_step_kernel(int nv, int nl, int n,
 // 24 MFMAs per K-step (Re and Im, hi*hi + hi*lo + lo*hi).  Each K-step costs
 // one workgroup barrier.
 //
-// One launch carries the two rank-K problems of a chunk (multiple and single
-// scattering, same N and tile order): work items [0, u0.nitems) update u0,
-// the rest u1.  The short single-scattering items come last and fill the
-// workgroup slots the long ones leave free at the end of the grid.
+// One launch carries several rank-K problems: the two of a chunk (multiple
+// and single scattering, whose short items come last and fill the workgroup
+// slots the long ones leave free at the end of the grid), or, in serving,
+// the chunks of several requests of different N.  Work items are numbered
+// problem after problem.
 struct MixedUpdate {
     int64_t nks;               // K-steps of the expanded rows (pad_ksteps)
     const uint4 *Z;            // expanded rows
     double2 *acc;              // [ksplit][n][n] accumulator copies
     const unsigned *bound;     // float bits of max |u| (scale), may be null
-    int ksplit;
-    int nitems;                // ntile * ksplit, 0 for an empty problem
+    const int2 *order;         // lower-triangle tiles in blocked order
+    int n, ntile, ksplit;
+    int nitems;                // ntile * ksplit
+};
+template <int MAXG>
+struct MixedGroup {
+    MixedUpdate u[MAXG];
+    int count;
 };
 
-__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(int n, MixedUpdate u0, MixedUpdate u1,
-                                                            const int2 *__restrict__ order, int ntile)
+// MAXG = 2 for a chunk's pair of problems, kMaxUpdateGroup for shared
+// launches (a smaller argument block and select chain for the common case)
+template <int MAXG>
+__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(MixedGroup<MAXG> G)
 {
     using namespace mixed;
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const bool second = (int)blockIdx.x >= u0.nitems;   // uniform
-    const MixedUpdate &U = second ? u1 : u0;
+    // the problem of this work item (uniform); fields are picked with
+    // selects so the argument block is never indexed dynamically
+    int item = (int)blockIdx.x, g = 0;
+#pragma unroll
+    for (int q = 0; q + 1 < MAXG; ++q)
+        if (g == q && q + 1 < G.count && item >= G.u[q].nitems) {
+            item -= G.u[q].nitems;
+            g = q + 1;
+        }
+    MixedUpdate U = G.u[0];
+#pragma unroll
+    for (int q = 1; q < MAXG; ++q)
+        if (g == q) U = G.u[q];
+    // a chunk's pair shares N and the tile order (launch_herk_mixed_update)
+    const int n = MAXG == 2 ? G.u[0].n : U.n, ntile = MAXG == 2 ? G.u[0].ntile : U.ntile, ksplit = U.ksplit;
     const int64_t nks_total = U.nks;
     const uint4 *__restrict__ Z = U.Z;
     double2 *__restrict__ acc = U.acc;
     const unsigned *__restrict__ bound = U.bound;
-    const int ksplit = U.ksplit;
+    const int2 *__restrict__ order = MAXG == 2 ? G.u[0].order : U.order;
     int s, t;
-    item_decode((int)blockIdx.x - (second ? u0.nitems : 0), ksplit, ntile, s, t);
+    item_decode(item, ksplit, ntile, s, t);
     const int ti = order[t].x, tj = order[t].y;
     const int npad = pad_rows(n);
     const int64_t npair = nks_total / 2;   // K-steps come in pairs (padded to 4)
@@ -539,26 +563,55 @@ void launch_expand_mixed(const HerkProblem &hp, hipStream_t s)
                            (uint4 *)hp.zbuf);
 }
 
+static MixedUpdate mixed_desc(const HerkProblem &p)
+{
+    const int nt = ntiles(p.n, kTileMixed);
+    MixedUpdate u{};
+    u.nks = mixed::pad_ksteps(p.k);
+    u.Z = (const uint4 *)p.zbuf;
+    u.acc = p.acc;
+    u.bound = p.bound;
+    u.order = (const int2 *)p.order;
+    u.n = p.n;
+    u.ntile = nt * (nt + 1) / 2;
+    u.ksplit = p.ksplit;
+    u.nitems = u.ntile * p.ksplit;
+    return u;
+}
+
+template <int MAXG>
+static void launch_group(const HerkProblem *const *hp, int count, hipStream_t s)
+{
+    MixedGroup<MAXG> G{};
+    int64_t grid = 0;
+    for (int i = 0; i < count; ++i) {
+        const HerkProblem *p = hp[i];
+        if (!p || p->k <= 0) continue;
+        if (G.count == MAXG) {   // full: launch and start another group
+            hipLaunchKernelGGL(herk_mixed_kernel<MAXG>, dim3((unsigned)grid), dim3(256), 0, s, G);
+            G = MixedGroup<MAXG>{};
+            grid = 0;
+        }
+        G.u[G.count] = mixed_desc(*p);
+        grid += G.u[G.count++].nitems;
+    }
+    if (grid > 0) hipLaunchKernelGGL(herk_mixed_kernel<MAXG>, dim3((unsigned)grid), dim3(256), 0, s, G);
+}
+
+void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s)
+{
+    launch_group<kMaxUpdateGroup>(hp, count, s);
+}
+
 void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStream_t s)
 {
-    const int nt = ntiles(a.n, kTileMixed);
-    const int ntile = nt * (nt + 1) / 2;
-    auto desc = [&](const HerkProblem *hp) {
-        MixedUpdate u{};
-        if (hp && hp->k > 0) {
-            u.nks = mixed::pad_ksteps(hp->k);
-            u.Z = (const uint4 *)hp->zbuf;
-            u.acc = hp->acc;
-            u.bound = hp->bound;
-            u.ksplit = hp->ksplit;
-            u.nitems = ntile * hp->ksplit;
-        }
-        return u;
-    };
-    const MixedUpdate u0 = desc(&a), u1 = desc(b);
-    const unsigned grid = (unsigned)(u0.nitems + u1.nitems);
-    if (grid == 0) return;
-    hipLaunchKernelGGL(herk_mixed_kernel, dim3(grid), dim3(256), 0, s, a.n, u0, u1, (const int2 *)a.order, ntile);
+    // the two-entry kernel takes N and the tile order from its first entry
+    if (a.k > 0 && b && b->k > 0 && (b->n != a.n || b->order != a.order)) {
+        launch_group<kMaxUpdateGroup>(std::initializer_list<const HerkProblem *>{&a, b}.begin(), 2, s);
+        return;
+    }
+    const HerkProblem *list[2] = {&a, b};
+    launch_group<2>(list, 2, s);
 }
 
 // Lower-triangle tiles (ti >= tj) enumerated by 8x8 super-blocks.

@@ -92,6 +92,10 @@ constexpr int kTileF64 = 64;
 // items follow a's in the same grid.
 void launch_expand_mixed(const HerkProblem &hp, hipStream_t s);
 void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStream_t s);
+// Up to kMaxUpdateGroup problems (each with its own N, tile order and
+// accumulators; null or empty entries skipped) per launch, more in several.
+constexpr int kMaxUpdateGroup = 8;
+void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s);
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the

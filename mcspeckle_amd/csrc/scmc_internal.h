@@ -4,8 +4,10 @@
 #include "kernels.h"
 #include "scmc.h"
 
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 namespace scmc {
@@ -19,9 +21,33 @@ int64_t chunk_paths(int n, int64_t remaining, double mean_len, bool mixed);
 int64_t field_chunk_paths(int nv, int nl, int64_t remaining, double mean_len);
 
 // Owning device allocation that only grows.
+// Serving: device blocks a session frees are kept and handed out again
+// instead of going back to the driver (hipFree waits for the whole device
+// and can stall every other slot).  A block freed during a request may still
+// be read by that request's queued kernels, so it becomes reusable only at
+// settle(), which the server calls once the request has fully synchronised.
+// DevBuf allocates from the pool that tls_block_pool names (the slot
+// thread's session while it runs a request) and returns a block to the pool
+// it came from.
+struct BlockPool {
+    struct Blk {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Blk> ready, retired;
+    size_t cap = (size_t)16 << 30;   // ready bytes kept past settle()
+    void *take(size_t want, size_t *got);
+    void put(void *p, size_t bytes) { retired.push_back({p, bytes}); }
+    void settle();
+    void trim(bool now = false);   // every block back to the driver (now: hipFree on this thread)
+    ~BlockPool() { trim(); }
+};
+extern thread_local BlockPool *tls_block_pool;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    BlockPool *pool = nullptr;   // where p goes back to (null: the driver)
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -62,7 +88,8 @@ struct TraceChunk {
 // the next chunk are built while the current one updates the accumulators).
 struct CovSlot {
     DevBuf A1, B1, A2, B2, SA, SB, Gp, Fp2, maxG, maxF2, U, ones, EV0, UL0, bound, zbuf_m, zbuf_s;
-    hipEvent_t ev[6] = {};   // factors start / NEE done / expansion done; update start / multi done / single done
+    hipEvent_t ev[7] = {};   // factors start / NEE done / expansion done; update start / update done /
+                             // bound copied; update inputs ready (the accumulators and tile order too)
     HerkProblem hp[2];       // multiple and single scattering
     bool used = false;       // events recorded and not yet harvested
     int64_t np = 0, k2 = 0;
@@ -70,7 +97,40 @@ struct CovSlot {
 };
 
 // Covariance accumulation state across chunks.
+// Serving: the mixed-precision updates of the requests running at the same
+// time go out as shared launches (launch_herk_mixed_group).  A slot thread
+// submits its chunk's two problems; whichever thread finds no launch in
+// progress and fewer than max_inflight groups still running on the
+// batcher's stream launches everything pending as one group, after making
+// the stream wait on each job's factors.  Each job's t1 event is recorded
+// after its group, and the submitting session's stream waits on it.  No
+// GPU-side waits are involved: a thread that cannot launch re-checks every
+// 100 us.
+// Largest N = Nv*Nl whose updates go through the shared launches (36 tiles
+// of 128 or fewer: a single such problem leaves most of the GPU idle).
+constexpr int kSharedUpdateMaxN = 1024;
+struct UpdateBatcher {
+    struct Job {
+        const HerkProblem *hm = nullptr, *hs = nullptr;
+        hipEvent_t ready = nullptr, t0 = nullptr, t1 = nullptr;   // t1: after the group
+        bool launched = false;
+    };
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Job *> pending;
+    std::vector<hipEvent_t> inflight;   // end events of launched groups
+    std::vector<hipEvent_t> spare;
+    bool launching = false;
+    int max_inflight = 2;
+    int64_t groups = 0, jobs = 0;       // launches and problems pairs (statistics)
+    int init();
+    ~UpdateBatcher();
+    int run(Job &j);   // returns once j's group is launched
+};
+
 struct CovEngine {
+    UpdateBatcher *batcher = nullptr;   // serving: shared update launches
     bool mixed = false;
     int nv = 0, nl = 0, n = 0, tile = 0;
     int ks_m = 0, ks_s = 0;
@@ -123,6 +183,7 @@ struct ProblemCopy {
 
 // A solve in progress: device accumulators plus the traced-chunk buffers.
 struct Session {
+    BlockPool pool;   // serving: reused device blocks (first member: destroyed last)
     int kind = 0;   // 0 MCcov, 1 MCfield
     bool mixed = false;
     ProblemCopy pc;

@@ -55,6 +55,8 @@ struct Server {
     std::vector<std::thread> slots;
     int64_t next_id = 1;
     bool stop = false;
+    bool shared_updates = false;   // mixed updates of concurrent small requests in shared launches
+    UpdateBatcher batcher;
     // A server still running when the process exits (no scmc_server_stop)
     // is stopped here: joinable threads would otherwise end the process in
     // std::terminate.  Running requests are cancelled at their next slice.
@@ -101,6 +103,9 @@ int64_t request_footprint(int nv, int nl, int precision, bool near)
 
 static void run_request(Server *S, Request *r, Session &sess)
 {
+    // device blocks come from and go back to the session's pool while the
+    // request runs; they become reusable once it has synchronised (below)
+    tls_block_pool = &sess.pool;
     int rc = sess.init(&r->pc.pr, 0, r->precision);
     const int64_t slice = S->slice > 0 ? S->slice : std::max<int64_t>(1000, r->max_itr / 8);
     for (int64_t p0 = 0; !rc && p0 < r->max_itr; p0 += slice) {
@@ -124,6 +129,14 @@ static void run_request(Server *S, Request *r, Session &sess)
     sess.fill_stats(&r->st);
     r->rc = rc;
     if (rc) r->err = scmc_last_error();
+    // every kernel of the request has finished: result() synchronised the
+    // session's stream after draining both (on an error path, wait here)
+    if (rc) {
+        if (sess.stream2) (void)hipStreamSynchronize(sess.stream2);
+        if (sess.stream) (void)hipStreamSynchronize(sess.stream);
+    }
+    sess.pool.settle();
+    tls_block_pool = nullptr;
 }
 
 // A slot keeps one session between requests: its streams and device buffers
@@ -157,6 +170,7 @@ static void slot_loop(Server *S)
             r->t_start = clk::now();
         }
         if (!sess) sess.reset(new Session());
+        sess->ce.batcher = S->shared_updates ? &S->batcher : nullptr;
         run_request(S, r, *sess);
         if (r->rc || r->footprint * 2 * std::max(1, S->cfg.max_active) > S->budget) sess.reset();
         {
@@ -226,6 +240,8 @@ int scmc_server_start(const scmc_server_config *cfg)
     // and atexit handlers run last-registered first.
     static std::once_flag at_exit;
     std::call_once(at_exit, [] { std::atexit([] { (void)scmc_server_stop(); }); });
+    S->shared_updates = S->cfg.shared_launches != 0;
+    if (S->shared_updates && (rc = S->batcher.init())) return rc;
     Server *raw = S.get();
     reaper_start();
     for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);

@@ -527,10 +527,11 @@ class Session:
 class Server:
     """Continuous-batching covariance server (scmc_server_* in scmc.h)."""
 
-    def __init__(self, max_active=4, hbm_budget_gb=0.0, paths_per_slice=0, device=0):
+    def __init__(self, max_active=4, hbm_budget_gb=0.0, paths_per_slice=0, device=0, shared_launches=False):
         cfg = N.ServerConfig()
         cfg.device, cfg.max_active = device, max_active
         cfg.hbm_budget_gb, cfg.paths_per_slice = hbm_budget_gb, paths_per_slice
+        cfg.shared_launches = 1 if shared_launches else 0
         N.check(N.lib().scmc_server_start(C.byref(cfg)))
         self._specs = {}
 

@@ -8,6 +8,8 @@
 * Admission by footprint: with a budget that fits one request at a time, the
   requests still all complete, one after another.
 * Cancel frees the slot and reports SCMC_E_CANCELLED.
+* Shared update launches (the chunks of concurrent requests in one
+  matrix-core launch) give the results of separate launches, to 1e-12.
 """
 import sys
 import time
@@ -62,6 +64,24 @@ def test_results_same_alone_and_in_the_mix():
         Ms1, Mm1 = E.mccov_spec(req(i, nv, nl, od, g, seed=i), n, prec)
         tol = 1e-4 if prec == "mixed" else 1e-12
         assert cs_err(Ms_m + Mm_m, Ms1 + Mm1) <= tol, i
+
+
+def test_shared_update_launches_match_separate_ones():
+    """Requests of different N whose chunks share matrix-core launches get
+    the same covariance as with one launch per request: every work item reads
+    and writes only its own problem's rows and accumulator copies."""
+    mix = [m for m in MIX if m[6] == "mixed"]
+    out = {}
+    for shared in (False, True):
+        with E.Server(max_active=4, paths_per_slice=300, shared_launches=shared) as S:
+            ids = {i: S.submit(req(i, nv, nl, od, g, seed=i), n, prec) for i, nv, nl, od, g, n, prec in mix}
+            say("submitted, shared launches", shared)
+            out[shared] = {i: S.wait(r) for i, r in ids.items()}
+    for i, *_ in mix:
+        Ms_a, Mm_a, _ = out[False][i]
+        Ms_b, Mm_b, _ = out[True][i]
+        assert cs_err(Ms_b, Ms_a) <= 1e-12, i
+        assert cs_err(Mm_b, Mm_a) <= 1e-12, i
 
 
 def test_admission_by_footprint_serialises_big_requests():
