@@ -505,6 +505,7 @@ int CovEngine::ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s)
     if (!hip_ok(hipStreamSynchronize(s), "acc sync", &rc)) return rc;
     std::swap(acc.p, nb.p);
     std::swap(acc.bytes, nb.bytes);
+    std::swap(acc.pool, nb.pool);   // each block goes back where it came from
     ks_have = ks;
     return SCMC_OK;
 }
@@ -633,9 +634,10 @@ int UpdateBatcher::run(Job &j)
     std::unique_lock<std::mutex> lk(mu);
     pending.push_back(&j);
     while (!j.launched) {
-        // retire finished groups (non-blocking queries)
+        // retire finished groups (non-blocking queries; an error status also
+        // retires the group, so a failed device cannot leave waiters spinning)
         for (size_t i = 0; i < inflight.size();) {
-            if (hipEventQuery(inflight[i]) == hipSuccess) {
+            if (hipEventQuery(inflight[i]) != hipErrorNotReady) {
                 spare.push_back(inflight[i]);
                 inflight.erase(inflight.begin() + (long)i);
             } else {
@@ -676,13 +678,18 @@ int UpdateBatcher::run(Job &j)
         if (end) inflight.push_back(end);
         groups += 1;
         jobs += (int64_t)take.size();
-        // a failed launch still releases the waiters: their sessions then
-        // report the error from their own next HIP call or this one
-        for (Job *q : take) q->launched = true;
+        // a failed launch releases its waiters with the error
+        for (Job *q : take) {
+            q->launched = true;
+            if (!ok) {
+                q->rc = rc;
+                q->err = scmc_last_error();
+            }
+        }
         cv.notify_all();
-        if (!ok) return rc;
     }
-    return rc;
+    if (j.rc) return set_error(j.rc, "%s", j.err.c_str());
+    return SCMC_OK;
 }
 
 int CovEngine::consume(hipStream_t sc)

@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace scmc {
@@ -114,6 +115,8 @@ struct UpdateBatcher {
         const HerkProblem *hm = nullptr, *hs = nullptr;
         hipEvent_t ready = nullptr, t0 = nullptr, t1 = nullptr;   // t1: after the group
         bool launched = false;
+        int rc = SCMC_OK;   // the group's launch status
+        std::string err;
     };
     hipStream_t stream = nullptr;
     std::mutex mu;

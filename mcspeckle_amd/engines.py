@@ -27,7 +27,9 @@ Outputs are numpy complex128 arrays in MATLAB's column-major layout
 from dataclasses import dataclass, field
 from typing import Optional
 
+import atexit
 import ctypes as C
+
 import numpy as np
 
 from . import _native as N
@@ -522,6 +524,16 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+def _stop_server_at_exit():
+    # Python's atexit hooks run before the C runtime's exit handlers, among
+    # which HIP tears itself down; a server still running then is stopped here
+    if N._lib is not None:
+        N._lib.scmc_server_stop()
+
+
+atexit.register(_stop_server_at_exit)
 
 
 class Server:
