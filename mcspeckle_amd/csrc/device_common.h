@@ -416,4 +416,31 @@ __device__ __forceinline__ float2 phase_att_mixed(const DevProblem &P, const dou
     return make_float2(mag * c, mag * s);
 }
 
+// Isotropic or HG amplitude (evalampfunc_general.m:5-18, evaluateHG.m:25-39)
+// in FP32 for the mixed variant: iso, or hg_k (1 + g^2 - 2 g c)^mu with
+// mu = -3/4 (3-D) or -1/2 (2-D).  Types other than 1 and 2 are HG, as in
+// eval_amp; tabulated amplitudes (type 2) are not handled here.
+struct AmpF32 {
+    bool hg;
+    float iso, hg_k, g2p1, g2;
+};
+template <int DIM>
+__device__ __forceinline__ AmpF32 amp_f32_setup(const DevPhase &ph)
+{
+    AmpF32 a;
+    a.hg = ph.type != 1;
+    a.iso = (float)ph.iso;
+    a.hg_k = (float)sqrt((1 - ph.g * ph.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi));
+    a.g2p1 = (float)(1 + ph.g * ph.g);
+    a.g2 = (float)(2 * ph.g);
+    return a;
+}
+template <int DIM>
+__device__ __forceinline__ float amp_f32(float c, const AmpF32 &a)
+{
+    if (!a.hg) return a.iso;
+    const float t = fmaxf(a.g2p1 - a.g2 * c, 1e-30f);
+    return a.hg_k * __builtin_amdgcn_exp2f((DIM == 3 ? -0.75f : -0.5f) * __builtin_amdgcn_logf(t));
+}
+
 } // namespace scmc

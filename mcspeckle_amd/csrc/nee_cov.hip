@@ -17,6 +17,8 @@
 #include "hetero.h"
 #include "kernels.h"
 
+#include <type_traits>
+
 #include <algorithm>
 
 namespace scmc {
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(256) void nee_single_near_kernel(DevProblem P, int6
 // and nl light items over its threads; the per-vertex maxima meet in LDS and
 // the workgroup adds one value to the chunk bound.
 constexpr int kNeeVertexThreads = 256, kNeeMaxVpb = 64;
-template <int DIM, typename T, bool HET>
+template <int DIM, typename T, bool HET, bool FAST = false>
 __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
     DevProblem P, HetMedium M, int64_t nvert, int vpb, const PathRec *paths, const VertexRec *verts,
     const int32_t *vpath, const int64_t *offsets, const T *Gp, const T *Fp2, const float *maxG, const float *maxF2,
@@ -175,6 +177,13 @@ __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
     for (int j = threadIdx.x; j < vpb; j += blockDim.x) mAv[j] = mBv[j] = badv[j] = 0u;
     __syncthreads();
     const int w = P.nv + P.nl;
+    // FAST (FP32 factors, far-field views and lights, homogeneous medium,
+    // isotropic or HG phase; checked by the launcher): E by the FP64 cycle
+    // reduction and FP32 sin/cos/exp, the amplitude in FP32 (the factors
+    // are rounded to FP32 anyway).  Its own instance, so the FP64 path's
+    // registers do not limit its occupancy.
+    static_assert(!FAST || (std::is_same<T, float2>::value && !HET), "FAST needs FP32 factors, no voxels");
+    const AmpF32 af32 = amp_f32_setup<DIM>(P.amp);
     for (int j = threadIdx.x; j < vpb * w; j += blockDim.x) {
         const int vl = j / w, i = j - vl * w;
         const int64_t vi = v0 + vl;
@@ -185,6 +194,38 @@ __global__ __launch_bounds__(kNeeVertexThreads) void nee_cov_vertex_kernel(
         const VertexRec r = verts[vi];
         const PathRec pr = paths[p];
         const double sw = sqrt(r.weight / pr.px);
+        if constexpr (FAST) {
+            const float swf = (float)sw;
+            float2 f;
+            if (i < P.nv) {
+                const double *vc = P.v + (size_t)i * DIM;   // far field: rv = v_i
+                f = phase_att_mixed<DIM>(P, r.x, vc, P.ff_sign, 1);
+                const float a = amp_f32<DIM>((float)dot<DIM>(r.ow, vc), af32) * swf;
+                f.x *= a;
+                f.y *= a;
+                A1[m * P.nv + i] = to_c<T>(make_double2(f.x, f.y));
+                const float mg = sqrtf(f.x * f.x + f.y * f.y);
+                if (mg > 0.f) atomicMax(&mAv[vl], __float_as_uint(mg));
+                if (!(isfinite(f.x) && isfinite(f.y))) atomicOr(&badv[vl], 1u);
+                if (P.do_cbs) A2[m * P.nv + i] = Fp2[(int64_t)p * P.nv + i];
+            } else {
+                const int li = i - P.nv;
+                B1[m * P.nl + li] = Gp[(int64_t)p * P.nl + li];
+                if (P.do_cbs) {
+                    const double *lc = P.l + (size_t)li * DIM;   // far field: rl = l_j
+                    f = phase_att_mixed<DIM>(P, r.x, lc, 1.0, 1);
+                    const double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
+                    const float a = amp_f32<DIM>((float)dot<DIM>(mow, lc), af32) * swf;
+                    f.x *= a;
+                    f.y *= a;
+                    B2[m * P.nl + li] = to_c<T>(make_double2(f.x, f.y));
+                    const float mg = sqrtf(f.x * f.x + f.y * f.y);
+                    if (mg > 0.f) atomicMax(&mBv[vl], __float_as_uint(mg));
+                    if (!(isfinite(f.x) && isfinite(f.y))) atomicOr(&badv[vl], 1u);
+                }
+            }
+            continue;
+        }
         if (i < P.nv) {
             const double *vc = P.v + (size_t)i * DIM;
             double rv[3];
@@ -248,10 +289,16 @@ static void nee_cov_dispatch(const DevProblem &P, const HetMedium &M, int64_t np
     if (nvert > 0) {
         // about four items per thread
         const int vpb = std::max(1, std::min(kNeeMaxVpb, 4 * kNeeVertexThreads / (P.nv + P.nl)));
-        hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET>), dim3((unsigned)((nvert + vpb - 1) / vpb)),
-                           dim3(kNeeVertexThreads), 0, s, P, M, nvert, vpb, paths, verts, vpath, offsets,
-                           (const T *)f.Gp, (const T *)f.Fp2, f.maxG, f.maxF2, (T *)f.A1, (T *)f.B1, (T *)f.A2,
-                           (T *)f.B2, f.bound, f.nonfinite);
+        const dim3 grid((unsigned)((nvert + vpb - 1) / vpb));
+        constexpr bool kCanFast = std::is_same<T, float2>::value && !HET;
+        if (kCanFast && P.is_ff_v && P.is_ff_l && P.amp.type != 2)
+            hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET, kCanFast>), grid, dim3(kNeeVertexThreads), 0, s, P,
+                               M, nvert, vpb, paths, verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2,
+                               f.maxG, f.maxF2, (T *)f.A1, (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound, f.nonfinite);
+        else
+            hipLaunchKernelGGL((nee_cov_vertex_kernel<DIM, T, HET, false>), grid, dim3(kNeeVertexThreads), 0, s, P, M,
+                               nvert, vpb, paths, verts, vpath, offsets, (const T *)f.Gp, (const T *)f.Fp2, f.maxG,
+                               f.maxF2, (T *)f.A1, (T *)f.B1, (T *)f.A2, (T *)f.B2, f.bound, f.nonfinite);
     }
 }
 
