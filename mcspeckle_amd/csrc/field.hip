@@ -36,11 +36,25 @@ constexpr int kFieldNL = 4;        // lights per launch pass
 // Per-path / per-vertex coefficients.  Block per path, lanes over lights.
 // HET: voxel medium (MCsampleHetro.m:141-176, evalphaseattHetro).
 // nonfinite += paths with a NaN/Inf coefficient (path weights, positions).
-template <int DIM, bool HET>
+// FAST (mixed precision, far-field lights, no voxels, isotropic or HG phase;
+// chosen by the launcher): the per-vertex e^{2 pi i xi} and the CBS light
+// terms in FP32 after the FP64 cycle reduction (the accumulation that uses
+// the coefficients stays FP64).
+template <int DIM>
+__device__ __forceinline__ double2 cn_f32(const VertexRec &r, const PathRec &pr)
+{
+    const float amp = sqrtf((float)(r.weight / pr.px));
+    const float fr = (float)(r.xi - rint(r.xi));   // xi in revolutions
+    return make_double2(amp * __builtin_amdgcn_cosf(fr), amp * __builtin_amdgcn_sinf(fr));
+}
+
+template <int DIM, bool HET, bool FAST = false>
 __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium M, int64_t np, const PathRec *paths,
                                                         const VertexRec *verts, const int64_t *offsets,
                                                         double2 *coef, double2 *coefR, unsigned long long *nonfinite)
 {
+    static_assert(!FAST || !HET, "FAST: homogeneous media only");
+    const AmpF32 af32 = amp_f32_setup<DIM>(P.amp);
     const int64_t p = blockIdx.x;
     if (p >= np) return;
     bool bad = false;
@@ -57,10 +71,15 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium 
         double2 R = make_double2(0.0, 0.0);
         for (int64_t vi = o; vi < e; ++vi) {
             const VertexRec r = verts[vi];
-            double amp = sqrt(r.weight / pr.px);
-            double s, c;
-            sincos(2 * kPi * r.xi, &s, &c);
-            double2 cn = make_double2(amp * c, amp * s);
+            double2 cn;
+            if constexpr (FAST) {
+                cn = cn_f32<DIM>(r, pr);
+            } else {
+                double amp = sqrt(r.weight / pr.px);
+                double s, c;
+                sincos(2 * kPi * r.xi, &s, &c);
+                cn = make_double2(amp * c, amp * s);
+            }
             double2 t;
             if (vi == o) {
                 t = cmul(el0, cn);
@@ -70,8 +89,15 @@ __global__ __launch_bounds__(64) void field_coef_kernel(DevProblem P, HetMedium 
                     t.x *= rs2;
                     t.y *= rs2;
                     double mow[3] = {-r.ow[0], -r.ow[1], -r.ow[2]};
-                    double2 el = cmul(nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l),
-                                      eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
+                    double2 el;
+                    if constexpr (FAST) {
+                        const float2 e32 = phase_att_mixed<DIM>(P, r.x, lc, 1.0, 1);
+                        const float a = amp_f32<DIM>((float)dot<DIM>(mow, rl), af32);
+                        el = make_double2(e32.x * a, e32.y * a);
+                    } else {
+                        el = cmul(nee_e<DIM, HET>(P, M, r.x, lc, 1.0, P.is_ff_l),
+                                  eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
+                    }
                     double2 q = cmul(el, cn);
                     R.x += q.x * rs2;
                     R.y += q.y * rs2;
@@ -102,12 +128,13 @@ __device__ __forceinline__ double2 wave_sum(double2 v)
 //   coefR   = sum_n c_n SL_n / sqrt 2, SL_n = sum_l e_l(x_n, l) a(-ow_n . rl_l) Wl(l)   (:199, :213)
 //   cl0[l]  = c_1 e_l0(l) Wl(l)     (first vertex, contracted with af_ang_vl in the
 //                                    view kernel, :214)
-template <int DIM>
+template <int DIM, bool FAST = false>
 __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64_t np, const PathRec *paths,
                                                              const VertexRec *verts, const int64_t *offsets,
                                                              const double2 *wl, double2 *coef, double2 *coefR,
                                                              double2 *cl0, unsigned long long *nonfinite)
 {
+    const AmpF32 af32 = amp_f32_setup<DIM>(P.amp);
     const int64_t p = blockIdx.x;
     if (p >= np) return;
     bool bad = false;
@@ -117,6 +144,7 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
     const int lane = threadIdx.x;
     auto cn_of = [&](int64_t vi) {
         const VertexRec r = verts[vi];
+        if constexpr (FAST) return cn_f32<DIM>(r, pr);
         double amp = sqrt(r.weight / pr.px);
         double sn, cs;
         sincos(2 * kPi * r.xi, &sn, &cs);
@@ -159,8 +187,14 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
                 const double *lc = P.l + (size_t)l * DIM;
                 double rl[3];
                 light_dir<DIM>(P, pr.x1, l, rl);
-                double2 el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l),
-                                  eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
+                double2 el;
+                if constexpr (FAST) {
+                    const float2 e32 = phase_att_mixed<DIM>(P, r.x, lc, 1.0, 1);
+                    const float a = amp_f32<DIM>((float)dot<DIM>(mow, rl), af32);
+                    el = make_double2(e32.x * a, e32.y * a);
+                } else {
+                    el = cmul(phase_att64<DIM>(P, r.x, lc, 1.0, P.is_ff_l), eval_amp<DIM>(dot<DIM>(mow, rl), P.amp));
+                }
                 double2 t = cmul(el, wl[l]);
                 sl.x += t.x;
                 sl.y += t.y;
@@ -670,12 +704,20 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
     double2 *aftab = coefR + (size_t)np * ncol;
     double2 *cl0 = aftab + (size_t)P.nv * P.nl;
     double2 *part = cl0 + (wave ? (size_t)np * P.nl : 0);
-    if (wave)
-        hipLaunchKernelGGL(field_wave_coef_kernel<DIM>, dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts,
+    // mixed precision, far-field lights, isotropic/HG phase: FP32 per-vertex terms
+    const bool fastc = mixed && P.is_ff_l && P.amp.type != 2;
+    if (wave && fastc)
+        hipLaunchKernelGGL((field_wave_coef_kernel<DIM, true>), dim3((unsigned)np), dim3(64), 0, s, P, np, paths, verts,
                            offsets, wl, coef, coefR, cl0, nonfinite);
+    else if (wave)
+        hipLaunchKernelGGL((field_wave_coef_kernel<DIM, false>), dim3((unsigned)np), dim3(64), 0, s, P, np, paths,
+                           verts, offsets, wl, coef, coefR, cl0, nonfinite);
     else if (het)
         hipLaunchKernelGGL((field_coef_kernel<DIM, true>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
                            offsets, coef, coefR, nonfinite);
+    else if (fastc)
+        hipLaunchKernelGGL((field_coef_kernel<DIM, false, true>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths,
+                           verts, offsets, coef, coefR, nonfinite);
     else
         hipLaunchKernelGGL((field_coef_kernel<DIM, false>), dim3((unsigned)np), dim3(64), 0, s, P, M, np, paths, verts,
                            offsets, coef, coefR, nonfinite);
