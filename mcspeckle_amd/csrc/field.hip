@@ -652,6 +652,110 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             if (q < nlg) hpart[((size_t)split * P.nl + l0 + q) * P.nv + v] = acc[q];
 }
 
+// The same contraction on the matrix cores (mixed precision): per path,
+// H[32 views, 32 lights] += E[views] (x) cl0[lights] as two real rank-2
+// products with v_mfma_f32_32x32x2f32, K = (re, im):
+//   Re H = [Er, -Ei] . [Cr; Ci],   Im H = [Ei, Er] . [Cr; Ci].
+// Lane l of a wave computes E(x1_p, v) for view l % 32 (A row) and supplies
+// K index l / 32; B is the staged cl0 row of the path.  A workgroup holds 128
+// views (32 per wave) x 32 NB lights; the FP32 accumulators of each 32-path
+// batch are added into FP64 ones, and the ranges' partials are reduced as
+// before.
+constexpr int kWmViews = 128, kWmPB = 32;
+constexpr int kWmBlocks = 1;   // 32-light blocks per workgroup (1: 148 registers, three waves per SIMD; 2 measured 9 % slower)
+typedef float wm_f32x16 __attribute__((ext_vector_type(16)));
+template <int DIM, int NB>
+__global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P, int64_t np, int nsplit,
+                                                                    const PathRec *paths, const double2 *cl0,
+                                                                    double2 *hpart)
+{
+    __shared__ double sx[kWmPB][3];
+    __shared__ float sdlo[kWmPB][3], sdhi[kWmPB][3];
+    constexpr int kWmLights = 32 * NB;
+    __shared__ float2 scl[kWmPB][kWmLights];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 31, kk = lane >> 5;
+    const int vbase = blockIdx.x * kWmViews + wave * 32;
+    const int v = vbase + m;
+    const bool vok = v < P.nv;
+    double vd[3] = {0, 0, 0};
+    float inv[3] = {0, 0, 0};
+    bool up[3] = {false, false, false};
+    for (int j = 0; j < DIM && vok; ++j) {
+        const double nd = P.v[(size_t)v * DIM + j];
+        vd[j] = nd;
+        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
+        up[j] = nd > 0;
+    }
+    const double nilam = -P.inv_lambda;
+    const float hs = (float)(P.sigt * 0.5);
+    const int l0 = blockIdx.y * kWmLights;
+    const int split = blockIdx.z;
+    const int64_t p0 = np * split / nsplit, p1 = np * (split + 1) / nsplit;
+    wm_f32x16 cr[NB], ci[NB];
+    double ar[NB][16], ai[NB][16];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            cr[b][e] = ci[b][e] = 0.f;
+            ar[b][e] = ai[b][e] = 0.0;
+        }
+    for (int64_t pb = p0; pb < p1; pb += kWmPB) {
+        const int cnt = (int)((p1 - pb) < kWmPB ? (p1 - pb) : kWmPB);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt * 3; t += blockDim.x) {
+            const int i = t / 3, j = t % 3;
+            const double xj = paths[pb + i].x1[j];
+            sx[i][j] = xj;
+            sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
+            sdhi[i][j] = j < DIM ? (float)(P.bmax[j] - xj) : 0.f;
+        }
+        for (int t = threadIdx.x; t < kWmPB * kWmLights; t += blockDim.x) {
+            const int i = t / kWmLights, q = t % kWmLights;
+            float2 c = make_float2(0.f, 0.f);
+            if (i < cnt && l0 + q < P.nl) {
+                const double2 d = cl0[(size_t)(pb + i) * P.nl + l0 + q];
+                c = make_float2((float)d.x, (float)d.y);
+            }
+            scl[i][q] = c;
+        }
+        __syncthreads();
+        for (int i = 0; i < cnt; ++i) {
+            const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
+            float2 ev = make_float2(0.f, 0.f);
+            if (vok) ev = e_far_mixed<DIM>(x, vd, nilam, sdlo[i], sdhi[i], up, inv, hs);
+            const float a1 = kk == 0 ? ev.x : -ev.y;
+            const float a2 = kk == 0 ? ev.y : ev.x;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2 c = scl[i][b * 32 + m];
+                const float bv = kk == 0 ? c.x : c.y;
+                cr[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, cr[b], 0, 0, 0);
+                ci[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, bv, ci[b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                ar[b][e] += (double)cr[b][e];
+                ai[b][e] += (double)ci[b][e];
+                cr[b][e] = ci[b][e] = 0.f;
+            }
+    }
+    // accumulator element e of lane l: view row 8 (e / 4) + 4 (l / 32) + e % 4,
+    // light column l % 32
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int vv = vbase + 8 * (e >> 2) + 4 * kk + (e & 3);
+            const int l = l0 + b * 32 + m;
+            if (vv < P.nv && l < P.nl) hpart[((size_t)split * P.nl + l) * P.nv + vv] = make_double2(ar[b][e], ai[b][e]);
+        }
+}
+
 __global__ void field_wave_first_reduce_kernel(int nv, int nl, int nsplit, const double2 *aftab, const double2 *hpart,
                                                double2 *acc)
 {
@@ -793,10 +897,17 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
         if (nsf > kWaveMaxSplit) nsf = kWaveMaxSplit;
         if (nsf < 1) nsf = 1;
         dim3 gf((unsigned)vt, (unsigned)ng, (unsigned)nsf);
-        if (mixed)
-            hipLaunchKernelGGL((field_wave_first_kernel<DIM, true>), gf, dim3(kFieldViews), 0, s, P, np, nsf, paths,
-                               cl0, hpart);
-        else
+        if (mixed) {
+            // matrix cores: 128 views x 64 lights per workgroup
+            const int vtm = (P.nv + kWmViews - 1) / kWmViews, ngm = (P.nl + 32 * kWmBlocks - 1) / (32 * kWmBlocks);
+            int nsm = (2048 + vtm * ngm - 1) / (vtm * ngm);
+            if (nsm > maxsf) nsm = (int)maxsf;
+            if (nsm > kWaveMaxSplit) nsm = kWaveMaxSplit;
+            if (nsm < 1) nsm = 1;
+            nsf = nsm;
+            hipLaunchKernelGGL((field_wave_first_mfma_kernel<DIM, kWmBlocks>), dim3((unsigned)vtm, (unsigned)ngm, (unsigned)nsm),
+                               dim3(256), 0, s, P, np, nsm, paths, cl0, hpart);
+        } else
             hipLaunchKernelGGL((field_wave_first_kernel<DIM, false>), gf, dim3(kFieldViews), 0, s, P, np, nsf, paths,
                                cl0, hpart);
         hipLaunchKernelGGL(field_wave_first_reduce_kernel, dim3((unsigned)((P.nv + 255) / 256)), dim3(256), 0, s, P.nv,
