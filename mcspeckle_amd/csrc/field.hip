@@ -317,6 +317,13 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     const int ncol = WAVE ? 1 : P.nl;   // columns of coef / coefR / part
     __shared__ FEntry se[kFieldBatch];
     __shared__ double2 sc[kFieldBatch][kFieldNL];
+    // F32ACC (mixed): FP32 copies of the coefficients; each batch's products
+    // are summed in FP32 (64 entries at most) and then added into the FP64
+    // accumulators.  Not for the two-views-per-lane one-column instance
+    // (MCfield), which would then exceed 128 registers (four waves per SIMD)
+    // and measured slower.
+    constexpr bool F32ACC = MIXED && (WAVE || HET || VPT == 1);
+    __shared__ float2 scf[F32ACC ? kFieldBatch : 1][F32ACC ? kFieldNL : 1];
     const int split = blockIdx.y;
     const int64_t e0 = nent * split / nsplit, e1 = nent * (split + 1) / nsplit;
 
@@ -374,9 +381,15 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             if (q < nlp)
                 c = (e < nvert) ? coef[e * ncol + l0 + q] : coefR[(e - nvert) * ncol + l0 + q];
             sc[ei][q] = c;
+            if (F32ACC) scf[ei][q] = make_float2((float)c.x, (float)c.y);
         }
         __syncthreads();
         if (!vok[0]) continue;
+        float2 accf[VPT][NLQ];
+#pragma unroll
+        for (int t = 0; t < VPT; ++t)
+#pragma unroll
+            for (int q = 0; q < NLQ; ++q) accf[t][q] = make_float2(0.f, 0.f);
         for (int ei = 0; ei < cnt; ++ei) {
             const FEntry &f = se[ei];
             // every lane reads the same entry: branch on its kind as a scalar
@@ -415,6 +428,19 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                         const float2 am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
                         evf = make_float2(evf.x * am.x - evf.y * am.y, evf.x * am.y + evf.y * am.x);
                     }
+                    if (F32ACC && kind != 0) {
+                        if (kind == 2) {
+                            const float dr = (float)f.div.x, di = (float)f.div.y;
+                            evf = make_float2(evf.x * dr - evf.y * di, evf.x * di + evf.y * dr);
+                        }
+#pragma unroll
+                        for (int q = 0; q < NLQ; ++q) {
+                            const float2 c = scf[ei][q];
+                            accf[t][q].x += evf.x * c.x - evf.y * c.y;
+                            accf[t][q].y += evf.x * c.y + evf.y * c.x;
+                        }
+                        continue;
+                    }
                     ev = make_double2(evf.x, evf.y);
                 } else {
                     ev = nee_e<DIM, HET>(P, M, x, P.v + (size_t)vv[t] * DIM, P.ff_sign, P.is_ff_v);
@@ -423,7 +449,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                         ev = cmul(ev, a);
                     }
                 }
-                if (kind == 2) ev = cmul(ev, f.div);
+                if (!F32ACC && kind == 2) ev = cmul(ev, f.div);
                 if (kind == 0) {
                     for (int q = 0; q < NLQ; ++q) {
                         const double2 tt = cmul(ev, afv[t][q]), c = sce[q];
@@ -438,6 +464,15 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     }
                 }
             }
+        }
+        if (F32ACC) {
+#pragma unroll
+            for (int t = 0; t < VPT; ++t)
+#pragma unroll
+                for (int q = 0; q < NLQ; ++q) {
+                    acc[t][q].x += (double)accf[t][q].x;
+                    acc[t][q].y += (double)accf[t][q].y;
+                }
         }
     }
 #pragma unroll
