@@ -32,6 +32,10 @@ int quad_grid_setup(const double *box_min, const double *box_max, double v_max, 
 void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
                          const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,
                          hipStream_t s);
+// Walk paths [p0, p0 + np) and scatter every vertex into the slab grids in
+// the same pass; *nvert (device) is incremented by the vertex count.
+void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
+                               unsigned long long *nvert, hipStream_t s);
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s);
 void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,

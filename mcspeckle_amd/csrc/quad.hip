@@ -58,27 +58,23 @@ __device__ __forceinline__ int slab_of(double z, const QuadGeom &g)
     return j;
 }
 
+// One vertex's contribution (MCfieldFourierQuad.m:263-273, and the light-
+// summed form of MCfieldFourierQuadOnWave.m:281-302): xv the vertex, x1 its
+// path's first vertex, amp = sqrt(W/px) [/sqrt 2 for later vertices with
+// CBS], xi its random-phase uniform.
 template <typename T>
-__global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeom g, int64_t nvert, const PathRec *paths,
-                                                           const VertexRec *verts, const int32_t *vpath,
-                                                           const int64_t *offsets, typename cx<T>::t *grid)
+__device__ __forceinline__ void quad_scatter_vertex(const DevProblem &P, const QuadGeom &g, const double *xv,
+                                                    const double *x1, double amp, double xi,
+                                                    typename cx<T>::t *grid)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nvert) return;
-    const VertexRec r = verts[e];
-    const int p = vpath[e];
-    const PathRec pr = paths[p];
-    const bool first = (e == offsets[p]);
-    double amp = sqrt(r.weight / pr.px);
-    if (P.do_cbs && !first) amp *= 1 / sqrt(2.0);
     if constexpr (std::is_same<T, float>::value) {
         // FP32 grid: every phase counted in cycles and reduced in FP64, then
         // hardware sin/cos/exp2 in FP32 (the FP64 grid keeps libm FP64)
-        const float2 ex = cis_cycles(r.xi);   // e^{2 pi i xi}
+        const float2 ex = cis_cycles(xi);   // e^{2 pi i xi}
         const float pre = (float)amp * ex.x, pim = (float)amp * ex.y;
         for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
-            const double *xs = pass == 0 ? pr.x1 : r.x;
-            const double *xl = pass == 0 ? r.x : pr.x1;
+            const double *xs = pass == 0 ? x1 : xv;
+            const double *xl = pass == 0 ? xv : x1;
             const int j = slab_of(xs[2], g);
             if (j < 0) continue;
             long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
@@ -113,11 +109,11 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
         return;
     }
     double sx, cxv;
-    sincos(2 * kPi * r.xi, &sx, &cxv);
+    sincos(2 * kPi * xi, &sx, &cxv);
     const double pre = amp * cxv, pim = amp * sx;
     for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
-        const double *xs = pass == 0 ? pr.x1 : r.x;   // scatter (view-side) position
-        const double *xl = pass == 0 ? r.x : pr.x1;   // light-side position
+        const double *xs = pass == 0 ? x1 : xv;   // scatter (view-side) position
+        const double *xl = pass == 0 ? xv : x1;   // light-side position
         const int j = slab_of(xs[2], g);
         if (j < 0) continue;
         long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
@@ -159,6 +155,65 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
             atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], tr, tim);
         }
     }
+}
+
+
+// Vertex-parallel scatter over stored records (one lane per vertex): for
+// many lights per vertex, where the walk's divergence would multiply the
+// per-vertex work in the fused kernel below.
+template <typename T>
+__global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeom g, int64_t nvert, const PathRec *paths,
+                                                           const VertexRec *verts, const int32_t *vpath,
+                                                           const int64_t *offsets, typename cx<T>::t *grid)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvert) return;
+    const VertexRec r = verts[e];
+    const int p = vpath[e];
+    const PathRec pr = paths[p];
+    double amp = sqrt(r.weight / pr.px);
+    if (P.do_cbs && e != offsets[p]) amp *= 1 / sqrt(2.0);
+    quad_scatter_vertex<T>(P, g, r.x, pr.x1, amp, r.xi, grid);
+}
+
+// The walk and the scatter in one pass (one lane per path): every vertex is
+// scattered as the walk reaches it, so no vertex records are stored and no
+// counting pass is needed.  The lane follows trace_emit_kernel's walk
+// exactly (vertex n: position, weight and xi = the first uniform of draw n,
+// taken before the step); *nvert receives the vertex count (the
+// normaliser), one atomic per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, QuadGeom g, uint64_t p0, int64_t np,
+                                                                 typename cx<T>::t *grid,
+                                                                 unsigned long long *nvert)
+{
+    __shared__ unsigned long long ws[4];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (i < np) {
+        const uint64_t path = P.path_offset + p0 + (uint64_t)i;
+        double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
+        double2 w0p;
+        path_start<3>(P, path, x, px, w, w0p);
+        const double x1[3] = {x[0], x[1], x[2]};
+        double weight = P.albedo;
+        uint32_t n = 0;
+        for (;;) {
+            ++n;
+            const double2 a = uniform_pair(P.seed, path, n, 0);
+            double amp = sqrt(weight / px);
+            if (P.do_cbs && n > 1) amp *= 1 / sqrt(2.0);
+            quad_scatter_vertex<T>(P, g, x, x1, amp, a.x, grid);
+            ++cnt;
+            if (!path_step<3>(P, path, n, x, w, ow, weight, a.y)) break;
+        }
+    }
+    // workgroup sum of the vertex counts
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(nvert, ws[0] + ws[1] + ws[2] + ws[3]);
 }
 
 // Batched complex GEMM C[b] = A[b] * B[b] (column-major, lda = M, ldb = K,
@@ -260,6 +315,17 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
                            offsets, (double2 *)grid);
 }
 
+void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
+                               unsigned long long *nvert, hipStream_t s)
+{
+    if (np <= 0) return;
+    dim3 gr((unsigned)((np + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(quad_trace_scatter_kernel<float>, gr, dim3(256), 0, s, P, g, p0, np, (float2 *)grid, nvert);
+    else
+        hipLaunchKernelGGL(quad_trace_scatter_kernel<double>, gr, dim3(256), 0, s, P, g, p0, np, (double2 *)grid,
+                           nvert);
+}
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s)
 {

@@ -149,8 +149,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
     memset(&st, 0, sizeof st);
     {
         ProblemState ps;
-        TraceChunk tc;
-        DevBuf dgz, grid, W, T1, H, zexp, u, dwl;
+        DevBuf dgz, grid, W, T1, H, zexp, u, dwl, dcount;
         const int nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
         const int nl = wave ? 1 : pr.nl;   // output planes
         const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
@@ -174,26 +173,53 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                 g.wl = (const double2 *)dwl.p;
             }
             launch_zero(grid.p, (cs * ngrid + 15) / 16 * 16 > grid.bytes ? grid.bytes : (cs * ngrid + 15) / 16 * 16, s);
-            int64_t nstore = 0;
-            double mean_len = 20;
-            for (int64_t p0 = 0; p0 < max_itr;) {
-                // ~3.2e7 vertices (2 GB of records) per chunk: a trace pass takes
-                // about the longest walk's step count times the step latency
-                // whatever the chunk size, so few large chunks amortise it
-                int64_t np = std::min<int64_t>(max_itr - p0, std::max<int64_t>(256, (int64_t)(3.2e7 / mean_len)));
+            // At most 4 lights per vertex: walk and scatter in one pass (no
+            // vertex records, no counting pass).  More: trace into records,
+            // then a vertex-parallel scatter (in the fused pass the walk's
+            // divergence would multiply the per-vertex light loop).
+            const bool fused = pr.nl <= 4;
+            if (fused) {
+                if (!dcount.ensure(sizeof(unsigned long long), &rc)) break;
+                launch_zero(dcount.p, 16, s);
                 auto a = std::chrono::steady_clock::now();
-                if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
-                launch_quad_scatter(ps.dp, g, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
-                                    (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, grid.p, fp32, s);
-                if (!hip_ok(hipStreamSynchronize(s), "quad scatter", &rc)) break;
+                const int64_t kLaunch = (int64_t)1 << 24;   // paths per launch
+                for (int64_t p0 = 0; p0 < max_itr; p0 += kLaunch) {
+                    const int64_t np = std::min<int64_t>(max_itr - p0, kLaunch);
+                    launch_quad_trace_scatter(ps.dp, g, (uint64_t)p0, np, grid.p, fp32,
+                                              (unsigned long long *)dcount.p, s);
+                    st.paths += np;
+                    st.chunks += 1;
+                }
+                unsigned long long hv = 0;
+                if (!hip_ok(hipMemcpyAsync(&hv, dcount.p, sizeof hv, hipMemcpyDeviceToHost, s), "vertex count", &rc) ||
+                    !hip_ok(hipStreamSynchronize(s), "quad scatter", &rc))
+                    break;
                 st.ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-                nstore += tc.nvert;
-                st.paths += np;
-                st.vertices += tc.nvert;
-                st.chunks += 1;
-                mean_len = (double)st.vertices / (double)st.paths;
-                p0 += np;
+                st.vertices = (int64_t)hv;
+            } else {
+                TraceChunk tc;
+                double mean_len = 20;
+                for (int64_t p0 = 0; p0 < max_itr;) {
+                    // ~3.2e7 vertices (2 GB of records) per chunk: a trace pass
+                    // takes about the longest walk's step count times the step
+                    // latency whatever the chunk size, so few large chunks
+                    // amortise it
+                    int64_t np = std::min<int64_t>(max_itr - p0, std::max<int64_t>(256, (int64_t)(3.2e7 / mean_len)));
+                    auto a = std::chrono::steady_clock::now();
+                    if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
+                    launch_quad_scatter(ps.dp, g, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
+                                        (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, grid.p, fp32, s);
+                    if (!hip_ok(hipStreamSynchronize(s), "quad scatter", &rc)) break;
+                    st.ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+                    st.paths += np;
+                    st.vertices += tc.nvert;
+                    st.chunks += 1;
+                    mean_len = (double)st.vertices / (double)st.paths;
+                    p0 += np;
+                }
+                if (rc) break;
             }
+            const int64_t nstore = st.vertices;
             if (rc) break;
             // DFT matrix of fliplr(flipud(fftshift(fft2(ifftshift(.))))) and the crop:
             // W[o][ix] = exp(+2 pi i ((o + nva_pad - a)(ix + nx_pad - h) mod n)/n),

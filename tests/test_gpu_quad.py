@@ -31,6 +31,9 @@ CASES = [
     (True, 0.3, 0.05, 0.5, 2.0, 0.6, -1.0, 1, 2),
     (True, 0.25, 0.01, 0.5, 4.0, 0.5, 1.0, 2, 1),
     (False, 0.2, 0.004, 0.2, 5.0, 1.0, -1.0, 1, 2),
+    # six lights: stored vertex records and the vertex-parallel scatter
+    # (up to four, the walk and the scatter run in one pass)
+    (True, 0.3, 0.05, 0.5, 2.0, 0.6, -1.0, 6, 2),
 ]
 
 
