@@ -83,13 +83,14 @@ def lib():
         L.orc_quad_grid_setup.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(OGrid)]
         L.orc_mcfield_fourier_quad.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double,
                                                C.c_double, C.c_int64, C.c_double, C.c_int, C.c_int, _dp,
-                                               C.c_double, C.c_uint64, _dp, _dp]
+                                               C.c_double, C.POINTER(OPhase), C.c_uint64, _dp, _dp]
         L.orc_mcfield_fourier_quad.restype = C.c_int64
         L.orc_mcfield_wave.argtypes = [P, _dp, _dp, C.c_int64, _dp, _dp]
         L.orc_quad_grid_setup_wave.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(OGrid)]
         L.orc_mcfield_fourier_quad_wave.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, _dp, _dp,
                                                     C.c_double, C.c_double, C.c_int64, C.c_double, C.c_int,
-                                                    C.c_int, _dp, C.c_double, C.c_uint64, _dp, _dp]
+                                                    C.c_int, _dp, C.c_double, C.POINTER(OPhase), C.c_uint64, _dp,
+                                                    _dp]
         L.orc_mcfield_fourier_quad_wave.restype = C.c_int64
         L.orc_tabulated_scatter.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_inv_cdf.argtypes = [_dp, C.c_int, C.c_int, _dp]
@@ -249,10 +250,22 @@ def quad_grid(box_min, box_max, v_max, v_stp, lam, wave=False):
     return g
 
 
+def _quad_phase(amp, keep):
+    """The walk's phase function (a PhaseSpec of mcspeckle_amd.engines, or
+    None for the reference's isotropic walk) as a pointer for the quad calls."""
+    if amp is None:
+        return None
+    o = _phase(amp, keep)
+    keep.append(o)
+    return C.pointer(o)
+
+
 def mcfield_fourier_quad_wave(sigt, albedo, box_min, box_max, l, wl, v_max, v_stp, max_itr, lam, do_cbs, smp_flg,
-                              lmean0=None, vsign=-1.0, seed=0):
+                              lmean0=None, vsign=-1.0, seed=0, amp=None):
     """MCfieldFourierQuadOnWave: (nv0, nv0) field of the wave sum_l wl[l] * light l."""
     L = lib()
+    keep = []
+    ph = _quad_phase(amp, keep)
     g = quad_grid(box_min, box_max, v_max, v_stp, lam, wave=True)
     l = np.asarray(l, dtype=np.float64).reshape(3, -1)
     nl = l.shape[1]
@@ -266,14 +279,16 @@ def mcfield_fourier_quad_wave(sigt, albedo, box_min, box_max, l, wl, v_max, v_st
     re, im = np.empty(n), np.empty(n)
     nstore = L.orc_mcfield_fourier_quad_wave(sigt, albedo, _p(bmin), _p(bmax), _p(lc), nl, _p(wr), _p(wi), v_max,
                                              v_stp, int(max_itr), lam, int(do_cbs), int(smp_flg),
-                                             _p(lm) if lm is not None else None, float(vsign), int(seed), _p(re),
-                                             _p(im))
+                                             _p(lm) if lm is not None else None, float(vsign), ph, int(seed),
+                                             _p(re), _p(im))
     return (re + 1j * im).reshape((g.nv0, g.nv0), order="F"), nstore
 
 
 def mcfield_fourier_quad(sigt, albedo, box_min, box_max, l, v_max, v_stp, max_itr, lam, do_cbs, smp_flg,
-                         lmean0=None, vsign=-1.0, seed=0):
+                         lmean0=None, vsign=-1.0, seed=0, amp=None):
     L = lib()
+    keep = []
+    ph = _quad_phase(amp, keep)
     g = quad_grid(box_min, box_max, v_max, v_stp, lam)
     l = np.asarray(l, dtype=np.float64).reshape(3, -1)
     nl = l.shape[1]
@@ -285,7 +300,7 @@ def mcfield_fourier_quad(sigt, albedo, box_min, box_max, l, v_max, v_stp, max_it
     re, im = np.empty(n), np.empty(n)
     nstore = L.orc_mcfield_fourier_quad(sigt, albedo, _p(bmin), _p(bmax), _p(lc), nl, v_max, v_stp, int(max_itr),
                                         lam, int(do_cbs), int(smp_flg), _p(lm) if lm is not None else None,
-                                        float(vsign), int(seed), _p(re), _p(im))
+                                        float(vsign), ph, int(seed), _p(re), _p(im))
     return (re + 1j * im).reshape((g.nv0, g.nv0, nl), order="F"), nstore
 
 

@@ -1054,7 +1054,7 @@ static void dft1(const cplx *in, size_t stride, int n, cplx *out, const cplx *tw
 static int64_t fourier_quad(double sigt, double albedo, const double *box_min_in, const double *box_max_in,
                             const double *l, int nl, const cplx *wl, double v_max, double v_stp, int64_t max_itr,
                             double lambda, int do_cbs, int smp_flg, const double *lmean0, double vsign,
-                            uint64_t seed, double *u_re, double *u_im)
+                            const orc_phase *amp, uint64_t seed, double *u_re, double *u_im)
 {
     orc_quad_grid G;
     quad_grid_setup(box_min_in, box_max_in, v_max, v_stp, lambda, wl != NULL, &G);
@@ -1082,7 +1082,13 @@ static int64_t fourier_quad(double sigt, double albedo, const double *box_min_in
     for (int j = 0; j < 3; ++j) { pr.box_min[j] = G.box_min[j]; pr.box_max[j] = G.box_max[j]; }
     pr.nl = nl; pr.l = l; pr.nv = 0; pr.v = NULL;
     pr.is_ff_l = 1; pr.is_ff_v = 1; pr.do_cbs = do_cbs; pr.smp_flg = smp_flg;
-    pr.amp.type = 1;
+    /* the walk's phase function: isotropic in the reference; amp selects
+     * HG or a tabulated table instead (directions only: the scatter itself
+     * carries no amplitude) */
+    if (amp)
+        pr.amp = *amp;
+    else
+        pr.amp.type = 1;
     pr.has_lmean0 = 1;
     double lm[3] = {0, 0, 1};
     if (lmean0) for (int j = 0; j < 3; ++j) lm[j] = lmean0[j];
@@ -1223,22 +1229,22 @@ static int64_t fourier_quad(double sigt, double albedo, const double *box_min_in
 int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_min, const double *box_max,
                                  const double *l, int nl, double v_max, double v_stp, int64_t max_itr,
                                  double lambda, int do_cbs, int smp_flg, const double *lmean0,
-                                 double vsign, uint64_t seed, double *u_re, double *u_im)
+                                 double vsign, const orc_phase *amp, uint64_t seed, double *u_re, double *u_im)
 {
     return fourier_quad(sigt, albedo, box_min, box_max, l, nl, NULL, v_max, v_stp, max_itr, lambda, do_cbs,
-                        smp_flg, lmean0, vsign, seed, u_re, u_im);
+                        smp_flg, lmean0, vsign, amp, seed, u_re, u_im);
 }
 
 int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *box_min, const double *box_max,
                                       const double *l, int nl, const double *wl_re, const double *wl_im,
                                       double v_max, double v_stp, int64_t max_itr, double lambda, int do_cbs,
-                                      int smp_flg, const double *lmean0, double vsign, uint64_t seed,
-                                      double *u_re, double *u_im)
+                                      int smp_flg, const double *lmean0, double vsign, const orc_phase *amp,
+                                      uint64_t seed, double *u_re, double *u_im)
 {
     cplx *wl = malloc(sizeof(cplx) * (size_t)nl);
     for (int j = 0; j < nl; ++j) wl[j] = CMPLX(wl_re[j], wl_im ? wl_im[j] : 0.0);
     int64_t r = fourier_quad(sigt, albedo, box_min, box_max, l, nl, wl, v_max, v_stp, max_itr, lambda, do_cbs,
-                             smp_flg, lmean0, vsign, seed, u_re, u_im);
+                             smp_flg, lmean0, vsign, amp, seed, u_re, u_im);
     free(wl);
     return r;
 }

@@ -173,21 +173,23 @@ int orc_quad_grid_setup(const double *box_min, const double *box_max, double v_m
  * the view grid, so the transform size may be even). */
 int orc_quad_grid_setup_wave(const double *box_min, const double *box_max, double v_max, double v_stp,
                              double lambda, orc_quad_grid *g);
-/* code/MCfieldFourierQuad.m:1-303 (isotropic only, 3-D).  l is 3 x nl,
- * lmean0 may be NULL (defaults to [0;0;1]).  Out: u nv0 x nv0 x nl complex,
+/* code/MCfieldFourierQuad.m:1-303 (3-D).  l is 3 x nl, lmean0 may be NULL
+ * (defaults to [0;0;1]).  amp: the walk's phase function, NULL for the
+ * reference's isotropic walk (HG or tabulated: directions are sampled as in
+ * MCcov, the scatter is unchanged).  Out: u nv0 x nv0 x nl complex,
  * column-major; returns the number of stored vertices (the normaliser). */
 int64_t orc_mcfield_fourier_quad(double sigt, double albedo, const double *box_min, const double *box_max,
                                  const double *l, int nl, double v_max, double v_stp, int64_t max_itr,
                                  double lambda, int do_cbs, int smp_flg, const double *lmean0,
-                                 double vsign, uint64_t seed, double *u_re, double *u_im);
+                                 double vsign, const orc_phase *amp, uint64_t seed, double *u_re, double *u_im);
 
 /* code/MCfieldFourierQuadOnWave.m:1-337 : the lights are combined per vertex
  * with the weights Wl (length nl, wl_im may be NULL).  Out: u nv0 x nv0. */
 int64_t orc_mcfield_fourier_quad_wave(double sigt, double albedo, const double *box_min, const double *box_max,
                                       const double *l, int nl, const double *wl_re, const double *wl_im,
                                       double v_max, double v_stp, int64_t max_itr, double lambda, int do_cbs,
-                                      int smp_flg, const double *lmean0, double vsign, uint64_t seed,
-                                      double *u_re, double *u_im);
+                                      int smp_flg, const double *lmean0, double vsign, const orc_phase *amp,
+                                      uint64_t seed, double *u_re, double *u_im);
 
 /* code/MCCovHetro.m:1-276 (heterogeneous media).  pr supplies everything but
  * sigt/albedo grids (pr->albedo is used when m->albedo is NULL; pr->sigt is

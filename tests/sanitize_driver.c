@@ -160,7 +160,14 @@ int main(void) {
             size_t nq = (size_t)nv0 * nv0 * nl, nqw = (size_t)nw0 * nw0;
             double *qre = zalloc(nq), *qim = zalloc(nq);
             int64_t nvq = orc_mcfield_fourier_quad(3.0, 0.95, bmin, bmax, lq, nl, 0.3, 0.05, 40, 0.5, 1, 2, NULL,
-                                                   -1.0, 4, qre, qim);
+                                                   -1.0, NULL, 4, qre, qim);
+            ok &= nvq > 0 && finite_all("fourier_quad", qre, nq);
+            orc_phase hg;
+            memset(&hg, 0, sizeof hg);
+            hg.type = 3;
+            hg.g = 0.8;
+            nvq = orc_mcfield_fourier_quad(3.0, 0.95, bmin, bmax, lq, nl, 0.3, 0.05, 40, 0.5, 1, 2, NULL, -1.0, &hg, 4,
+                                           qre, qim);
             ok &= nvq > 0 && finite_all("fourier_quad", qre, nq);
             free(qre);
             free(qim);
@@ -168,7 +175,7 @@ int main(void) {
             qim = zalloc(nqw);
             const double wre[2] = {1.0, 0.5};
             int64_t nvw = orc_mcfield_fourier_quad_wave(3.0, 0.95, bmin, bmax, lq, nl, wre, NULL, 0.3, 0.05, 40, 0.5,
-                                                        0, 1, NULL, 1.0, 4, qre, qim);
+                                                        0, 1, NULL, 1.0, NULL, 4, qre, qim);
             ok &= nvw > 0 && finite_all("fourier_quad_wave", qre, nqw);
             free(qre);
             free(qim);

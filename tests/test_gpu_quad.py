@@ -53,3 +53,44 @@ def test_gpu_quad_matches_oracle(oracle, cbs, vmax, vstp, lam, hw, T, vsign, nl,
     err = np.abs(u - u0).max() / np.abs(u0).max()
     say("  err", err, "grid", u.shape)
     assert err <= tol
+
+
+PHASE_CASES = [
+    # sct_type, nl (wave: None), cbs, angle bins: HG and tabulated walks, the fused
+    # pass (nl <= 4), stored records (six lights), the wave variant, and a table
+    # too large for LDS (read from HBM)
+    (3, 2, True, 0), (2, 2, True, 256), (2, 1, False, 256), (2, 6, True, 256), (3, None, True, 0),
+    (2, None, False, 256), (2, 2, True, 2001),
+]
+
+
+@pytest.mark.parametrize("sct_type,nl,cbs,nbins", PHASE_CASES)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-4)])
+def test_gpu_quad_walk_phase_functions(oracle, sct_type, nl, cbs, nbins, precision, tol):
+    """HG and tabulated walks (the Fourier benchmark walks with a tabulated
+    three-lobe phase function; the reference only walks isotropically)."""
+    from test_oracle_quad import tab_phase
+    amp = 0.85 if sct_type == 3 else tab_phase(nbins)
+    ph = E._phase_from_arg(sct_type, amp, "ampfunc")
+    bmin, bmax = [-2.0, -2.0, 0.0], [2.0, 2.0, 0.6]
+    n = 300
+    say("quad phase", sct_type, nl, cbs, precision)
+    st = {}
+    if nl is None:
+        l = lights(3)
+        wl = np.array([0.7 * np.exp(0.3j), 0.4 * np.exp(-1.1j), 0.9 * np.exp(2.0j)])
+        u0, K = oracle.mcfield_fourier_quad_wave(4.0, 0.97, bmin, bmax, l, wl, 0.3, 0.05, n, 0.5, cbs, 2, None, -1.0,
+                                                 9, amp=ph)
+        u = E.MCfieldFourierQuadOnWave(4.0, 0.97, bmin, bmax, l, 0.3, 0.05, n, 0.5, cbs, 2, None, wl, -1.0,
+                                       sct_type=sct_type, ampfunc=amp, seed=9, precision=precision, stats=st)
+    else:
+        l = lights(nl)
+        u0, K = oracle.mcfield_fourier_quad(4.0, 0.97, bmin, bmax, l, 0.3, 0.05, n, 0.5, cbs, 2, None, -1.0, 9,
+                                            amp=ph)
+        u = E.MCfieldFourierQuad(4.0, 0.97, bmin, bmax, l, 0.3, 0.05, n, 0.5, cbs, 2, None, -1.0, sct_type=sct_type,
+                                 ampfunc=amp, seed=9, precision=precision, stats=st)
+    assert st["vertices"] == K
+    assert u.shape == u0.shape
+    err = np.abs(u - u0).max() / np.abs(u0).max()
+    say("  err", err, "grid", u.shape)
+    assert err <= tol

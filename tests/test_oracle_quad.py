@@ -13,13 +13,15 @@ import pytest
 from mcspeckle_amd import engines as E
 
 
-def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, cbs, smp, vsign, seed, wl=None):
-    """MCfieldFourierQuad.m, or with wl MCfieldFourierQuadOnWave.m, in numpy."""
+def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, cbs, smp, vsign, seed, wl=None,
+               sct_type=1, ampfunc=0):
+    """MCfieldFourierQuad.m, or with wl MCfieldFourierQuadOnWave.m, in numpy.
+    sct_type/ampfunc: the walk's phase function (isotropic in the reference)."""
     wave = wl is not None
     g = oracle.quad_grid(np.array(bmin, float), np.array(bmax, float), v_max, v_stp, lam, wave=wave)
     nv0, nx = g.nv0, g.nx
     Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
-    spec = E.make_problem(sigt, albedo, Bmin, Bmax, l, np.zeros((3, 0)), 1, 1, lam, cbs, smp, 1, 0,
+    spec = E.make_problem(sigt, albedo, Bmin, Bmax, l, np.zeros((3, 0)), 1, 1, lam, cbs, smp, sct_type, ampfunc,
                           lmean0=[0, 0, 1], seed=seed, need_views=False)
     ps = oracle.trace(spec, n_paths)
     off = ps["offset"]
@@ -97,6 +99,41 @@ def test_oracle_quad_matches_numpy_fft_restatement(oracle, cbs, vmax, vsign):
                                            150, 0.5, cbs, 2, None, vsign, 4)
     assert K == K2
     np.testing.assert_allclose(u_or, u_np, rtol=0, atol=1e-10 * np.abs(u_np).max())
+
+
+def tab_phase(n=256, seed=3):
+    """A tabulated phase function like the Fourier benchmark's: a seeded
+    mixture of three HG lobes on n angle bins."""
+    from mcspeckle_amd.interface import tabulatedAmplitudeScatter
+    rng = np.random.default_rng(seed)
+    g, w = rng.uniform(-0.3, 0.95, 3), rng.dirichlet(np.ones(3))
+    th = np.linspace(0, np.pi, n)
+    hg = sum(wi * (1 - gi ** 2) / (1 + gi ** 2 - 2 * gi * np.cos(th)) ** 1.5 for gi, wi in zip(g, w))
+    return tabulatedAmplitudeScatter(th, np.sqrt(hg))["ampfunc"]
+
+
+@pytest.mark.parametrize("sct_type,wave", [(3, False), (2, False), (3, True), (2, True)])
+def test_oracle_quad_walk_phase_functions(oracle, sct_type, wave):
+    """HG and tabulated walks (an extension: the reference walks isotropically):
+    the same numpy restatement over the oracle's walks with that phase function."""
+    amp = 0.8 if sct_type == 3 else tab_phase()
+    ph = E._phase_from_arg(sct_type, amp, "ampfunc")
+    args = dict(sigt=3.0, albedo=0.95, bmin=[-2.0, -2.0, 0.0], bmax=[2.0, 2.0, 0.6], l=L2 if wave else L2[:, :2],
+                v_max=0.3, v_stp=0.05, n_paths=120, lam=0.5, cbs=True, smp=2, vsign=-1.0, seed=8)
+    u_np, K = quad_numpy(oracle, wl=WL if wave else None, sct_type=sct_type, ampfunc=amp, **args)
+    a = [args[k] for k in ("sigt", "albedo", "bmin", "bmax", "l")]
+    if wave:
+        u_or, K2 = oracle.mcfield_fourier_quad_wave(*a, WL, 0.3, 0.05, 120, 0.5, True, 2, None, -1.0, 8, amp=ph)
+    else:
+        u_or, K2 = oracle.mcfield_fourier_quad(*a, 0.3, 0.05, 120, 0.5, True, 2, None, -1.0, 8, amp=ph)
+    assert K == K2
+    np.testing.assert_allclose(u_or, u_np, rtol=0, atol=1e-10 * np.abs(u_np).max())
+    # the phase function reaches the walk: the isotropic walk stores another vertex set
+    if wave:
+        _, K0 = oracle.mcfield_fourier_quad_wave(*a, WL, 0.3, 0.05, 120, 0.5, True, 2, None, -1.0, 8)
+    else:
+        _, K0 = oracle.mcfield_fourier_quad(*a, 0.3, 0.05, 120, 0.5, True, 2, None, -1.0, 8)
+    assert K0 != K
 
 
 @pytest.mark.parametrize("bw,vmax,vstp,lam", [(4.0, 0.3, 0.05, 0.5), (10.0, 0.25, 0.5 / 510, 0.0098),
