@@ -338,7 +338,7 @@ int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevB
     out.n_amp = 0;
     out.amp = nullptr;
     out.n_icdf = 0;
-    out.icdf = nullptr;
+    out.cos_icdf = nullptr;
     if (type != SCMC_TABULATED) return SCMC_OK;
     std::vector<double> a((size_t)ph.n_amp * 2);
     for (int i = 0; i < ph.n_amp; ++i) {
@@ -351,13 +351,15 @@ int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevB
     if (!hip_ok(hipMemcpyAsync(amp.p, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy evalAmp",
                 &rc) ||
         !hip_ok(hipMemcpyAsync(icdf.p, ph.icdf, sizeof(double) * ph.n_icdf, hipMemcpyHostToDevice, s),
-                "copy sampleIcdf", &rc) ||
-        !hip_ok(hipStreamSynchronize(s), "table sync", &rc))
+                "copy sampleIcdf", &rc))
+        return rc;
+    launch_cos_table((double *)icdf.p, ph.n_icdf, s);
+    if (!hip_ok(hipGetLastError(), "cos table", &rc) || !hip_ok(hipStreamSynchronize(s), "table sync", &rc))
         return rc;
     out.n_amp = ph.n_amp;
     out.amp = (const double2 *)amp.p;
     out.n_icdf = ph.n_icdf;
-    out.icdf = (const double *)icdf.p;
+    out.cos_icdf = (const double *)icdf.p;
     return SCMC_OK;
 }
 

@@ -21,7 +21,7 @@ struct DevPhase {
     int n_amp;
     const double2 *amp;  // evalAmp (re, im) on device
     int n_icdf;
-    const double *icdf;  // sampleIcdf on device
+    const double *cos_icdf;  // cos(sampleIcdf), formed on device once per problem
 };
 
 // Problem constants passed by value to every kernel.
@@ -246,10 +246,24 @@ __device__ __forceinline__ void iso_dir(double d0, double d1, double d2, double 
     }
 }
 
-// smpampfunc_general.m:5-20, smpampfunc.m:3-7
+// The walking kernels stage the tabulated direction table (the cosines of
+// sampleIcdf) in LDS when it has at most kIcdfLds entries.  Every thread of
+// the workgroup calls stage_icdf (a tabulated problem ends it with a
+// barrier); it returns the LDS table, or null to read the table from HBM.
+constexpr int kIcdfLds = 1024;
+__device__ __forceinline__ const double *stage_icdf(const DevPhase &ph, double *lcos)
+{
+    if (ph.type != 2 || ph.n_icdf > kIcdfLds) return nullptr;
+    for (int j = threadIdx.x; j < ph.n_icdf; j += blockDim.x) lcos[j] = ph.cos_icdf[j];
+    __syncthreads();
+    return lcos;
+}
+
+// smpampfunc_general.m:5-20, smpampfunc.m:3-7.  lcos: the LDS copy of
+// ph.cos_icdf (stage_icdf), or null.
 template <int DIM>
 __device__ __forceinline__ void sample_dir(const double *ow, const DevPhase &ph, double d0, double d1, double d2,
-                                           double d3, double *w)
+                                           double d3, double *w, const double *lcos = nullptr)
 {
     if (ph.type == 1) {
         iso_dir<DIM>(d0, d1, d2, d3, w);
@@ -261,7 +275,7 @@ __device__ __forceinline__ void sample_dir(const double *ow, const DevPhase &ph,
         double stp = 1.0 / (n - 1);
         long i = lround(d0 / stp);
         i = i > n - 1 ? n - 1 : i;
-        cosa = cos(ph.icdf[i]);
+        cosa = lcos ? lcos[i] : ph.cos_icdf[i];
     } else {
         cosa = sample_hg<DIM>(ph.g, d0);
     }
@@ -320,10 +334,10 @@ __device__ __forceinline__ void path_start(const DevProblem &P, uint64_t path, d
 }
 
 // One transition after the vertex n (MCcov.m:266-288).  Returns false when
-// the path ends.  x, w, weight, ow are updated in place.
+// the path ends.  x, w, weight, ow are updated in place; lcos as in sample_dir.
 template <int DIM>
 __device__ __forceinline__ bool path_step(const DevProblem &P, uint64_t path, uint32_t n, double *x, double *w,
-                                          double *ow, double &weight, double flight_u)
+                                          double *ow, double &weight, double flight_u, const double *lcos = nullptr)
 {
     double d = -log(-flight_u + 1) / P.sigt;
 #pragma unroll
@@ -339,7 +353,7 @@ __device__ __forceinline__ bool path_step(const DevProblem &P, uint64_t path, ui
     double2 e = uniform_pair(P.seed, path, n, 3);
 #pragma unroll
     for (int j = 0; j < DIM; ++j) ow[j] = w[j];
-    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w);
+    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w, lcos);
     return true;
 }
 

@@ -310,7 +310,7 @@ __device__ __forceinline__ bool hetero_path_start(const DevProblem &P, const Het
 template <int DIM>
 __device__ __forceinline__ bool hetero_path_step(const DevProblem &P, const HetMedium &M, uint64_t path, uint32_t n,
                                                  double *x, double *w, double *ow, double &weight, double ud0,
-                                                 double ua0)
+                                                 double ua0, const double *lcos = nullptr)
 {
     const double d = woodcock<DIM>(P, M, x, w, path, n, ud0, ua0);
 #pragma unroll
@@ -326,7 +326,7 @@ __device__ __forceinline__ bool hetero_path_step(const DevProblem &P, const HetM
     const double2 e = uniform_pair(P.seed, path, n, 3);
 #pragma unroll
     for (int j = 0; j < DIM; ++j) ow[j] = w[j];
-    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w);
+    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w, lcos);
     return true;
 }
 

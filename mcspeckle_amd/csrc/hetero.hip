@@ -17,6 +17,8 @@ template <int DIM>
 __global__ __launch_bounds__(256) void hetero_count_kernel(DevProblem P, HetMedium M, uint64_t p0, int64_t np,
                                                            int32_t *counts)
 {
+    __shared__ double lcos[kIcdfLds];
+    const double *tab = stage_icdf(P.amp, lcos);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint64_t path = P.path_offset + p0 + (uint64_t)i;
@@ -32,7 +34,7 @@ __global__ __launch_bounds__(256) void hetero_count_kernel(DevProblem P, HetMedi
         ++n;
         const double2 a = uniform_pair(P.seed, path, n, 0);
         const double2 e = uniform_pair(P.seed, path, n, 3);
-        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y)) break;
+        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) break;
     }
     counts[i] = (int32_t)n;
 }
@@ -42,6 +44,8 @@ __global__ __launch_bounds__(256) void hetero_emit_kernel(DevProblem P, HetMediu
                                                           const int64_t *offsets, PathRec *paths, VertexRec *verts,
                                                           int32_t *vpath)
 {
+    __shared__ double lcos[kIcdfLds];
+    const double *tab = stage_icdf(P.amp, lcos);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint64_t path = P.path_offset + p0 + (uint64_t)i;
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(256) void hetero_emit_kernel(DevProblem P, HetMediu
             vpath[o] = (int32_t)i;
             ++o;
         }
-        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y)) break;
+        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) break;
     }
 }
 

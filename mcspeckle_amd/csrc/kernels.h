@@ -122,5 +122,7 @@ size_t field_work_elems(int64_t np, int64_t nvert, int nv, int nl, bool wave);
 
 // Device-wide helpers.
 void launch_zero(void *p, size_t bytes, hipStream_t s);
+// In place: t[i] = cos(t[i]) (the tabulated sampleIcdf angles -> cosines).
+void launch_cos_table(double *t, int n, hipStream_t s);
 
 } // namespace scmc

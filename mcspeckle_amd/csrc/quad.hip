@@ -188,6 +188,8 @@ __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, Q
                                                                  unsigned long long *nvert)
 {
     __shared__ unsigned long long ws[4];
+    __shared__ double lcos[kIcdfLds];
+    const double *tab = stage_icdf(P.amp, lcos);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cnt = 0;
     if (i < np) {
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, Q
             if (P.do_cbs && n > 1) amp *= 1 / sqrt(2.0);
             quad_scatter_vertex<T>(P, g, x, x1, amp, a.x, grid);
             ++cnt;
-            if (!path_step<3>(P, path, n, x, w, ow, weight, a.y)) break;
+            if (!path_step<3>(P, path, n, x, w, ow, weight, a.y, tab)) break;
         }
     }
     // workgroup sum of the vertex counts

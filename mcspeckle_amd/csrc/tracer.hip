@@ -15,6 +15,8 @@ namespace scmc {
 template <int DIM>
 __global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t p0, int64_t np, int32_t *counts)
 {
+    __shared__ double lcos[kIcdfLds];
+    const double *tab = stage_icdf(P.amp, lcos);
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     uint64_t path = P.path_offset + p0 + (uint64_t)i;
@@ -26,7 +28,7 @@ __global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t
     for (;;) {
         ++n;
         double2 a = uniform_pair(P.seed, path, n, 0);
-        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y)) break;
+        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) break;
     }
     counts[i] = (int32_t)n;
 }
@@ -35,6 +37,8 @@ template <int DIM>
 __global__ __launch_bounds__(256) void trace_emit_kernel(DevProblem P, uint64_t p0, int64_t np, const int64_t *offsets,
                                                          PathRec *paths, VertexRec *verts, int32_t *vpath)
 {
+    __shared__ double lcos[kIcdfLds];
+    const double *tab = stage_icdf(P.amp, lcos);
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     uint64_t path = P.path_offset + p0 + (uint64_t)i;
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(256) void trace_emit_kernel(DevProblem P, uint64_t 
             vpath[o] = (int32_t)i;
             ++o;
         }
-        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y)) break;
+        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) break;
     }
 }
 
@@ -182,6 +186,20 @@ void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64
         hipLaunchKernelGGL(trace_emit_kernel<3>, g, dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath);
     else
         hipLaunchKernelGGL(trace_emit_kernel<2>, g, dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath);
+}
+
+// The tabulated direction table: each sampleIcdf angle replaced by its
+// cosine, with the same cos the walk used per draw, so a walk is unchanged.
+__global__ void cos_table_kernel(double *t, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] = cos(t[i]);
+}
+
+void launch_cos_table(double *t, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(cos_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, n);
 }
 
 __global__ void zero_kernel(uint4 *p, size_t n16)
