@@ -106,12 +106,12 @@ def serving_stream(n_req, rate, path_scale, seed=5):
     return out
 
 
-def serve(n_req, rate, path_scale, max_active):
+def serve(n_req, rate, path_scale, max_active, shared=False):
     import threading
     from mcspeckle_amd import engines as E
     stream = serving_stream(n_req, rate, path_scale)
     lat, work = [], 0.0
-    S = E.Server(max_active=max_active)
+    S = E.Server(max_active=max_active, shared_launches=shared)
     ids = []
     t0 = time.perf_counter()
 
@@ -140,7 +140,7 @@ def serve(n_req, rate, path_scale, max_active):
     return {"requests": n_req, "arrival_rate_per_s": rate, "path_scale": path_scale, "max_active": max_active,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "throughput": work / t_end, "unit": "paths*entries/s",
-            "makespan_s": t_end, "offered_span_s": stream[-1][0]}
+            "makespan_s": t_end, "offered_span_s": stream[-1][0], "shared_launches": bool(shared)}
 
 
 def field_leg(n_paths, precision="mixed"):
@@ -427,6 +427,9 @@ def main():
     # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
     if not args.no_serve:
         out["serving"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale, args.serve_active)
+        # the same stream with the slots' small updates packed into shared launches
+        out["serving_shared"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                      args.serve_active, shared=True)
 
     # ---- CPU baseline: FP64 restatement with the reference's loop structure
     if not args.no_cpu:
