@@ -281,8 +281,9 @@ __device__ __forceinline__ float2 e_far_mixed(const double *x, const double *v, 
     const double cyc = dot<DIM>(x, v) * nilam;
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
-    float best = INFINITY;
-    for (int j = 0; j < DIM; ++j) best = fminf(best, (up[j] ? dhi[j] : dlo[j]) * inv[j]);
+    // the first face starts the minimum (a start at +inf costs a min per call)
+    float best = (up[0] ? dhi[0] : dlo[0]) * inv[0];
+    for (int j = 1; j < DIM; ++j) best = fminf(best, (up[j] ? dhi[j] : dlo[j]) * inv[j]);
     const float mag = __builtin_amdgcn_exp2f(fmaf(-hs * 1.4426950408889634f, best, lamp));
     return make_float2(mag * cs, mag * sn);
 }
@@ -435,9 +436,10 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                         }
 #pragma unroll
                         for (int q = 0; q < NLQ; ++q) {
+                            // the complex product added as four FMAs into the sum
                             const float2 c = scf[ei][q];
-                            accf[t][q].x += evf.x * c.x - evf.y * c.y;
-                            accf[t][q].y += evf.x * c.y + evf.y * c.x;
+                            accf[t][q].x = fmaf(-evf.y, c.y, fmaf(evf.x, c.x, accf[t][q].x));
+                            accf[t][q].y = fmaf(evf.y, c.x, fmaf(evf.x, c.y, accf[t][q].y));
                         }
                         continue;
                     }
@@ -458,9 +460,10 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     }
                 } else {
                     for (int q = 0; q < NLQ; ++q) {
+                        // the complex product added as four FMAs into the sum
                         const double2 c = sce[q];
-                        acc[t][q].x += ev.x * c.x - ev.y * c.y;
-                        acc[t][q].y += ev.x * c.y + ev.y * c.x;
+                        acc[t][q].x = fma(-ev.y, c.y, fma(ev.x, c.x, acc[t][q].x));
+                        acc[t][q].y = fma(ev.y, c.x, fma(ev.x, c.y, acc[t][q].y));
                     }
                 }
             }
