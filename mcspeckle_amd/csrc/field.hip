@@ -306,8 +306,10 @@ __device__ __forceinline__ float2 e_far_mixed_het(const DevProblem &P, const Het
 // the first-vertex entries are left to field_wave_first_kernel.  NLQ:
 // light columns accumulated per pass (1 for one column, else kFieldNL).
 // VPT views per lane (v, v + 256, ...): the entry reads from LDS and the
-// per-entry scalar work are shared between them.
-template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false, int VPT = 1>
+// per-entry scalar work are shared between them.  AMPT: 0 reads the
+// amplitude type at run time; 3 is an instance for HG amplitudes only (the
+// uniform type tests then fold away).
+template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false, int VPT = 1, int AMPT = 0>
 __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
@@ -361,7 +363,8 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     // t = 1 + g^2 - 2 g c, mu = -3/4 (3-D) or -1/2 (2-D).  Tabulated
     // amplitudes are complex (amp_tab).  Types other than 1 and 2 are HG,
     // as in eval_amp.
-    const bool tab = P.amp.type == 2, hg = P.amp.type != 1 && !tab;
+    const bool tab = AMPT == 0 ? P.amp.type == 2 : false;
+    const bool hg = AMPT == 0 ? P.amp.type != 1 && !tab : AMPT == 3;
     // uniform: kept in a scalar register
     const float lk = __builtin_bit_cast(
         float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
@@ -903,7 +906,17 @@ static void field_dispatch(const DevProblem &P, const HetMedium *Mp, int64_t np,
                 SCMC_FIELD_NEAR((field_near_kernel<DIM, false>));
         } else if (mixed) {
             dim3 g2((unsigned)((P.nv + 2 * kFieldViews - 1) / (2 * kFieldViews)), (unsigned)nsplit);
-            if (wave)
+            // types other than isotropic (1) and tabulated (2) are HG, as in eval_amp
+            const bool hg_only = P.amp.type != 1 && P.amp.type != 2;
+            if (wave && hg_only)
+                hipLaunchKernelGGL((field_kernel<DIM, true, true, 1, false, 2, 3>), g2, dim3(kFieldViews), 0, s, P, M,
+                                   np, nvert, nent, nsplit, l0, nlp, paths, verts, vpath, offsets, coef, coefR, aftab,
+                                   cl0, part);
+            else if (ncol == 1 && !wave && hg_only)
+                hipLaunchKernelGGL((field_kernel<DIM, true, false, 1, false, 2, 3>), g2, dim3(kFieldViews), 0, s, P, M,
+                                   np, nvert, nent, nsplit, l0, nlp, paths, verts, vpath, offsets, coef, coefR, aftab,
+                                   cl0, part);
+            else if (wave)
                 hipLaunchKernelGGL((field_kernel<DIM, true, true, 1, false, 2>), g2, dim3(kFieldViews), 0, s, P, M, np,
                                    nvert, nent, nsplit, l0, nlp, paths, verts, vpath, offsets, coef, coefR, aftab, cl0,
                                    part);

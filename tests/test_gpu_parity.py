@@ -388,3 +388,18 @@ def test_gpu_nonfinite_path_weights_are_flagged(oracle, precision):
     u = E.mcfield_replay(spec, bad, n, precision, stats=st)
     assert st["nonfinite"] == 1
     assert not np.isfinite(u).all()
+
+
+@pytest.mark.parametrize("kind,dim,cbs", [("hg", 3, True), ("hg", 2, False), ("tab", 3, True), ("iso", 3, False)])
+def test_gpu_mcfield_one_light(oracle, kind, dim, cbs):
+    """One light: the two-views-per-lane field instance (HG amplitudes have their
+    own instance), 700 views so a lane's second view block is ragged."""
+    spec = build(kind, dim, cbs, 2, None, nv=700, nl=1)
+    say("mcfield one light", kind, dim, cbs)
+    n = 400
+    ps = oracle.trace(spec, n)
+    u0 = oracle.mcfield_replay(spec, ps, n)
+    assert field_err(E.mcfield_replay(spec, ps, n, "mixed"), u0) <= 2e-5
+    assert field_err(E.mcfield_replay(spec, ps, n, "fp64"), u0) <= 1e-12
+    u1 = oracle.mcfield(spec, n)
+    assert field_err(E.mcfield_spec(spec, n, "mixed"), u1) <= 2e-5
