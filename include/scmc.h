@@ -206,7 +206,10 @@ int scmc_mcfield_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t m
  * relative to pr->path_offset) and returns when they are accumulated;
  * scmc_session_result normalises by max_itr without consuming the sums, so a
  * run can be checkpointed, resumed or sharded (the parfor repeats of
- * scmc.m:343-367 are disjoint path ranges of one session). */
+ * scmc.m:343-367 are disjoint path ranges of one session).  Calls on one
+ * session from several threads run one after another; destroying a session
+ * while another thread's call on it is in progress releases the handle at
+ * once and the session itself when that call returns. */
 int scmc_session_create(const scmc_problem *pr, int32_t kind, int32_t precision, int64_t *session);
 int scmc_session_run(int64_t session, int64_t first, int64_t n_paths);
 int scmc_session_result(int64_t session, int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im);

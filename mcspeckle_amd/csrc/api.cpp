@@ -853,7 +853,7 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
         if (!acc.p) {
             launch_zero(out.p, sizeof(double) * 2 * ne, s);
         } else {
-            launch_cov_finalize((const double2 *)acc.p, ks, nv, nl, tile, sc, rowf, o, d.im ? o + ne : nullptr, s);
+            launch_cov_finalize((const double2 *)acc.p, ks, nv, nl, sc, rowf, o, d.im ? o + ne : nullptr, s);
         }
         int r2 = SCMC_OK;
         if (d.im) {
@@ -1049,15 +1049,23 @@ void Session::fill_stats(scmc_stats *out) const
     }
 }
 
+// Session handles.  A call holds a reference to its session and its call
+// lock for the whole call: calls on one session from several threads run one
+// after another, and scmc_session_destroy during another thread's call only
+// drops the handle -- the session is freed when that call returns.
+struct SessionHandle {
+    std::mutex call_mu;
+    std::unique_ptr<Session> s;
+};
 static std::mutex g_sess_mu;
-static std::map<int64_t, std::unique_ptr<Session>> g_sessions;
+static std::map<int64_t, std::shared_ptr<SessionHandle>> g_sessions;
 static int64_t g_next_session = 1;
 
-static Session *find_session(int64_t h)
+static std::shared_ptr<SessionHandle> find_session(int64_t h)
 {
     std::lock_guard<std::mutex> g(g_sess_mu);
     auto it = g_sessions.find(h);
-    return it == g_sessions.end() ? nullptr : it->second.get();
+    return it == g_sessions.end() ? nullptr : it->second;
 }
 
 } // namespace scmc
@@ -1075,44 +1083,49 @@ int scmc_session_create(const scmc_problem *pr, int32_t kind, int32_t precision,
     static std::once_flag at_exit;
     std::call_once(at_exit, [] {
         std::atexit([] {
-            std::map<int64_t, std::unique_ptr<Session>> left;
+            std::map<int64_t, std::shared_ptr<SessionHandle>> left;
             {
                 std::lock_guard<std::mutex> g(g_sess_mu);
                 left.swap(g_sessions);
             }
         });
     });
+    std::shared_ptr<SessionHandle> h(new SessionHandle());
+    h->s = std::move(s);
     std::lock_guard<std::mutex> g(g_sess_mu);
     *session = g_next_session++;
-    g_sessions[*session] = std::move(s);
+    g_sessions[*session] = std::move(h);
     return SCMC_OK;
 }
 
 int scmc_session_run(int64_t session, int64_t first, int64_t n_paths)
 {
-    Session *s = find_session(session);
-    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
-    return s->run(first, n_paths);
+    auto h = find_session(session);
+    if (!h) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    std::lock_guard<std::mutex> g(h->call_mu);
+    return h->s->run(first, n_paths);
 }
 
 int scmc_session_result(int64_t session, int64_t max_itr, double *a_re, double *a_im, double *b_re, double *b_im)
 {
-    Session *s = find_session(session);
-    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
-    return s->result(max_itr, a_re, a_im, b_re, b_im);
+    auto h = find_session(session);
+    if (!h) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    std::lock_guard<std::mutex> g(h->call_mu);
+    return h->s->result(max_itr, a_re, a_im, b_re, b_im);
 }
 
 int scmc_session_stats(int64_t session, scmc_stats *stats)
 {
-    Session *s = find_session(session);
-    if (!s) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
-    if (stats) s->fill_stats(stats);
+    auto h = find_session(session);
+    if (!h) return set_error(SCMC_E_NOTFOUND, "unknown session %lld", (long long)session);
+    std::lock_guard<std::mutex> g(h->call_mu);
+    if (stats) h->s->fill_stats(stats);
     return SCMC_OK;
 }
 
 int scmc_session_destroy(int64_t session)
 {
-    std::unique_ptr<Session> victim;
+    std::shared_ptr<SessionHandle> victim;
     {
         std::lock_guard<std::mutex> g(g_sess_mu);
         auto it = g_sessions.find(session);

@@ -648,7 +648,7 @@ void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
 }
 
 // ------------------------------------------------------------ finaliser
-__global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale,
+__global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int nl, double scale,
                                     const double2 *rowf, double *out_re, double *out_im)
 {
     const int n = nv * nl;
@@ -662,7 +662,11 @@ __global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int 
     const int l1 = (int)(t % nl); t /= nl;
     const int l2 = (int)t;
     const int i = v1 + nv * l1, j = v2 + nv * l2;
-    const bool lower = (i / tile) >= (j / tile);
+    // Every entry is read from the lower triangle of the matrix (a >= b lies
+    // in a computed tile, ti >= tj), the upper one as its conjugate, and the
+    // diagonal is real: the result is exactly Hermitian even though a
+    // diagonal tile's two triangles were summed independently.
+    const bool lower = i >= j;
     const int a = lower ? i : j, b = lower ? j : i;
     double re = 0.0, im = 0.0;
     for (int s = 0; s < ksplit; ++s) {
@@ -671,6 +675,7 @@ __global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int 
         im += c.y;
     }
     if (!lower) im = -im;
+    if (i == j) im = 0.0;
     if (rowf) {
         // * rowf[i] * conj(rowf[j])
         double2 fi = rowf[i], fj = rowf[j];
@@ -690,12 +695,12 @@ __global__ void cov_finalize_kernel(const double2 *acc, int ksplit, int nv, int 
     }
 }
 
-void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale, const double2 *rowf,
+void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, double scale, const double2 *rowf,
                          double *out_re, double *out_im, hipStream_t s)
 {
     const int64_t total = (int64_t)nv * nl * nv * nl;
     dim3 g((unsigned)((total + 255) / 256));
-    hipLaunchKernelGGL(cov_finalize_kernel, g, dim3(256), 0, s, acc, ksplit, nv, nl, tile, scale, rowf, out_re, out_im);
+    hipLaunchKernelGGL(cov_finalize_kernel, g, dim3(256), 0, s, acc, ksplit, nv, nl, scale, rowf, out_re, out_im);
 }
 
 } // namespace scmc

@@ -101,7 +101,7 @@ void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the
 // optional per-row factor rowf[(v,l)] (single scattering amplitude), and write
 // the MATLAB nv x nv x nl x nl column-major layout (split or interleaved).
-void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, int tile, double scale,
+void launch_cov_finalize(const double2 *acc, int ksplit, int nv, int nl, double scale,
                          const double2 *rowf, double *out_re, double *out_im, hipStream_t s);
 // Single-scatter amplitude table af_ang_vl(v,l) = a(l_l . v_v) (far field).
 void launch_af_table(const DevProblem &P, double2 *af, hipStream_t s);

@@ -78,8 +78,11 @@ struct Server {
     }
 };
 
+// The running server.  Callers of scmc_wait/scmc_poll hold a reference while
+// they use it, so scmc_server_stop can drop the global one while a waiter is
+// still inside: the Server is destroyed by whoever leaves it last.
 static std::mutex g_srv_mu;
-static std::unique_ptr<Server> g_srv;
+static std::shared_ptr<Server> g_srv;
 
 int64_t request_footprint(int nv, int nl, int precision, bool near)
 {
@@ -228,7 +231,7 @@ int scmc_server_start(const scmc_server_config *cfg)
     if (g_srv) return set_error(SCMC_E_BUSY, "server already running");
     int rc = ensure_device(nullptr);
     if (rc) return rc;
-    std::unique_ptr<Server> S(new Server());
+    std::shared_ptr<Server> S(new Server());
     if (cfg) S->cfg = *cfg;
     if (S->cfg.max_active <= 0) S->cfg.max_active = 4;
     size_t freeb = 0, total = 0;
@@ -251,7 +254,7 @@ int scmc_server_start(const scmc_server_config *cfg)
 
 int scmc_server_stop(void)
 {
-    std::unique_ptr<Server> S;
+    std::shared_ptr<Server> S;
     {
         std::lock_guard<std::mutex> g(g_srv_mu);
         if (!g_srv) return SCMC_OK;
@@ -261,10 +264,22 @@ int scmc_server_stop(void)
         std::lock_guard<std::mutex> lk(S->mu);
         S->stop = true;
         for (auto &kv : S->all) kv.second->cancel = true;
+        // requests never admitted finish now, as cancelled; running ones
+        // stop at their next slice boundary and finish in their slot
+        for (Request *r : S->pending) {
+            r->rc = SCMC_E_CANCELLED;
+            r->err = "server stopped before the request was admitted";
+            r->state = 2;
+            r->t_done = clk::now();
+        }
+        S->pending.clear();
     }
     S->cv_work.notify_all();
+    S->cv_done.notify_all();
     for (auto &t : S->slots) t.join();   // their sessions' buffers go to the reaper
     reaper_stop();
+    // every request is finished now: wake any waiter that is still asleep
+    S->cv_done.notify_all();
     return SCMC_OK;
 }
 
@@ -295,11 +310,11 @@ int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision
 
 static int finish(int64_t id, bool block, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)
 {
-    Server *S;
+    std::shared_ptr<Server> S;   // keeps the server alive across a concurrent stop
     {
         std::lock_guard<std::mutex> g(g_srv_mu);
         if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
-        S = g_srv.get();
+        S = g_srv;
     }
     std::unique_ptr<Request> done;
     {

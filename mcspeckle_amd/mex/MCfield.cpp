@@ -35,7 +35,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     }
     if (nrhs == 16 && !mxIsEmpty(prhs[15])) {
         p.has_lmean0 = 1;
-        for (int j = 0; j < p.dim; ++j) p.lmean0[j] = mxGetPr(prhs[15])[j];
+        read_vector(prhs[15], p.dim, p.lmean0, id, "lmean0");
     }
     p.seed = seed_from_matlab();
     mxArray *u = mxCreateDoubleMatrix((mwSize)p.nv, (mwSize)p.nl, mxCOMPLEX);

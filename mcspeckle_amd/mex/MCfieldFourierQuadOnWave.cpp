@@ -29,7 +29,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     p.amp.type = SCMC_ISOTROPIC;
     if (!mxIsEmpty(prhs[11])) {
         p.has_lmean0 = 1;
-        for (int j = 0; j < 3; ++j) p.lmean0[j] = mxGetPr(prhs[11])[j];
+        read_vector(prhs[11], 3, p.lmean0, id, "lmean0");
     }
     std::vector<double> wr, wi;
     read_weights(prhs[12], p.nl, wr, wi, id);

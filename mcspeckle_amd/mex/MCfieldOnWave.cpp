@@ -37,7 +37,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     }
     if (!mxIsEmpty(prhs[15])) {
         p.has_lmean0 = 1;
-        for (int j = 0; j < p.dim; ++j) p.lmean0[j] = mxGetPr(prhs[15])[j];
+        read_vector(prhs[15], p.dim, p.lmean0, id, "lmean0");
     }
     std::vector<double> wr, wi;
     read_weights(prhs[16], p.nl, wr, wi, id);

@@ -39,9 +39,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     }
     // MCsampleHetro.m:35-40
     if (nrhs >= 17 && !mxIsEmpty(prhs[16])) {
-        if ((int)mxGetNumberOfElements(prhs[16]) != p.dim) mexErrMsgIdAndTxt(id, "Invalid lmean0 size");
         p.has_lmean0 = 1;
-        for (int j = 0; j < p.dim; ++j) p.lmean0[j] = mxGetPr(prhs[16])[j];
+        read_vector(prhs[16], p.dim, p.lmean0, id, "lmean0");
     }
     p.seed = seed_from_matlab();
     mxArray *u = mxCreateDoubleMatrix((mwSize)p.nv, (mwSize)p.nl, mxCOMPLEX);

@@ -28,6 +28,16 @@ inline double scalar(const mxArray *a, const char *id, const char *what)
     return mxGetScalar(a);
 }
 
+// A real double array with exactly n elements (lmean0: MCfield.m:77-82 reads
+// it as a dim-vector), copied into out; anything else raises.
+inline void read_vector(const mxArray *a, int n, double *out, const char *id, const char *what)
+{
+    if (!mxIsDouble(a) || mxIsComplex(a) || (int)mxGetNumberOfElements(a) != n)
+        mexErrMsgIdAndTxt(id, "%s must be a real double vector of %d elements", what, n);
+    const double *p = mxGetPr(a);
+    for (int j = 0; j < n; ++j) out[j] = p[j];
+}
+
 // MCcov.m:48-53
 inline int read_box(const mxArray *bmin, const mxArray *bmax, double *lo, double *hi, const char *id)
 {

@@ -134,3 +134,80 @@ def test_server_left_running_at_exit_ends_cleanly():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=180)
     assert "submitted" in r.stdout, r.stderr[-2000:]
     assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+
+
+def test_stop_wakes_a_waiter_on_a_pending_request():
+    """scmc_server_stop while another thread waits on a request that was never
+    admitted (more requests than slots): the wait returns SCMC_E_CANCELLED and
+    the server is freed only after the waiter has left it."""
+    import ctypes as C
+    import threading
+    S = E.Server(max_active=1, paths_per_slice=100)
+    try:
+        spec_big = req(0, 64, 32, 10.0, 0.9, seed=1)
+        S.submit(spec_big, 200000, "fp64")
+        rids = [S.submit(req(i, 4, 4, 3.0, 0.5, seed=i), 500, "mixed") for i in (1, 2)]
+        ne = 16 * 16
+        bufs = [np.empty(ne) for _ in range(4)]
+        out = {}
+
+        def waiter():
+            out["rc"] = N.lib().scmc_wait(int(rids[-1]), *[N.dptr(b) for b in bufs], None)
+
+        t = threading.Thread(target=waiter)
+        t.start()
+        time.sleep(0.5)
+        assert t.is_alive()            # the last request is still pending
+        say("stop with a waiter")
+    finally:
+        S.stop()
+    t.join(timeout=60)
+    assert not t.is_alive()
+    assert out["rc"] == N.SCMC_E_CANCELLED
+    # the next server starts normally
+    with E.Server(max_active=1) as S2:
+        Ms, Mm, st = S2.wait(S2.submit(req(3, 4, 4, 3.0, 0.5, seed=3), 300, "mixed"))
+        assert np.all(np.isfinite(Ms + Mm)) and st["paths"] == 300
+
+
+def c5_requests(n_req, path_scale, seed=2024):
+    """Requests drawn from the serving workload's distributions: Nv, Nl
+    uniform in {4..64}, optical depth uniform in [1, 20] (slab T = 1, so
+    sigma = OD), HG g uniform in [0, 0.95], paths log-uniform in [1e5, 1e7]
+    (times path_scale), directions uniform on a 15 degree cone."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_req):
+        nv, nl = int(rng.integers(4, 65)), int(rng.integers(4, 65))
+        od, g = float(rng.uniform(1, 20)), float(rng.uniform(0, 0.95))
+        paths = int(10 ** rng.uniform(5, 7) * path_scale)
+        out.append((i, nv, nl, od, g, max(paths, 200)))
+    return out
+
+
+def test_serving_stream_requests_match_fp64_one_shot():
+    """Twelve requests of the serving workload's distributions, arriving as a
+    Poisson stream (50 req/s) at a server with 4 slots, mixed precision; each
+    result is checked against the one-shot FP64 engine on the same paths
+    (<= 1e-4 in the Cauchy-Schwarz metric)."""
+    reqs = c5_requests(12, 0.002)
+    rng = np.random.default_rng(7)
+    specs = {i: E.make_problem(od, 0.99, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(nl, 15, np.random.default_rng(i)),
+                               cone_dirs(nv, 15, np.random.default_rng(1000 + i)), 1, 1, 1.0, True, 2, 3, g,
+                               seed=50 + i)
+             for i, nv, nl, od, g, _ in reqs}
+    with E.Server(max_active=4) as S:
+        ids = {}
+        for i, nv, nl, od, g, n in reqs:
+            ids[i] = S.submit(specs[i], n, "mixed")
+            time.sleep(rng.exponential(1 / 50))
+        say("stream submitted", [(nv, nl, n) for _, nv, nl, _, _, n in reqs])
+        res = {i: S.wait(r) for i, r in ids.items()}
+    for i, nv, nl, od, g, n in reqs:
+        Ms, Mm, st = res[i]
+        assert st["paths"] == n
+        Ms0, Mm0 = E.mccov_spec(specs[i], n, "fp64")
+        err = cs_err(Ms + Mm, Ms0 + Mm0)
+        say("  request", i, "N", nv * nl, "paths", n, "err", err)
+        assert err <= 1e-4, (i, err)
+        assert cs_err(Ms, Ms0) <= 1e-4, i

@@ -53,6 +53,27 @@ def test_reference_argument_errors(mex, gw):
     assert mex.call(gw, 1, *a)[1][1] == "Invalid view source input"
 
 
+def test_lmean0_is_checked_before_it_is_read(mex):
+    """lmean0 must be a real double vector of dim (MCfield, MCfieldOnWave) or
+    3 (the Fourier gateways) elements; a short, complex or non-double value
+    raises instead of being read past its end."""
+    a = base_args()
+    bad = [np.array([[0.0], [1.0]]), np.array([[0.0], [0.0], [1.0], [0.0]]), np.array([[0.0], [0.0], [1.0 + 1j]]),
+           {"x": 1.0}]
+    for lm in bad:
+        _, err = mex.call("MCfield", 1, *(a + [[], lm]))
+        assert err is not None and "lmean0" in err[1], (lm, err)
+        _, err = mex.call("MCfieldOnWave", 1, *(a + [[], lm, np.array([[1.0, 0.5]])]))
+        assert err is not None and "lmean0" in err[1], (lm, err)
+    q = [2.0, 0.9, np.array([[-5.0], [-5.0], [0.0]]), np.array([[5.0], [5.0], [1.0]]), np.array([[0.0], [0.0], [1.0]]),
+         0.1, 0.01, 100, 1.0, 0, 2]
+    for lm in bad:
+        _, err = mex.call("MCfieldFourierQuad", 1, *(q + [lm]))
+        assert err is not None and "lmean0" in err[1], (lm, err)
+        _, err = mex.call("MCfieldFourierQuadOnWave", 1, *(q + [lm, np.array([[1.0]])]))
+        assert err is not None and "lmean0" in err[1], (lm, err)
+
+
 @pytest.mark.gpu
 def test_mccov_gateway_equals_engine(mex):
     a = base_args()
