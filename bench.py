@@ -38,17 +38,8 @@ PEAK_FP64 = 78.6e12     # MI355X FP64, flop/s
 
 
 def workload(seed_dirs=2024):
-    from mcspeckle_amd import engines as E
-    rng = np.random.default_rng(seed_dirs)
-
-    def cone(n, deg):
-        ct = rng.uniform(np.cos(np.deg2rad(deg)), 1.0, n)
-        st = np.sqrt(1 - ct ** 2)
-        ph = rng.uniform(0, 2 * np.pi, n)
-        return np.vstack([st * np.cos(ph), st * np.sin(ph), ct])
-
-    l, v = cone(64, 15), cone(64, 15)
-    return E.make_problem(20.0, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.9, seed=0)
+    from mcspeckle_amd import workloads as W
+    return W.cov_4096(seed_dirs=seed_dirs)
 
 
 def profiled_traffic():
@@ -82,28 +73,8 @@ def cpu_baseline(spec, n_paths, threads):
 
 
 def serving_stream(n_req, rate, path_scale, seed=5):
-    """Requests shaped like the serving configuration: Nv, Nl ~ U{4..64},
-    optical depth ~ U[1,20] (T = 1), HG g ~ U[0,0.95], paths ~ logU[1e5,1e7]
-    (times path_scale), Poisson arrivals at `rate` per second."""
-    from mcspeckle_amd import engines as E
-    rng = np.random.default_rng(seed)
-    t, out = 0.0, []
-    for i in range(n_req):
-        nv, nl = int(rng.integers(4, 65)), int(rng.integers(4, 65))
-        od, g = rng.uniform(1, 20), rng.uniform(0, 0.95)
-        paths = max(1, int(round(np.exp(rng.uniform(np.log(1e5), np.log(1e7))) * path_scale)))
-
-        def cone(n):
-            ct = rng.uniform(np.cos(np.deg2rad(15)), 1.0, n)
-            st = np.sqrt(1 - ct ** 2)
-            ph = rng.uniform(0, 2 * np.pi, n)
-            return np.vstack([st * np.cos(ph), st * np.sin(ph), ct])
-
-        spec = E.make_problem(od, 1.0, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone(nl), cone(nv), 1, 1, 1.0, True, 2, 3, g,
-                              seed=1000 + i)
-        out.append((t, spec, paths))
-        t += rng.exponential(1.0 / rate)
-    return out
+    from mcspeckle_amd import workloads as W
+    return W.request_stream(n_req, rate, path_scale, seed)
 
 
 def serve(n_req, rate, path_scale, max_active, shared=False):
@@ -148,40 +119,30 @@ def field_leg(n_paths, precision="mixed"):
     sigma_a = 0.05, HG g = 0.8, one plane wave at normal incidence, 128 x 128
     far-field views on a uniform grid over +-10 degrees, CBS off."""
     from mcspeckle_amd import engines as E
-    th = np.deg2rad(np.linspace(-10, 10, 128))
-    X, Y = np.meshgrid(th, th)
-    v = np.vstack([np.sin(X.ravel()), np.sin(Y.ravel()), np.ones(X.size)])
-    v /= np.linalg.norm(v, axis=0)
-    sigt = 10.05
-    spec = E.make_problem(sigt, 10 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 2], [[0], [0], [1]], v, 1, 1, 1.0, False, 2,
-                          3, 0.8, seed=0)
+    from mcspeckle_amd import workloads as W
+    spec = W.speckle_image_128()
     st = {}
     t0 = time.perf_counter()
     E.mcfield_spec(spec, n_paths, precision, st)
     dt = time.perf_counter() - t0
-    return {"paths": n_paths, "views": v.shape[1], "value": n_paths * v.shape[1] / dt, "unit": "paths*entries/s",
+    return {"paths": n_paths, "views": spec.nv, "value": n_paths * spec.nv / dt, "unit": "paths*entries/s",
             "seconds": dt, "ms_accum": st["ms_accum"], "vertices": st["vertices"], "precision": precision}
 
 
-def quad_leg(n_paths, precision="mixed", seed=3):
+def quad_leg(n_paths, precision="mixed"):
     """Fourier-accelerated field: box 10 x 10 x 1, sigma_s = 8, sigma_a = 0.02,
     tabulated phase function (256 angle bins, mixture of 3 HG lobes with
     g ~ U[-0.3, 0.95] and Dirichlet(1) weights, seeded), one light at normal
     incidence, 511 x 511 view grid (the reference's grids are odd)."""
     from mcspeckle_amd import engines as E
-    from mcspeckle_amd.interface import tabulatedAmplitudeScatter
-    rng = np.random.default_rng(seed)
-    gs, ws = rng.uniform(-0.3, 0.95, 3), rng.dirichlet([1, 1, 1])
-    th = np.linspace(0, np.pi, 256)
-    hg = sum(w * (1 - g * g) / (4 * np.pi * (1 + g * g - 2 * g * np.cos(th)) ** 1.5) for g, w in zip(gs, ws))
-    tab = tabulatedAmplitudeScatter(th, np.sqrt(hg))["ampfunc"]
-    v_max, v_stp = 0.25, 0.5 / 510
-    lam = 10 * (2 * v_max + v_stp) / 511
-    sigt = 8.02
+    from mcspeckle_amd import workloads as W
+    a = W.fourier_field_511()
     st = {}
     t0 = time.perf_counter()
-    u = E.MCfieldFourierQuad(sigt, 8 / sigt, [-5, -5, 0], [5, 5, 1], [[0], [0], [1]], v_max, v_stp, n_paths, lam,
-                             False, 2, None, -1, sct_type=2, ampfunc=tab, seed=0, precision=precision, stats=st)
+    u = E.MCfieldFourierQuad(a["sigt"], a["albedo"], a["box_min"], a["box_max"], a["l"], a["v_max"], a["v_stp"],
+                             n_paths, a["lambda_"], a["doCBS"], a["smpFlg"], a["lmean0"], a["vsign"],
+                             sct_type=a["sct_type"], ampfunc=a["ampfunc"], seed=a["seed"], precision=precision,
+                             stats=st)
     dt = time.perf_counter() - t0
     return {"paths": n_paths, "grid": list(u.shape[:2]), "value": n_paths * u.shape[0] * u.shape[1] / dt,
             "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_scatter": st["ms_accum"],

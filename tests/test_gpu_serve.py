@@ -170,44 +170,29 @@ def test_stop_wakes_a_waiter_on_a_pending_request():
         assert np.all(np.isfinite(Ms + Mm)) and st["paths"] == 300
 
 
-def c5_requests(n_req, path_scale, seed=2024):
-    """Requests drawn from the serving workload's distributions: Nv, Nl
-    uniform in {4..64}, optical depth uniform in [1, 20] (slab T = 1, so
-    sigma = OD), HG g uniform in [0, 0.95], paths log-uniform in [1e5, 1e7]
-    (times path_scale), directions uniform on a 15 degree cone."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n_req):
-        nv, nl = int(rng.integers(4, 65)), int(rng.integers(4, 65))
-        od, g = float(rng.uniform(1, 20)), float(rng.uniform(0, 0.95))
-        paths = int(10 ** rng.uniform(5, 7) * path_scale)
-        out.append((i, nv, nl, od, g, max(paths, 200)))
-    return out
-
-
 def test_serving_stream_requests_match_fp64_one_shot():
-    """Twelve requests of the serving workload's distributions, arriving as a
-    Poisson stream (50 req/s) at a server with 4 slots, mixed precision; each
-    result is checked against the one-shot FP64 engine on the same paths
-    (<= 1e-4 in the Cauchy-Schwarz metric)."""
-    reqs = c5_requests(12, 0.002)
-    rng = np.random.default_rng(7)
-    specs = {i: E.make_problem(od, 0.99, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(nl, 15, np.random.default_rng(i)),
-                               cone_dirs(nv, 15, np.random.default_rng(1000 + i)), 1, 1, 1.0, True, 2, 3, g,
-                               seed=50 + i)
-             for i, nv, nl, od, g, _ in reqs}
+    """Twelve requests of the serving workload (Nv, Nl uniform in {4..64},
+    optical depth in [1, 20], HG g in [0, 0.95], paths log-uniform in
+    [1e5, 1e7] scaled by 0.002, Poisson arrivals at 50 req/s) through a
+    server with 4 slots in mixed precision; each result is checked against
+    the one-shot FP64 engine on the same paths (<= 1e-4 in the
+    Cauchy-Schwarz metric)."""
+    from mcspeckle_amd import workloads as W
+    stream = W.request_stream(12, 50.0, 0.002, seed=2024)
     with E.Server(max_active=4) as S:
-        ids = {}
-        for i, nv, nl, od, g, n in reqs:
-            ids[i] = S.submit(specs[i], n, "mixed")
-            time.sleep(rng.exponential(1 / 50))
-        say("stream submitted", [(nv, nl, n) for _, nv, nl, _, _, n in reqs])
-        res = {i: S.wait(r) for i, r in ids.items()}
-    for i, nv, nl, od, g, n in reqs:
-        Ms, Mm, st = res[i]
+        ids = []
+        t0 = time.perf_counter()
+        for ta, spec, n in stream:
+            d = ta - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids.append(S.submit(spec, n, "mixed"))
+        say("stream submitted", [(spec.nv, spec.nl, n) for _, spec, n in stream])
+        res = [S.wait(r) for r in ids]
+    for (ta, spec, n), (Ms, Mm, st) in zip(stream, res):
         assert st["paths"] == n
-        Ms0, Mm0 = E.mccov_spec(specs[i], n, "fp64")
+        Ms0, Mm0 = E.mccov_spec(spec, n, "fp64")
         err = cs_err(Ms + Mm, Ms0 + Mm0)
-        say("  request", i, "N", nv * nl, "paths", n, "err", err)
-        assert err <= 1e-4, (i, err)
-        assert cs_err(Ms, Ms0) <= 1e-4, i
+        say("  request N", spec.nv * spec.nl, "paths", n, "err", err)
+        assert err <= 1e-4, (spec.nv, spec.nl, err)
+        assert cs_err(Ms, Ms0) <= 1e-4
