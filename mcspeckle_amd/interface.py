@@ -18,11 +18,12 @@ def boxArea(wavelength, MFP, z, x, y=None):
     """interface/targetArea/boxArea.m:24-63."""
     if not (np.isscalar(wavelength) and np.isscalar(MFP)):
         raise ScmcArgumentError("wavelength and MFP are scalars")
-    z, x = np.asarray(z, dtype=float).reshape(-1), np.asarray(x, dtype=float).reshape(-1)
+    z, x = np.asarray(z).reshape(-1), np.asarray(x).reshape(-1)
     if z.size != 2 or x.size != 2 or (y is not None and np.asarray(y).size != 2):
         raise ScmcArgumentError("x, y and z defined as 2 elements vectors")
     if np.iscomplexobj(z) or np.iscomplexobj(x) or (y is not None and np.iscomplexobj(y)):
         raise ScmcArgumentError("x, y and z are real values")
+    z, x = z.astype(float), x.astype(float)
     if z[0] > z[1]:
         raise ScmcArgumentError("z(1) must be smaller than z(2)")
     if x[0] > x[1]:

@@ -411,3 +411,20 @@ def test_gpu_mcfield_one_light(oracle, kind, dim, cbs):
     assert field_err(E.mcfield_replay(spec, ps, n, "fp64"), u0) <= 1e-12
     u1 = oracle.mcfield(spec, n)
     assert field_err(E.mcfield_spec(spec, n, "mixed"), u1) <= 2e-5
+
+
+@pytest.mark.parametrize("kind,dim", [("hg", 3), ("tab", 3), ("hg", 2)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_gpu_cbs_reciprocity(kind, dim, precision, tol):
+    """The GPU's CBS update is reciprocal: the exchanged problem (lights -v,
+    views -l) walks the same paths (smpFlg = 1) and gives the covariances
+    with the view and light index pairs exchanged (MCcov.m:244-254)."""
+    from test_oracle_engines import swap_lights_and_views
+    spec = build(kind, dim, True, 1, None, nv=6, nl=4, seed=23)
+    sw = swap_lights_and_views(spec)
+    say("cbs reciprocity", kind, dim, precision)
+    n = 3000
+    Ms, Mm = E.mccov_spec(spec, n, precision)
+    Ms2, Mm2 = E.mccov_spec(sw, n, precision)
+    assert cs_err(Ms2, Ms.transpose(2, 3, 0, 1)) <= tol
+    assert cs_err(Mm2, Mm.transpose(2, 3, 0, 1)) <= tol

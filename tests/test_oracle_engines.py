@@ -310,3 +310,42 @@ def test_mcfield_wave_is_mcfield_contracted_with_the_weights(oracle, kind, dim, 
     uw = oracle.mcfield_wave(spec, wl, n)
     want = u @ wl
     np.testing.assert_allclose(uw, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def swap_lights_and_views(spec):
+    """The exchanged problem: lights -v, views -l (same walks when the first
+    vertex is drawn uniformly, smpFlg = 1, and the first direction
+    isotropically)."""
+    import dataclasses
+    return dataclasses.replace(spec, l=np.ascontiguousarray(-spec.v), v=np.ascontiguousarray(-spec.l))
+
+
+def cs_rel(A, B):
+    from conftest import cov_matrix, cs_scale
+    C, R = cov_matrix(A), cov_matrix(B)
+    return float(np.max(np.abs(C - R) / np.maximum(cs_scale(R), 1e-300)))
+
+
+@pytest.mark.parametrize("kind,dim", [("iso", 3), ("hg", 3), ("tab", 3), ("hg", 2)])
+def test_cbs_reciprocity(oracle, kind, dim):
+    """With CBS (MCcov.m:244-254) each vertex adds (f + r)(f + r)^H, and the
+    reversed path r of the problem (l, v) is the forward path f of the
+    exchanged problem (lights -v, views -l): e_l(x)[-v] = e_v(x)[v],
+    e_v(x)[-l] = e_l(x)[l], a(ow.(-l)) = a(-ow.l), a(w1.(-v)) = a(-w1.v).  On
+    the same walks, Ms and Mm of the exchanged problem are therefore those of
+    the original with the view and light index pairs exchanged,
+    M'(l1, l2, v1, v2) = M(v1, v2, l1, l2).  Without CBS Mm has no such
+    symmetry (the check has the power to see a missing reversed path)."""
+    tab = tab3d() if kind == "tab" else None
+    spec = small_spec(kind, dim, True, nv=4, nl=3, seed=21, smp=1, tab=tab)
+    sw = swap_lights_and_views(spec)
+    n = 400
+    Ms, Mm = oracle.mccov(spec, n)
+    Ms2, Mm2 = oracle.mccov(sw, n)
+    assert cs_rel(Ms2, Ms.transpose(2, 3, 0, 1)) <= 1e-12
+    assert cs_rel(Mm2, Mm.transpose(2, 3, 0, 1)) <= 1e-12
+    import dataclasses
+    no = dataclasses.replace(spec, do_cbs=False)
+    _, Mm0 = oracle.mccov(no, n)
+    _, Mm0s = oracle.mccov(swap_lights_and_views(no), n)
+    assert cs_rel(Mm0s, Mm0.transpose(2, 3, 0, 1)) > 1e-3

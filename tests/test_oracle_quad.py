@@ -19,11 +19,21 @@ def quad_numpy(oracle, sigt, albedo, bmin, bmax, l, v_max, v_stp, n_paths, lam, 
     sct_type/ampfunc: the walk's phase function (isotropic in the reference)."""
     wave = wl is not None
     g = oracle.quad_grid(np.array(bmin, float), np.array(bmax, float), v_max, v_stp, lam, wave=wave)
-    nv0, nx = g.nv0, g.nx
     Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
     spec = E.make_problem(sigt, albedo, Bmin, Bmax, l, np.zeros((3, 0)), 1, 1, lam, cbs, smp, sct_type, ampfunc,
                           lmean0=[0, 0, 1], seed=seed, need_views=False)
     ps = oracle.trace(spec, n_paths)
+    return quad_sum(g, ps, sigt, l, v_max, v_stp, lam, cbs, vsign, wl)
+
+
+def quad_sum(g, ps, sigt, l, v_max, v_stp, lam, cbs, vsign, wl=None):
+    """The vertex store, slab scatter and transforms of MCfieldFourierQuad.m:
+    218-303 (MCfieldFourierQuadOnWave.m with wl) over a given path set in the
+    grid's box g.  Returns (u, K)."""
+    wave = wl is not None
+    nv0, nx = g.nv0, g.nx
+    Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
+    n_paths = ps["offset"].size - 1
     off = ps["offset"]
     xL, x1L, pxL = [], [], []
     for p in range(n_paths):
@@ -179,3 +189,114 @@ def test_oracle_quad_wave_is_linear_in_the_weights(oracle):
     whole = oracle.mcfield_fourier_quad_wave(*args, WL, *rest)[0]
     want = sum(WL[j] * parts[j] for j in range(3))
     np.testing.assert_allclose(whole, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def _subset(ps, idx):
+    """The paths idx of a path set (oracle layout)."""
+    off = ps["offset"]
+    out = {k: [] for k in ("x1", "px", "w1", "w0p_re", "w0p_im", "x", "ow", "weight", "xi")}
+    counts = []
+    for p in idx:
+        a, b = off[p], off[p + 1]
+        counts.append(b - a)
+        for k in ("x1", "w1"):
+            out[k].append(ps[k][3 * p:3 * p + 3])
+        for k in ("px", "w0p_re", "w0p_im"):
+            out[k].append(ps[k][p:p + 1])
+        for k in ("x", "ow"):
+            out[k].append(ps[k][3 * a:3 * b])
+        for k in ("weight", "xi"):
+            out[k].append(ps[k][a:b])
+    r = {k: np.ascontiguousarray(np.concatenate(v)) for k, v in out.items()}
+    r["offset"] = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return r
+
+
+def _paths_on_cell_centres(g, slabs, n_paths, seed):
+    """A path set whose first vertices sit exactly on cell centres of the
+    Fourier grid (x, y) and on slab centres (z), the positions the Fourier
+    scatter quantises to; later vertices anywhere near the box centre."""
+    rng = np.random.default_rng(seed)
+    Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
+    gz = np.concatenate([Bmin[2] + np.arange(g.nz - 1) * g.z_bin, [Bmax[2]]])
+    tz = (gz[:-1] + gz[1:]) / 2
+    c = (g.nx - 1) // 2
+    a_iso = 1 / np.sqrt(4 * np.pi)
+    ps = {k: [] for k in ("x1", "px", "w1", "w0p_re", "w0p_im", "x", "ow", "weight", "xi")}
+    counts, slab_of = [], []
+    for p in range(n_paths):
+        j = int(rng.choice(slabs))
+        slab_of.append(j)
+        ix, iy = c + rng.integers(-1, 2), c + rng.integers(-1, 2)
+        x1 = np.array([g.gx0 + ix * g.box_stp, g.gy0 + iy * g.box_stp, tz[j]])
+        nvtx = int(rng.integers(1, 6))
+        counts.append(nvtx)
+        w1 = rng.normal(size=3)
+        ps["x1"].append(x1)
+        ps["w1"].append(w1 / np.linalg.norm(w1))
+        ps["px"].append([rng.uniform(0.5, 2.0)])
+        ps["w0p_re"].append([a_iso])
+        ps["w0p_im"].append([0.0])
+        for n in range(nvtx):
+            xn = x1 if n == 0 else np.array([rng.uniform(-1, 1), rng.uniform(-1, 1),
+                                             rng.uniform(Bmin[2] + 0.02, Bmax[2] - 0.02)])
+            ow = rng.normal(size=3)
+            ps["x"].append(xn)
+            ps["ow"].append(ow / np.linalg.norm(ow))
+            ps["weight"].append([rng.uniform(0.2, 1.0)])
+            ps["xi"].append([rng.uniform()])
+    r = {k: np.ascontiguousarray(np.concatenate(v)) for k, v in ps.items()}
+    r["offset"] = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return r, np.array(slab_of), tz
+
+
+@pytest.mark.parametrize("vsign", [1.0, -1.0])
+def test_fourier_field_equals_direct_field_on_cell_centres(oracle, vsign):
+    """MCfieldFourierQuad against MCfield on one path set: when every first
+    vertex lies on a cell centre and a slab centre (where the Fourier scatter
+    puts it), the Fourier field is the direct field after the normalisation
+    map, with lights and views exchanged (the Fourier sum puts the view phase
+    at the first vertex and the light at the current one, MCfield.m's
+    reversed-path term; by reciprocity that is MCfield with lights -v and view
+    -l):
+
+        u_F(o) = sqrt(maxItr / K) / a_iso * e^{i phi_j(o)} * u_MC,j(o)  summed over slabs j
+
+    sqrt(maxItr/K): the vertex-count normalisation (MCfieldFourierQuad.m:214,
+    302-303 vs MCfield.m:252-254); 1/a_iso: the isotropic amplitude the Fourier
+    sum leaves out; phi_j: k vz zmax (vsign = +1, viewing towards +z: one
+    constant per pixel), or -2 k |vz| t_j (vsign = -1: the reference's
+    slab-centre phase -k|vz| t_j (zexp0, :134) has the opposite sign of the
+    in-slab phase +k(z1 - t_j) and of MCfield's +k|vz| z1 for a view towards
+    -z).  This checks the DFT index folding (fftshift/flip/crop), the lateral
+    and slab phases, both attenuations and the normalisation independently of
+    a transliteration of the Fourier code."""
+    lam, v_max, v_stp = 0.5, 0.3, 0.05
+    bmin, bmax = [-2.0, -2.0, 0.0], [2.0, 2.0, 1.2]
+    sigt, albedo = 1.5, 0.9
+    l = np.array([[0.04], [-0.03], [np.sqrt(1 - 0.04 ** 2 - 0.03 ** 2)]])
+    g = oracle.quad_grid(np.array(bmin), np.array(bmax), v_max, v_stp, lam)
+    assert g.nfft == g.nv0 == 13 and g.nz - 1 >= 2      # no spatial padding beyond the view grid
+    n_paths = 60
+    ps, slab_of, tz = _paths_on_cell_centres(g, [0, 1, g.nz - 2], n_paths, seed=3 if vsign > 0 else 4)
+    u_F, K = quad_sum(g, ps, sigt, l, v_max, v_stp, lam, False, vsign)
+    assert K == ps["offset"][-1]
+    # the view grid of MCfieldFourierQuad.m:103-105 and the exchanged problem
+    theta_v = -v_max + np.arange(g.nv0) * v_stp
+    vx, vy = np.meshgrid(-theta_v, -theta_v, indexing="ij")
+    vz = vsign * np.sqrt(1 - vx ** 2 - vy ** 2)
+    vdirs = np.vstack([vx.ravel(order="F"), vy.ravel(order="F"), vz.ravel(order="F")])
+    Bmin, Bmax = np.array(g.box_min[:]), np.array(g.box_max[:])
+    spec = E.make_problem(sigt, albedo, Bmin, Bmax, -vdirs, -l, 1, 1, lam, False, 2, 1, 0, seed=0)
+    k = 2 * np.pi / lam
+    a_iso = 1 / np.sqrt(4 * np.pi)
+    pred = np.zeros((g.nv0, g.nv0), complex)
+    for j in np.unique(slab_of):
+        u_mc = oracle.mcfield_replay(spec, _subset(ps, np.flatnonzero(slab_of == j)), n_paths)
+        u_mc = u_mc.reshape(-1).reshape((g.nv0, g.nv0), order="F")
+        if vsign > 0:
+            phi = k * vz * Bmax[2]
+        else:
+            phi = -2 * k * np.abs(vz) * tz[j]
+        pred += np.sqrt(n_paths / K) / a_iso * np.exp(1j * phi) * u_mc
+    np.testing.assert_allclose(u_F[:, :, 0], pred, rtol=0, atol=1e-11 * np.abs(pred).max())
