@@ -200,6 +200,20 @@ int scmc_mccov_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max
 int scmc_mcfield_hetro(const scmc_problem *pr, const scmc_medium *med, int64_t max_itr, int32_t precision,
                        double *u_re, double *u_im, scmc_stats *stats);
 
+/* u = MCfield(...) without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, walks and output as scmc_mcfield (homogeneous media); every
+ * device buffer lies in one file-scope device array of 128 MiB per GPU
+ * (held until the process ends), so the call issues no hipMalloc/hipFree.
+ * Problems whose fixed part (accumulators and per-view work, about
+ * 1 KiB x nv x nl) leaves no room for 64 paths are rejected with SCMC_E_ARG
+ * before any device work.  Calls in one process run one after another; calls
+ * from several threads wait for each other, and the array keeps the last
+ * call's contents (each call clears what must start at zero). */
+int scmc_mcfield_noalloc(const scmc_problem *pr, int64_t max_itr, int32_t precision, double *u_re, double *u_im,
+                         scmc_stats *stats);
+
 /* ---- sessions: accumulate path ranges, read results at any time -------- */
 /* A session owns the device accumulators of one MCcov (kind 0) or MCfield
  * (kind 1) problem.  scmc_session_run adds paths [first, first + n) (ids

@@ -25,7 +25,7 @@ MIXED = 1
 PRECISIONS = {"fp64": FP64, "mixed": MIXED}
 
 EXPORTS = [
-    "scmc_mccov", "scmc_mcfield", "scmc_mcfield_fourier_quad", "scmc_quad_grid",
+    "scmc_mccov", "scmc_mcfield", "scmc_mcfield_noalloc", "scmc_mcfield_fourier_quad", "scmc_quad_grid",
     "scmc_mcfield_wave", "scmc_mcfield_fourier_quad_wave", "scmc_quad_grid_wave", "scmc_mccov_hetro", "scmc_mcfield_hetro",
     "scmc_trace", "scmc_mccov_replay", "scmc_mcfield_replay",
     "scmc_session_create", "scmc_session_run", "scmc_session_result", "scmc_session_stats",
@@ -116,6 +116,7 @@ def lib():
     sig = {
         "scmc_mccov": ([P, C.c_int64, C.c_int32, _dp, _dp, _dp, _dp, S], C.c_int),
         "scmc_mcfield": ([P, C.c_int64, C.c_int32, _dp, _dp, S], C.c_int),
+        "scmc_mcfield_noalloc": ([P, C.c_int64, C.c_int32, _dp, _dp, S], C.c_int),
         "scmc_mcfield_fourier_quad": ([P, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_int32, _dp, _dp, S], C.c_int),
         "scmc_quad_grid": ([_dp, _dp, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32),
                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),

@@ -164,10 +164,12 @@ void BlockPool::trim(bool now)
     retired.clear();
 }
 
+thread_local Arena *tls_arena = nullptr;
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release()
 {
-    if (p) {
+    if (p && !borrowed) {
         if (pool)
             pool->put(p, bytes);
         else
@@ -176,11 +178,26 @@ void DevBuf::release()
     p = nullptr;
     bytes = 0;
     pool = nullptr;
+    borrowed = false;
 }
 bool DevBuf::ensure(size_t b, int *rc)
 {
     if (b <= bytes && p) return true;
     release();
+    if (Arena *A = tls_arena) {
+        const size_t want = ((b ? b : 16) + 255) & ~(size_t)255;
+        if (want > A->cap - A->used) {
+            *rc = set_error(SCMC_E_NOMEM,
+                            "allocation-free call: %zu more bytes needed, %zu of the %zu-byte scratch are left",
+                            want, A->cap - A->used, A->cap);
+            return false;
+        }
+        p = A->base + A->used;
+        A->used += want;
+        bytes = want;
+        borrowed = true;
+        return true;
+    }
     // 1/8 headroom: the next chunk, a few percent larger, fits without a
     // reallocation (and its device-wide wait)
     size_t want = b ? b + b / 8 : 16;

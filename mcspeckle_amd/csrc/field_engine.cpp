@@ -2,6 +2,7 @@
 #include "scmc_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -50,6 +51,19 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
     int rc = SCMC_OK;
     const size_t need = field_work_elems(tc.np, tc.nvert, nv, nl, wave);
     if (!work.ensure(sizeof(double2) * need, &rc)) return rc;
+    if (tls_arena) {
+        // allocation-free calls create no events: host time around the
+        // launch and its completion
+        auto t0 = std::chrono::steady_clock::now();
+        launch_field(ps.dp, ps.het ? &ps.hm : nullptr, tc.np, tc.nvert, (const PathRec *)tc.paths.p,
+                     (const VertexRec *)tc.verts.p, (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p,
+                     (double2 *)work.p, (double2 *)acc.p, mixed, wave ? (const double2 *)wl.p : nullptr,
+                     (unsigned long long *)nonfinite.p, s);
+        if (!hip_ok(hipGetLastError(), "field launch", &rc) || !hip_ok(hipStreamSynchronize(s), "field sync", &rc))
+            return rc;
+        *ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return SCMC_OK;
+    }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);

@@ -45,10 +45,22 @@ struct BlockPool {
 };
 extern thread_local BlockPool *tls_block_pool;
 
+// Allocation-free calls: while a thread's tls_arena is set, DevBuf::ensure
+// carves its blocks from that arena -- device memory that outlives the call
+// and that the library does not allocate per call -- instead of asking the
+// driver, and release() gives nothing back.  A block is carved at its exact
+// size (256-byte aligned); running out is an error, never an allocation.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+extern thread_local Arena *tls_arena;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     BlockPool *pool = nullptr;   // where p goes back to (null: the driver)
+    bool borrowed = false;       // p lies in an arena (nothing to free)
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;

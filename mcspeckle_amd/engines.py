@@ -218,6 +218,21 @@ def mcfield_spec(spec, max_itr, precision="fp64", stats=None):
     return (u_re + 1j * u_im).reshape((nv, nl), order="F")
 
 
+def mcfield_noalloc_spec(spec, max_itr, precision="fp64", stats=None):
+    """mcfield_spec through scmc_mcfield_noalloc: the same field, with every
+    device buffer in the library's fixed scratch array (no device allocation)."""
+    L = N.lib()
+    nv, nl = spec.nv, spec.nl
+    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    st = N.Stats()
+    p = c_problem(spec)
+    N.check(L.scmc_mcfield_noalloc(C.byref(p), int(max_itr), _prec(precision), N.dptr(u_re), N.dptr(u_im),
+                                   C.byref(st)))
+    if stats is not None:
+        stats.update(st.as_dict())
+    return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+
+
 def MCcov(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg, sct_type,
           ampfunc, ampfunc0=None, *, seed=0, path_offset=0, precision="fp64", stats=None):
     """code/MCcov.m:1-299 on the GPU. Returns (Ms, Mm)."""
