@@ -15,12 +15,18 @@ synchronises its stream), so the timed region is bracketed by device
 synchronisation on both sides.  value = K * paths_per_step * 4096^2 / time,
 mixed precision (FP16 hi/lo MFMA, FP32 accumulation, FP64 tracer/phase).
 
-Also reported: the plain FP64 GPU path on the same workload (fewer paths),
-the error of the mixed result against the FP64 result on identical paths,
-the roofline of the dominant kernel (the mixed covariance update, timed with
-HIP events on its stream), and the FP64 CPU restatement timed on the host on a
-bounded sample (the oracle; test infrastructure used here only as the
-baseline being timed).
+By default the timed region is 20 steps of 5e5 paths: the workload's full
+1e7 paths.
+
+Also reported: the plain FP64 GPU path on the same workload (1e5 paths),
+the error of the mixed result against the FP64 result on those identical
+paths, the roofline of the dominant kernel (the mixed covariance update, timed
+with HIP events on its stream), the field and Fourier-field workloads, the
+serving loop (capacity with all requests submitted at once, latency of a
+Poisson stream offering 70 % of that throughput, and the 50 req/s overload
+stream with paths scaled down), and the FP64 CPU restatement timed on the
+host single-threaded and on all cores on bounded samples (the oracle; test
+infrastructure used here only as the baseline being timed).
 """
 import argparse
 import json
@@ -72,16 +78,51 @@ def cpu_baseline(spec, n_paths, threads):
     return n_paths * N * N / dt, dt
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_legs(spec, paths_all, paths_one, threads):
+    """The FP64 CPU restatement (the oracle, with the reference's per-vertex
+    Nl^2 x Nv^2 update loop) on the host cores: the headline workload on a
+    sample of paths single-threaded and on `threads` threads (OpenMP over path
+    shards, one private accumulator per thread), and the 64 x 64 workload
+    (1e5 paths) in full on `threads` threads plus a 1e4-path single-thread
+    sample."""
+    from mcspeckle_amd import workloads as W
+    v_all, t_all = cpu_baseline(spec, paths_all, threads)
+    v_one, t_one = cpu_baseline(spec, paths_one, 1)
+    c1 = W.plain_cov_64()
+    v1_all, t1_all = cpu_baseline(c1, 100000, threads)
+    v1_one, t1_one = cpu_baseline(c1, 10000, 1)
+    return {"value": v_all, "unit": "paths*entries/s", "cores": threads, "kind": "port",
+            "sample": f"first {paths_all} paths of the same workload (seed 0), {t_all:.1f} s, OpenMP over path "
+                      f"shards on {threads} threads",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "single_thread": {"value": v_one, "cores": 1, "sample": f"first {paths_one} paths, {t_one:.1f} s"},
+            "plain_cov_64": {"paths": 100000, "seconds": t1_all, "value": v1_all, "cores": threads,
+                             "single_thread": {"paths": 10000, "seconds": t1_one, "value": v1_one}}}
+
+
 def serving_stream(n_req, rate, path_scale, seed=5):
     from mcspeckle_amd import workloads as W
     return W.request_stream(n_req, rate, path_scale, seed)
 
 
-def serve(n_req, rate, path_scale, max_active, shared=False):
+def serve(n_req, rate, path_scale, max_active, shared=False, seed=5, timeline=None):
+    """Requests of the serving workload through the continuous-batching
+    server, submitted at their Poisson arrival times (rate = inf: all at
+    once); latency is submit to result."""
     import threading
     from mcspeckle_amd import engines as E
-    stream = serving_stream(n_req, rate, path_scale)
-    lat, work = [], 0.0
+    stream = serving_stream(n_req, rate if np.isfinite(rate) else 1e9, path_scale, seed)
+    lat, queued, work = [], [], 0.0
     S = E.Server(max_active=max_active, shared_launches=shared)
     ids = []
     t0 = time.perf_counter()
@@ -102,16 +143,25 @@ def serve(n_req, rate, path_scale, max_active, shared=False):
         rid, spec, paths = ids[done]
         Ms, Mm, st = S.wait(rid)
         lat.append(st["ms_total"])
+        queued.append(st["ms_queued"])
         work += paths * float(spec.nv * spec.nl) ** 2
         done += 1
     t_end = time.perf_counter() - t0
     th.join()
     S.stop()
-    lat = np.array(lat)
-    return {"requests": n_req, "arrival_rate_per_s": rate, "path_scale": path_scale, "max_active": max_active,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "throughput": work / t_end, "unit": "paths*entries/s",
-            "makespan_s": t_end, "offered_span_s": stream[-1][0], "shared_launches": bool(shared)}
+    lat, queued = np.array(lat), np.array(queued)
+    out = {"requests": n_req, "arrival_rate_per_s": rate if np.isfinite(rate) else None,
+           "saturated": not np.isfinite(rate), "path_scale": path_scale, "max_active": max_active,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "queued_p50_ms": float(np.percentile(queued, 50)),
+           "queued_p99_ms": float(np.percentile(queued, 99)), "throughput": work / t_end, "unit": "paths*entries/s",
+           "requests_per_s": n_req / t_end, "makespan_s": t_end, "offered_span_s": stream[-1][0],
+           "shared_launches": bool(shared)}
+    if timeline is not None:
+        # per request: arrival, admission and completion (s from the start), N, paths
+        timeline.extend((ta, ta + q / 1e3, ta + l / 1e3, spec.nv * spec.nl, paths)
+                        for (ta, spec, paths), q, l in zip(stream, queued, lat))
+    return out
 
 
 def field_leg(n_paths, precision="mixed"):
@@ -269,11 +319,13 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--paths-per-step", type=int, default=50000)
-    ap.add_argument("--fp64-paths", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    # 20 steps of 5e5 paths: the workload's full 1e7 paths in the timed region
+    ap.add_argument("--paths-per-step", type=int, default=500000)
+    ap.add_argument("--fp64-paths", type=int, default=100000)
     ap.add_argument("--cpu-paths", type=int, default=32)
+    ap.add_argument("--cpu-paths-single", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -281,6 +333,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-path-scale", type=float, default=0.01)
     ap.add_argument("--serve-active", type=int, default=4)
+    ap.add_argument("--capacity-requests", type=int, default=16)
+    ap.add_argument("--sustained-requests", type=int, default=24)
+    ap.add_argument("--sustained-load", type=float, default=0.7)
+    ap.add_argument("--timeline", default="", help="write the per-request serving timeline (JSON) here")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--field-paths", type=int, default=1000000)
     ap.add_argument("--quad-paths", type=int, default=5000000)
@@ -334,7 +390,8 @@ def main():
         "dtype": "fp16x3+fp32acc (fp64 tracer/phase)", "data": "synthetic (device-sampled Philox paths, seed 0)",
         "config": {"workload": "mixed-precision covariance 4096^2: slab T=1 sigma=20 albedo=1 HG g=0.9, "
                                "64 lights x 64 views on a 15 deg cone, CBS on",
-                   "paths_per_step": P, "vertices_per_path": verts / (args.steps * P)},
+                   "paths_per_step": P, "paths_timed": args.steps * P,
+                   "vertices_per_path": verts / (args.steps * P)},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP16, "traffic": profiled_traffic(),
                      "kernel": "herk_mixed_kernel", "expand_ms_per_launch": (st1["ms_expand"] - st0["ms_expand"]) / max(herk_launches, 1), "launches": herk_launches,
@@ -368,6 +425,7 @@ def main():
         }
         out["mixed_vs_fp64_speedup"] = value / out["fp64_plain"]["value"]
         out["mixed_vs_fp64_max_cs_rel_err"] = cs_err(Msm + Mmm, Ms64 + Mm64)
+        out["mixed_vs_fp64_paths"] = Pf
 
     # ---- the field and Fourier-field configurations
     if not args.no_legs:
@@ -385,21 +443,43 @@ def main():
             key, fn = legs[name]
             out[key] = fn()
 
-    # ---- serving: continuous batching, p50/p99 latency (scaled-down stream)
+    # ---- serving: continuous batching, p50/p99 latency
     if not args.no_serve:
-        out["serving"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale, args.serve_active)
-        # the same stream with the slots' small updates packed into shared launches
-        out["serving_shared"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
-                                      args.serve_active, shared=True)
+        tl = {} if args.timeline else None
+        # capacity: requests of the workload's distributions at their full path
+        # counts, all submitted at once; the server's completion rate is what
+        # the GPU sustains
+        cap = serve(args.capacity_requests, float("inf"), 1.0, args.serve_active, seed=11,
+                    timeline=tl.setdefault("capacity", []) if tl is not None else None)
+        out["serving_capacity"] = cap
+        # a Poisson stream offering a stated fraction of that throughput
+        # (paths * entries per second; the request sizes do not depend on the
+        # rate, so the stream's own mean work per request sets the rate):
+        # latency when the server keeps up
+        probe = serving_stream(args.sustained_requests, 1.0, 1.0, 12)
+        mean_work = float(np.mean([p * float(sp.nv * sp.nl) ** 2 for _, sp, p in probe]))
+        rate = args.sustained_load * cap["throughput"] / mean_work
+        out["serving"] = serve(args.sustained_requests, rate, 1.0, args.serve_active, seed=12,
+                               timeline=tl.setdefault("sustained", []) if tl is not None else None)
+        out["serving"]["offered_fraction_of_capacity"] = args.sustained_load
+        # the workload's stated arrival rate (50 req/s) is ~50x what one GPU
+        # completes: run it with paths scaled down, as an overload figure
+        out["serving_overload"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                        args.serve_active,
+                                        timeline=tl.setdefault("overload", []) if tl is not None else None)
+        # the same overload stream with the slots' small updates in shared launches
+        out["serving_overload_shared"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                               args.serve_active, shared=True)
+        if tl is not None:
+            with open(args.timeline, "w") as f:
+                json.dump(tl, f)
 
     # ---- CPU baseline: FP64 restatement with the reference's loop structure
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, t = cpu_baseline(spec, args.cpu_paths, threads)
-        out["cpu_baseline"] = {"value": v, "unit": "paths*entries/s", "cores": threads, "kind": "port",
-                               "sample": f"first {args.cpu_paths} paths of the same workload (seed 0), "
-                                         f"{t:.1f} s, OpenMP over path shards"}
-        out["vs_cpu_baseline"] = value / v
+        out["cpu_baseline"] = cpu_legs(spec, args.cpu_paths, args.cpu_paths_single, threads)
+        out["vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+        out["vs_cpu_single_thread"] = value / out["cpu_baseline"]["single_thread"]["value"]
     print(json.dumps(out))
 
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SCMC_ABI_VERSION 1
+#define SCMC_ABI_VERSION 2
 
 enum scmc_status {
     SCMC_OK = 0,
@@ -117,6 +117,8 @@ typedef struct {
                                 factors came out NaN/Inf (e.g. a NaN path weight
                                 in a replayed set); the results then hold NaN,
                                 as the reference's would                        */
+    double ms_queued;        /* serving: submit to admission (the rest of
+                                ms_total is the request's own run)              */
 } scmc_stats;
 
 /* Replayable path set (vertex-major), the format of the oracle's sampler and

@@ -65,7 +65,7 @@ class Stats(C.Structure):
         ("ms_trace", C.c_double), ("ms_nee", C.c_double), ("ms_accum", C.c_double),
         ("ms_total", C.c_double), ("fp16_scale_log2", C.c_double),
         ("ms_herk", C.c_double), ("herk_launches", C.c_int64), ("herk_flops", C.c_double),
-        ("ms_expand", C.c_double), ("nonfinite", C.c_int64),
+        ("ms_expand", C.c_double), ("nonfinite", C.c_int64), ("ms_queued", C.c_double),
     ]
 
     def as_dict(self):

@@ -930,7 +930,7 @@ extern "C" {
 
 const char *scmc_last_error(void) { return g_err.c_str(); }
 
-const char *scmc_version(void) { return "scmc-mi355x 0.1 (abi 1, gfx950)"; }
+const char *scmc_version(void) { return "scmc-mi355x 0.2 (abi 2, gfx950)"; }
 
 int scmc_device_count(void)
 {

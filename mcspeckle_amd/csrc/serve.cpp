@@ -192,7 +192,9 @@ static int copy_out(Request *r, double *ms_re, double *ms_im, double *mm_re, dou
     if (stats) {
         *stats = r->st;
         stats->ms_total = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
-        // queueing delay before admission in ms_trace's place is not reported; ms_total is latency
+        // a request cancelled before admission was never started
+        const clk::time_point adm = r->t_start.time_since_epoch().count() ? r->t_start : r->t_done;
+        stats->ms_queued = std::chrono::duration<double, std::milli>(adm - r->t_submit).count();
     }
     if (r->rc) return set_error(r->rc, "%s", r->err.c_str());
     const size_t ne = r->ms_re.size();
