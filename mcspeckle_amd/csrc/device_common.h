@@ -246,6 +246,48 @@ __device__ __forceinline__ void iso_dir(double d0, double d1, double d2, double 
     }
 }
 
+// Path regeneration.  A walking kernel with resident lanes hands each lane a
+// new path index as soon as its path ends, instead of letting a wave run as
+// long as its longest path (for the slab workloads the longest of 64 walks is
+// ~6x the mean, so most lanes would idle).  A wave takes indices in blocks of
+// kPathBlock from a global counter (one atomic per block) and gives them to
+// its idle lanes in lane order.  A path's walk depends only on its index, so
+// the results do not depend on which lane walks it.
+constexpr int kPathBlock = 256;
+struct WavePaths {
+    int64_t next = 0, end = 0;   // the wave's current block (wave-uniform)
+    bool exhausted = false;
+};
+// Called by every lane of the wave; `want` lanes receive the next index in
+// [0, np), or -1 once the counter has passed np.  *ctr starts at 0.
+__device__ __forceinline__ int64_t take_path(WavePaths &wp, bool want, int64_t np, unsigned long long *ctr)
+{
+    int64_t id = -1;
+    const unsigned lane = __lane_id();
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned long long m = __ballot(want);
+    while (m && !wp.exhausted) {
+        if (wp.next >= wp.end) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(ctr, (unsigned long long)kPathBlock);
+            base = __shfl(base, 0);
+            if ((int64_t)base >= np) {
+                wp.exhausted = true;
+                break;
+            }
+            wp.next = (int64_t)base;
+            wp.end = (int64_t)base + kPathBlock < np ? (int64_t)base + kPathBlock : np;
+        }
+        const int64_t avail = wp.end - wp.next;
+        const int rank = __popcll(m & below);
+        if (((m >> lane) & 1ull) && rank < avail) id = wp.next + rank;
+        const int nwant = __popcll(m);
+        wp.next += nwant < avail ? nwant : avail;
+        m = __ballot(want && id < 0);
+    }
+    return id;
+}
+
 // The walking kernels stage the tabulated direction table (the cosines of
 // sampleIcdf) in LDS when it has at most kIcdfLds entries.  Every thread of
 // the workgroup calls stage_icdf (a tabulated problem ends it with a

@@ -45,19 +45,21 @@ __device__ __forceinline__ void tri_decode(int t, int &ti, int &tj)
 }
 
 // Work item -> (k-split s, tile rank t).  Workgroups are dispatched to the 8
-// XCDs round robin (XCD = blockIdx % 8).  When ksplit is a multiple of 8 each
-// XCD gets its own k-splits and walks all tiles of one split in the blocked
-// order of `order` before the next, so the workgroups resident on one XCD
-// share a k-range and an 8x8 block of tiles: their rows stay in that XCD's L2.
+// XCDs round robin (XCD = blockIdx % 8).  The splits run one after another
+// across the whole chip, so the expanded rows being read at any time are one
+// split's (about 225 MB at N = 4096, within the 256 MB MALL); inside a split
+// each XCD walks its own contiguous run of ntile/8 tiles in the blocked order
+// of `order` (an 8x8 block of tiles), so its rows are shared in its L2.
 __device__ __forceinline__ void item_decode(int b, int ksplit, int ntile, int &s, int &t)
 {
-    if ((ksplit & 7) == 0) {
-        const int x = b & 7, j = b >> 3;
-        s = x + 8 * (j / ntile);
-        t = j % ntile;
+    (void)ksplit;
+    if ((ntile & 7) == 0) {
+        const int x = b & 7, g = b >> 3, per = ntile >> 3;
+        s = g / per;
+        t = x * per + g % per;
     } else {
-        s = b % ksplit;
-        t = b / ksplit;
+        s = b / ntile;
+        t = b % ntile;
     }
 }
 

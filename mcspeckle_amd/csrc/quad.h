@@ -5,6 +5,7 @@
 namespace scmc {
 
 constexpr int kQuadMaxSlabs = 4096;
+constexpr int kQuadFusedLights = 4;   // lights per vertex the fused walk-and-scatter kernel handles
 
 // Grid geometry shared by the host set-up and the scatter kernel.
 struct QuadGeom {
@@ -33,9 +34,10 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
                          const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,
                          hipStream_t s);
 // Walk paths [p0, p0 + np) and scatter every vertex into the slab grids in
-// the same pass; *nvert (device) is incremented by the vertex count.
+// the same pass; *nvert (device) is incremented by the vertex count; *ctr
+// (device, zeroed before the launch) hands out the path indices.
 void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
-                               unsigned long long *nvert, hipStream_t s);
+                               unsigned long long *nvert, unsigned long long *ctr, hipStream_t s);
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s);
 void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,

@@ -19,6 +19,7 @@
 #include "kernels.h"
 #include "quad.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace scmc {
@@ -58,105 +59,186 @@ __device__ __forceinline__ int slab_of(double z, const QuadGeom &g)
     return j;
 }
 
-// One vertex's contribution (MCfieldFourierQuad.m:263-273, and the light-
-// summed form of MCfieldFourierQuadOnWave.m:281-302): xv the vertex, x1 its
-// path's first vertex, amp = sqrt(W/px) [/sqrt 2 for later vertices with
-// CBS], xi its random-phase uniform.
+// One vertex's contribution in one pass (MCfieldFourierQuad.m:263-273, and
+// the light-summed form of MCfieldFourierQuadOnWave.m:281-302): xs the
+// scatter (view-side) position whose cell and slab receive it, xl the
+// light-side position, amp = sqrt(W/px) [/sqrt 2 for later vertices with
+// CBS], xi the vertex's random-phase uniform.  Pass 1 has xs = x1, xl = the
+// vertex; pass 2 (CBS) swaps them (:224-232).
 template <typename T>
-__device__ __forceinline__ void quad_scatter_vertex(const DevProblem &P, const QuadGeom &g, const double *xv,
-                                                    const double *x1, double amp, double xi,
-                                                    typename cx<T>::t *grid)
+__device__ __forceinline__ void quad_scatter_pass(const DevProblem &P, const QuadGeom &g, const double *xs,
+                                                  const double *xl, double amp, double xi, typename cx<T>::t *grid)
 {
+    const int j = slab_of(xs[2], g);
+    if (j < 0) return;
+    long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
+    ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
+    const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
+    const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
     if constexpr (std::is_same<T, float>::value) {
         // FP32 grid: every phase counted in cycles and reduced in FP64, then
         // hardware sin/cos/exp2 in FP32 (the FP64 grid keeps libm FP64)
         const float2 ex = cis_cycles(xi);   // e^{2 pi i xi}
         const float pre = (float)amp * ex.x, pim = (float)amp * ex.y;
-        for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
-            const double *xs = pass == 0 ? x1 : xv;
-            const double *xl = pass == 0 ? xv : x1;
-            const int j = slab_of(xs[2], g);
-            if (j < 0) continue;
-            long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
-            ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
-            iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
-            const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
-            const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
-            const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
-            if (g.wave) {
-                float twr = 0.f, twi = 0.f;
-                for (int l = 0; l < P.nl; ++l) {
-                    const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
-                    const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
-                    twr += wr * c2.x - wi * c2.y;
-                    twi += wr * c2.y + wi * c2.x;
-                }
-                const float2 e1 = cis_cycles(g.vsign * (tz - xs[2]) * P.inv_lambda);
-                const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
-                atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], (ar * twr - ai * twi) * m,
-                             (ar * twi + ai * twr) * m);
-                continue;
-            }
-            const float2 e1 = cis_cycles(-(tz - xs[2]) * P.inv_lambda);
-            const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
+        const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
+        if (g.wave) {
+            float twr = 0.f, twi = 0.f;
             for (int l = 0; l < P.nl; ++l) {
-                const float2 c2 = cis_cycles(dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
-                const float br = m * c2.x, bi = m * c2.y;
-                atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], ar * br - ai * bi,
-                             ar * bi + ai * br);
+                const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+                const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
+                twr += wr * c2.x - wi * c2.y;
+                twi += wr * c2.y + wi * c2.x;
             }
+            const float2 e1 = cis_cycles(g.vsign * (tz - xs[2]) * P.inv_lambda);
+            const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
+            atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], (ar * twr - ai * twi) * m,
+                         (ar * twi + ai * twr) * m);
+            return;
+        }
+        const float2 e1 = cis_cycles(-(tz - xs[2]) * P.inv_lambda);
+        const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
+        for (int l = 0; l < P.nl; ++l) {
+            const float2 c2 = cis_cycles(dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+            const float br = m * c2.x, bi = m * c2.y;
+            atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], ar * br - ai * bi,
+                         ar * bi + ai * br);
         }
         return;
     }
     double sx, cxv;
     sincos(2 * kPi * xi, &sx, &cxv);
     const double pre = amp * cxv, pim = amp * sx;
-    for (int pass = 0; pass < (P.do_cbs ? 2 : 1); ++pass) {
-        const double *xs = pass == 0 ? x1 : xv;   // scatter (view-side) position
-        const double *xl = pass == 0 ? xv : x1;   // light-side position
-        const int j = slab_of(xs[2], g);
-        if (j < 0) continue;
-        long ix = lround((xs[0] - g.gx0) / g.box_stp), iy = lround((xs[1] - g.gy0) / g.box_stp);
-        ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
-        iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
-        const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
-        const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
-        if (g.wave) {
-            // MCfieldFourierQuadOnWave.m:281-302: pxL exp(i vsign k (tz - z_s)) twl atl,
-            // twl = sum_l Wl(l) exp(i vsign k l.x_L)
-            double twr = 0, twi = 0;
-            for (int l = 0; l < P.nl; ++l) {
-                double s2, c2;
-                sincos(g.vsign * P.k * dot<3>(P.l + 3 * l, xl), &s2, &c2);
-                const double2 w = g.wl[l];
-                twr += w.x * c2 - w.y * s2;
-                twi += w.x * s2 + w.y * c2;
-            }
-            double s1, c1;
-            sincos(g.vsign * P.k * (-xs[2] + tz), &s1, &c1);
-            const double m = exp(att);
-            double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
-            double tr = (ar * twr - ai * twi) * m, tim = (ar * twi + ai * twr) * m;
-            atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], tr, tim);
-            continue;
+    if (g.wave) {
+        // MCfieldFourierQuadOnWave.m:281-302: pxL exp(i vsign k (tz - z_s)) twl atl,
+        // twl = sum_l Wl(l) exp(i vsign k l.x_L)
+        double twr = 0, twi = 0;
+        for (int l = 0; l < P.nl; ++l) {
+            double s2, c2;
+            sincos(g.vsign * P.k * dot<3>(P.l + 3 * l, xl), &s2, &c2);
+            const double2 w = g.wl[l];
+            twr += w.x * c2 - w.y * s2;
+            twi += w.x * s2 + w.y * c2;
         }
         double s1, c1;
-        sincos(-P.k * (-xs[2] + tz), &s1, &c1);
-        for (int l = 0; l < P.nl; ++l) {
-            const double *lc = P.l + 3 * l;
-            double ph = P.k * dot<3>(lc, xl);
-            double s2, c2;
-            sincos(ph, &s2, &c2);
-            const double m2 = exp(att);
-            // pxL * e1 * e2
-            double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
-            double br = m2 * c2, bi = m2 * s2;
-            double tr = ar * br - ai * bi, tim = ar * bi + ai * br;
-            atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], tr, tim);
-        }
+        sincos(g.vsign * P.k * (-xs[2] + tz), &s1, &c1);
+        const double m = exp(att);
+        double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
+        double tr = (ar * twr - ai * twi) * m, tim = (ar * twi + ai * twr) * m;
+        atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], tr, tim);
+        return;
+    }
+    double s1, c1;
+    sincos(-P.k * (-xs[2] + tz), &s1, &c1);
+    for (int l = 0; l < P.nl; ++l) {
+        const double *lc = P.l + 3 * l;
+        double ph = P.k * dot<3>(lc, xl);
+        double s2, c2;
+        sincos(ph, &s2, &c2);
+        const double m2 = exp(att);
+        // pxL * e1 * e2
+        double ar = pre * c1 - pim * s1, ai = pre * s1 + pim * c1;
+        double br = m2 * c2, bi = m2 * s2;
+        double tr = ar * br - ai * bi, tim = ar * bi + ai * br;
+        atomic_add_c(&grid[(((size_t)j * P.nl + l) * g.nx + iy) * g.nx + ix], tr, tim);
     }
 }
 
+// Pass 1 of a whole path lands in one cell: its scatter position is the
+// path's first vertex for every vertex.  The fused kernel therefore sums a
+// path's pass-1 terms in registers -- per vertex the light-side factor
+// amp e^{2 pi i xi} e^{i k l.x_n} att(x_n) (the wave: the Wl-weighted sum over
+// lights) -- and adds the sum times the first vertex's slab phase with one
+// atomic pair per light when the path ends (quad_flush_path).
+template <typename T>
+struct QuadPathSum {
+    T re[kQuadFusedLights], im[kQuadFusedLights];
+};
+
+template <typename T>
+__device__ __forceinline__ void quad_forward_term(const DevProblem &P, const QuadGeom &g, const double *xl, double amp,
+                                                  double xi, QuadPathSum<T> &S)
+{
+    const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
+    if constexpr (std::is_same<T, float>::value) {
+        const float2 ex = cis_cycles(xi);
+        const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
+        const float pre = (float)amp * m * ex.x, pim = (float)amp * m * ex.y;
+        if (g.wave) {
+            float twr = 0.f, twi = 0.f;
+            for (int l = 0; l < P.nl; ++l) {
+                const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+                const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
+                twr += wr * c2.x - wi * c2.y;
+                twi += wr * c2.y + wi * c2.x;
+            }
+            S.re[0] += pre * twr - pim * twi;
+            S.im[0] += pre * twi + pim * twr;
+            return;
+        }
+#pragma unroll
+        for (int l = 0; l < kQuadFusedLights; ++l)
+            if (l < P.nl) {
+                const float2 c2 = cis_cycles(dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
+                S.re[l] += pre * c2.x - pim * c2.y;
+                S.im[l] += pre * c2.y + pim * c2.x;
+            }
+        return;
+    }
+    double sx, cxv;
+    sincos(2 * kPi * xi, &sx, &cxv);
+    const double m = exp(att);
+    const double pre = amp * m * cxv, pim = amp * m * sx;
+    if (g.wave) {
+        double twr = 0, twi = 0;
+        for (int l = 0; l < P.nl; ++l) {
+            double s2, c2;
+            sincos(g.vsign * P.k * dot<3>(P.l + 3 * l, xl), &s2, &c2);
+            const double2 w = g.wl[l];
+            twr += w.x * c2 - w.y * s2;
+            twi += w.x * s2 + w.y * c2;
+        }
+        S.re[0] += pre * twr - pim * twi;
+        S.im[0] += pre * twi + pim * twr;
+        return;
+    }
+#pragma unroll
+    for (int l = 0; l < kQuadFusedLights; ++l)
+        if (l < P.nl) {
+            double s2, c2;
+            sincos(P.k * dot<3>(P.l + 3 * l, xl), &s2, &c2);
+            S.re[l] += pre * c2 - pim * s2;
+            S.im[l] += pre * s2 + pim * c2;
+        }
+}
+
+template <typename T>
+__device__ __forceinline__ void quad_flush_path(const DevProblem &P, const QuadGeom &g, const double *x1,
+                                                const QuadPathSum<T> &S, typename cx<T>::t *grid)
+{
+    const int j = slab_of(x1[2], g);
+    if (j < 0) return;
+    long ix = lround((x1[0] - g.gx0) / g.box_stp), iy = lround((x1[1] - g.gy0) / g.box_stp);
+    ix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > g.nx - 1 ? g.nx - 1 : iy);
+    const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
+    double er, ei;
+    if constexpr (std::is_same<T, float>::value) {
+        const float2 e1 = cis_cycles((g.wave ? g.vsign : -1.0) * (tz - x1[2]) * P.inv_lambda);
+        er = e1.x;
+        ei = e1.y;
+    } else {
+        sincos((g.wave ? g.vsign : -1.0) * P.k * (tz - x1[2]), &ei, &er);
+    }
+    const int nout = g.wave ? 1 : P.nl;
+#pragma unroll
+    for (int l = 0; l < kQuadFusedLights; ++l)
+        if (l < nout) {
+            const T sr = S.re[l], si = S.im[l];
+            atomic_add_c(&grid[(((size_t)j * nout + l) * g.nx + iy) * g.nx + ix], (double)(sr * (T)er - si * (T)ei),
+                         (double)(sr * (T)ei + si * (T)er));
+        }
+}
 
 // Vertex-parallel scatter over stored records (one lane per vertex): for
 // many lights per vertex, where the walk's divergence would multiply the
@@ -173,41 +255,61 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
     const PathRec pr = paths[p];
     double amp = sqrt(r.weight / pr.px);
     if (P.do_cbs && e != offsets[p]) amp *= 1 / sqrt(2.0);
-    quad_scatter_vertex<T>(P, g, r.x, pr.x1, amp, r.xi, grid);
+    quad_scatter_pass<T>(P, g, pr.x1, r.x, amp, r.xi, grid);
+    if (P.do_cbs) quad_scatter_pass<T>(P, g, r.x, pr.x1, amp, r.xi, grid);
 }
 
-// The walk and the scatter in one pass (one lane per path): every vertex is
-// scattered as the walk reaches it, so no vertex records are stored and no
-// counting pass is needed.  The lane follows trace_emit_kernel's walk
-// exactly (vertex n: position, weight and xi = the first uniform of draw n,
-// taken before the step); *nvert receives the vertex count (the
-// normaliser), one atomic per workgroup.
+// The walk and the scatter in one pass: every vertex is scattered as the walk
+// reaches it, so no vertex records are stored and no counting pass is needed.
+// Lanes stay resident and take a new path as soon as theirs ends
+// (take_path): a wave no longer waits for its longest walk.  A lane follows
+// trace_emit_kernel's walk exactly (vertex n: position, weight and xi = the
+// first uniform of draw n, taken before the step); *nvert receives the vertex
+// count (the normaliser), one atomic per workgroup.  *ctr (zeroed) hands out
+// the path indices [0, np).
 template <typename T>
 __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, QuadGeom g, uint64_t p0, int64_t np,
                                                                  typename cx<T>::t *grid,
-                                                                 unsigned long long *nvert)
+                                                                 unsigned long long *nvert, unsigned long long *ctr)
 {
     __shared__ unsigned long long ws[4];
     __shared__ double lcos[kIcdfLds];
     const double *tab = stage_icdf(P.amp, lcos);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cnt = 0;
-    if (i < np) {
-        const uint64_t path = P.path_offset + p0 + (uint64_t)i;
-        double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
-        double2 w0p;
-        path_start<3>(P, path, x, px, w, w0p);
-        const double x1[3] = {x[0], x[1], x[2]};
-        double weight = P.albedo;
-        uint32_t n = 0;
-        for (;;) {
+    WavePaths wp;
+    bool active = false;
+    uint64_t path = 0;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, x1[3] = {0, 0, 0}, px = 1, weight = 0;
+    uint32_t n = 0;
+    QuadPathSum<T> S;
+    for (;;) {
+        if (__ballot(!active)) {
+            const int64_t id = take_path(wp, !active, np, ctr);
+            if (id >= 0) {
+                path = P.path_offset + p0 + (uint64_t)id;
+                double2 w0p;
+                path_start<3>(P, path, x, px, w, w0p);
+                for (int j = 0; j < 3; ++j) x1[j] = x[j];
+                weight = P.albedo;
+                n = 0;
+#pragma unroll
+                for (int l = 0; l < kQuadFusedLights; ++l) S.re[l] = S.im[l] = 0;
+                active = true;
+            }
+        }
+        if (!__ballot(active)) break;
+        if (active) {
             ++n;
             const double2 a = uniform_pair(P.seed, path, n, 0);
             double amp = sqrt(weight / px);
             if (P.do_cbs && n > 1) amp *= 1 / sqrt(2.0);
-            quad_scatter_vertex<T>(P, g, x, x1, amp, a.x, grid);
+            quad_forward_term<T>(P, g, x, amp, a.x, S);
+            if (P.do_cbs) quad_scatter_pass<T>(P, g, x, x1, amp, a.x, grid);
             ++cnt;
-            if (!path_step<3>(P, path, n, x, w, ow, weight, a.y, tab)) break;
+            if (!path_step<3>(P, path, n, x, w, ow, weight, a.y, tab)) {
+                quad_flush_path<T>(P, g, x1, S, grid);
+                active = false;
+            }
         }
     }
     // workgroup sum of the vertex counts
@@ -317,16 +419,32 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
                            offsets, (double2 *)grid);
 }
 
+// Resident workgroups for a regenerating walk kernel: as many as fit on the
+// device at once (occupancy query), but no more than the paths need.
+static unsigned resident_blocks(const void *kernel, int64_t np)
+{
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0);
+    int64_t b = (int64_t)std::max(cus, 1) * std::max(per, 1);
+    const int64_t need = (np + 255) / 256;
+    return (unsigned)(b < need ? b : need);
+}
+
 void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
-                               unsigned long long *nvert, hipStream_t s)
+                               unsigned long long *nvert, unsigned long long *ctr, hipStream_t s)
 {
     if (np <= 0) return;
-    dim3 gr((unsigned)((np + 255) / 256));
-    if (fp32)
-        hipLaunchKernelGGL(quad_trace_scatter_kernel<float>, gr, dim3(256), 0, s, P, g, p0, np, (float2 *)grid, nvert);
-    else
-        hipLaunchKernelGGL(quad_trace_scatter_kernel<double>, gr, dim3(256), 0, s, P, g, p0, np, (double2 *)grid,
-                           nvert);
+    if (fp32) {
+        const void *k = (const void *)quad_trace_scatter_kernel<float>;
+        hipLaunchKernelGGL(quad_trace_scatter_kernel<float>, dim3(resident_blocks(k, np)), dim3(256), 0, s, P, g, p0,
+                           np, (float2 *)grid, nvert, ctr);
+    } else {
+        const void *k = (const void *)quad_trace_scatter_kernel<double>;
+        hipLaunchKernelGGL(quad_trace_scatter_kernel<double>, dim3(resident_blocks(k, np)), dim3(256), 0, s, P, g, p0,
+                           np, (double2 *)grid, nvert, ctr);
+    }
 }
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s)

@@ -173,20 +173,23 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                 g.wl = (const double2 *)dwl.p;
             }
             launch_zero(grid.p, (cs * ngrid + 15) / 16 * 16 > grid.bytes ? grid.bytes : (cs * ngrid + 15) / 16 * 16, s);
-            // At most 4 lights per vertex: walk and scatter in one pass (no
+            // At most kQuadFusedLights lights: walk and scatter in one pass (no
             // vertex records, no counting pass).  More: trace into records,
             // then a vertex-parallel scatter (in the fused pass the walk's
             // divergence would multiply the per-vertex light loop).
-            const bool fused = pr.nl <= 4;
+            const bool fused = pr.nl <= kQuadFusedLights;
             if (fused) {
-                if (!dcount.ensure(sizeof(unsigned long long), &rc)) break;
-                launch_zero(dcount.p, 16, s);
-                auto a = std::chrono::steady_clock::now();
+                // [0]: vertex count; [1 + launch]: the launch's path counter
                 const int64_t kLaunch = (int64_t)1 << 24;   // paths per launch
-                for (int64_t p0 = 0; p0 < max_itr; p0 += kLaunch) {
+                const int64_t nlaunch = (max_itr + kLaunch - 1) / kLaunch;
+                if (!dcount.ensure(sizeof(unsigned long long) * (size_t)(1 + nlaunch), &rc)) break;
+                launch_zero(dcount.p, sizeof(unsigned long long) * (size_t)(1 + nlaunch), s);
+                auto a = std::chrono::steady_clock::now();
+                for (int64_t p0 = 0, li = 0; p0 < max_itr; p0 += kLaunch, ++li) {
                     const int64_t np = std::min<int64_t>(max_itr - p0, kLaunch);
                     launch_quad_trace_scatter(ps.dp, g, (uint64_t)p0, np, grid.p, fp32,
-                                              (unsigned long long *)dcount.p, s);
+                                              (unsigned long long *)dcount.p, (unsigned long long *)dcount.p + 1 + li,
+                                              s);
                     st.paths += np;
                     st.chunks += 1;
                 }
