@@ -40,7 +40,18 @@ void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t 
                                unsigned long long *nvert, unsigned long long *ctr, hipStream_t s);
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s);
-void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,
+// The view grid of MCfieldFourierQuad.m:80-138 as the combine kernel needs
+// it: slab phases zexp0, the box factor boxexp0 and the final scale.
+struct QuadView {
+    double v_max, v_stp, vsign, sigt, lambda, zmin, zmax;
+    double scale;   // sqrt(V sigt / stored vertices), MCfieldFourierQuad.m:302-303
+    int wave;       // MCfieldFourierQuadOnWave.m:143 slab-phase sign
+};
+// W[ix * nv0 + o] = tw[((o + nva_pad - a)(ix + nx_pad - h)) mod n], the DFT
+// matrix with the shifts, flips and crop folded in (tw: n twiddles e^{2 pi i r/n}).
+void launch_quad_dft_matrix(const QuadGeom &g, const double2 *tw, void *W, bool fp32, hipStream_t s);
+// u[o, l] = scale * boxexp0[o] * sum_s H[s, l][o] exp(zexp0[o] tz_s)
+void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double *gz, const QuadView &v, double2 *u,
                          bool fp32, hipStream_t s);
 
 } // namespace scmc

@@ -379,30 +379,70 @@ __global__ __launch_bounds__(256) void cgemm_kernel(int M, int N, int K, const t
         }
 }
 
-// u[o, l] += sum_s H[s, l][o] * exp(zexp0[o] * tz_s)   (MCfieldFourierQuad.m:281-282)
+// u[o, l] = scale * boxexp0[o] * sum_s H[s, l][o] * exp(zexp0[o] * tz_s)
+// (MCfieldFourierQuad.m:124-138, 281-282, 291-303), one thread per output
+// pixel o = i + nv0 j and plane l; vx, vy, vz of MCfieldFourierQuad.m:103-105.
 template <typename T>
-__global__ void quad_combine_kernel(int nv0, int nl, int nz, const typename cx<T>::t *H, const double2 *zexp0,
-                                    const double *gz, double2 *u)
+__global__ void quad_combine_kernel(int nv0, int nl, int nz, const typename cx<T>::t *H, const double *gz, QuadView v,
+                                    double2 *u)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nvv = (int64_t)nv0 * nv0;
     if (idx >= nvv * nl) return;
     const int64_t o = idx % nvv;
     const int l = (int)(idx / nvv);
-    const double2 ze = zexp0[o];
+    const int i = (int)(o % nv0), j = (int)(o / nv0);
+    const double vx = -(-v.v_max + i * v.v_stp), vy = -(-v.v_max + j * v.v_stp);
+    const double vz = v.vsign * sqrt(1 - vx * vx - vy * vy);
+    const double dir = vz < 0 ? 1 / fabs(vz) : 0, idir = vz > 0 ? 1 / fabs(vz) : 0;
+    // zexp0: MCfieldFourierQuad.m:134 (-) / MCfieldFourierQuadOnWave.m:143 (+)
+    const double zr = -v.sigt / 2 * (dir - idir);
+    const double zi = (v.wave ? 2 : -2) * kPi / v.lambda * vz * v.vsign;
     double ur = 0, ui = 0;
     for (int s = 0; s < nz - 1; ++s) {
         const double tz = 0.5 * (gz[s] + gz[s + 1]);
         const typename cx<T>::t h = H[((size_t)s * nl + l) * nvv + o];
-        const double m = exp(ze.x * tz);
+        const double m = exp(zr * tz);
         double sn, cs;
-        sincos(ze.y * tz, &sn, &cs);
+        sincos(zi * tz, &sn, &cs);
         const double er = m * cs, ei = m * sn;
         ur += (double)h.x * er - (double)h.y * ei;
         ui += (double)h.x * ei + (double)h.y * er;
     }
-    u[idx].x += ur;
-    u[idx].y += ui;
+    // boxexp0 (MCfieldFourierQuad.m:137-138)
+    const double bm = exp(-v.sigt / 2 * (-v.zmin * dir + v.zmax * idir)) * v.scale;
+    double bs, bc;
+    sincos(2 * kPi / v.lambda * (-v.zmin * (vz < 0) * vz + v.zmax * (vz > 0) * vz), &bs, &bc);
+    const double br = bm * bc, bi = bm * bs;
+    u[idx].x = ur * br - ui * bi;
+    u[idx].y = ur * bi + ui * br;
+}
+
+template <typename T>
+__global__ void quad_dft_matrix_kernel(int nv0, int nx, int n, int nva_pad, int nx_pad, const double2 *tw,
+                                       typename cx<T>::t *W)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nv0 * nx) return;
+    const int o = (int)(idx % nv0), ix = (int)(idx / nv0);
+    const int h = n / 2, am = n - 1 - h;
+    const int64_t f = o + nva_pad - am, q = ix + nx_pad - h;
+    const int64_t r = ((f * q) % n + n) % n;
+    const double2 t = tw[r];
+    W[idx].x = (T)t.x;
+    W[idx].y = (T)t.y;
+}
+
+void launch_quad_dft_matrix(const QuadGeom &g, const double2 *tw, void *W, bool fp32, hipStream_t s)
+{
+    const int64_t n = (int64_t)g.nv0 * g.nx;
+    dim3 gr((unsigned)((n + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(quad_dft_matrix_kernel<float>, gr, dim3(256), 0, s, g.nv0, g.nx, g.nfft, g.nva_pad, g.nx_pad,
+                           tw, (float2 *)W);
+    else
+        hipLaunchKernelGGL(quad_dft_matrix_kernel<double>, gr, dim3(256), 0, s, g.nv0, g.nx, g.nfft, g.nva_pad,
+                           g.nx_pad, tw, (double2 *)W);
 }
 
 void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
@@ -458,17 +498,16 @@ void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B,
                            (const double2 *)B, sb, (double2 *)C, sc, transB);
 }
 
-void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double2 *zexp0, const double *gz, double2 *u,
+void launch_quad_combine(int nv0, int nl, int nz, const void *H, const double *gz, const QuadView &v, double2 *u,
                          bool fp32, hipStream_t s)
 {
     const int64_t n = (int64_t)nv0 * nv0 * nl;
     dim3 gr((unsigned)((n + 255) / 256));
     if (fp32)
-        hipLaunchKernelGGL(quad_combine_kernel<float>, gr, dim3(256), 0, s, nv0, nl, nz, (const float2 *)H, zexp0, gz,
-                           u);
+        hipLaunchKernelGGL(quad_combine_kernel<float>, gr, dim3(256), 0, s, nv0, nl, nz, (const float2 *)H, gz, v, u);
     else
-        hipLaunchKernelGGL(quad_combine_kernel<double>, gr, dim3(256), 0, s, nv0, nl, nz, (const double2 *)H, zexp0,
-                           gz, u);
+        hipLaunchKernelGGL(quad_combine_kernel<double>, gr, dim3(256), 0, s, nv0, nl, nz, (const double2 *)H, gz, v,
+                           u);
 }
 
 } // namespace scmc

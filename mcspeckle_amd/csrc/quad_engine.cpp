@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace scmc {
@@ -50,10 +51,13 @@ int quad_grid_setup(const double *box_min, const double *box_max, double v_max, 
     g.nx_pad = std::max((g.nv0 - g.nx) / 2, 0);
     g.nva_pad = std::max((g.nx - g.nv0) / 2, 0);
     g.nfft = g.nx + 2 * g.nx_pad;
-    // MCfieldFourierQuad.m:108-121: z_bin from the widest view of the grid
+    // MCfieldFourierQuad.m:108-121: z_bin from the widest view of the grid,
+    // max(1 - |vz|) over the grid.  |vx| = |-v_max + i v_stp| is largest at
+    // i = 0 or nv0 - 1 (a rounded sum is monotone in each term), and so are
+    // vx^2 + vy^2 and 1 - |vz|: the four corners give the maximum exactly.
     double worst = 0;
-    for (int i = 0; i < g.nv0; ++i)
-        for (int j = 0; j < g.nv0; ++j) {
+    for (int i : {0, g.nv0 - 1})
+        for (int j : {0, g.nv0 - 1}) {
             double vx = -(-v_max + i * v_stp), vy = -(-v_max + j * v_stp);
             double q = 1 - fabs(sqrt(1 - vx * vx - vy * vy));
             worst = std::max(worst, q);
@@ -103,6 +107,23 @@ int scmc_quad_grid_wave(const double box_min[3], const double box_max[3], double
     return quad_grid_query(box_min, box_max, v_max, v_stp, lambda, true, nv0, nx, nz);
 }
 
+// Device buffers and the stream of the Fourier solves, kept from call to
+// call (they only grow): the slab grids and transforms of a 511^2 problem are
+// ~0.4 GB, and allocating and freeing them per call cost ~2 ms of device-wide
+// waits.  Calls in one process take turns (mu).  Never destroyed: the HIP
+// runtime may be gone by the time static destructors run.
+struct QuadWorkspace {
+    std::mutex mu;
+    hipStream_t s = nullptr;
+    ProblemState ps;
+    DevBuf dgz, grid, W, tw, T1, H, u, dwl, dcount;
+};
+static QuadWorkspace *quad_workspace()
+{
+    static QuadWorkspace *w = new QuadWorkspace();
+    return w;
+}
+
 // MCfieldFourierQuad.m:1-303, or with wl_re MCfieldFourierQuadOnWave.m:1-337.
 static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, const double *wl_im, double v_max,
                             double v_stp, int64_t max_itr, double vsign, int32_t precision, double *u_re, double *u_im,
@@ -143,13 +164,16 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
     const double lz = pr.l[3 * (pr.nl - 1) + 2];   // l(end)
     g.l_diratt = lz > 0 ? 1 / fabs(lz) : 0;
     g.l_idiratt = lz < 0 ? 1 / fabs(lz) : 0;
-    hipStream_t s;
-    if (!hip_ok(hipStreamCreate(&s), "stream", &rc)) return rc;
+    QuadWorkspace &ws = *quad_workspace();
+    std::lock_guard<std::mutex> lock(ws.mu);
+    if (!ws.s && !hip_ok(hipStreamCreate(&ws.s), "stream", &rc)) return rc;
+    hipStream_t s = ws.s;
     scmc_stats st;
     memset(&st, 0, sizeof st);
     {
-        ProblemState ps;
-        DevBuf dgz, grid, W, T1, H, zexp, u, dwl, dcount;
+        ProblemState &ps = ws.ps;
+        DevBuf &dgz = ws.dgz, &grid = ws.grid, &W = ws.W, &tw = ws.tw, &T1 = ws.T1, &H = ws.H, &u = ws.u,
+               &dwl = ws.dwl, &dcount = ws.dcount;
         const int nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
         const int nl = wave ? 1 : pr.nl;   // output planes
         const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
@@ -157,7 +181,9 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
         do {
             if ((rc = ps.setup(&pr, s))) break;
             if (!dgz.ensure(sizeof(double) * gz.size(), &rc) || !grid.ensure(cs * ngrid, &rc)) break;
-            if (!hip_ok(hipMemcpy(dgz.p, gz.data(), sizeof(double) * gz.size(), hipMemcpyHostToDevice), "gz", &rc)) break;
+            if (!hip_ok(hipMemcpyAsync(dgz.p, gz.data(), sizeof(double) * gz.size(), hipMemcpyHostToDevice, s), "gz",
+                        &rc))
+                break;
             g.gz = (const double *)dgz.p;
             if (wave) {
                 std::vector<double> hw((size_t)pr.nl * 2);
@@ -166,7 +192,9 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                     hw[2 * (size_t)l + 1] = wl_im ? wl_im[l] : 0.0;
                 }
                 if (!dwl.ensure(sizeof(double) * hw.size(), &rc) ||
-                    !hip_ok(hipMemcpy(dwl.p, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice), "wl", &rc))
+                    !hip_ok(hipMemcpyAsync(dwl.p, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, s),
+                            "wl", &rc) ||
+                    !hip_ok(hipStreamSynchronize(s), "wl sync", &rc))
                     break;
                 g.wave = 1;
                 g.vsign = vsign;
@@ -226,80 +254,62 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             if (rc) break;
             // DFT matrix of fliplr(flipud(fftshift(fft2(ifftshift(.))))) and the crop:
             // W[o][ix] = exp(+2 pi i ((o + nva_pad - a)(ix + nx_pad - h) mod n)/n),
-            // h = floor(n/2) (the shift of fftshift/ifftshift), a = n - 1 - h
-            const int n = g.nfft, h = n / 2, am = n - 1 - h;
-            std::vector<double> w((size_t)nv0 * nx * 2);
-            for (int ix = 0; ix < nx; ++ix)
-                for (int o = 0; o < nv0; ++o) {
-                    long long f = o + g.nva_pad - am, q = ix + g.nx_pad - h;
-                    long long r = ((f * q) % n + n) % n;
-                    double ang = 2 * M_PI * (double)r / n;
-                    w[2 * ((size_t)ix * nv0 + o)] = cos(ang);
-                    w[2 * ((size_t)ix * nv0 + o) + 1] = sin(ang);
-                }
-            if (!W.ensure(cs * (size_t)nv0 * nx, &rc) || !T1.ensure(cs * (size_t)nsl * nl * nv0 * nx, &rc) ||
-                !H.ensure(cs * (size_t)nsl * nl * nv0 * nv0, &rc) ||
-                !zexp.ensure(sizeof(double2) * (size_t)nv0 * nv0, &rc) ||
+            // h = floor(n/2) (the shift of fftshift/ifftshift), a = n - 1 - h;
+            // built on the device from the n twiddles
+            const int n = g.nfft;
+            std::vector<double> htw((size_t)n * 2);
+            for (int r = 0; r < n; ++r) {
+                const double ang = 2 * M_PI * (double)r / n;
+                htw[2 * (size_t)r] = cos(ang);
+                htw[2 * (size_t)r + 1] = sin(ang);
+            }
+            if (!W.ensure(cs * (size_t)nv0 * nx, &rc) || !tw.ensure(sizeof(double) * htw.size(), &rc) ||
+                !T1.ensure(cs * (size_t)nsl * nl * nv0 * nx, &rc) || !H.ensure(cs * (size_t)nsl * nl * nv0 * nv0, &rc) ||
                 !u.ensure(sizeof(double2) * (size_t)nv0 * nv0 * nl, &rc))
                 break;
-            if (fp32) {
-                std::vector<float> wf(w.begin(), w.end());
-                if (!hip_ok(hipMemcpy(W.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice), "W", &rc)) break;
-            } else if (!hip_ok(hipMemcpy(W.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice), "W", &rc)) {
-                break;
-            }
-            // zexp0 and boxexp0 (MCfieldFourierQuad.m:124-138)
-            std::vector<double> ze((size_t)nv0 * nv0 * 2);
-            std::vector<std::complex<double>> be((size_t)nv0 * nv0);
-            for (int j = 0; j < nv0; ++j)
-                for (int i = 0; i < nv0; ++i) {
-                    double vx = -(-v_max + i * v_stp), vy = -(-v_max + j * v_stp);
-                    double vz = vsign * sqrt(1 - vx * vx - vy * vy);
-                    double dir = vz < 0 ? 1 / fabs(vz) : 0, idir = vz > 0 ? 1 / fabs(vz) : 0;
-                    size_t o = (size_t)i + (size_t)nv0 * j;
-                    ze[2 * o] = -pr.sigt / 2 * (dir - idir);
-                    // MCfieldFourierQuad.m:134 (-) / MCfieldFourierQuadOnWave.m:143 (+)
-                    ze[2 * o + 1] = (wave ? 2 : -2) * M_PI / pr.lambda * vz * vsign;
-                    be[o] = exp(-pr.sigt / 2 * (-bmin[2] * dir + bmax[2] * idir)) *
-                            std::exp(std::complex<double>(0, 2 * M_PI / pr.lambda *
-                                                                 (-bmin[2] * (vz < 0) * vz + bmax[2] * (vz > 0) * vz)));
-                }
-            if (!hip_ok(hipMemcpy(zexp.p, ze.data(), sizeof(double) * ze.size(), hipMemcpyHostToDevice), "zexp", &rc))
+            if (!hip_ok(hipMemcpyAsync(tw.p, htw.data(), sizeof(double) * htw.size(), hipMemcpyHostToDevice, s), "tw",
+                        &rc))
                 break;
             auto a = std::chrono::steady_clock::now();
+            launch_quad_dft_matrix(g, (const double2 *)tw.p, W.p, fp32, s);
             // T1[b] = W (nv0 x nx) * X[b] (nx x nx);  H[b] = T1[b] * W^T
             launch_cgemm(nv0, nx, nx, W.p, 0, grid.p, (int64_t)nx * nx, T1.p, (int64_t)nv0 * nx, nsl * nl, false, fp32, s);
             launch_cgemm(nv0, nv0, nx, T1.p, (int64_t)nv0 * nx, W.p, 0, H.p, (int64_t)nv0 * nv0, nsl * nl, true, fp32, s);
-            launch_zero(u.p, sizeof(double2) * (size_t)nv0 * nv0 * nl, s);
-            launch_quad_combine(nv0, nl, g.nz, H.p, (const double2 *)zexp.p, g.gz, (double2 *)u.p, fp32, s);
+            // slab phases exp(zexp0 tz), boxexp0 (MCfieldFourierQuad.m:124-138,
+            // 281-282) and MCfieldFourierQuad.m:302-303 (maxItr = stored
+            // vertices), per output pixel on the device
+            QuadView view;
+            view.v_max = v_max;
+            view.v_stp = v_stp;
+            view.vsign = vsign;
+            view.sigt = pr.sigt;
+            view.lambda = pr.lambda;
+            view.zmin = bmin[2];
+            view.zmax = bmax[2];
+            view.wave = wave ? 1 : 0;
+            const double bwn = g.nx * g.box_stp;
+            const double V = bwn * bwn * (bmax[2] - bmin[2]);
+            view.scale = nstore ? sqrt(1.0 / (double)nstore * V * pr.sigt) : 0.0;
+            launch_quad_combine(nv0, nl, g.nz, H.p, g.gz, view, (double2 *)u.p, fp32, s);
             std::vector<double> hu((size_t)nv0 * nv0 * nl * 2);
             if (!hip_ok(hipMemcpyAsync(hu.data(), u.p, sizeof(double) * hu.size(), hipMemcpyDeviceToHost, s), "copy u",
                         &rc) ||
                 !hip_ok(hipStreamSynchronize(s), "quad sync", &rc))
                 break;
             st.ms_nee += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-            // boxexp0 and MCfieldFourierQuad.m:302-303 (maxItr = stored vertices)
-            const double bwn = g.nx * g.box_stp;
-            const double V = bwn * bwn * (bmax[2] - bmin[2]);
-            const double sc = nstore ? sqrt(1.0 / (double)nstore * V * pr.sigt) : 0.0;
-            const size_t nvv = (size_t)nv0 * nv0;
-            for (int l = 0; l < nl; ++l)
-                for (size_t o = 0; o < nvv; ++o) {
-                    size_t k = o + nvv * l;
-                    std::complex<double> z(hu[2 * k], hu[2 * k + 1]);
-                    z *= be[o];
-                    z *= sc;
-                    if (u_im) {
-                        u_re[k] = z.real();
-                        u_im[k] = z.imag();
-                    } else {
-                        u_re[2 * k] = z.real();
-                        u_re[2 * k + 1] = z.imag();
-                    }
+            const size_t ne = (size_t)nv0 * nv0 * nl;
+            if (u_im) {
+                for (size_t k = 0; k < ne; ++k) {
+                    u_re[k] = hu[2 * k];
+                    u_im[k] = hu[2 * k + 1];
                 }
+            } else {
+                memcpy(u_re, hu.data(), sizeof(double) * 2 * ne);
+            }
         } while (0);
+        // an error leaves queued work behind: finish it before the next call
+        if (rc) (void)hipStreamSynchronize(s);
     }
-    (void)hipStreamDestroy(s);
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
     return rc;
