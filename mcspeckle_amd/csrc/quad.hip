@@ -445,6 +445,180 @@ void launch_quad_dft_matrix(const QuadGeom &g, const double2 *tw, void *W, bool 
                            g.nx_pad, tw, (double2 *)W);
 }
 
+// One pass of the slab transform as a factored DFT (four-step Cooley-Tukey):
+// for n = n1 n2, x[q1 + n1 q2] -> Y[f2 + n2 f1] =
+//   sum_q1 w^(q1 f2) w1^(q1 f1) sum_q2 w2^(q2 f2) x[q1 + n1 q2],
+// w = e^{2 pi i/n}, w1 = w^n2, w2 = w^n1: n (n1 + n2) products per vector
+// instead of the n^2 of the DFT matrix.  Vector j of batch b holds nin input
+// elements at in[b in_bs + j in_vs + e in_es], placed at q = (e + qoff) mod n
+// (the other positions are zero); output o < nout is Y[(o + foff) mod n],
+// stored at out[b out_bs + j out_vs + o out_es].  With qoff = nx_pad - h and
+// foff = nva_pad - a this is the row (or column) transform of H = W X W^T with
+// W[o][ix] = w^((o + nva_pad - a)(ix + nx_pad - h) mod n).
+// A workgroup takes nv consecutive vectors of one batch in LDS (two buffers of
+// nv x n) with the n twiddles.
+struct DftPass {
+    int n, n1, n2, nin, nout, nvec, nv, qoff, foff;
+    int64_t in_bs, in_vs, in_es, out_bs, out_vs, out_es;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void dft_pass_kernel(DftPass d, const typename cx<T>::t *__restrict__ in,
+                                                       typename cx<T>::t *__restrict__ out,
+                                                       const double2 *__restrict__ tw)
+{
+    typedef typename cx<T>::t C2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char dft_lds[];
+    C2 *tws = (C2 *)dft_lds;          // n twiddles
+    C2 *xa = tws + d.n;               // nv x n: input, then stage 2 output
+    C2 *xb = xa + (size_t)d.nv * d.n; // nv x n: stage 1 output
+    const int b = blockIdx.y, j0 = blockIdx.x * d.nv;
+    const int nv = min(d.nv, d.nvec - j0);
+    const int n = d.n, n1 = d.n1, n2 = d.n2;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        tws[r].x = (T)tw[r].x;
+        tws[r].y = (T)tw[r].y;
+    }
+    for (int e = threadIdx.x; e < d.nv * n; e += blockDim.x) {
+        xa[e].x = 0;
+        xa[e].y = 0;
+    }
+    __syncthreads();
+    // load: consecutive threads read consecutive addresses
+    const C2 *ib = in + (size_t)b * d.in_bs;
+    for (int t = threadIdx.x; t < nv * d.nin; t += blockDim.x) {
+        int v, e;
+        if (d.in_es == 1) {
+            v = t / d.nin;
+            e = t - v * d.nin;
+        } else {
+            e = t / nv;
+            v = t - e * nv;
+        }
+        int q = (e + d.qoff) % n;
+        if (q < 0) q += n;
+        xa[v * n + q] = ib[(size_t)(j0 + v) * d.in_vs + (size_t)e * d.in_es];
+    }
+    __syncthreads();
+    // stage 1: Z[q1 n2 + f2] = w^(q1 f2) sum_q2 w2^(q2 f2) x[q1 + n1 q2]
+    for (int t = threadIdx.x; t < nv * n; t += blockDim.x) {
+        const int v = t / n, k = t - v * n, q1 = k / n2, f2 = k - q1 * n2;
+        const C2 *x = xa + v * n;
+        T sr = 0, si = 0;
+        int r = 0;   // (q2 f2) mod n2
+        for (int q2 = 0; q2 < n2; ++q2) {
+            const C2 w = tws[r * n1];
+            const C2 a = x[q1 + n1 * q2];
+            sr += a.x * w.x - a.y * w.y;
+            si += a.x * w.y + a.y * w.x;
+            r += f2;
+            if (r >= n2) r -= n2;
+        }
+        const C2 w = tws[(q1 * f2) % n];
+        xb[v * n + k].x = sr * w.x - si * w.y;
+        xb[v * n + k].y = sr * w.y + si * w.x;
+    }
+    __syncthreads();
+    // stage 2: Y[f2 + n2 f1] = sum_q1 w1^(q1 f1) Z[q1 n2 + f2]
+    for (int t = threadIdx.x; t < nv * n; t += blockDim.x) {
+        const int v = t / n, k = t - v * n, f1 = k / n2, f2 = k - f1 * n2;
+        const C2 *z = xb + v * n;
+        T sr = 0, si = 0;
+        int r = 0;   // (q1 f1) mod n1
+        for (int q1 = 0; q1 < n1; ++q1) {
+            const C2 w = tws[r * n2];
+            const C2 a = z[q1 * n2 + f2];
+            sr += a.x * w.x - a.y * w.y;
+            si += a.x * w.y + a.y * w.x;
+            r += f1;
+            if (r >= n1) r -= n1;
+        }
+        xa[v * n + f2 + n2 * f1].x = sr;
+        xa[v * n + f2 + n2 * f1].y = si;
+    }
+    __syncthreads();
+    C2 *ob = out + (size_t)b * d.out_bs;
+    for (int t = threadIdx.x; t < nv * d.nout; t += blockDim.x) {
+        int v, o;
+        if (d.out_es == 1) {
+            v = t / d.nout;
+            o = t - v * d.nout;
+        } else {
+            o = t / nv;
+            v = t - o * nv;
+        }
+        int f = (o + d.foff) % n;
+        if (f < 0) f += n;
+        ob[(size_t)(j0 + v) * d.out_vs + (size_t)o * d.out_es] = xa[v * n + f];
+    }
+}
+
+// n = n1 n2 with n1 + n2 smallest (n1 >= n2 >= 2); false if n is prime or the
+// factored form (n (n1 + n2) products per vector against n^2, with more index
+// work per product) saves less than a factor 3.
+bool dft_factor(int n, int *n1, int *n2)
+{
+    int best = 0;
+    for (int f = 2; f * f <= n; ++f)
+        if (n % f == 0) best = f;
+    if (!best) return false;
+    *n2 = best;
+    *n1 = n / best;
+    return 3 * (*n1 + *n2) <= n;
+}
+
+// H[b] = W X[b] W^T for every slab (batch b) by two factored-DFT passes:
+// along ix into T (nv0 x nx per batch), then along iy into H (nv0 x nv0).
+// Returns false (nothing launched) when n does not factor usefully.
+bool launch_quad_dft2(const QuadGeom &g, int batch, const void *X, void *T1, void *H, const double2 *tw, bool fp32,
+                      hipStream_t s)
+{
+    int n1, n2;
+    const int n = g.nfft;
+    if (!dft_factor(n, &n1, &n2)) return false;
+    const int h = n / 2, am = n - 1 - h;
+    const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+    // two nv x n buffers and the twiddles within 64 KB of LDS
+    int nv = (int)((65536 / cs - n) / (2 * (size_t)n));
+    nv = std::max(1, std::min(nv, 16));
+    const size_t lds = cs * ((size_t)n + 2 * (size_t)nv * n);
+    DftPass a;
+    a.n = n;
+    a.n1 = n1;
+    a.n2 = n2;
+    a.qoff = g.nx_pad - h;
+    a.foff = g.nva_pad - am;
+    a.nv = nv;
+    // pass 1: vectors = columns iy of X (ix contiguous), out T[o + nv0 iy]
+    a.nin = g.nx;
+    a.nout = g.nv0;
+    a.nvec = g.nx;
+    a.in_bs = (int64_t)g.nx * g.nx;
+    a.in_vs = g.nx;
+    a.in_es = 1;
+    a.out_bs = (int64_t)g.nv0 * g.nx;
+    a.out_vs = g.nv0;
+    a.out_es = 1;
+    DftPass c = a;
+    // pass 2: vectors = rows o of T (iy strided by nv0), out H[o + nv0 o']
+    c.nvec = g.nv0;
+    c.in_bs = (int64_t)g.nv0 * g.nx;
+    c.in_vs = 1;
+    c.in_es = g.nv0;
+    c.out_bs = (int64_t)g.nv0 * g.nv0;
+    c.out_vs = 1;
+    c.out_es = g.nv0;
+    dim3 ga((unsigned)((a.nvec + nv - 1) / nv), (unsigned)batch), gc((unsigned)((c.nvec + nv - 1) / nv), (unsigned)batch);
+    if (fp32) {
+        hipLaunchKernelGGL(dft_pass_kernel<float>, ga, dim3(256), lds, s, a, (const float2 *)X, (float2 *)T1, tw);
+        hipLaunchKernelGGL(dft_pass_kernel<float>, gc, dim3(256), lds, s, c, (const float2 *)T1, (float2 *)H, tw);
+    } else {
+        hipLaunchKernelGGL(dft_pass_kernel<double>, ga, dim3(256), lds, s, a, (const double2 *)X, (double2 *)T1, tw);
+        hipLaunchKernelGGL(dft_pass_kernel<double>, gc, dim3(256), lds, s, c, (const double2 *)T1, (double2 *)H, tw);
+    }
+    return true;
+}
+
 void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, const PathRec *paths,
                          const VertexRec *verts, const int32_t *vpath, const int64_t *offsets, void *grid, bool fp32,
                          hipStream_t s)

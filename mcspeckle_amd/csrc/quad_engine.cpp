@@ -117,6 +117,8 @@ struct QuadWorkspace {
     hipStream_t s = nullptr;
     ProblemState ps;
     DevBuf dgz, grid, W, tw, T1, H, u, dwl, dcount;
+    double *host_u = nullptr;   // pinned staging of the result (a pageable copy ran at ~2 GB/s)
+    size_t host_u_bytes = 0;
 };
 static QuadWorkspace *quad_workspace()
 {
@@ -271,10 +273,15 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                         &rc))
                 break;
             auto a = std::chrono::steady_clock::now();
-            launch_quad_dft_matrix(g, (const double2 *)tw.p, W.p, fp32, s);
-            // T1[b] = W (nv0 x nx) * X[b] (nx x nx);  H[b] = T1[b] * W^T
-            launch_cgemm(nv0, nx, nx, W.p, 0, grid.p, (int64_t)nx * nx, T1.p, (int64_t)nv0 * nx, nsl * nl, false, fp32, s);
-            launch_cgemm(nv0, nv0, nx, T1.p, (int64_t)nv0 * nx, W.p, 0, H.p, (int64_t)nv0 * nv0, nsl * nl, true, fp32, s);
+            // H[b] = W X[b] W^T: factored DFT passes when n factors (513 =
+            // 27 x 19), else T1[b] = W (nv0 x nx) X[b] (nx x nx), H[b] = T1[b] W^T
+            if (!launch_quad_dft2(g, nsl * nl, grid.p, T1.p, H.p, (const double2 *)tw.p, fp32, s)) {
+                launch_quad_dft_matrix(g, (const double2 *)tw.p, W.p, fp32, s);
+                launch_cgemm(nv0, nx, nx, W.p, 0, grid.p, (int64_t)nx * nx, T1.p, (int64_t)nv0 * nx, nsl * nl, false,
+                             fp32, s);
+                launch_cgemm(nv0, nv0, nx, T1.p, (int64_t)nv0 * nx, W.p, 0, H.p, (int64_t)nv0 * nv0, nsl * nl, true,
+                             fp32, s);
+            }
             // slab phases exp(zexp0 tz), boxexp0 (MCfieldFourierQuad.m:124-138,
             // 281-282) and MCfieldFourierQuad.m:302-303 (maxItr = stored
             // vertices), per output pixel on the device
@@ -291,20 +298,29 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             const double V = bwn * bwn * (bmax[2] - bmin[2]);
             view.scale = nstore ? sqrt(1.0 / (double)nstore * V * pr.sigt) : 0.0;
             launch_quad_combine(nv0, nl, g.nz, H.p, g.gz, view, (double2 *)u.p, fp32, s);
-            std::vector<double> hu((size_t)nv0 * nv0 * nl * 2);
-            if (!hip_ok(hipMemcpyAsync(hu.data(), u.p, sizeof(double) * hu.size(), hipMemcpyDeviceToHost, s), "copy u",
+            const size_t ne = (size_t)nv0 * nv0 * nl;
+            if (ws.host_u_bytes < sizeof(double) * 2 * ne) {
+                if (ws.host_u) (void)hipHostFree(ws.host_u);
+                ws.host_u = nullptr;
+                ws.host_u_bytes = 0;
+                if (!hip_ok(hipHostMalloc((void **)&ws.host_u, sizeof(double) * 2 * ne, hipHostMallocDefault),
+                            "pinned result", &rc))
+                    break;
+                ws.host_u_bytes = sizeof(double) * 2 * ne;
+            }
+            const double *hu = ws.host_u;
+            if (!hip_ok(hipMemcpyAsync(ws.host_u, u.p, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy u",
                         &rc) ||
                 !hip_ok(hipStreamSynchronize(s), "quad sync", &rc))
                 break;
             st.ms_nee += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-            const size_t ne = (size_t)nv0 * nv0 * nl;
             if (u_im) {
                 for (size_t k = 0; k < ne; ++k) {
                     u_re[k] = hu[2 * k];
                     u_im[k] = hu[2 * k + 1];
                 }
             } else {
-                memcpy(u_re, hu.data(), sizeof(double) * 2 * ne);
+                memcpy(u_re, hu, sizeof(double) * 2 * ne);
             }
         } while (0);
         // an error leaves queued work behind: finish it before the next call

@@ -34,6 +34,11 @@ CASES = [
     # six lights: stored vertex records and the vertex-parallel scatter
     # (up to four, the walk and the scatter run in one pass)
     (True, 0.3, 0.05, 0.5, 2.0, 0.6, -1.0, 6, 2),
+    # transforms that factor (the slab DFT in two LDS passes instead of the
+    # DFT-matrix GEMMs): 63 = 9 x 7 with the cell grid padded (13 cells), and
+    # 77 = 11 x 7 with the transform cropped to the 63 views
+    (True, 0.31, 0.01, 0.5, 4.0, 0.6, -1.0, 2, 2),
+    (False, 0.31, 0.01, 0.5, 30.0, 0.6, 1.0, 1, 2),
 ]
 
 

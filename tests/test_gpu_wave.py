@@ -87,6 +87,9 @@ QUAD_CASES = [
     (True, 0.325, 0.05, 0.5, 2.0, 0.6, -1.0, 2),
     (True, 0.25, 0.01, 0.5, 4.0, 0.5, 1.0, 1),
     (False, 0.325, 0.05, 0.5, 3.0, 0.8, 1.0, 2),
+    # even transforms that factor (two LDS passes): 64 = 8 x 8 padded, 80 = 10 x 8 cropped
+    (True, 0.315, 0.01, 0.5, 4.0, 0.6, -1.0, 2),
+    (False, 0.315, 0.01, 0.5, 30.0, 0.5, 1.0, 2),
 ]
 
 
