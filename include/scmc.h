@@ -249,7 +249,11 @@ int scmc_mccov_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t ma
 int scmc_mcfield_replay(const scmc_problem *pr, const scmc_pathset *ps, int64_t max_itr, int32_t precision,
                         double *u_re, double *u_im, scmc_stats *stats);
 
-/* ---- serving: device-resident continuous batching ---------------------- */
+/* ---- serving: continuous batching of covariance and field solves -------- */
+/* A host scheduler with max_active slots: each admitted request runs on its
+ * slot's own HIP streams, so the work of all admitted requests overlaps on the
+ * GPU; requests are admitted in arrival order as slots and HBM budget free
+ * up (a later request that fits may pass one that does not). */
 
 typedef struct {
     int32_t device;          /* HIP device ordinal                             */
@@ -266,9 +270,14 @@ int scmc_server_stop(void);
 /* Submit an MCcov request (mixed or FP64).  The problem's arrays are copied
  * before return.  *request_id receives the handle. */
 int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id);
+/* Submit an MCfield request (MCfield.m:1-254, mixed or FP64); its field u
+ * (nv x nl, layout as scmc_mcfield) comes back in the first output pair of
+ * scmc_poll / scmc_wait, and the second pair is left untouched. */
+int scmc_submit_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id);
 /* 0 when finished (results copied out, handle released), SCMC_E_BUSY when
- * still running.  Output layout as scmc_mccov; timing in *stats (ms_total =
- * submit-to-finish latency). */
+ * still running.  Output layout as scmc_mccov (covariance requests) or
+ * scmc_mcfield (field requests, first pair); timing in *stats (ms_total =
+ * submit-to-finish latency, ms_queued = submit to admission). */
 int scmc_poll(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, double *mm_im,
               scmc_stats *stats);
 int scmc_wait(int64_t request_id, double *ms_re, double *ms_im, double *mm_re, double *mm_im,

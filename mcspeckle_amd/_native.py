@@ -30,7 +30,7 @@ EXPORTS = [
     "scmc_trace", "scmc_mccov_replay", "scmc_mcfield_replay",
     "scmc_session_create", "scmc_session_run", "scmc_session_result", "scmc_session_stats",
     "scmc_session_destroy",
-    "scmc_server_start", "scmc_server_stop", "scmc_submit_mccov", "scmc_poll",
+    "scmc_server_start", "scmc_server_stop", "scmc_submit_mccov", "scmc_submit_mcfield", "scmc_poll",
     "scmc_wait", "scmc_cancel", "scmc_request_footprint",
     "scmc_last_error", "scmc_version", "scmc_device_count",
 ]
@@ -138,6 +138,7 @@ def lib():
         "scmc_server_start": ([C.POINTER(ServerConfig)], C.c_int),
         "scmc_server_stop": ([], C.c_int),
         "scmc_submit_mccov": ([P, C.c_int64, C.c_int32, _i64p], C.c_int),
+        "scmc_submit_mcfield": ([P, C.c_int64, C.c_int32, _i64p], C.c_int),
         "scmc_poll": ([C.c_int64, _dp, _dp, _dp, _dp, S], C.c_int),
         "scmc_wait": ([C.c_int64, _dp, _dp, _dp, _dp, S], C.c_int),
         "scmc_cancel": ([C.c_int64], C.c_int),

@@ -1,7 +1,10 @@
-// serve.cpp -- device-resident continuous batching of covariance solves.
+// serve.cpp -- continuous batching of covariance and field solves on one GPU.
 //
-// Requests (MCcov problems of any size, precision and path count) arrive with
-// scmc_submit_mccov.  The server keeps `max_active` slots; a slot takes the
+// Requests (MCcov or MCfield problems of any size, precision and path count)
+// arrive with scmc_submit_mccov / scmc_submit_mcfield.  The scheduler runs on
+// the host: `max_active` slot threads, each driving its own HIP streams, so
+// the kernels of all admitted requests are in flight on the device together
+// (they share the GPU through its hardware queues).  A slot takes the
 // first pending request whose device footprint fits the remaining HBM budget
 // (FIFO with backfill), so requests are admitted as slots and memory free up.
 // Each admitted request runs in its slot's session (its own HIP streams and
@@ -32,6 +35,7 @@ using clk = std::chrono::steady_clock;
 
 struct Request {
     int64_t id = 0;
+    int kind = 0;   // 0 MCcov, 1 MCfield
     ProblemCopy pc;
     int64_t max_itr = 0;
     int precision = 0;
@@ -104,12 +108,25 @@ int64_t request_footprint(int nv, int nl, int precision, bool near)
     return (int64_t)(acc + chunk + 64.0 * 1024 * 1024);
 }
 
+// Device memory of an admitted MCfield request: accumulator, per-view work
+// (64 entry splits) and the records of its largest chunk (field_chunk_paths).
+int64_t request_footprint_field(int nv, int nl, int precision)
+{
+    (void)precision;   // both modes keep FP64 accumulators and records
+    const double n = (double)nv * nl;
+    const double kchunk = std::max(2097152.0, std::min(8388608.0, 67108864.0 / std::max(nl, 1)));   // vertices
+    const double paths = std::min(1048576.0, kchunk / 10.0);
+    const double work = 16.0 * (kchunk * nl + paths * nl + n + 64.0 * n);
+    const double records = kchunk * (64.0 + 4.0) + paths * (80.0 + 12.0);
+    return (int64_t)(16.0 * n + work + records + 64.0 * 1024 * 1024);
+}
+
 static void run_request(Server *S, Request *r, Session &sess)
 {
     // device blocks come from and go back to the session's pool while the
     // request runs; they become reusable once it has synchronised (below)
     tls_block_pool = &sess.pool;
-    int rc = sess.init(&r->pc.pr, 0, r->precision);
+    int rc = sess.init(&r->pc.pr, r->kind, r->precision);
     const int64_t slice = S->slice > 0 ? S->slice : std::max<int64_t>(1000, r->max_itr / 8);
     for (int64_t p0 = 0; !rc && p0 < r->max_itr; p0 += slice) {
         if (r->cancel.load()) {
@@ -121,13 +138,19 @@ static void run_request(Server *S, Request *r, Session &sess)
         const int64_t len = std::min(slice, r->max_itr - p0);
         rc = sess.run(p0, len, p0 + len >= r->max_itr);
     }
-    if (!rc) {
+    if (!rc && r->kind == 0) {
         const size_t ne = (size_t)r->pc.pr.nv * r->pc.pr.nv * r->pc.pr.nl * r->pc.pr.nl;
         r->ms_re.resize(ne);
         r->ms_im.resize(ne);
         r->mm_re.resize(ne);
         r->mm_im.resize(ne);
         rc = sess.result(r->max_itr, r->ms_re.data(), r->ms_im.data(), r->mm_re.data(), r->mm_im.data());
+    } else if (!rc) {
+        // the field u (nv x nl) in the first output pair
+        const size_t ne = (size_t)r->pc.pr.nv * r->pc.pr.nl;
+        r->ms_re.resize(ne);
+        r->ms_im.resize(ne);
+        rc = sess.result(r->max_itr, r->ms_re.data(), r->ms_im.data(), nullptr, nullptr);
     }
     sess.fill_stats(&r->st);
     r->rc = rc;
@@ -199,7 +222,7 @@ static int copy_out(Request *r, double *ms_re, double *ms_im, double *mm_re, dou
     if (r->rc) return set_error(r->rc, "%s", r->err.c_str());
     const size_t ne = r->ms_re.size();
     auto put = [&](double *re, double *im, const std::vector<double> &a, const std::vector<double> &b) {
-        if (!re) return;
+        if (!re || a.size() != ne) return;
         if (im) {
             memcpy(re, a.data(), ne * sizeof(double));
             memcpy(im, b.data(), ne * sizeof(double));
@@ -285,7 +308,7 @@ int scmc_server_stop(void)
     return SCMC_OK;
 }
 
-int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id)
+static int submit(const scmc_problem *pr, int kind, int64_t max_itr, int32_t precision, int64_t *request_id)
 {
     int rc = validate_problem(pr, max_itr, true);
     if (rc) return rc;
@@ -294,10 +317,12 @@ int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision
     if (!g_srv) return set_error(SCMC_E_ARG, "server not started");
     Server *S = g_srv.get();
     std::unique_ptr<Request> r(new Request());
+    r->kind = kind;
     r->pc.assign(pr);
     r->max_itr = max_itr;
     r->precision = precision;
-    r->footprint = request_footprint(pr->nv, pr->nl, precision, !(pr->is_ff_l && pr->is_ff_v));
+    r->footprint = kind == 0 ? request_footprint(pr->nv, pr->nl, precision, !(pr->is_ff_l && pr->is_ff_v))
+                             : request_footprint_field(pr->nv, pr->nl, precision);
     r->t_submit = clk::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -332,6 +357,16 @@ static int finish(int64_t id, bool block, double *ms_re, double *ms_im, double *
         S->all.erase(it);
     }
     return copy_out(done.get(), ms_re, ms_im, mm_re, mm_im, stats);
+}
+
+int scmc_submit_mccov(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id)
+{
+    return submit(pr, 0, max_itr, precision, request_id);
+}
+
+int scmc_submit_mcfield(const scmc_problem *pr, int64_t max_itr, int32_t precision, int64_t *request_id)
+{
+    return submit(pr, 1, max_itr, precision, request_id);
 }
 
 int scmc_poll(int64_t id, double *ms_re, double *ms_im, double *mm_re, double *mm_im, scmc_stats *stats)

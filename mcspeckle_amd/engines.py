@@ -562,31 +562,37 @@ class Server:
         N.check(N.lib().scmc_server_start(C.byref(cfg)))
         self._specs = {}
 
-    def submit(self, spec, max_itr, precision="mixed"):
+    def submit(self, spec, max_itr, precision="mixed", kind="cov"):
+        """kind "cov" (MCcov) or "field" (MCfield)."""
         p = c_problem(spec)
         h = C.c_int64()
-        N.check(N.lib().scmc_submit_mccov(C.byref(p), int(max_itr), _prec(precision), C.byref(h)))
-        self._specs[h.value] = spec
+        fn = N.lib().scmc_submit_mccov if kind == "cov" else N.lib().scmc_submit_mcfield
+        N.check(fn(C.byref(p), int(max_itr), _prec(precision), C.byref(h)))
+        self._specs[h.value] = (spec, kind)
         return h.value
 
     def _finish(self, rid, block):
-        spec = self._specs[rid]
+        spec, kind = self._specs[rid]
         nv, nl = spec.nv, spec.nl
-        ne = (nv * nl) ** 2
-        a = [np.empty(ne) for _ in range(4)]
+        ne = (nv * nl) ** 2 if kind == "cov" else nv * nl
+        a = [np.empty(ne) for _ in range(4 if kind == "cov" else 2)]
         st = N.Stats()
         fn = N.lib().scmc_wait if block else N.lib().scmc_poll
-        rc = fn(int(rid), *[N.dptr(x) for x in a], C.byref(st))
+        ptrs = [N.dptr(x) for x in a] + ([None, None] if kind != "cov" else [])
+        rc = fn(int(rid), *ptrs, C.byref(st))
         if rc == N.SCMC_E_BUSY:
             return None
         self._specs.pop(rid, None)
         N.check(rc)
+        if kind != "cov":
+            return (a[0] + 1j * a[1]).reshape((nv, nl), order="F"), st.as_dict()
         shape = (nv, nv, nl, nl)
         return ((a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F"),
                 st.as_dict())
 
     def poll(self, rid):
-        """(Ms, Mm, stats) when finished, None while running."""
+        """(Ms, Mm, stats) -- (u, stats) for a field request -- when finished,
+        None while running."""
         return self._finish(rid, False)
 
     def wait(self, rid):

@@ -196,3 +196,35 @@ def test_serving_stream_requests_match_fp64_one_shot():
         say("  request N", spec.nv * spec.nl, "paths", n, "err", err)
         assert err <= 1e-4, (spec.nv, spec.nl, err)
         assert cs_err(Ms, Ms0) <= 1e-4
+
+
+def test_field_requests_alone_in_the_mix_and_against_mcfield():
+    """Field requests (MCfield) served beside covariance requests: each field
+    is the same alone and in the mix (1e-12), and equals the one-shot
+    scmc_mcfield on the same paths (FP64 1e-12, mixed 2e-5, max |u - u*| /
+    max |u*|: the slices chunk the paths differently)."""
+    rng = np.random.default_rng(31)
+    fields = [(E.make_problem(6.0, 0.97, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(nl, 10, rng),
+                              cone_dirs(nv, 12, rng), 1, 1, 1.0, cbs, 2, 3, 0.8, seed=60 + k), n, prec)
+              for k, (nv, nl, cbs, n, prec) in enumerate([(900, 1, False, 3000, "mixed"), (256, 3, True, 2000, "fp64"),
+                                                          (4096, 1, True, 1500, "mixed")])]
+    alone = []
+    with E.Server(max_active=1, paths_per_slice=700) as S:
+        for spec, n, prec in fields:
+            alone.append(S.wait(S.submit(spec, n, prec, kind="field")))
+    with E.Server(max_active=4, paths_per_slice=700) as S:
+        ids = []
+        for k, (spec, n, prec) in enumerate(fields):
+            ids.append(S.submit(spec, n, prec, kind="field"))
+            i, nv, nl, od, g, nc, pc = MIX[k]
+            S.submit(req(i, nv, nl, od, g, seed=i), nc, pc)    # covariance requests in between
+        say("field mix submitted")
+        mixed = [S.wait(r) for r in ids]
+    for (spec, n, prec), (ua, sta), (um, stm) in zip(fields, alone, mixed):
+        assert ua.shape == (spec.nv, spec.nl) and stm["paths"] == n
+        assert np.abs(um - ua).max() <= 1e-12 * np.abs(ua).max()
+        u1 = E.mcfield_spec(spec, n, prec)
+        tol = 2e-5 if prec == "mixed" else 1e-12
+        err = np.abs(um - u1).max() / np.abs(u1).max()
+        say("  field", spec.nv, spec.nl, prec, "err", err)
+        assert err <= tol
