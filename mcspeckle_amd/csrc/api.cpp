@@ -394,12 +394,13 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
     np = n;
     if (!counts.ensure(sizeof(int32_t) * (size_t)n, &rc) || !offsets.ensure(sizeof(int64_t) * (size_t)(n + 1), &rc) ||
         !paths.ensure(sizeof(PathRec) * (size_t)n, &rc) ||
-        !tsum.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(scan_tiles(n), 1), &rc))
+        !tsum.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(scan_tiles(n), 1), &rc) ||
+        !ctr.ensure(sizeof(unsigned long long), &rc))
         return rc;
     if (M)
         launch_hetero_trace(P, *M, p0, n, (int32_t *)counts.p, nullptr, nullptr, nullptr, nullptr, false, s);
     else
-        launch_trace_count(P, p0, n, (int32_t *)counts.p, s);
+        launch_trace_count(P, p0, n, (int32_t *)counts.p, (unsigned long long *)ctr.p, s);
     launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, (int64_t *)tsum.p, s);
     int64_t total = 0;
     if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)offsets.p + n, sizeof total, hipMemcpyDeviceToHost, s), "copy total",
@@ -415,7 +416,7 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
                             (int32_t *)vpath.p, true, s);
     else
         launch_trace_emit(P, p0, n, (const int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
-                          (int32_t *)vpath.p, s);
+                          (int32_t *)vpath.p, (unsigned long long *)ctr.p, s);
     return hip_ok(hipGetLastError(), "trace launch", &rc) ? SCMC_OK : rc;
 }
 

@@ -25,14 +25,31 @@ struct PathRec {
 };
 
 // ---------------------------------------------------------------- tracer
-void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, hipStream_t s);
+// Workgroups of 256 lanes for a kernel whose lanes stay resident and take
+// paths from a counter (take_path): as many as fit on the device at once
+// (occupancy query), but no more than np paths need.
+inline unsigned resident_blocks(const void *kernel, int64_t np)
+{
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0);
+    const int64_t b = (int64_t)(cus > 1 ? cus : 1) * (per > 1 ? per : 1);
+    const int64_t need = (np + 255) / 256;
+    return (unsigned)(b < need ? b : need);
+}
+
+// Vertex counts of paths [p0, p0 + np) (ids relative to P.path_offset).
+// ctr: one device counter the launch clears and the walk hands paths out from.
+void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, unsigned long long *ctr,
+                        hipStream_t s);
 // offsets[np + 1] = exclusive scan of counts (offsets[np] = total).
 // Exclusive scan of counts[np] into offsets[np + 1]; tsum: scratch of
 // scan_tiles(np) int64 (at least one).
 int64_t scan_tiles(int64_t np);
 void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, int64_t *tsum, hipStream_t s);
 void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64_t *offsets, PathRec *paths,
-                       VertexRec *verts, int32_t *vpath, hipStream_t s);
+                       VertexRec *verts, int32_t *vpath, unsigned long long *ctr, hipStream_t s);
 
 // -------------------------------------------------- covariance factors
 // Factor arrays for the rank-K update C += sum_k u_k u_k^H with

@@ -84,6 +84,7 @@ static size_t fixed_bytes(const scmc_problem *pr)
     b += tab(pr->amp, pr->amp.type);
     if (pr->has_amp0) b += tab(pr->amp0, pr->amp.type);
     b += al(16 * (size_t)pr->nv * (size_t)pr->nl) + al(sizeof(unsigned long long));
+    b += al(sizeof(unsigned long long));   // the walks' path counter
     return b;
 }
 
@@ -155,7 +156,8 @@ static int field_noalloc(const scmc_problem *pr, int64_t max_itr, int32_t precis
     // and the counter: the scratch holds the previous call's values)
     if ((rc = ps.setup(pr, s)) || (rc = fe.init(ps, s))) return rc;
     // the chunk buffers at their largest, so later chunks reuse them
-    if (!tc.counts.ensure(4 * (size_t)np_max, &rc) || !tc.offsets.ensure(8 * (size_t)(np_max + 1), &rc) ||
+    if (!tc.ctr.ensure(sizeof(unsigned long long), &rc) || !tc.counts.ensure(4 * (size_t)np_max, &rc) ||
+        !tc.offsets.ensure(8 * (size_t)(np_max + 1), &rc) ||
         !tc.paths.ensure(sizeof(PathRec) * (size_t)np_max, &rc) ||
         !tc.tsum.ensure(8 * (size_t)std::max<int64_t>(scan_tiles(np_max), 1), &rc) ||
         !tc.verts.ensure(sizeof(VertexRec) * (size_t)nvert_max, &rc) ||
@@ -168,7 +170,7 @@ static int field_noalloc(const scmc_problem *pr, int64_t max_itr, int32_t precis
     for (int64_t done = 0; done < max_itr;) {
         const auto t0 = std::chrono::steady_clock::now();
         const int64_t np = std::min(np_max, max_itr - done);
-        launch_trace_count(ps.dp, (uint64_t)done, np, (int32_t *)tc.counts.p, s);
+        launch_trace_count(ps.dp, (uint64_t)done, np, (int32_t *)tc.counts.p, (unsigned long long *)tc.ctr.p, s);
         launch_scan((const int32_t *)tc.counts.p, np, (int64_t *)tc.offsets.p, (int64_t *)tc.tsum.p, s);
         int64_t total = 0;
         if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)tc.offsets.p + np, sizeof total, hipMemcpyDeviceToHost, s),
@@ -192,7 +194,7 @@ static int field_noalloc(const scmc_problem *pr, int64_t max_itr, int32_t precis
             total = off[(size_t)use];
         }
         launch_trace_emit(ps.dp, (uint64_t)done, use, (const int64_t *)tc.offsets.p, (PathRec *)tc.paths.p,
-                          (VertexRec *)tc.verts.p, (int32_t *)tc.vpath.p, s);
+                          (VertexRec *)tc.verts.p, (int32_t *)tc.vpath.p, (unsigned long long *)tc.ctr.p, s);
         if (!hip_ok(hipGetLastError(), "trace launch", &rc)) return rc;
         tc.np = use;
         tc.nvert = total;

@@ -289,6 +289,7 @@ __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, Q
                 path = P.path_offset + p0 + (uint64_t)id;
                 double2 w0p;
                 path_start<3>(P, path, x, px, w, w0p);
+                ow[0] = ow[1] = ow[2] = 0;   // a new path has no incoming direction
                 for (int j = 0; j < 3; ++j) x1[j] = x[j];
                 weight = P.albedo;
                 n = 0;
@@ -631,19 +632,6 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
     else
         hipLaunchKernelGGL(quad_scatter_kernel<double>, gr, dim3(256), 0, s, P, g, nvert, paths, verts, vpath,
                            offsets, (double2 *)grid);
-}
-
-// Resident workgroups for a regenerating walk kernel: as many as fit on the
-// device at once (occupancy query), but no more than the paths need.
-static unsigned resident_blocks(const void *kernel, int64_t np)
-{
-    int dev = 0, cus = 0, per = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0);
-    int64_t b = (int64_t)std::max(cus, 1) * std::max(per, 1);
-    const int64_t need = (np + 255) / 256;
-    return (unsigned)(b < need ? b : need);
 }
 
 void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,

@@ -93,6 +93,7 @@ struct ProblemState {
 struct TraceChunk {
     int64_t np = 0, nvert = 0;
     DevBuf counts, offsets, paths, verts, vpath, tsum;   // tsum: scan scratch
+    DevBuf ctr;                                         // the walks' path counter
     int run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, const HetMedium *M = nullptr);
     int load(const scmc_pathset *ps, hipStream_t s);
 };

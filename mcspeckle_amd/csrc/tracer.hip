@@ -12,68 +12,105 @@
 
 namespace scmc {
 
+// Both passes keep their lanes resident: a lane takes the next path index
+// (take_path) as soon as its walk ends, so a wave does not idle while its
+// longest walk finishes (for the slab workloads the longest of 64 walks is
+// ~6x the mean).  Outputs are indexed by path, so they do not depend on which
+// lane walked which path.
 template <int DIM>
-__global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t p0, int64_t np, int32_t *counts)
+__global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t p0, int64_t np, int32_t *counts,
+                                                          unsigned long long *ctr)
 {
     __shared__ double lcos[kIcdfLds];
     const double *tab = stage_icdf(P.amp, lcos);
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    uint64_t path = P.path_offset + p0 + (uint64_t)i;
-    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
-    double2 w0p;
-    path_start<DIM>(P, path, x, px, w, w0p);
-    double weight = P.albedo;
+    WavePaths wp;
+    bool active = false;
+    int64_t id = 0;
+    uint64_t path = 0;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        ++n;
-        double2 a = uniform_pair(P.seed, path, n, 0);
-        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) break;
+        if (__ballot(!active)) {
+            const int64_t t = take_path(wp, !active, np, ctr);
+            if (t >= 0) {
+                id = t;
+                path = P.path_offset + p0 + (uint64_t)t;
+                double2 w0p;
+                path_start<DIM>(P, path, x, px, w, w0p);
+                ow[0] = ow[1] = ow[2] = 0;   // a new path has no incoming direction
+                weight = P.albedo;
+                n = 0;
+                active = true;
+            }
+        }
+        if (!__ballot(active)) break;
+        if (active) {
+            ++n;
+            const double2 a = uniform_pair(P.seed, path, n, 0);
+            if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) {
+                counts[id] = (int32_t)n;
+                active = false;
+            }
+        }
     }
-    counts[i] = (int32_t)n;
 }
 
 template <int DIM>
 __global__ __launch_bounds__(256) void trace_emit_kernel(DevProblem P, uint64_t p0, int64_t np, const int64_t *offsets,
-                                                         PathRec *paths, VertexRec *verts, int32_t *vpath)
+                                                         PathRec *paths, VertexRec *verts, int32_t *vpath,
+                                                         unsigned long long *ctr)
 {
     __shared__ double lcos[kIcdfLds];
     const double *tab = stage_icdf(P.amp, lcos);
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    uint64_t path = P.path_offset + p0 + (uint64_t)i;
-    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
-    double2 w0p;
-    path_start<DIM>(P, path, x, px, w, w0p);
-    PathRec pr;
-    for (int j = 0; j < 3; ++j) {
-        pr.x1[j] = x[j];
-        pr.w1[j] = w[j];
-    }
-    pr.px = px;
-    pr.pad = 0;
-    pr.w0p = w0p;
-    paths[i] = pr;
-    double weight = P.albedo;
-    int64_t o = offsets[i];
-    const int64_t end = offsets[i + 1];
+    WavePaths wp;
+    bool active = false;
+    int64_t id = 0, o = 0, end = 0;
+    uint64_t path = 0;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        ++n;
-        double2 a = uniform_pair(P.seed, path, n, 0);
-        if (o < end) {
-            VertexRec r;
-            for (int j = 0; j < 3; ++j) {
-                r.x[j] = x[j];
-                r.ow[j] = ow[j];
+        if (__ballot(!active)) {
+            const int64_t t = take_path(wp, !active, np, ctr);
+            if (t >= 0) {
+                id = t;
+                path = P.path_offset + p0 + (uint64_t)t;
+                double2 w0p;
+                path_start<DIM>(P, path, x, px, w, w0p);
+                ow[0] = ow[1] = ow[2] = 0;   // a new path has no incoming direction
+                PathRec pr;
+                for (int j = 0; j < 3; ++j) {
+                    pr.x1[j] = x[j];
+                    pr.w1[j] = w[j];
+                }
+                pr.px = px;
+                pr.pad = 0;
+                pr.w0p = w0p;
+                paths[t] = pr;
+                o = offsets[t];
+                end = offsets[t + 1];
+                weight = P.albedo;
+                n = 0;
+                active = true;
             }
-            r.weight = weight;
-            r.xi = a.x;
-            verts[o] = r;
-            vpath[o] = (int32_t)i;
-            ++o;
         }
-        if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) break;
+        if (!__ballot(active)) break;
+        if (active) {
+            ++n;
+            const double2 a = uniform_pair(P.seed, path, n, 0);
+            if (o < end) {
+                VertexRec r;
+                for (int j = 0; j < 3; ++j) {
+                    r.x[j] = x[j];
+                    r.ow[j] = ow[j];
+                }
+                r.weight = weight;
+                r.xi = a.x;
+                verts[o] = r;
+                vpath[o] = (int32_t)id;
+                ++o;
+            }
+            if (!path_step<DIM>(P, path, n, x, w, ow, weight, a.y, tab)) active = false;
+        }
     }
 }
 
@@ -157,14 +194,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_tile_apply_kernel(const int
 
 int64_t scan_tiles(int64_t np) { return (np + kScanTile - 1) / kScanTile; }
 
-void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, hipStream_t s)
+void launch_trace_count(const DevProblem &P, uint64_t p0, int64_t np, int32_t *counts, unsigned long long *ctr,
+                        hipStream_t s)
 {
     if (np <= 0) return;
-    dim3 g((unsigned)((np + 255) / 256));
+    launch_zero(ctr, sizeof(unsigned long long), s);
     if (P.dim == 3)
-        hipLaunchKernelGGL(trace_count_kernel<3>, g, dim3(256), 0, s, P, p0, np, counts);
+        hipLaunchKernelGGL(trace_count_kernel<3>, dim3(resident_blocks((const void *)trace_count_kernel<3>, np)),
+                           dim3(256), 0, s, P, p0, np, counts, ctr);
     else
-        hipLaunchKernelGGL(trace_count_kernel<2>, g, dim3(256), 0, s, P, p0, np, counts);
+        hipLaunchKernelGGL(trace_count_kernel<2>, dim3(resident_blocks((const void *)trace_count_kernel<2>, np)),
+                           dim3(256), 0, s, P, p0, np, counts, ctr);
 }
 
 void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, int64_t *tsum, hipStream_t s)
@@ -178,14 +218,16 @@ void launch_scan(const int32_t *counts, int64_t np, int64_t *offsets, int64_t *t
 }
 
 void launch_trace_emit(const DevProblem &P, uint64_t p0, int64_t np, const int64_t *offsets, PathRec *paths,
-                       VertexRec *verts, int32_t *vpath, hipStream_t s)
+                       VertexRec *verts, int32_t *vpath, unsigned long long *ctr, hipStream_t s)
 {
     if (np <= 0) return;
-    dim3 g((unsigned)((np + 255) / 256));
+    launch_zero(ctr, sizeof(unsigned long long), s);
     if (P.dim == 3)
-        hipLaunchKernelGGL(trace_emit_kernel<3>, g, dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath);
+        hipLaunchKernelGGL(trace_emit_kernel<3>, dim3(resident_blocks((const void *)trace_emit_kernel<3>, np)),
+                           dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath, ctr);
     else
-        hipLaunchKernelGGL(trace_emit_kernel<2>, g, dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath);
+        hipLaunchKernelGGL(trace_emit_kernel<2>, dim3(resident_blocks((const void *)trace_emit_kernel<2>, np)),
+                           dim3(256), 0, s, P, p0, np, offsets, paths, verts, vpath, ctr);
 }
 
 // The tabulated direction table: each sampleIcdf angle replaced by its
