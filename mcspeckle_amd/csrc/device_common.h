@@ -206,7 +206,8 @@ __device__ __forceinline__ void rotate_by_theta(const double *ow, double costhet
         return;
     }
     double phi = kPi * 2 * u;
-    double sinphi = sin(phi), cosphi = cos(phi);
+    double sinphi, cosphi;
+    sincos(phi, &sinphi, &cosphi);
     if (fabs(ow[2]) > 0.999999999) {
         w[0] = sintheta * cosphi;
         w[1] = sintheta * sinphi;
@@ -222,9 +223,10 @@ __device__ __forceinline__ void rotate_by_theta(const double *ow, double costhet
 __device__ __forceinline__ void box_muller(double ua, double ub, double &n0, double &n1)
 {
     double r = sqrt(-2.0 * log(1.0 - ua));
-    double t = 2.0 * kPi * ub;
-    n0 = r * cos(t);
-    n1 = r * sin(t);
+    double t = 2.0 * kPi * ub, st, ct;
+    sincos(t, &st, &ct);
+    n0 = r * ct;
+    n1 = r * st;
 }
 
 // randn(dim,1)/norm: smpampfunc_general.m:7-8, MCcov.m:148
@@ -246,9 +248,9 @@ __device__ __forceinline__ void iso_dir(double d0, double d1, double d2, double 
     }
 }
 
-// Path regeneration.  A walking kernel with resident lanes hands each lane a
-// new path index as soon as its path ends, instead of letting a wave run as
-// long as its longest path (for the slab workloads the longest of 64 walks is
+// Path regeneration.  A walking kernel with resident lanes hands its idle
+// lanes new path indices (at the wave's refills, refill_due below), instead of
+// letting a wave run as long as its longest path (for the slab workloads the longest of 64 walks is
 // ~6x the mean, so most lanes would idle).  A wave takes indices in blocks of
 // kPathBlock from a global counter (one atomic per block) and gives them to
 // its idle lanes in lane order.  A path's walk depends only on its index, so
@@ -286,6 +288,20 @@ __device__ __forceinline__ int64_t take_path(WavePaths &wp, bool want, int64_t n
         m = __ballot(want && id < 0);
     }
     return id;
+}
+
+// A refill runs the path-start code (four Philox draws, the first vertex and
+// direction) for the whole wave, masked to the lanes that start a path, so a
+// wave that refilled whenever one lane was idle would pay it nearly every
+// step (a slab walk of ~18 vertices ends in ~3.6 of 64 lanes per step).  A
+// wave therefore refills once kRefillIdle lanes are idle, or when none is
+// active, and stops asking once the counter is exhausted.
+constexpr int kRefillIdle = 16;
+__device__ __forceinline__ bool refill_due(const WavePaths &wp, bool active)
+{
+    const unsigned long long act = __ballot(active);
+    const int idle = 64 - __popcll(act);
+    return !wp.exhausted && idle > 0 && (idle >= kRefillIdle || act == 0ull);
 }
 
 // The walking kernels stage the tabulated direction table (the cosines of
@@ -392,10 +408,11 @@ __device__ __forceinline__ bool path_step(const DevProblem &P, uint64_t path, ui
         weight = weight * P.albedo;
     }
     double2 c = uniform_pair(P.seed, path, n, 2);
-    double2 e = uniform_pair(P.seed, path, n, 3);
+    // the fourth uniform of the draw is used by the isotropic phase only
+    const double e3 = P.amp.type == 1 ? uniform_pair(P.seed, path, n, 3).x : 0.0;
 #pragma unroll
     for (int j = 0; j < DIM; ++j) ow[j] = w[j];
-    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w, lcos);
+    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e3, w, lcos);
     return true;
 }
 

@@ -323,10 +323,11 @@ __device__ __forceinline__ bool hetero_path_step(const DevProblem &P, const HetM
         weight = weight * med_albedo<DIM>(P, M, x);
     }
     const double2 c = uniform_pair(P.seed, path, n, 2);
-    const double2 e = uniform_pair(P.seed, path, n, 3);
+    // the fourth uniform of the draw is used by the isotropic phase only
+    const double e3 = P.amp.type == 1 ? uniform_pair(P.seed, path, n, 3).x : 0.0;
 #pragma unroll
     for (int j = 0; j < DIM; ++j) ow[j] = w[j];
-    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e.x, w, lcos);
+    sample_dir<DIM>(ow, P.amp, b.y, c.x, c.y, e3, w, lcos);
     return true;
 }
 

@@ -261,8 +261,9 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
 
 // The walk and the scatter in one pass: every vertex is scattered as the walk
 // reaches it, so no vertex records are stored and no counting pass is needed.
-// Lanes stay resident and take a new path as soon as theirs ends
-// (take_path): a wave no longer waits for its longest walk.  A lane follows
+// Lanes stay resident and take a new path at the wave's next refill once
+// theirs ends (refill_due, take_path): a wave no longer waits for its longest
+// walk.  A lane follows
 // trace_emit_kernel's walk exactly (vertex n: position, weight and xi = the
 // first uniform of draw n, taken before the step); *nvert receives the vertex
 // count (the normaliser), one atomic per workgroup.  *ctr (zeroed) hands out
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, Q
     uint32_t n = 0;
     QuadPathSum<T> S;
     for (;;) {
-        if (__ballot(!active)) {
+        if (refill_due(wp, active)) {
             const int64_t id = take_path(wp, !active, np, ctr);
             if (id >= 0) {
                 path = P.path_offset + p0 + (uint64_t)id;

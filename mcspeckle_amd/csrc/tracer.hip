@@ -12,11 +12,12 @@
 
 namespace scmc {
 
-// Both passes keep their lanes resident: a lane takes the next path index
-// (take_path) as soon as its walk ends, so a wave does not idle while its
-// longest walk finishes (for the slab workloads the longest of 64 walks is
-// ~6x the mean).  Outputs are indexed by path, so they do not depend on which
-// lane walked which path.
+// Both passes keep their lanes resident: a lane whose walk has ended takes the
+// next path index (take_path) at the wave's next refill (refill_due: once a
+// quarter of the wave is idle), so a wave does not idle while its longest walk
+// finishes (for the slab workloads the longest of 64 walks is ~6x the mean).
+// Outputs are indexed by path, so they do not depend on which lane walked
+// which path.
 template <int DIM>
 __global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t p0, int64_t np, int32_t *counts,
                                                           unsigned long long *ctr)
@@ -30,7 +31,7 @@ __global__ __launch_bounds__(256) void trace_count_kernel(DevProblem P, uint64_t
     double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        if (__ballot(!active)) {
+        if (refill_due(wp, active)) {
             const int64_t t = take_path(wp, !active, np, ctr);
             if (t >= 0) {
                 id = t;
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(256) void trace_emit_kernel(DevProblem P, uint64_t 
     double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        if (__ballot(!active)) {
+        if (refill_due(wp, active)) {
             const int64_t t = take_path(wp, !active, np, ctr);
             if (t >= 0) {
                 id = t;
