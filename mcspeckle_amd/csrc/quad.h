@@ -7,6 +7,14 @@ namespace scmc {
 constexpr int kQuadMaxSlabs = 4096;
 constexpr int kQuadFusedLights = 4;   // lights per vertex the fused walk-and-scatter kernel handles
 
+// One light of an illumination wave as the FP32-grid kernels read it: the
+// phase gradient vsign l / lambda (cycles per unit length) and the weight Wl.
+struct WaveLight {
+    double kx, ky, kz;
+    float wr, wi;
+};
+constexpr int kWaveLightsLds = 512;   // a workgroup stages up to this many lights in LDS
+
 // Grid geometry shared by the host set-up and the scatter kernel.
 struct QuadGeom {
     int nv0;          // views per axis
@@ -23,6 +31,7 @@ struct QuadGeom {
     int wave = 0;
     double vsign = -1.0;
     const double2 *wl = nullptr;  // nl weights (device)
+    const WaveLight *wlk = nullptr;   // nl lights with their weights (device)
 };
 
 // Host restatement of MCfieldFourierQuad.m:80-121 (with parity_fix, of

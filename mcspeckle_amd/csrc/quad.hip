@@ -36,6 +36,33 @@ __device__ __forceinline__ float2 cis_cycles(double cyc)
     return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
 }
 
+// twl = sum_l Wl(l) e^{2 pi i (vsign l.x / lambda)} over the wave's lights
+// (MCfieldFourierQuadOnWave.m:281-302) in FP32 after the FP64 reduction of
+// each phase; wl is the LDS copy of the lights (stage_wave) or the global one.
+__device__ __forceinline__ float2 wave_sum(const WaveLight *wl, int nl, const double *x)
+{
+    float twr = 0.f, twi = 0.f;
+#pragma unroll 4
+    for (int l = 0; l < nl; ++l) {
+        const WaveLight w = wl[l];
+        const float2 c = cis_cycles(fma(w.kz, x[2], fma(w.ky, x[1], w.kx * x[0])));
+        twr = fmaf(w.wr, c.x, fmaf(-w.wi, c.y, twr));
+        twi = fmaf(w.wr, c.y, fmaf(w.wi, c.x, twi));
+    }
+    return make_float2(twr, twi);
+}
+
+// Copies the wave's lights into LDS when there are at most kWaveLightsLds of
+// them; every thread of the workgroup calls it (it ends with a barrier).
+// Returns whether the LDS copy holds them.
+__device__ __forceinline__ bool stage_wave(const DevProblem &P, const QuadGeom &g, WaveLight *lds)
+{
+    if (!g.wave || P.nl > kWaveLightsLds) return false;
+    for (int l = threadIdx.x; l < P.nl; l += blockDim.x) lds[l] = g.wlk[l];
+    __syncthreads();
+    return true;
+}
+
 __device__ __forceinline__ void atomic_add_c(double2 *p, double re, double im)
 {
     unsafeAtomicAdd(&p->x, re);
@@ -66,8 +93,9 @@ __device__ __forceinline__ int slab_of(double z, const QuadGeom &g)
 // CBS], xi the vertex's random-phase uniform.  Pass 1 has xs = x1, xl = the
 // vertex; pass 2 (CBS) swaps them (:224-232).
 template <typename T>
-__device__ __forceinline__ void quad_scatter_pass(const DevProblem &P, const QuadGeom &g, const double *xs,
-                                                  const double *xl, double amp, double xi, typename cx<T>::t *grid)
+__device__ __forceinline__ void quad_scatter_pass(const DevProblem &P, const QuadGeom &g, bool wave,
+                                                  const WaveLight *wls, const double *xs, const double *xl, double amp,
+                                                  double xi, typename cx<T>::t *grid)
 {
     const int j = slab_of(xs[2], g);
     if (j < 0) return;
@@ -82,14 +110,9 @@ __device__ __forceinline__ void quad_scatter_pass(const DevProblem &P, const Qua
         const float2 ex = cis_cycles(xi);   // e^{2 pi i xi}
         const float pre = (float)amp * ex.x, pim = (float)amp * ex.y;
         const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
-        if (g.wave) {
-            float twr = 0.f, twi = 0.f;
-            for (int l = 0; l < P.nl; ++l) {
-                const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
-                const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
-                twr += wr * c2.x - wi * c2.y;
-                twi += wr * c2.y + wi * c2.x;
-            }
+        if (wave) {
+            const float2 tw = wave_sum(wls, P.nl, xl);
+            const float twr = tw.x, twi = tw.y;
             const float2 e1 = cis_cycles(g.vsign * (tz - xs[2]) * P.inv_lambda);
             const float ar = pre * e1.x - pim * e1.y, ai = pre * e1.y + pim * e1.x;
             atomic_add_c(&grid[((size_t)j * g.nx + iy) * g.nx + ix], (ar * twr - ai * twi) * m,
@@ -109,7 +132,7 @@ __device__ __forceinline__ void quad_scatter_pass(const DevProblem &P, const Qua
     double sx, cxv;
     sincos(2 * kPi * xi, &sx, &cxv);
     const double pre = amp * cxv, pim = amp * sx;
-    if (g.wave) {
+    if (wave) {
         // MCfieldFourierQuadOnWave.m:281-302: pxL exp(i vsign k (tz - z_s)) twl atl,
         // twl = sum_l Wl(l) exp(i vsign k l.x_L)
         double twr = 0, twi = 0;
@@ -156,22 +179,18 @@ struct QuadPathSum {
 };
 
 template <typename T>
-__device__ __forceinline__ void quad_forward_term(const DevProblem &P, const QuadGeom &g, const double *xl, double amp,
-                                                  double xi, QuadPathSum<T> &S)
+__device__ __forceinline__ void quad_forward_term(const DevProblem &P, const QuadGeom &g, bool wave,
+                                                  const WaveLight *wls, const double *xl, double amp, double xi,
+                                                  QuadPathSum<T> &S)
 {
     const double att = -P.sigt / 2 * ((xl[2] - P.bmin[2]) * g.l_diratt + (P.bmax[2] - xl[2]) * g.l_idiratt);
     if constexpr (std::is_same<T, float>::value) {
         const float2 ex = cis_cycles(xi);
         const float m = __builtin_amdgcn_exp2f((float)(att * 1.4426950408889634));
         const float pre = (float)amp * m * ex.x, pim = (float)amp * m * ex.y;
-        if (g.wave) {
-            float twr = 0.f, twi = 0.f;
-            for (int l = 0; l < P.nl; ++l) {
-                const float2 c2 = cis_cycles(g.vsign * dot<3>(P.l + 3 * l, xl) * P.inv_lambda);
-                const float wr = (float)g.wl[l].x, wi = (float)g.wl[l].y;
-                twr += wr * c2.x - wi * c2.y;
-                twi += wr * c2.y + wi * c2.x;
-            }
+        if (wave) {
+            const float2 tw = wave_sum(wls, P.nl, xl);
+            const float twr = tw.x, twi = tw.y;
             S.re[0] += pre * twr - pim * twi;
             S.im[0] += pre * twi + pim * twr;
             return;
@@ -189,7 +208,7 @@ __device__ __forceinline__ void quad_forward_term(const DevProblem &P, const Qua
     sincos(2 * kPi * xi, &sx, &cxv);
     const double m = exp(att);
     const double pre = amp * m * cxv, pim = amp * m * sx;
-    if (g.wave) {
+    if (wave) {
         double twr = 0, twi = 0;
         for (int l = 0; l < P.nl; ++l) {
             double s2, c2;
@@ -213,7 +232,7 @@ __device__ __forceinline__ void quad_forward_term(const DevProblem &P, const Qua
 }
 
 template <typename T>
-__device__ __forceinline__ void quad_flush_path(const DevProblem &P, const QuadGeom &g, const double *x1,
+__device__ __forceinline__ void quad_flush_path(const DevProblem &P, const QuadGeom &g, bool wave, const double *x1,
                                                 const QuadPathSum<T> &S, typename cx<T>::t *grid)
 {
     const int j = slab_of(x1[2], g);
@@ -224,13 +243,13 @@ __device__ __forceinline__ void quad_flush_path(const DevProblem &P, const QuadG
     const double tz = 0.5 * (g.gz[j] + g.gz[j + 1]);
     double er, ei;
     if constexpr (std::is_same<T, float>::value) {
-        const float2 e1 = cis_cycles((g.wave ? g.vsign : -1.0) * (tz - x1[2]) * P.inv_lambda);
+        const float2 e1 = cis_cycles((wave ? g.vsign : -1.0) * (tz - x1[2]) * P.inv_lambda);
         er = e1.x;
         ei = e1.y;
     } else {
-        sincos((g.wave ? g.vsign : -1.0) * P.k * (tz - x1[2]), &ei, &er);
+        sincos((wave ? g.vsign : -1.0) * P.k * (tz - x1[2]), &ei, &er);
     }
-    const int nout = g.wave ? 1 : P.nl;
+    const int nout = wave ? 1 : P.nl;
 #pragma unroll
     for (int l = 0; l < kQuadFusedLights; ++l)
         if (l < nout) {
@@ -248,6 +267,8 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
                                                            const VertexRec *verts, const int32_t *vpath,
                                                            const int64_t *offsets, typename cx<T>::t *grid)
 {
+    __shared__ WaveLight wls[kWaveLightsLds];
+    const bool staged = stage_wave(P, g, wls);
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nvert) return;
     const VertexRec r = verts[e];
@@ -255,8 +276,13 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
     const PathRec pr = paths[p];
     double amp = sqrt(r.weight / pr.px);
     if (P.do_cbs && e != offsets[p]) amp *= 1 / sqrt(2.0);
-    quad_scatter_pass<T>(P, g, pr.x1, r.x, amp, r.xi, grid);
-    if (P.do_cbs) quad_scatter_pass<T>(P, g, r.x, pr.x1, amp, r.xi, grid);
+    if (staged) {
+        quad_scatter_pass<T>(P, g, g.wave, wls, pr.x1, r.x, amp, r.xi, grid);
+        if (P.do_cbs) quad_scatter_pass<T>(P, g, g.wave, wls, r.x, pr.x1, amp, r.xi, grid);
+    } else {
+        quad_scatter_pass<T>(P, g, g.wave, g.wlk, pr.x1, r.x, amp, r.xi, grid);
+        if (P.do_cbs) quad_scatter_pass<T>(P, g, g.wave, g.wlk, r.x, pr.x1, amp, r.xi, grid);
+    }
 }
 
 // The walk and the scatter in one pass: every vertex is scattered as the walk
@@ -268,14 +294,16 @@ __global__ __launch_bounds__(256) void quad_scatter_kernel(DevProblem P, QuadGeo
 // first uniform of draw n, taken before the step); *nvert receives the vertex
 // count (the normaliser), one atomic per workgroup.  *ctr (zeroed) hands out
 // the path indices [0, np).
-template <typename T>
+template <typename T, bool WAVE>
 __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, QuadGeom g, uint64_t p0, int64_t np,
                                                                  typename cx<T>::t *grid,
                                                                  unsigned long long *nvert, unsigned long long *ctr)
 {
     __shared__ unsigned long long ws[4];
     __shared__ double lcos[kIcdfLds];
+    __shared__ WaveLight wls[WAVE ? kWaveLightsLds : 1];
     const double *tab = stage_icdf(P.amp, lcos);
+    if (WAVE) stage_wave(P, g, wls);   // a fused wave has at most kWaveLightsLds lights
     unsigned long long cnt = 0;
     WavePaths wp;
     bool active = false;
@@ -305,11 +333,11 @@ __global__ __launch_bounds__(256) void quad_trace_scatter_kernel(DevProblem P, Q
             const double2 a = uniform_pair(P.seed, path, n, 0);
             double amp = sqrt(weight / px);
             if (P.do_cbs && n > 1) amp *= 1 / sqrt(2.0);
-            quad_forward_term<T>(P, g, x, amp, a.x, S);
-            if (P.do_cbs) quad_scatter_pass<T>(P, g, x, x1, amp, a.x, grid);
+            quad_forward_term<T>(P, g, WAVE, wls, x, amp, a.x, S);
+            if (P.do_cbs) quad_scatter_pass<T>(P, g, WAVE, wls, x, x1, amp, a.x, grid);
             ++cnt;
             if (!path_step<3>(P, path, n, x, w, ow, weight, a.y, tab)) {
-                quad_flush_path<T>(P, g, x1, S, grid);
+                quad_flush_path<T>(P, g, WAVE, x1, S, grid);
                 active = false;
             }
         }
@@ -639,14 +667,20 @@ void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t 
                                unsigned long long *nvert, unsigned long long *ctr, hipStream_t s)
 {
     if (np <= 0) return;
+    auto go = [&](auto kernel, auto *gr) {
+        hipLaunchKernelGGL(kernel, dim3(resident_blocks((const void *)kernel, np)), dim3(256), 0, s, P, g, p0, np, gr,
+                           nvert, ctr);
+    };
     if (fp32) {
-        const void *k = (const void *)quad_trace_scatter_kernel<float>;
-        hipLaunchKernelGGL(quad_trace_scatter_kernel<float>, dim3(resident_blocks(k, np)), dim3(256), 0, s, P, g, p0,
-                           np, (float2 *)grid, nvert, ctr);
+        if (g.wave)
+            go(quad_trace_scatter_kernel<float, true>, (float2 *)grid);
+        else
+            go(quad_trace_scatter_kernel<float, false>, (float2 *)grid);
     } else {
-        const void *k = (const void *)quad_trace_scatter_kernel<double>;
-        hipLaunchKernelGGL(quad_trace_scatter_kernel<double>, dim3(resident_blocks(k, np)), dim3(256), 0, s, P, g, p0,
-                           np, (double2 *)grid, nvert, ctr);
+        if (g.wave)
+            go(quad_trace_scatter_kernel<double, true>, (double2 *)grid);
+        else
+            go(quad_trace_scatter_kernel<double, false>, (double2 *)grid);
     }
 }
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,

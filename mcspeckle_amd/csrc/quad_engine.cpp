@@ -116,7 +116,7 @@ struct QuadWorkspace {
     std::mutex mu;
     hipStream_t s = nullptr;
     ProblemState ps;
-    DevBuf dgz, grid, W, tw, T1, H, u, dwl, dcount;
+    DevBuf dgz, grid, W, tw, T1, H, u, dwl, dwlk, dcount;
     double *host_u = nullptr;   // pinned staging of the result (a pageable copy ran at ~2 GB/s)
     size_t host_u_bytes = 0;
 };
@@ -175,7 +175,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
     {
         ProblemState &ps = ws.ps;
         DevBuf &dgz = ws.dgz, &grid = ws.grid, &W = ws.W, &tw = ws.tw, &T1 = ws.T1, &H = ws.H, &u = ws.u,
-               &dwl = ws.dwl, &dcount = ws.dcount;
+               &dwl = ws.dwl, &dwlk = ws.dwlk, &dcount = ws.dcount;
         const int nx = g.nx, nv0 = g.nv0, nsl = g.nz - 1;
         const int nl = wave ? 1 : pr.nl;   // output planes
         const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
@@ -189,25 +189,35 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             g.gz = (const double *)dgz.p;
             if (wave) {
                 std::vector<double> hw((size_t)pr.nl * 2);
+                // the FP32-grid kernels' form: phase gradients in cycles per
+                // unit length, weights in FP32
+                std::vector<WaveLight> hk((size_t)pr.nl);
+                const double kc = vsign * (1.0 / pr.lambda);
                 for (int l = 0; l < pr.nl; ++l) {
                     hw[2 * (size_t)l] = wl_re[l];
                     hw[2 * (size_t)l + 1] = wl_im ? wl_im[l] : 0.0;
+                    hk[(size_t)l] = WaveLight{kc * pr.l[3 * (size_t)l], kc * pr.l[3 * (size_t)l + 1],
+                                              kc * pr.l[3 * (size_t)l + 2], (float)hw[2 * (size_t)l],
+                                              (float)hw[2 * (size_t)l + 1]};
                 }
-                if (!dwl.ensure(sizeof(double) * hw.size(), &rc) ||
+                if (!dwl.ensure(sizeof(double) * hw.size(), &rc) || !dwlk.ensure(sizeof(WaveLight) * hk.size(), &rc) ||
                     !hip_ok(hipMemcpyAsync(dwl.p, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, s),
+                            "wl", &rc) ||
+                    !hip_ok(hipMemcpyAsync(dwlk.p, hk.data(), sizeof(WaveLight) * hk.size(), hipMemcpyHostToDevice, s),
                             "wl", &rc) ||
                     !hip_ok(hipStreamSynchronize(s), "wl sync", &rc))
                     break;
                 g.wave = 1;
                 g.vsign = vsign;
                 g.wl = (const double2 *)dwl.p;
+                g.wlk = (const WaveLight *)dwlk.p;
             }
             launch_zero(grid.p, (cs * ngrid + 15) / 16 * 16 > grid.bytes ? grid.bytes : (cs * ngrid + 15) / 16 * 16, s);
-            // At most kQuadFusedLights lights: walk and scatter in one pass (no
-            // vertex records, no counting pass).  More: trace into records,
-            // then a vertex-parallel scatter (in the fused pass the walk's
-            // divergence would multiply the per-vertex light loop).
-            const bool fused = pr.nl <= kQuadFusedLights;
+            // At most kQuadFusedLights lights (a wave: kWaveLightsLds, its one
+            // output plane and its lights staged in LDS): walk and scatter in
+            // one pass (no vertex records, no counting pass).  More: trace
+            // into records, then a vertex-parallel scatter.
+            const bool fused = wave ? pr.nl <= kWaveLightsLds : pr.nl <= kQuadFusedLights;
             if (fused) {
                 // [0]: vertex count; [1 + launch]: the launch's path counter
                 const int64_t kLaunch = (int64_t)1 << 24;   // paths per launch

@@ -112,3 +112,27 @@ def test_gpu_quad_wave_matches_oracle(oracle, cbs, vmax, vstp, lam, hw, T, vsign
     err = np.abs(u - u0).max() / np.abs(u0).max()
     say("  err", err, "grid", u.shape)
     assert err <= tol
+
+
+@pytest.mark.parametrize("nl,cbs", [(40, True), (600, False), (600, True)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-4)])
+def test_gpu_quad_wave_many_lights(oracle, nl, cbs, precision, tol):
+    """Waves of many plane waves: up to 512 lights the walk and the scatter
+    run in one pass with the lights staged in LDS; beyond that the walks are
+    stored as vertex records and the vertex-parallel scatter reads the lights
+    from HBM."""
+    rng = np.random.default_rng(nl)
+    lx, ly = rng.uniform(-0.05, 0.05, nl), rng.uniform(-0.05, 0.05, nl)
+    l = np.vstack([lx, ly, np.sqrt(1 - lx ** 2 - ly ** 2)])
+    wl = weights(nl, seed=6)
+    bmin, bmax = [-2.0, -2.0, 0.0], [2.0, 2.0, 0.6]
+    n = 200
+    say("quad wave lights", nl, cbs, precision)
+    u0, K = oracle.mcfield_fourier_quad_wave(4.0, 0.97, bmin, bmax, l, wl, 0.3, 0.05, n, 0.5, cbs, 2, None, -1.0, 9)
+    st = {}
+    u = E.MCfieldFourierQuadOnWave(4.0, 0.97, bmin, bmax, l, 0.3, 0.05, n, 0.5, cbs, 2, None, wl, -1.0, seed=9,
+                                   precision=precision, stats=st)
+    assert st["vertices"] == K
+    err = np.abs(u - u0).max() / np.abs(u0).max()
+    say("  err", err, "grid", u.shape)
+    assert err <= tol
