@@ -60,8 +60,9 @@ struct QuadView {
 // matrix with the shifts, flips and crop folded in (tw: n twiddles e^{2 pi i r/n}).
 void launch_quad_dft_matrix(const QuadGeom &g, const double2 *tw, void *W, bool fp32, hipStream_t s);
 // H[b] = W X[b] W^T for batch slabs by two factored-DFT passes in LDS (n =
-// n1 n2); false when n is prime or factors too unevenly (then nothing is
-// launched and the DFT-matrix GEMMs are used).  tw: the n twiddles.
+// n1 n2); false when n is prime, factors too unevenly or one vector of it
+// does not fit in LDS (then nothing is launched and the DFT-matrix GEMMs are
+// used).  tw: the n twiddles.
 bool launch_quad_dft2(const QuadGeom &g, int batch, const void *X, void *T1, void *H, const double2 *tw, bool fp32,
                       hipStream_t s);
 // u[o, l] = scale * boxexp0[o] * sum_s H[s, l][o] exp(zexp0[o] tz_s)

@@ -486,11 +486,44 @@ void launch_quad_dft_matrix(const QuadGeom &g, const double2 *tw, void *W, bool 
 // foff = nva_pad - a this is the row (or column) transform of H = W X W^T with
 // W[o][ix] = w^((o + nva_pad - a)(ix + nx_pad - h) mod n).
 // A workgroup takes nv consecutive vectors of one batch in LDS (two buffers of
-// nv x n) with the n twiddles.
+// nv x n) with the n twiddles and a table of the index split k = c n2 + f
+// (both stages walk the outputs k in that form; stage 1's twiddle exponent is
+// c f < n).  Every per-element index is stepped, not divided: a thread's
+// elements are t, t + 256, ..., and (t / m, t mod m) advances by a constant.
 struct DftPass {
     int n, n1, n2, nin, nout, nvec, nv, qoff, foff;
     int64_t in_bs, in_vs, in_es, out_bs, out_vs, out_es;
 };
+
+// (t / m, t mod m) for t = t0, t0 + stride, ...
+struct DivStep {
+    int q, r, dq, dr, m;
+    __device__ DivStep(int t0, int m_, int stride) : m(m_)
+    {
+        q = t0 / m;
+        r = t0 - q * m;
+        dq = stride / m;
+        dr = stride - dq * m;
+    }
+    __device__ void next()
+    {
+        q += dq;
+        r += dr;
+        if (r >= m) {
+            r -= m;
+            ++q;
+        }
+    }
+};
+
+// (e + off) mod n for 0 <= e < n and -n < off < n
+__device__ __forceinline__ int wrap_index(int e, int off, int n)
+{
+    int q = e + off;
+    q += q < 0 ? n : 0;
+    q -= q >= n ? n : 0;
+    return q;
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void dft_pass_kernel(DftPass d, const typename cx<T>::t *__restrict__ in,
@@ -502,12 +535,15 @@ __global__ __launch_bounds__(256) void dft_pass_kernel(DftPass d, const typename
     C2 *tws = (C2 *)dft_lds;          // n twiddles
     C2 *xa = tws + d.n;               // nv x n: input, then stage 2 output
     C2 *xb = xa + (size_t)d.nv * d.n; // nv x n: stage 1 output
+    int *kd = (int *)(xb + (size_t)d.nv * d.n);   // n entries: c | f << 16
     const int b = blockIdx.y, j0 = blockIdx.x * d.nv;
     const int nv = min(d.nv, d.nvec - j0);
     const int n = d.n, n1 = d.n1, n2 = d.n2;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         tws[r].x = (T)tw[r].x;
         tws[r].y = (T)tw[r].y;
+        const int c = r / n2;
+        kd[r] = c | ((r - c * n2) << 16);
     }
     for (int e = threadIdx.x; e < d.nv * n; e += blockDim.x) {
         xa[e].x = 0;
@@ -516,70 +552,70 @@ __global__ __launch_bounds__(256) void dft_pass_kernel(DftPass d, const typename
     __syncthreads();
     // load: consecutive threads read consecutive addresses
     const C2 *ib = in + (size_t)b * d.in_bs;
-    for (int t = threadIdx.x; t < nv * d.nin; t += blockDim.x) {
-        int v, e;
-        if (d.in_es == 1) {
-            v = t / d.nin;
-            e = t - v * d.nin;
-        } else {
-            e = t / nv;
-            v = t - e * nv;
-        }
-        int q = (e + d.qoff) % n;
-        if (q < 0) q += n;
-        xa[v * n + q] = ib[(size_t)(j0 + v) * d.in_vs + (size_t)e * d.in_es];
+    if (d.in_es == 1) {
+        DivStep it(threadIdx.x, d.nin, blockDim.x);   // (v, e)
+        for (int t = threadIdx.x; t < nv * d.nin; t += blockDim.x, it.next())
+            xa[it.q * n + wrap_index(it.r, d.qoff, n)] = ib[(size_t)(j0 + it.q) * d.in_vs + it.r];
+    } else {
+        DivStep it(threadIdx.x, nv, blockDim.x);   // (e, v)
+        for (int t = threadIdx.x; t < nv * d.nin; t += blockDim.x, it.next())
+            xa[it.r * n + wrap_index(it.q, d.qoff, n)] = ib[(size_t)(j0 + it.r) * d.in_vs + (size_t)it.q * d.in_es];
     }
     __syncthreads();
     // stage 1: Z[q1 n2 + f2] = w^(q1 f2) sum_q2 w2^(q2 f2) x[q1 + n1 q2]
-    for (int t = threadIdx.x; t < nv * n; t += blockDim.x) {
-        const int v = t / n, k = t - v * n, q1 = k / n2, f2 = k - q1 * n2;
-        const C2 *x = xa + v * n;
-        T sr = 0, si = 0;
-        int r = 0;   // (q2 f2) mod n2
-        for (int q2 = 0; q2 < n2; ++q2) {
-            const C2 w = tws[r * n1];
-            const C2 a = x[q1 + n1 * q2];
-            sr += a.x * w.x - a.y * w.y;
-            si += a.x * w.y + a.y * w.x;
-            r += f2;
-            if (r >= n2) r -= n2;
+    {
+        DivStep it(threadIdx.x, n, blockDim.x);   // (v, k)
+        for (int t = threadIdx.x; t < nv * n; t += blockDim.x, it.next()) {
+            const int dk = kd[it.r], q1 = dk & 0xffff, f2 = dk >> 16;
+            const C2 *x = xa + it.q * n + q1;
+            const int inc = f2 * n1;
+            T sr = 0, si = 0;
+            int rn = 0;   // ((q2 f2) mod n2) n1
+            for (int q2 = 0; q2 < n2; ++q2, x += n1) {
+                const C2 w = tws[rn];
+                const C2 a = *x;
+                sr += a.x * w.x - a.y * w.y;
+                si += a.x * w.y + a.y * w.x;
+                rn += inc;
+                rn -= rn >= n ? n : 0;
+            }
+            const C2 w = tws[q1 * f2];
+            xb[t].x = sr * w.x - si * w.y;
+            xb[t].y = sr * w.y + si * w.x;
         }
-        const C2 w = tws[(q1 * f2) % n];
-        xb[v * n + k].x = sr * w.x - si * w.y;
-        xb[v * n + k].y = sr * w.y + si * w.x;
     }
     __syncthreads();
     // stage 2: Y[f2 + n2 f1] = sum_q1 w1^(q1 f1) Z[q1 n2 + f2]
-    for (int t = threadIdx.x; t < nv * n; t += blockDim.x) {
-        const int v = t / n, k = t - v * n, f1 = k / n2, f2 = k - f1 * n2;
-        const C2 *z = xb + v * n;
-        T sr = 0, si = 0;
-        int r = 0;   // (q1 f1) mod n1
-        for (int q1 = 0; q1 < n1; ++q1) {
-            const C2 w = tws[r * n2];
-            const C2 a = z[q1 * n2 + f2];
-            sr += a.x * w.x - a.y * w.y;
-            si += a.x * w.y + a.y * w.x;
-            r += f1;
-            if (r >= n1) r -= n1;
+    {
+        DivStep it(threadIdx.x, n, blockDim.x);   // (v, k)
+        for (int t = threadIdx.x; t < nv * n; t += blockDim.x, it.next()) {
+            const int dk = kd[it.r], f1 = dk & 0xffff, f2 = dk >> 16;
+            const C2 *z = xb + it.q * n + f2;
+            const int inc = f1 * n2;
+            T sr = 0, si = 0;
+            int rn = 0;   // ((q1 f1) mod n1) n2
+            for (int q1 = 0; q1 < n1; ++q1, z += n2) {
+                const C2 w = tws[rn];
+                const C2 a = *z;
+                sr += a.x * w.x - a.y * w.y;
+                si += a.x * w.y + a.y * w.x;
+                rn += inc;
+                rn -= rn >= n ? n : 0;
+            }
+            xa[it.q * n + f2 + n2 * f1].x = sr;
+            xa[it.q * n + f2 + n2 * f1].y = si;
         }
-        xa[v * n + f2 + n2 * f1].x = sr;
-        xa[v * n + f2 + n2 * f1].y = si;
     }
     __syncthreads();
     C2 *ob = out + (size_t)b * d.out_bs;
-    for (int t = threadIdx.x; t < nv * d.nout; t += blockDim.x) {
-        int v, o;
-        if (d.out_es == 1) {
-            v = t / d.nout;
-            o = t - v * d.nout;
-        } else {
-            o = t / nv;
-            v = t - o * nv;
-        }
-        int f = (o + d.foff) % n;
-        if (f < 0) f += n;
-        ob[(size_t)(j0 + v) * d.out_vs + (size_t)o * d.out_es] = xa[v * n + f];
+    if (d.out_es == 1) {
+        DivStep it(threadIdx.x, d.nout, blockDim.x);   // (v, o)
+        for (int t = threadIdx.x; t < nv * d.nout; t += blockDim.x, it.next())
+            ob[(size_t)(j0 + it.q) * d.out_vs + it.r] = xa[it.q * n + wrap_index(it.r, d.foff, n)];
+    } else {
+        DivStep it(threadIdx.x, nv, blockDim.x);   // (o, v)
+        for (int t = threadIdx.x; t < nv * d.nout; t += blockDim.x, it.next())
+            ob[(size_t)(j0 + it.r) * d.out_vs + (size_t)it.q * d.out_es] = xa[it.r * n + wrap_index(it.q, d.foff, n)];
     }
 }
 
@@ -599,7 +635,8 @@ bool dft_factor(int n, int *n1, int *n2)
 
 // H[b] = W X[b] W^T for every slab (batch b) by two factored-DFT passes:
 // along ix into T (nv0 x nx per batch), then along iy into H (nv0 x nv0).
-// Returns false (nothing launched) when n does not factor usefully.
+// Returns false (nothing launched) when n does not factor usefully or one
+// vector of it does not fit in LDS.
 bool launch_quad_dft2(const QuadGeom &g, int batch, const void *X, void *T1, void *H, const double2 *tw, bool fp32,
                       hipStream_t s)
 {
@@ -608,10 +645,12 @@ bool launch_quad_dft2(const QuadGeom &g, int batch, const void *X, void *T1, voi
     if (!dft_factor(n, &n1, &n2)) return false;
     const int h = n / 2, am = n - 1 - h;
     const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
-    // two nv x n buffers and the twiddles within 64 KB of LDS
-    int nv = (int)((65536 / cs - n) / (2 * (size_t)n));
-    nv = std::max(1, std::min(nv, 16));
-    const size_t lds = cs * ((size_t)n + 2 * (size_t)nv * n);
+    // two nv x n buffers, the twiddles and the index table within 64 KB of
+    // LDS; a transform too long for one vector goes to the DFT-matrix GEMMs
+    const int64_t fixed = (int64_t)(cs + sizeof(int)) * n, per = 2 * (int64_t)cs * n;
+    if (fixed + per > 65536) return false;
+    const int nv = (int)std::min<int64_t>(16, (65536 - fixed) / per);
+    const size_t lds = (size_t)(fixed + per * nv);
     DftPass a;
     a.n = n;
     a.n1 = n1;
