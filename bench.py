@@ -316,6 +316,7 @@ def run_legs(args):
         for _ in range(args.leg_repeats - 1):
             fn()                      # earlier calls: first-use costs (code objects, allocations)
         out[key] = fn()
+        out[key]["call"] = f"{args.leg_repeats} of {args.leg_repeats}"
     return out
 
 
@@ -362,7 +363,7 @@ def main():
     ap.add_argument("--quad-paths", type=int, default=5000000)
     ap.add_argument("--hetro-paths", type=int, default=200000)
     ap.add_argument("--no-legs", action="store_true")
-    ap.add_argument("--leg-repeats", type=int, default=1, help="calls per leg; the last one is reported")
+    ap.add_argument("--leg-repeats", type=int, default=2, help="calls per leg; the last one is reported")
     ap.add_argument("--no-headline", action="store_true",
                     help="skip the headline covariance (profiling the other legs on their own)")
     ap.add_argument("--legs", default="all",

@@ -393,14 +393,15 @@ def MCfieldFourierQuad(sigt, albedo, box_min, box_max, l, v_max, v_stp, maxItr, 
         raise ScmcArgumentError("MCfieldFourierQuad is 3-D only")
     nv0, _, _ = quad_grid(spec.box_min, spec.box_max, v_max, v_stp, lambda_)
     nl = spec.nl
-    u_re, u_im = np.empty(nv0 * nv0 * nl), np.empty(nv0 * nv0 * nl)
+    # interleaved complex output (the imaginary pointer NULL)
+    u = np.empty(nv0 * nv0 * nl, dtype=np.complex128)
     st = N.Stats()
     p = c_problem(spec)
     N.check(N.lib().scmc_mcfield_fourier_quad(C.byref(p), float(v_max), float(v_stp), int(maxItr), float(vsign),
-                                              _prec(precision), N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+                                              _prec(precision), N.dptr(u.view(np.float64)), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (u_re + 1j * u_im).reshape((nv0, nv0, nl), order="F")
+    return u.reshape((nv0, nv0, nl), order="F")
 
 
 def MCfieldFourierQuadOnWave(sigt, albedo, box_min, box_max, l, v_max, v_stp, maxItr, lambda_, doCBS, smpFlg,
@@ -417,15 +418,15 @@ def MCfieldFourierQuadOnWave(sigt, albedo, box_min, box_max, l, v_max, v_stp, ma
         raise ScmcArgumentError("Wl (one weight per light) is required")
     wr, wi = _weights(Wl, spec.nl)
     nv0, _, _ = quad_grid(spec.box_min, spec.box_max, v_max, v_stp, lambda_, wave=True)
-    u_re, u_im = np.empty(nv0 * nv0), np.empty(nv0 * nv0)
+    u = np.empty(nv0 * nv0, dtype=np.complex128)   # interleaved complex output
     st = N.Stats()
     p = c_problem(spec)
     N.check(N.lib().scmc_mcfield_fourier_quad_wave(C.byref(p), N.dptr(wr), N.dptr(wi), float(v_max), float(v_stp),
-                                                   int(maxItr), float(vsign), _prec(precision), N.dptr(u_re),
-                                                   N.dptr(u_im), C.byref(st)))
+                                                   int(maxItr), float(vsign), _prec(precision),
+                                                   N.dptr(u.view(np.float64)), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (u_re + 1j * u_im).reshape((nv0, nv0), order="F")
+    return u.reshape((nv0, nv0), order="F")
 
 
 # ------------------------------------------------------------ path sets

@@ -99,3 +99,24 @@ def test_gpu_quad_walk_phase_functions(oracle, sct_type, nl, cbs, nbins, precisi
     err = np.abs(u - u0).max() / np.abs(u0).max()
     say("  err", err, "grid", u.shape)
     assert err <= tol
+
+
+def test_gpu_quad_split_and_interleaved_outputs_agree():
+    """The C-ABI writes split complex (real and imaginary arrays) or, with a
+    NULL imaginary pointer, interleaved complex; the Python wrapper uses the
+    latter.  Both hold the same field (FP64 grid: only the atomics' order
+    differs between the two runs)."""
+    import ctypes as C
+    from mcspeckle_amd import _native as N
+    l = lights(2)
+    bmin, bmax = [-2.0, -2.0, 0.0], [2.0, 2.0, 0.6]
+    u = E.MCfieldFourierQuad(4.0, 0.97, bmin, bmax, l, 0.3, 0.05, 300, 0.5, True, 2, None, -1.0, seed=9,
+                             precision="fp64")
+    spec = E.make_problem(4.0, 0.97, bmin, bmax, l, np.zeros((3, 0)), 1, 1, 0.5, True, 2, 1, 0, None, seed=9,
+                          need_views=False)
+    n = u.size
+    re, im = np.empty(n), np.empty(n)
+    p = E.c_problem(spec)
+    N.check(N.lib().scmc_mcfield_fourier_quad(C.byref(p), 0.3, 0.05, 300, -1.0, N.FP64, N.dptr(re), N.dptr(im), None))
+    v = (re + 1j * im).reshape(u.shape, order="F")
+    assert np.abs(v - u).max() <= 1e-12 * np.abs(u).max()
