@@ -565,6 +565,8 @@ class Server:
 
     def submit(self, spec, max_itr, precision="mixed", kind="cov"):
         """kind "cov" (MCcov) or "field" (MCfield)."""
+        if kind not in ("cov", "field"):
+            raise ScmcArgumentError(f'kind must be "cov" or "field", not {kind!r}')
         p = c_problem(spec)
         h = C.c_int64()
         fn = N.lib().scmc_submit_mccov if kind == "cov" else N.lib().scmc_submit_mcfield
