@@ -75,6 +75,7 @@ def lib():
         L.orc_trace_count.restype = C.c_int64
         L.orc_trace_fill.argtypes = [P, C.c_uint64, C.c_int64, C.POINTER(OPathSet)]
         L.orc_mccov.argtypes = [P, C.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_mccov_mt.argtypes = [P, C.c_int64, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_mccov_sums.argtypes = [P, C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_mccov_replay.argtypes = [P, C.POINTER(OPathSet), C.c_int64, _dp, _dp, _dp, _dp]
         L.orc_mcfield.argtypes = [P, C.c_int64, _dp, _dp]
@@ -172,13 +173,17 @@ def _pathset(ps):
     return c
 
 
-def mccov(spec, max_itr):
-    """code/MCcov.m restated, paths [0, max_itr) of spec.seed (path_offset must be 0)."""
+def mccov(spec, max_itr, nthreads=1):
+    """code/MCcov.m restated, paths [0, max_itr) of spec.seed (path_offset must be 0);
+    nthreads > 1 sums the paths in OpenMP shards."""
     L = lib()
     p, keep = problem(spec)
     ne = (spec.nv * spec.nl) ** 2
     a = [np.empty(ne) for _ in range(4)]
-    L.orc_mccov(C.byref(p), int(max_itr), *[_p(x) for x in a])
+    if nthreads > 1:
+        L.orc_mccov_mt(C.byref(p), int(max_itr), int(nthreads), *[_p(x) for x in a])
+    else:
+        L.orc_mccov(C.byref(p), int(max_itr), *[_p(x) for x in a])
     shape = (spec.nv, spec.nv, spec.nl, spec.nl)
     return (a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F")
 

@@ -655,12 +655,12 @@ static void split_out(const cplx *src, size_t n, double scale, double *re, doubl
     for (size_t i = 0; i < n; ++i) { re[i] = creal(src[i]) * scale; im[i] = cimag(src[i]) * scale; }
 }
 
-void orc_mccov_sums(const orc_problem *pr, uint64_t p0, int64_t np, int nthreads,
-                    double *ms_re, double *ms_im, double *mm_re, double *mm_im)
+/* Unnormalised sums over paths [p0, p0 + np) into Ms, Mm (zeroed), over
+ * nthreads OpenMP path shards each with its own accumulators. */
+static void mccov_sums_into(const orc_problem *pr, uint64_t p0, int64_t np, int nthreads, cplx *Ms, cplx *Mm)
 {
     size_t ne = (size_t)pr->nv * pr->nv * pr->nl * pr->nl;
     if (nthreads < 1) nthreads = 1;
-    cplx *Ms = calloc(ne, sizeof(cplx)), *Mm = calloc(ne, sizeof(cplx));
 #ifdef _OPENMP
     if (nthreads > 1) {
 #pragma omp parallel num_threads(nthreads)
@@ -676,6 +676,15 @@ void orc_mccov_sums(const orc_problem *pr, uint64_t p0, int64_t np, int nthreads
     } else
 #endif
         mccov_range(pr, p0, np, Ms, Mm);
+    (void)ne;
+}
+
+void orc_mccov_sums(const orc_problem *pr, uint64_t p0, int64_t np, int nthreads,
+                    double *ms_re, double *ms_im, double *mm_re, double *mm_im)
+{
+    size_t ne = (size_t)pr->nv * pr->nv * pr->nl * pr->nl;
+    cplx *Ms = calloc(ne, sizeof(cplx)), *Mm = calloc(ne, sizeof(cplx));
+    mccov_sums_into(pr, p0, np, nthreads, Ms, Mm);
     split_out(Ms, ne, 1.0, ms_re, ms_im);
     split_out(Mm, ne, 1.0, mm_re, mm_im);
     free(Ms); free(Mm);
@@ -702,6 +711,19 @@ void orc_mccov(const orc_problem *pr, int64_t max_itr,
     size_t ne = (size_t)pr->nv * pr->nv * pr->nl * pr->nl;
     cplx *Ms = calloc(ne, sizeof(cplx)), *Mm = calloc(ne, sizeof(cplx));
     mccov_range(pr, 0, max_itr, Ms, Mm);
+    mccov_normalise(pr, max_itr, Ms, Mm, ne, ms_re, ms_im, mm_re, mm_im);
+    free(Ms); free(Mm);
+}
+
+/* orc_mccov with the paths summed over nthreads shards (the sums of the
+ * shards are added in a thread-dependent order, so results agree with
+ * orc_mccov to rounding). */
+void orc_mccov_mt(const orc_problem *pr, int64_t max_itr, int nthreads,
+                  double *ms_re, double *ms_im, double *mm_re, double *mm_im)
+{
+    size_t ne = (size_t)pr->nv * pr->nv * pr->nl * pr->nl;
+    cplx *Ms = calloc(ne, sizeof(cplx)), *Mm = calloc(ne, sizeof(cplx));
+    mccov_sums_into(pr, 0, max_itr, nthreads, Ms, Mm);
     mccov_normalise(pr, max_itr, Ms, Mm, ne, ms_re, ms_im, mm_re, mm_im);
     free(Ms); free(Mm);
 }

@@ -138,6 +138,8 @@ void orc_trace_fill(const orc_problem *pr, uint64_t p0, int64_t np, orc_pathset 
  * Out: Ms, Mm as nv x nv x nl x nl complex, column-major. */
 void orc_mccov(const orc_problem *pr, int64_t max_itr,
                double *ms_re, double *ms_im, double *mm_re, double *mm_im);
+void orc_mccov_mt(const orc_problem *pr, int64_t max_itr, int nthreads,
+                  double *ms_re, double *ms_im, double *mm_re, double *mm_im);
 /* Same, over paths [p0, p0+np), UNnormalised sums (for sharding and timing),
  * nthreads > 1 splits the paths over OpenMP threads with private sums. */
 void orc_mccov_sums(const orc_problem *pr, uint64_t p0, int64_t np, int nthreads,
