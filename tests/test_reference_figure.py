@@ -1,22 +1,31 @@
-"""The reference's own published output as a pin.
+"""The reference's own published outputs as pins.
 
-examples/buildFarFieldCovMatrix/farFieldCov.m runs scmc() on a 2-D target
-(boxArea(1, 10, [-10,10], [-10,10]): wavelength 1, MFP 10, so OD 2 over the
-20-unit slab) with the measured tabulated amplitude of scatteringAmplitude.mat,
-views at 0..359 deg and a light at 0 deg, with and without CBS, and plots
-diag(abs(C(:,:,1,1))) (farFieldCov.m:111-122).  The figure in the committed
-farFieldCov.pdf, digitised by tests/golden/make_farfieldcov_curve.py into
-tests/golden/farFieldCov_diagonal.json, is MATLAB's result for 1.2e4 paths.
+The reference ships, beside four example scripts, the figures its author
+made by running them in MATLAB (1e3 paths x 12 parfor repeats each);
+tests/golden/make_reference_figures.py digitises them from the PDFs' rasters
+into tests/golden/*.json.  The oracle (CPU) and the GPU engines (through
+scmc(), as the scripts call it) must reproduce them within each figure's own
+accuracy: MATLAB's Monte Carlo noise at 1.2e4 paths (for the covariance
+diagonals, its CBS and no-CBS curves, equal in expectation away from 180 deg,
+differ by 1-2 %) plus the digitisation (half a pixel: about 0.005 in
+intensity, 0.5 deg in angle, which matters on steep flanks).  Together they
+pin, against the reference's actual output and independently of the
+restatement:
 
-Both the oracle (CPU) and the GPU engines (through scmc()) must reproduce it
-within the figure's own accuracy: MATLAB's Monte Carlo noise (its CBS and
-no-CBS curves, equal in expectation away from 180 deg, differ by 1-2 %) plus
-the digitisation (half a pixel: 0.005 in intensity, 0.5 deg in angle, which
-matters on the steep flanks).  This pins the whole MCcov chain -- tabulated
-table build and sampling, 2-D far-field NEE, CBS (Mm halved, peak at the
-backscatter direction), the V sigma / maxItr normalisation and C = Ms + Mm --
-against the reference's actual output, independently of the restatement.
+* farFieldCov.m:111-122 -- the 2-D far-field covariance with the measured
+  tabulated amplitude of scatteringAmplitude.mat: table build and sampling,
+  far-field NEE, CBS (Mm halved, the backscatter peak), V sigma / maxItr,
+  C = Ms + Mm;
+* nearFieldCov.m -- the same with near-field sensors on a circle (point-source
+  NEE, spreading and entrance directions) and an HG(-0.5) scatterer;
+* memoryEffectMeasurements.m -- the angular memory effect of a 3-D slab of
+  OD 10 (the phase structure across lights and views), also against the
+  closed form the script plots, (2 pi L dtheta / sinh(2 pi L dtheta))^2;
+* multipleAndSingleScattering.m -- the split of the correlation under a
+  rotation into single scattering (Csingle = Ms, flat) and multiple
+  scattering (decaying), in 3-D.
 """
+import importlib
 import json
 import os
 
@@ -26,77 +35,242 @@ import pytest
 from mcspeckle_amd import interface as I
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIG = json.load(open(os.path.join(HERE, "golden", "farFieldCov_diagonal.json")))
-
-# per-point and mean relative tolerances (see the module docstring)
-TOL_POINT, TOL_MEAN_NCBS, TOL_MEAN_CBS, TOL_PEAK = 0.07, 0.02, 0.035, 0.04
 
 
-def example_inputs():
+def fixture(name):
+    return json.load(open(os.path.join(HERE, "golden", name + ".json")))
+
+
+FAR, NEAR = fixture("farFieldCov_diagonal"), fixture("nearFieldCov_diagonal")
+MEMORY, ROTATION = fixture("memoryEffect_curve"), fixture("multipleAndSingleScattering_curves")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------------ inputs
+
+def scattering_amplitude():
     g = np.load(os.path.join(HERE, "golden", "scatteringAmplitude.npz"))
-    scatter = I.tabulatedAmplitudeScatter(g["theta"], g["farField_re"] + 1j * g["farField_im"])
+    return I.tabulatedAmplitudeScatter(g["theta"], g["farField_re"] + 1j * g["farField_im"])
+
+
+def diagonal_inputs(kind):
+    """farFieldCov.m:3-26 / nearFieldCov.m: target, views, light 0 deg, scatter."""
     target = I.boxArea(1, 10, [-10, 10], [-10, 10])
-    views = I.farFieldSource(np.deg2rad(np.arange(360.0)), 0)
     light = I.farFieldSource(np.deg2rad([0.0]), 0)
-    return target, views, light, scatter
+    if kind == "far":
+        return target, I.farFieldSource(np.deg2rad(np.arange(360.0)), 0), light, scattering_amplitude()
+    return target, I.nearFieldSource(20, np.deg2rad(np.arange(360.0))), light, I.HGScatter(-0.5)
 
 
-def check_against_figure(diag_cbs, diag_ncbs):
-    ref_n = np.array(FIG["without_cbs"], dtype=float)
-    dev_n = (diag_ncbs - ref_n) / ref_n
-    ref_c = np.array([np.nan if v is None else v for v in FIG["with_cbs"]])
-    seen = ~np.isnan(ref_c)
-    dev_c = (diag_cbs[seen] - ref_c[seen]) / ref_c[seen]
-    peak = (diag_cbs[180] - FIG["cbs_peak_180"]) / FIG["cbs_peak_180"]
-    print("without CBS: mean |dev| %.4f max %.4f; with CBS: mean |dev| %.4f max %.4f; peak %.4f (%.3f vs %.3f)"
-          % (np.abs(dev_n).mean(), np.abs(dev_n).max(), np.abs(dev_c).mean(), np.abs(dev_c).max(), peak,
-             diag_cbs[180], FIG["cbs_peak_180"]))
-    assert np.abs(dev_n).max() <= TOL_POINT
-    assert np.abs(dev_n).mean() <= TOL_MEAN_NCBS
-    assert np.abs(dev_c).mean() <= TOL_MEAN_CBS
-    assert abs(peak) <= TOL_PEAK
-    # the coherent backscattering cone: 180 deg lies ~1.5x above the no-CBS curve
-    assert 1.35 <= diag_cbs[180] / diag_ncbs[180] <= 1.65
+def memory_inputs():
+    """memoryEffectMeasurements.m: the target, scatterer and the (views, lights)
+    pair of every tilt."""
+    theta0, L, OD = 0.002, 2000.0, 10.0
+    theta = np.deg2rad(np.arange(21) * 0.001 + theta0)
+    views_vec = np.deg2rad(np.arange(-10, 0) * 0.1)
+    target = I.boxArea(1, L / OD, [0, L], [-50000, 50000], [-50000, 50000])
+    scatter = I.HGScatter(0, 0)
+    runs = []
+    for t2 in theta:
+        dt = t2 - theta[0]
+        runs.append((I.farFieldSource(np.concatenate([views_vec, views_vec - dt]), 0),
+                     I.farFieldSource(np.array([theta[0], t2]), 0)))
+    return target, scatter, runs, theta - theta[0], L
 
 
-def test_figure_fixture_is_consistent():
-    """The digitised curves have the figure's landmarks: the minima near 78 and
-    282 deg, the forward maximum at 0/359 deg, the CBS peak at 180 deg."""
-    n = np.array(FIG["without_cbs"])
+def memory_correlation(C, nvec=10):
+    """memoryEffectMeasurements.m: the intensity correlation from C."""
+    s1, s2 = slice(0, nvec), slice(nvec, 2 * nvec)
+    d = lambda M: np.mean(np.diagonal(M))
+    sig = np.zeros((2, 2), complex)
+    sig[0, 0] = d(C[s1, s1, 0, 0])
+    sig[1, 1] = d(C[s2, s2, 1, 1])
+    if abs(d(C[s2, s1, 0, 1])) > abs(d(C[s1, s2, 0, 1])):
+        sig[0, 1], sig[1, 0] = d(C[s2, s1, 0, 1]), d(C[s1, s2, 1, 0])
+    else:
+        sig[0, 1], sig[1, 0] = d(C[s1, s2, 0, 1]), d(C[s2, s1, 1, 0])
+    sI = np.real(np.array([[sig[0, 0] ** 2, sig[0, 1] * sig[1, 0]], [0, sig[1, 1] ** 2]]))
+    return sI[0, 1] / np.sqrt(sI[1, 1] * sI[0, 0])
+
+
+def rotation_inputs():
+    """multipleAndSingleScattering.m: target, scatterer and (views, lights) per rotation."""
+    span = np.deg2rad(22)
+    target = I.boxArea(1, 20, [-50, 50], [-50, 50], [-50, 50])
+    l1 = np.array([-np.sin(span), 0, np.cos(span)])
+    v1 = np.array([-np.sin(span), 0, -np.cos(span)])
+    runs = []
+    for a in np.deg2rad(ROTATION["x"]):
+        R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+        runs.append((I.farFieldSource(np.column_stack([v1, R @ v1])), I.farFieldSource(np.column_stack([l1, R @ l1]))))
+    return target, I.isotropicScatter(), runs
+
+
+def rotation_curves(results):
+    """(single, multiple, total) / max(total) from (Csingle, C) per rotation."""
+    single, mul, total = [], [], []
+    for Cs, C in results:
+        s, t = Cs[:, :, 0, 1], C[:, :, 0, 1]
+        k = np.argmax(np.abs(t).ravel(order="F"))
+        sv, tv = s.ravel(order="F")[k], t.ravel(order="F")[k]
+        single.append(abs(sv))
+        mul.append(abs(tv - sv))
+        total.append(abs(tv))
+    m = max(total)
+    return np.array(single) / m, np.array(mul) / m, np.array(total) / m
+
+
+# ------------------------------------------------------------------ checks
+
+def check_diagonal(fig, diag_cbs, diag_ncbs, tol_point=0.07, tol_mean=0.02, tol_mean_cbs=0.035, tol_peak=0.04,
+                   cbs_ratio=(1.35, 1.65)):
+    ref_n = np.array([np.nan if v is None else v for v in fig["without_cbs"]])
+    ok_n = ~np.isnan(ref_n)
+    dev_n = (diag_ncbs[ok_n] - ref_n[ok_n]) / ref_n[ok_n]
+    ref_c = np.array([np.nan if v is None else v for v in fig["with_cbs"]])
+    ok_c = ~np.isnan(ref_c)
+    dev_c = (diag_cbs[ok_c] - ref_c[ok_c]) / ref_c[ok_c]
+    peak = (diag_cbs[180] - fig["cbs_peak"]) / fig["cbs_peak"]
+    say("without CBS: mean |dev| %.4f max %.4f; with CBS: mean |dev| %.4f max %.4f; peak %.4f (%.4g vs %.4g), "
+        "CBS / no CBS at 180: %.3f" % (np.abs(dev_n).mean(), np.abs(dev_n).max(), np.abs(dev_c).mean(),
+                                       np.abs(dev_c).max(), peak, diag_cbs[180], fig["cbs_peak"],
+                                       diag_cbs[180] / diag_ncbs[180]))
+    assert np.abs(dev_n).max() <= tol_point
+    assert np.abs(dev_n).mean() <= tol_mean
+    assert np.abs(dev_c).mean() <= tol_mean_cbs
+    assert abs(peak) <= tol_peak
+    assert cbs_ratio[0] <= diag_cbs[180] / diag_ncbs[180] <= cbs_ratio[1]
+
+
+def check_memory(c, dtheta, L):
+    ref = np.array([np.nan if v is None else v for v in MEMORY["measured"]])
+    x = 2 * np.pi * L * dtheta
+    theory = np.where(x == 0, 1.0, (x / np.sinh(np.where(x == 0, 1.0, x))) ** 2)
+    ok = ~np.isnan(ref)
+    dr, dt = c[ok] - ref[ok], c - theory
+    say("memory effect:", np.round(c, 3).tolist())
+    say("  vs the reference's measured curve: mean |dev| %.4f max %.4f; vs the closed form: mean |dev| %.4f max %.4f"
+        % (np.abs(dr).mean(), np.abs(dr).max(), np.abs(dt).mean(), np.abs(dt).max()))
+    assert abs(c[0] - 1) < 1e-9
+    assert np.abs(dr).max() <= 0.06 and np.abs(dr).mean() <= 0.025
+    assert np.abs(dt).max() <= 0.08 and np.abs(dt).mean() <= 0.04
+    assert np.all(np.diff(c[c > 0.1]) < 0.02)          # decays with the tilt
+
+
+def check_rotation(single, mul, total):
+    ref = {k: np.array(ROTATION[k], dtype=float) for k in ("single", "multiple", "total")}
+    say("rotation: single", np.round(single, 3).tolist())
+    say("  multiple", np.round(mul, 3).tolist())
+    for name, ours in (("single", single), ("multiple", mul), ("total", total)):
+        d = ours - ref[name]
+        say("  %s vs the reference: mean |dev| %.4f max %.4f" % (name, np.abs(d[1:]).mean(), np.abs(d[1:]).max()))
+        # at 0 deg the figure's lines meet the frame (total = 1 by construction)
+        assert np.abs(d[1:]).max() <= 0.05 and np.abs(d[1:]).mean() <= 0.02, name
+    assert abs(total[0] - 1) < 1e-12
+
+
+def test_figure_fixtures_are_consistent():
+    """The digitised curves carry the figures' landmarks."""
+    n = np.array(FAR["without_cbs"], dtype=float)
     assert len(n) == 360 and np.all(np.isfinite(n))
     assert 70 <= int(np.argmin(n[:180])) <= 85 and 275 <= 180 + int(np.argmin(n[180:])) <= 290
-    assert n[0] > 3.7 and n[359] > 3.7
-    assert 4.5 <= FIG["cbs_peak_180"] <= 4.9
+    assert 4.5 <= FAR["cbs_peak"] <= 4.9
+    nn = np.array([np.nan if v is None else v for v in NEAR["without_cbs"]])
+    assert 170 <= int(np.nanargmax(nn)) <= 190 and 0.3 <= NEAR["cbs_peak"] <= 0.36
+    m = [v for v in MEMORY["measured"] if v is not None]
+    assert m[0] > 0.95 and m[-1] < 0.05
+    assert abs(np.mean(ROTATION["single"]) - 0.326) < 0.01 and ROTATION["multiple"][-1] < 0.05
 
 
-def oracle_diag(oracle, cbs, n_paths):
-    import importlib
+# ------------------------------------------------------------------ oracle
+
+def oracle_cov(oracle, target, views, lights, scatter, cbs, n, seed=0):
     from mcspeckle_amd import engines as E
     S = importlib.import_module("mcspeckle_amd.scmc")
-    target, views, light, scatter = example_inputs()
-    Config = {"targetArea": target, "views": dict(views, directions=views["directions2D"]),
-              "lights": dict(light, directions=light["directions2D"]), "scatter": scatter, "CBS": cbs,
+    tag = "3D" if target["D"] == 3 else "2D"
+    srcs = []
+    for src in (views, lights):
+        src = dict(src)
+        for base in ("directions", "positions"):
+            if base + tag in src:
+                src[base] = src[base + tag]
+        srcs.append(src)
+    Config = {"targetArea": target, "views": srcs[0], "lights": srcs[1], "scatter": scatter, "CBS": cbs,
               "uniformFirstScatter": False}
     a = S._engine_args(Config)
     spec = E.make_problem(a["sigt"], a["albedo"], a["box_min"], a["box_max"], a["l"], a["v"], a["is_ff_l"],
                           a["is_ff_v"], a["lambda_"], a["doCBS"], a["smpFlg"], a["sct_type"], a["ampfunc"],
-                          a["ampfunc0"], seed=0)
-    Ms, Mm = oracle.mccov(spec, n_paths, nthreads=min(8, os.cpu_count() or 1))
+                          a["ampfunc0"], seed=seed)
+    return oracle.mccov(spec, n, nthreads=min(8, os.cpu_count() or 1))
+
+
+def oracle_diag(oracle, kind, cbs, n):
+    target, views, light, scatter = diagonal_inputs(kind)
+    Ms, Mm = oracle_cov(oracle, target, views, light, scatter, cbs, n)
     return np.abs(np.diagonal((Ms + Mm)[:, :, 0, 0]))
 
 
-def test_oracle_reproduces_published_figure(oracle):
+def test_oracle_reproduces_far_field_figure(oracle):
     n = 30000
-    check_against_figure(oracle_diag(oracle, True, n), oracle_diag(oracle, False, n))
+    check_diagonal(FAR, oracle_diag(oracle, "far", True, n), oracle_diag(oracle, "far", False, n))
 
+
+def test_oracle_reproduces_near_field_figure(oracle):
+    n = 20000
+    check_diagonal(NEAR, oracle_diag(oracle, "near", True, n), oracle_diag(oracle, "near", False, n),
+                   tol_mean=0.03, cbs_ratio=(1.0, 1.12))
+
+
+def test_oracle_reproduces_rotation_figure(oracle):
+    target, scatter, runs = rotation_inputs()
+    res = []
+    for views, lights in runs:
+        Ms, Mm = oracle_cov(oracle, target, views, lights, scatter, True, 30000)
+        res.append((Ms, Ms + Mm))
+    check_rotation(*rotation_curves(res))
+
+
+# ------------------------------------------------------------------ GPU
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["far", "near"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_reproduces_published_figure(precision):
+def test_gpu_reproduces_diagonal_figures(kind, precision):
     from mcspeckle_amd import scmc
-    target, views, light, scatter = example_inputs()
+    target, views, light, scatter = diagonal_inputs(kind)
     out = {}
     for cbs in (True, False):
         res = scmc(target, views, light, scatter, 200000, "CBS", cbs, "rng", 0, precision=precision)
         out[cbs] = np.abs(np.diagonal(res["C"][:, :, 0, 0]))
-    check_against_figure(out[True], out[False])
+    if kind == "far":
+        check_diagonal(FAR, out[True], out[False])
+    else:
+        check_diagonal(NEAR, out[True], out[False], tol_mean=0.03, cbs_ratio=(1.0, 1.12))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_reproduces_memory_effect_figure(precision):
+    from mcspeckle_amd import scmc
+    target, scatter, runs, dtheta, L = memory_inputs()
+    c = []
+    for k, (views, lights) in enumerate(runs):
+        res = scmc(target, views, lights, scatter, 100000, "CBS", False, "uniformFirstScatter", False, "rng", 7,
+                   precision=precision)
+        c.append(memory_correlation(res["C"]))
+    check_memory(np.array(c), dtheta, L)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_reproduces_rotation_figure(precision):
+    from mcspeckle_amd import scmc
+    target, scatter, runs = rotation_inputs()
+    res = []
+    for views, lights in runs:
+        r = scmc(target, views, lights, scatter, 200000, "singleScattering", True, "rng", 3, precision=precision)
+        res.append((r["Csingle"], r["C"]))
+    check_rotation(*rotation_curves(res))
