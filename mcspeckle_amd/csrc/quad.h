@@ -44,8 +44,10 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
                          hipStream_t s);
 // Walk paths [p0, p0 + np) and scatter every vertex into the slab grids in
 // the same pass; *nvert (device) is incremented by the vertex count; *ctr
-// (device, zeroed before the launch) hands out the path indices.
-void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
+// (device, zeroed before the launch) hands out the path indices.  False (and
+// nothing launched) for more lights than the pass holds: kQuadFusedLights
+// separate lights, or a wave of more than kWaveLightsLds.
+bool launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
                                unsigned long long *nvert, unsigned long long *ctr, hipStream_t s);
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s);

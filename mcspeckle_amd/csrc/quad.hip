@@ -702,10 +702,11 @@ void launch_quad_scatter(const DevProblem &P, const QuadGeom &g, int64_t nvert, 
                            offsets, (double2 *)grid);
 }
 
-void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
+bool launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t p0, int64_t np, void *grid, bool fp32,
                                unsigned long long *nvert, unsigned long long *ctr, hipStream_t s)
 {
-    if (np <= 0) return;
+    if (g.wave ? P.nl > kWaveLightsLds : P.nl > kQuadFusedLights) return false;
+    if (np <= 0) return true;
     auto go = [&](auto kernel, auto *gr) {
         hipLaunchKernelGGL(kernel, dim3(resident_blocks((const void *)kernel, np)), dim3(256), 0, s, P, g, p0, np, gr,
                            nvert, ctr);
@@ -721,6 +722,7 @@ void launch_quad_trace_scatter(const DevProblem &P, const QuadGeom &g, uint64_t 
         else
             go(quad_trace_scatter_kernel<double, false>, (double2 *)grid);
     }
+    return true;
 }
 void launch_cgemm(int M, int N, int K, const void *A, int64_t sa, const void *B, int64_t sb, void *C, int64_t sc,
                   int batch, bool transB, bool fp32, hipStream_t s)

@@ -227,12 +227,16 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                 auto a = std::chrono::steady_clock::now();
                 for (int64_t p0 = 0, li = 0; p0 < max_itr; p0 += kLaunch, ++li) {
                     const int64_t np = std::min<int64_t>(max_itr - p0, kLaunch);
-                    launch_quad_trace_scatter(ps.dp, g, (uint64_t)p0, np, grid.p, fp32,
-                                              (unsigned long long *)dcount.p, (unsigned long long *)dcount.p + 1 + li,
-                                              s);
+                    if (!launch_quad_trace_scatter(ps.dp, g, (uint64_t)p0, np, grid.p, fp32,
+                                                   (unsigned long long *)dcount.p,
+                                                   (unsigned long long *)dcount.p + 1 + li, s)) {
+                        rc = set_error(SCMC_E_UNSUPPORTED, "fused Fourier pass for %d lights", pr.nl);
+                        break;
+                    }
                     st.paths += np;
                     st.chunks += 1;
                 }
+                if (rc) break;
                 unsigned long long hv = 0;
                 if (!hip_ok(hipMemcpyAsync(&hv, dcount.p, sizeof hv, hipMemcpyDeviceToHost, s), "vertex count", &rc) ||
                     !hip_ok(hipStreamSynchronize(s), "quad scatter", &rc))
