@@ -398,7 +398,8 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
         !ctr.ensure(sizeof(unsigned long long), &rc))
         return rc;
     if (M)
-        launch_hetero_trace(P, *M, p0, n, (int32_t *)counts.p, nullptr, nullptr, nullptr, nullptr, false, s);
+        launch_hetero_trace(P, *M, p0, n, (int32_t *)counts.p, nullptr, nullptr, nullptr, nullptr, false,
+                            (unsigned long long *)ctr.p, s);
     else
         launch_trace_count(P, p0, n, (int32_t *)counts.p, (unsigned long long *)ctr.p, s);
     launch_scan((const int32_t *)counts.p, n, (int64_t *)offsets.p, (int64_t *)tsum.p, s);
@@ -413,7 +414,7 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
         return rc;
     if (M)
         launch_hetero_trace(P, *M, p0, n, nullptr, (int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
-                            (int32_t *)vpath.p, true, s);
+                            (int32_t *)vpath.p, true, (unsigned long long *)ctr.p, s);
     else
         launch_trace_emit(P, p0, n, (const int64_t *)offsets.p, (PathRec *)paths.p, (VertexRec *)verts.p,
                           (int32_t *)vpath.p, (unsigned long long *)ctr.p, s);

@@ -13,95 +13,137 @@
 
 namespace scmc {
 
+// Both passes keep their lanes resident and refill them from a path counter
+// (take_path, refill_due), as the homogeneous tracer does: Woodcock walks in
+// a voxel medium vary in length as much as free flights in a slab.  Outputs
+// are indexed by path, so they do not depend on which lane walked a path.
 template <int DIM>
 __global__ __launch_bounds__(256) void hetero_count_kernel(DevProblem P, HetMedium M, uint64_t p0, int64_t np,
-                                                           int32_t *counts)
+                                                           int32_t *counts, unsigned long long *ctr)
 {
     __shared__ double lcos[kIcdfLds];
     const double *tab = stage_icdf(P.amp, lcos);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const uint64_t path = P.path_offset + p0 + (uint64_t)i;
-    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
-    double2 w0p;
-    if (!hetero_path_start<DIM>(P, M, path, x, px, w, w0p)) {
-        counts[i] = 1;
-        return;
-    }
-    double weight = med_albedo<DIM>(P, M, x);
+    WavePaths wp;
+    bool active = false;
+    int64_t id = 0;
+    uint64_t path = 0;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        ++n;
-        const double2 a = uniform_pair(P.seed, path, n, 0);
-        const double2 e = uniform_pair(P.seed, path, n, 3);
-        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) break;
+        if (refill_due(wp, active)) {
+            const int64_t t = take_path(wp, !active, np, ctr);
+            if (t >= 0) {
+                id = t;
+                path = P.path_offset + p0 + (uint64_t)t;
+                double2 w0p;
+                ow[0] = ow[1] = ow[2] = 0;   // a new path has no incoming direction
+                n = 0;
+                if (hetero_path_start<DIM>(P, M, path, x, px, w, w0p)) {
+                    weight = med_albedo<DIM>(P, M, x);
+                    active = true;
+                } else {
+                    counts[t] = 1;   // stored as one zero-weight vertex
+                }
+            }
+        }
+        if (!__ballot(active)) {
+            // a path whose first vertex misses the box ends at its start, so
+            // an idle wave ends only once the counter is exhausted
+            if (wp.exhausted) break;
+            continue;
+        }
+        if (active) {
+            ++n;
+            const double2 a = uniform_pair(P.seed, path, n, 0);
+            const double2 e = uniform_pair(P.seed, path, n, 3);
+            if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) {
+                counts[id] = (int32_t)n;
+                active = false;
+            }
+        }
     }
-    counts[i] = (int32_t)n;
 }
 
 template <int DIM>
 __global__ __launch_bounds__(256) void hetero_emit_kernel(DevProblem P, HetMedium M, uint64_t p0, int64_t np,
                                                           const int64_t *offsets, PathRec *paths, VertexRec *verts,
-                                                          int32_t *vpath)
+                                                          int32_t *vpath, unsigned long long *ctr)
 {
     __shared__ double lcos[kIcdfLds];
     const double *tab = stage_icdf(P.amp, lcos);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const uint64_t path = P.path_offset + p0 + (uint64_t)i;
-    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px;
-    double2 w0p;
-    const bool ok = hetero_path_start<DIM>(P, M, path, x, px, w, w0p);
-    int64_t o = offsets[i];
-    PathRec pr;
-    if (!ok) {
-        for (int j = 0; j < 3; ++j) {
-            pr.x1[j] = j < DIM ? 0.5 * (P.bmin[j] + P.bmax[j]) : 0.0;
-            pr.w1[j] = P.meanl[j];
-        }
-        pr.px = 1;
-        pr.pad = 0;
-        pr.w0p = make_double2(P.w0p_iso, 0.0);
-        paths[i] = pr;
-        VertexRec r;
-        for (int j = 0; j < 3; ++j) {
-            r.x[j] = pr.x1[j];
-            r.ow[j] = 0;
-        }
-        r.weight = 0;
-        r.xi = 0;
-        verts[o] = r;
-        vpath[o] = (int32_t)i;
-        return;
-    }
-    for (int j = 0; j < 3; ++j) {
-        pr.x1[j] = x[j];
-        pr.w1[j] = w[j];
-    }
-    pr.px = px;
-    pr.pad = 0;
-    pr.w0p = w0p;
-    paths[i] = pr;
-    double weight = med_albedo<DIM>(P, M, x);
-    const int64_t end = offsets[i + 1];
+    WavePaths wp;
+    bool active = false;
+    int64_t id = 0, o = 0, end = 0;
+    uint64_t path = 0;
+    double x[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ow[3] = {0, 0, 0}, px = 1, weight = 0;
     uint32_t n = 0;
     for (;;) {
-        ++n;
-        const double2 a = uniform_pair(P.seed, path, n, 0);
-        const double2 e = uniform_pair(P.seed, path, n, 3);
-        if (o < end) {
-            VertexRec r;
-            for (int j = 0; j < 3; ++j) {
-                r.x[j] = x[j];
-                r.ow[j] = ow[j];
+        if (refill_due(wp, active)) {
+            const int64_t t = take_path(wp, !active, np, ctr);
+            if (t >= 0) {
+                id = t;
+                path = P.path_offset + p0 + (uint64_t)t;
+                double2 w0p;
+                ow[0] = ow[1] = ow[2] = 0;   // a new path has no incoming direction
+                o = offsets[t];
+                end = offsets[t + 1];
+                n = 0;
+                PathRec pr;
+                pr.pad = 0;
+                if (hetero_path_start<DIM>(P, M, path, x, px, w, w0p)) {
+                    for (int j = 0; j < 3; ++j) {
+                        pr.x1[j] = x[j];
+                        pr.w1[j] = w[j];
+                    }
+                    pr.px = px;
+                    pr.w0p = w0p;
+                    weight = med_albedo<DIM>(P, M, x);
+                    active = true;
+                } else {
+                    // one vertex of weight 0 at the box centre
+                    for (int j = 0; j < 3; ++j) {
+                        pr.x1[j] = j < DIM ? 0.5 * (P.bmin[j] + P.bmax[j]) : 0.0;
+                        pr.w1[j] = P.meanl[j];
+                    }
+                    pr.px = 1;
+                    pr.w0p = make_double2(P.w0p_iso, 0.0);
+                    VertexRec r;
+                    for (int j = 0; j < 3; ++j) {
+                        r.x[j] = pr.x1[j];
+                        r.ow[j] = 0;
+                    }
+                    r.weight = 0;
+                    r.xi = 0;
+                    verts[o] = r;
+                    vpath[o] = (int32_t)t;
+                }
+                paths[t] = pr;
             }
-            r.weight = weight;
-            r.xi = a.x;
-            verts[o] = r;
-            vpath[o] = (int32_t)i;
-            ++o;
         }
-        if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) break;
+        if (!__ballot(active)) {
+            // a path whose first vertex misses the box ends at its start, so
+            // an idle wave ends only once the counter is exhausted
+            if (wp.exhausted) break;
+            continue;
+        }
+        if (active) {
+            ++n;
+            const double2 a = uniform_pair(P.seed, path, n, 0);
+            const double2 e = uniform_pair(P.seed, path, n, 3);
+            if (o < end) {
+                VertexRec r;
+                for (int j = 0; j < 3; ++j) {
+                    r.x[j] = x[j];
+                    r.ow[j] = ow[j];
+                }
+                r.weight = weight;
+                r.xi = a.x;
+                verts[o] = r;
+                vpath[o] = (int32_t)id;
+                ++o;
+            }
+            if (!hetero_path_step<DIM>(P, M, path, n, x, w, ow, weight, a.y, e.y, tab)) active = false;
+        }
     }
 }
 
@@ -153,21 +195,26 @@ __global__ void hetero_mean_reduce_kernel(int64_t n, int nsplit, const double2 *
 }
 
 void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
-                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit, hipStream_t s)
+                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit,
+                         unsigned long long *ctr, hipStream_t s)
 {
     if (np <= 0) return;
-    dim3 g((unsigned)((np + 255) / 256));
+    launch_zero(ctr, sizeof(unsigned long long), s);
+    auto go = [&](auto kernel, auto... args) {
+        hipLaunchKernelGGL(kernel, dim3(resident_blocks((const void *)kernel, np)), dim3(256), 0, s, P, M, p0, np,
+                           args..., ctr);
+    };
     if (!emit) {
         if (P.dim == 3)
-            hipLaunchKernelGGL(hetero_count_kernel<3>, g, dim3(256), 0, s, P, M, p0, np, counts);
+            go(hetero_count_kernel<3>, counts);
         else
-            hipLaunchKernelGGL(hetero_count_kernel<2>, g, dim3(256), 0, s, P, M, p0, np, counts);
+            go(hetero_count_kernel<2>, counts);
         return;
     }
     if (P.dim == 3)
-        hipLaunchKernelGGL(hetero_emit_kernel<3>, g, dim3(256), 0, s, P, M, p0, np, offsets, paths, verts, vpath);
+        go(hetero_emit_kernel<3>, (const int64_t *)offsets, paths, verts, vpath);
     else
-        hipLaunchKernelGGL(hetero_emit_kernel<2>, g, dim3(256), 0, s, P, M, p0, np, offsets, paths, verts, vpath);
+        go(hetero_emit_kernel<2>, (const int64_t *)offsets, paths, verts, vpath);
 }
 
 void launch_hetero_mean(const DevProblem &P, int64_t np, const PathRec *paths, const double2 *EV0,

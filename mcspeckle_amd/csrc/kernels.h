@@ -75,9 +75,11 @@ void launch_nee_cov(const DevProblem &P, const HetMedium *M, int64_t np, int64_t
                     bool mixed, hipStream_t s);
 
 // ------------------------------------------------------ voxel media
-// Walks of MCCovHetro.m: count pass (emit = false, fills counts) or emit pass.
+// Walks of MCCovHetro.m: count pass (emit = false, fills counts) or emit pass;
+// ctr: one device counter the launch clears and the walk hands paths out from.
 void launch_hetero_trace(const DevProblem &P, const HetMedium &M, uint64_t p0, int64_t np, int32_t *counts,
-                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit, hipStream_t s);
+                         int64_t *offsets, PathRec *paths, VertexRec *verts, int32_t *vpath, bool emit,
+                         unsigned long long *ctr, hipStream_t s);
 // mean1 += sum_p af_ang_vl EV0[p] UL0[p]^T (MCCovHetro.m:210-211); part holds
 // kHetMeanSplits * nv * nl double2.
 constexpr int kHetMeanSplits = 64;
