@@ -164,7 +164,7 @@ def serve(n_req, rate, path_scale, max_active, shared=False, seed=5, timeline=No
     return out
 
 
-def field_leg(n_paths, precision="mixed"):
+def field_leg(n_paths, precision="mixed", out=None):
     """MCfield speckle image: slab T=2 (x, y in +-5e4), sigma_s = 10,
     sigma_a = 0.05, HG g = 0.8, one plane wave at normal incidence, 128 x 128
     far-field views on a uniform grid over +-10 degrees, CBS off."""
@@ -173,13 +173,15 @@ def field_leg(n_paths, precision="mixed"):
     spec = W.speckle_image_128()
     st = {}
     t0 = time.perf_counter()
-    E.mcfield_spec(spec, n_paths, precision, st)
+    u = E.mcfield_spec(spec, n_paths, precision, st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(u)
     return {"paths": n_paths, "views": spec.nv, "value": n_paths * spec.nv / dt, "unit": "paths*entries/s",
             "seconds": dt, "ms_accum": st["ms_accum"], "vertices": st["vertices"], "precision": precision}
 
 
-def quad_leg(n_paths, precision="mixed"):
+def quad_leg(n_paths, precision="mixed", out=None):
     """Fourier-accelerated field: box 10 x 10 x 1, sigma_s = 8, sigma_a = 0.02,
     tabulated phase function (256 angle bins, mixture of 3 HG lobes with
     g ~ U[-0.3, 0.95] and Dirichlet(1) weights, seeded), one light at normal
@@ -194,12 +196,14 @@ def quad_leg(n_paths, precision="mixed"):
                              sct_type=a["sct_type"], ampfunc=a["ampfunc"], seed=a["seed"], precision=precision,
                              stats=st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(u)
     return {"paths": n_paths, "grid": list(u.shape[:2]), "value": n_paths * u.shape[0] * u.shape[1] / dt,
             "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_scatter": st["ms_accum"],
             "ms_dft": st["ms_nee"], "precision": precision}
 
 
-def field_wave_leg(n_paths, precision="mixed"):
+def field_wave_leg(n_paths, precision="mixed", out=None):
     """MCfieldOnWave: the field's view grid (128 x 128 over +-10 degrees, slab
     T=2, sigma_s = 10, sigma_a = 0.05, HG g = 0.8) under one illumination wave
     of 64 plane waves (8 x 8 directions within +-5 degrees, unit-modulus
@@ -218,16 +222,18 @@ def field_wave_leg(n_paths, precision="mixed"):
     sigt = 10.05
     st = {}
     t0 = time.perf_counter()
-    E.MCfieldOnWave(sigt, 10 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 2], l, v, 1, 1, n_paths, 1.0, True, 2, 3, 0.8,
-                    Wl=wl, seed=0, precision=precision, stats=st)
+    u = E.MCfieldOnWave(sigt, 10 / sigt, [-5e4, -5e4, 0], [5e4, 5e4, 2], l, v, 1, 1, n_paths, 1.0, True, 2, 3, 0.8,
+                        Wl=wl, seed=0, precision=precision, stats=st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(u)
     return {"paths": n_paths, "views": v.shape[1], "lights": l.shape[1],
             "value": n_paths * v.shape[1] / dt, "unit": "paths*entries/s",
             "light_equivalent_value": n_paths * v.shape[1] * l.shape[1] / dt,
             "seconds": dt, "ms_accum": st["ms_accum"], "vertices": st["vertices"], "precision": precision}
 
 
-def quad_wave_leg(n_paths, precision="mixed"):
+def quad_wave_leg(n_paths, precision="mixed", out=None):
     """MCfieldFourierQuadOnWave: the Fourier leg's slab (box 10 x 10 x 1,
     sigma_s = 8, sigma_a = 0.02, isotropic as in the reference) under a wave of
     81 plane waves (9 x 9 sines within +-0.02, the layout of
@@ -246,12 +252,14 @@ def quad_wave_leg(n_paths, precision="mixed"):
     u = E.MCfieldFourierQuadOnWave(sigt, 8 / sigt, [-5, -5, 0], [5, 5, 1], l, v_max, v_stp, n_paths, lam, False, 2,
                                    None, wl, -1, seed=0, precision=precision, stats=st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(u)
     return {"paths": n_paths, "grid": list(u.shape), "lights": l.shape[1],
             "value": n_paths * u.shape[0] * u.shape[1] / dt, "unit": "paths*entries/s", "seconds": dt,
             "vertices": st["vertices"], "ms_scatter": st["ms_accum"], "ms_dft": st["ms_nee"], "precision": precision}
 
 
-def hetro_leg(n_paths, precision="mixed", seed=4):
+def hetro_leg(n_paths, precision="mixed", seed=4, out=None):
     """MCCovHetro: a voxel medium of 8 x 8 x 4 cells over x, y in +-10, z in
     [0, 2] with sigma ~ U[2, 10] per cell (seeded), albedo 0.95, HG g = 0.9,
     32 lights x 32 views on a 15 degree cone (a 1024^2 covariance), CBS on,
@@ -269,16 +277,18 @@ def hetro_leg(n_paths, precision="mixed", seed=4):
     l, v = cone(32), cone(32)
     st = {}
     t0 = time.perf_counter()
-    E.MCCovHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], l, v, 1, 1, n_paths, 1.0, True, 2, 3, 0.9,
-                 seed=0, precision=precision, stats=st)
+    res = E.MCCovHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], l, v, 1, 1, n_paths, 1.0, True, 2, 3,
+                       0.9, seed=0, precision=precision, stats=st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(res[0] + res[1])
     N = 32 * 32
     return {"paths": n_paths, "n": N, "cells": list(grid.shape), "value": n_paths * N * N / dt,
             "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_trace": st["ms_trace"],
             "ms_nee": st["ms_nee"], "ms_accum": st["ms_accum"], "precision": precision}
 
 
-def sample_hetro_leg(n_paths, precision="mixed", seed=4):
+def sample_hetro_leg(n_paths, precision="mixed", seed=4, out=None):
     """MCsampleHetro: the same 8 x 8 x 4-cell medium as the MCCovHetro leg,
     one light at normal incidence, 64 x 64 far-field views on +-10 degrees
     (the MCfield speckle-image convention), CBS on, smpFlg 2."""
@@ -291,32 +301,52 @@ def sample_hetro_leg(n_paths, precision="mixed", seed=4):
     v /= np.linalg.norm(v, axis=0)
     st = {}
     t0 = time.perf_counter()
-    E.MCsampleHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], [[0.0], [0.0], [1.0]], v, 1, 1, n_paths,
-                    1.0, True, 2, 3, 0.9, seed=0, precision=precision, stats=st)
+    u = E.MCsampleHetro(grid, 0.95, [-10, -10, 0], [10, 10, 2], [2.5, 2.5, 0.5], [[0.0], [0.0], [1.0]], v, 1, 1,
+                        n_paths, 1.0, True, 2, 3, 0.9, seed=0, precision=precision, stats=st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out.append(u)
     return {"paths": n_paths, "views": int(v.shape[1]), "cells": list(grid.shape), "value": n_paths * v.shape[1] / dt,
             "unit": "paths*entries/s", "seconds": dt, "vertices": st["vertices"], "ms_trace": st["ms_trace"],
             "ms_accum": st["ms_accum"], "precision": precision}
 
 
+def leg_error(fn, n_paths, cov):
+    """Mixed result against the FP64 GPU path (itself held to the FP64 oracle
+    at 1e-10 by the tests) on the same n_paths paths: max |a - b| / max |b|
+    for fields, the Cauchy-Schwarz metric for covariances."""
+    a, b = [], []
+    fn(n_paths, "mixed", out=a)
+    fn(n_paths, "fp64", out=b)
+    a, b = a[0], b[0]
+    err = cs_err(a, b) if cov else float(np.abs(a - b).max() / np.abs(b).max())
+    return {"mixed_vs_fp64_rel_err": err, "error_paths": n_paths,
+            "error_metric": "cauchy-schwarz" if cov else "max |du| / max |u|"}
+
+
 def run_legs(args):
-    legs = {"mcfield": ("mcfield", lambda: field_leg(args.field_paths)),
-            "quad": ("mcfield_fourier_quad", lambda: quad_leg(args.quad_paths)),
-            "wave": ("mcfield_on_wave", lambda: field_wave_leg(args.field_paths)),
-            "quad_wave": ("mcfield_fourier_quad_on_wave", lambda: quad_wave_leg(args.quad_paths)),
-            "hetro": ("mccov_hetro", lambda: hetro_leg(args.hetro_paths)),
-            "sample_hetro": ("mcsample_hetro", lambda: sample_hetro_leg(args.hetro_paths))}
+    # name: (key, timed call, paths of the mixed-vs-FP64 comparison, covariance?)
+    legs = {"mcfield": ("mcfield", lambda: field_leg(args.field_paths), field_leg, 100000, False),
+            "quad": ("mcfield_fourier_quad", lambda: quad_leg(args.quad_paths), quad_leg, 1000000, False),
+            "wave": ("mcfield_on_wave", lambda: field_wave_leg(args.field_paths), field_wave_leg, 100000, False),
+            "quad_wave": ("mcfield_fourier_quad_on_wave", lambda: quad_wave_leg(args.quad_paths), quad_wave_leg,
+                          1000000, False),
+            "hetro": ("mccov_hetro", lambda: hetro_leg(args.hetro_paths), hetro_leg, 20000, True),
+            "sample_hetro": ("mcsample_hetro", lambda: sample_hetro_leg(args.hetro_paths), sample_hetro_leg, 20000,
+                             False)}
     pick = list(legs) if args.legs == "all" else [x.strip() for x in args.legs.split(",") if x.strip()]
     out = {}
     for name in pick:
         if name not in legs:
             print(f"unknown leg {name!r} (known: {', '.join(legs)})", file=sys.stderr)
             sys.exit(2)
-        key, fn = legs[name]
+        key, fn, leg, n_err, cov = legs[name]
         for _ in range(args.leg_repeats - 1):
             fn()                      # earlier calls: first-use costs (code objects, allocations)
         out[key] = fn()
         out[key]["call"] = f"{args.leg_repeats} of {args.leg_repeats}"
+        if not args.no_leg_errors:
+            out[key].update(leg_error(leg, n_err, cov))
     return out
 
 
@@ -364,6 +394,8 @@ def main():
     ap.add_argument("--hetro-paths", type=int, default=200000)
     ap.add_argument("--no-legs", action="store_true")
     ap.add_argument("--leg-repeats", type=int, default=2, help="calls per leg; the last one is reported")
+    ap.add_argument("--no-leg-errors", action="store_true",
+                    help="skip the mixed-vs-FP64 comparison of each leg (untimed, after its timed calls)")
     ap.add_argument("--no-headline", action="store_true",
                     help="skip the headline covariance (profiling the other legs on their own)")
     ap.add_argument("--legs", default="all",
