@@ -197,3 +197,32 @@ def test_exp_smp_x_is_a_density_ratio(oracle, lz):
     assert abs(inv.mean() - 1) < 4 * se
     se2 = zs.std() / np.sqrt(n)
     assert abs(zs.mean() - 1.5) < 4 * se2
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_isotropic_directions_are_uniform(oracle, dim):
+    """smpampfunc_general.m:7-8 / MCcov.m:148: randn(dim,1) normalised is uniform
+    on the sphere (circle): unit length, zero mean, second moment I/dim, and in
+    3-D every coordinate uniform on [-1, 1] (Archimedes), in 2-D the angle
+    uniform on [0, 2 pi).  The oracle's Box-Muller pairs feed it."""
+    L = oracle.lib()
+    ph = O.OPhase()
+    ph.type = 1
+    ow = np.ascontiguousarray([0.0, 0.0, 1.0][:dim])
+    rng = np.random.default_rng(dim)
+    n = 20000
+    W = np.empty((n, dim))
+    w = np.zeros(dim)
+    for i in range(n):
+        u = np.ascontiguousarray(rng.uniform(size=4))
+        L.orc_sample_dir(O._p(ow), C.byref(ph), dim, O._p(u), O._p(w))
+        W[i] = w
+    assert np.allclose(np.linalg.norm(W, axis=1), 1, atol=1e-12)
+    assert np.abs(W.mean(axis=0)).max() < 4.5 / np.sqrt(n)
+    assert np.abs(W.T @ W / n - np.eye(dim) / dim).max() < 5 / np.sqrt(n)
+    if dim == 3:
+        for j in range(3):
+            assert stats.kstest(W[:, j], stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    else:
+        ang = np.mod(np.arctan2(W[:, 1], W[:, 0]), 2 * np.pi)
+        assert stats.kstest(ang, stats.uniform(0, 2 * np.pi).cdf).pvalue > 1e-3
