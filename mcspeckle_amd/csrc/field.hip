@@ -214,8 +214,7 @@ struct FEntry {
     double x[3];
     double ad[3];     // amplitude direction: ow (vertex) or -w1 (reversed)
     double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
-    float dlo[3];     // x - box_min and box_max - x, rounded to FP32 (mixed
-    float dhi[3];     // attenuation: the distance to the face a view leaves by)
+    float dlo[3];     // x - box_min rounded to FP32 (mixed attenuation)
     float adf[3];     // ad rounded to FP32 (mixed HG amplitude)
     int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
     int p;            // path of the entry
@@ -249,7 +248,6 @@ __device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64
     }
     for (int j = 0; j < 3; ++j) {
         f.dlo[j] = j < DIM ? (float)(f.x[j] - P.bmin[j]) : 0.f;
-        f.dhi[j] = j < DIM ? (float)(P.bmax[j] - f.x[j]) : 0.f;
         f.adf[j] = (float)f.ad[j];
     }
 }
@@ -264,27 +262,48 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
     return make_float2((float)a.x, (float)a.y);
 }
 
-// Far-field E(x, v) of the mixed variant: FP64 x.v and range reduction to a
-// fraction of a cycle, hardware sin/cos (input in revolutions), FP32
-// attenuation over the distance to the face the view leaves by (dlo/dhi: the
-// entry's x - box_min and box_max - x in FP32; inv, up: per view).
-// The view's direction v enters as x.v times nilam = -1/lambda: the far-field
-// phase is k x.(-v), and negating either factor is exact, so the cycle count
-// is bit-identical to (x.(-v))/lambda.  lamp: log2 of a real amplitude factor
-// folded into the same exp2 (0 for none), so an isotropic or HG amplitude
-// costs no extra exponential and no complex multiply.
+// Per-view constants of the mixed far-field E: the direction times
+// -1/lambda (the phase in cycles is then x . vn: the far-field phase is
+// k x.(-v)), and the distance over which the view leaves the box as an affine
+// function of the entry's x - box_min (dlo):
+//   (v_j > 0 ? W_j - dlo_j : dlo_j) / max(|v_j|, 1e-7) = fa_j dlo_j + fb_j,
+// W_j the box width (cubeDist.m:4-11 for a point inside the box).
+struct FarView {
+    double vn[3];
+    float fa[3], fb[3];
+};
 template <int DIM>
-__device__ __forceinline__ float2 e_far_mixed(const double *x, const double *v, double nilam, const float *dlo,
-                                              const float *dhi, const bool *up, const float *inv, float hs,
+__device__ __forceinline__ FarView far_view(const DevProblem &P, const double *v)
+{
+    FarView f;
+    for (int j = 0; j < 3; ++j) {
+        const double nd = j < DIM ? v[j] : 0.0;
+        f.vn[j] = nd * -P.inv_lambda;
+        const float inv = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
+        const float w = j < DIM ? (float)(P.bmax[j] - P.bmin[j]) : 0.f;
+        f.fa[j] = nd > 0 ? -inv : inv;
+        f.fb[j] = nd > 0 ? w * inv : 0.f;
+    }
+    return f;
+}
+
+// Far-field E(x, v) of the mixed variant: FP64 x.vn and range reduction to a
+// fraction of a cycle, hardware sin/cos (input in revolutions), FP32
+// attenuation over the distance to the face the view leaves by (one FMA per
+// face from the entry's dlo).  nhs = -sigma/2 log2(e).  lamp: log2 of a real
+// amplitude factor folded into the same exp2 (0 for none), so an isotropic
+// or HG amplitude costs no extra exponential and no complex multiply.
+template <int DIM>
+__device__ __forceinline__ float2 e_far_mixed(const double *x, const FarView &fv, const float *dlo, float nhs,
                                               float lamp = 0.f)
 {
-    const double cyc = dot<DIM>(x, v) * nilam;
+    const double cyc = dot<DIM>(x, fv.vn);
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
     // the first face starts the minimum (a start at +inf costs a min per call)
-    float best = (up[0] ? dhi[0] : dlo[0]) * inv[0];
-    for (int j = 1; j < DIM; ++j) best = fminf(best, (up[j] ? dhi[j] : dlo[j]) * inv[j]);
-    const float mag = __builtin_amdgcn_exp2f(fmaf(-hs * 1.4426950408889634f, best, lamp));
+    float best = fmaf(fv.fa[0], dlo[0], fv.fb[0]);
+    for (int j = 1; j < DIM; ++j) best = fminf(best, fmaf(fv.fa[j], dlo[j], fv.fb[j]));
+    const float mag = __builtin_amdgcn_exp2f(fmaf(nhs, best, lamp));
     return make_float2(mag * cs, mag * sn);
 }
 
@@ -334,8 +353,8 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     int vv[VPT];
     bool vok[VPT];
     double vd[VPT][3], mv[VPT][3];   // mv = -v: the far-field march direction (HET)
-    float vf[VPT][3], inv[VPT][3];
-    bool up[VPT][3];   // nd_j > 0 -> distance to the max face
+    float vf[VPT][3];
+    FarView fv[VPT];
     double2 afv[VPT][NLQ], acc[VPT][NLQ];
 #pragma unroll
     for (int t = 0; t < VPT; ++t) {
@@ -346,9 +365,8 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
         for (int j = 0; j < 3; ++j) {
             mv[t][j] = -vd[t][j];
             vf[t][j] = (float)vd[t][j];
-            inv[t][j] = (float)(1.0 / fmax(fabs(vd[t][j]), 1.0000000000000004e-07));
-            up[t][j] = vd[t][j] > 0;
         }
+        fv[t] = far_view<DIM>(P, vd[t]);
         for (int q = 0; q < NLQ; ++q) {
             afv[t][q] = (!WAVE && vok[t] && q < nlp) ? aftab[(size_t)v + (size_t)P.nv * (l0 + q)]
                                                       : make_double2(0.0, 0.0);
@@ -357,7 +375,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
     }
     const float g = (float)P.amp.g;
     const float g2p1 = 1.f + g * g, g2 = 2.f * g;
-    const float hs = (float)(P.sigt * 0.5);
+    const float nhs = (float)(P.sigt * 0.5) * -1.4426950408889634f;
     // FP32 amplitude in the log2 domain (evalampfunc_general.m:5-18,
     // evaluateHG.m:25-39): isotropic a = iso; HG a = hg_k t^mu with
     // t = 1 + g^2 - 2 g c, mu = -3/4 (3-D) or -1/2 (2-D).  Tabulated
@@ -370,7 +388,6 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
         float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
                    int, hg ? (float)log2(sqrt((1 - P.amp.g * P.amp.g) / ((DIM == 3) ? 4 * kPi : 2 * kPi)))
                            : (float)log2(P.amp.iso))));
-    const double nilam = -P.inv_lambda;
     const float mu = (DIM == 3) ? -0.75f : -0.5f;
 
     for (int64_t eb = e0; eb < e1; eb += kFieldBatch) {
@@ -401,10 +418,9 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             // wave first vertices: field_wave_first_kernel
             if (WAVE && kind == 0) continue;
             const double x[3] = {f.x[0], f.x[1], f.x[2]};
-            float dlo[3], dhi[3], adf[3];
+            float dlo[3], adf[3];
             for (int j = 0; j < 3; ++j) {
                 dlo[j] = f.dlo[j];
-                dhi[j] = f.dhi[j];
                 adf[j] = f.adf[j];
             }
             double2 sce[NLQ];
@@ -426,7 +442,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     if (HET) {
                         evf = e_far_mixed_het<DIM>(P, M, x, mv[t], lamp);
                     } else {
-                        evf = e_far_mixed<DIM>(x, vd[t], nilam, dlo, dhi, up[t], inv[t], hs, lamp);
+                        evf = e_far_mixed<DIM>(x, fv[t], dlo, nhs, lamp);
                     }
                     if (kind != 0 && tab) {
                         const float2 am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
@@ -632,21 +648,14 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
                                                                        double2 *hpart)
 {
     __shared__ double sx[kWavePB][3];
-    __shared__ float sdlo[kWavePB][3], sdhi[kWavePB][3];
+    __shared__ float sdlo[kWavePB][3];
     __shared__ double2 scl[kWavePB][kWaveLG];
     const int v = blockIdx.x * kFieldViews + threadIdx.x;
     const bool vok = v < P.nv;
     double vd[3] = {0, 0, 0};
-    float inv[3] = {0, 0, 0};
-    bool up[3] = {false, false, false};
-    const double nilam = -P.inv_lambda;
-    for (int j = 0; j < DIM && vok; ++j) {
-        const double nd = P.v[(size_t)v * DIM + j];
-        vd[j] = nd;
-        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
-        up[j] = nd > 0;
-    }
-    const float hs = (float)(P.sigt * 0.5);
+    for (int j = 0; j < DIM && vok; ++j) vd[j] = P.v[(size_t)v * DIM + j];
+    const FarView fv = far_view<DIM>(P, vd);
+    const float nhs = (float)(P.sigt * 0.5) * -1.4426950408889634f;
     const int l0 = blockIdx.y * kWaveLG;
     const int nlg = (P.nl - l0) < kWaveLG ? (P.nl - l0) : kWaveLG;
     const int split = blockIdx.z;
@@ -662,7 +671,6 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             const double xj = paths[pb + i].x1[j];
             sx[i][j] = xj;
             sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
-            sdhi[i][j] = j < DIM ? (float)(P.bmax[j] - xj) : 0.f;
         }
         for (int t = threadIdx.x; t < cnt * kWaveLG; t += blockDim.x) {
             const int i = t / kWaveLG, q = t % kWaveLG;
@@ -674,7 +682,7 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
             double2 ev;
             if (MIXED) {
-                const float2 e = e_far_mixed<DIM>(x, vd, nilam, sdlo[i], sdhi[i], up, inv, hs);
+                const float2 e = e_far_mixed<DIM>(x, fv, sdlo[i], nhs);
                 ev = make_double2(e.x, e.y);
             } else {
                 ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
@@ -711,7 +719,7 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
                                                                     double2 *hpart)
 {
     __shared__ double sx[kWmPB][3];
-    __shared__ float sdlo[kWmPB][3], sdhi[kWmPB][3];
+    __shared__ float sdlo[kWmPB][3];
     constexpr int kWmLights = 32 * NB;
     __shared__ float2 scl[kWmPB][kWmLights];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -720,16 +728,9 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
     const int v = vbase + m;
     const bool vok = v < P.nv;
     double vd[3] = {0, 0, 0};
-    float inv[3] = {0, 0, 0};
-    bool up[3] = {false, false, false};
-    for (int j = 0; j < DIM && vok; ++j) {
-        const double nd = P.v[(size_t)v * DIM + j];
-        vd[j] = nd;
-        inv[j] = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
-        up[j] = nd > 0;
-    }
-    const double nilam = -P.inv_lambda;
-    const float hs = (float)(P.sigt * 0.5);
+    for (int j = 0; j < DIM && vok; ++j) vd[j] = P.v[(size_t)v * DIM + j];
+    const FarView fv = far_view<DIM>(P, vd);
+    const float nhs = (float)(P.sigt * 0.5) * -1.4426950408889634f;
     const int l0 = blockIdx.y * kWmLights;
     const int split = blockIdx.z;
     const int64_t p0 = np * split / nsplit, p1 = np * (split + 1) / nsplit;
@@ -750,7 +751,6 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
             const double xj = paths[pb + i].x1[j];
             sx[i][j] = xj;
             sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
-            sdhi[i][j] = j < DIM ? (float)(P.bmax[j] - xj) : 0.f;
         }
         for (int t = threadIdx.x; t < kWmPB * kWmLights; t += blockDim.x) {
             const int i = t / kWmLights, q = t % kWmLights;
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
         for (int i = 0; i < cnt; ++i) {
             const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
             float2 ev = make_float2(0.f, 0.f);
-            if (vok) ev = e_far_mixed<DIM>(x, vd, nilam, sdlo[i], sdhi[i], up, inv, hs);
+            if (vok) ev = e_far_mixed<DIM>(x, fv, sdlo[i], nhs);
             const float a1 = kk == 0 ? ev.x : -ev.y;
             const float a2 = kk == 0 ? ev.y : ev.x;
 #pragma unroll
