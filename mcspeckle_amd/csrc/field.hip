@@ -210,11 +210,29 @@ __global__ __launch_bounds__(64) void field_wave_coef_kernel(DevProblem P, int64
     if (__syncthreads_or(bad) && threadIdx.x == 0 && nonfinite) atomicAdd(nonfinite, 1ULL);
 }
 
+// The entry side of the exit distance: the signed distance to the nearer
+// face (the exit distance of the mixed far-field E), from the FP64 position.  A point on the max face gives
+// -0.0, whose sign bit still selects fbn.
+__device__ __forceinline__ float near_face(double x, double lo, double hi)
+{
+    const double dlo = x - lo, dhi = hi - x;
+    return dlo <= dhi ? (float)dlo : -(float)dhi;
+}
+
+__device__ __forceinline__ int qneg_bits(const float *q)
+{
+    return (int)((unsigned)__float_as_int(q[0]) >> 31 | ((unsigned)__float_as_int(q[1]) >> 31) << 1 |
+                 ((unsigned)__float_as_int(q[2]) >> 31) << 2);
+}
+
 struct FEntry {
     double x[3];
     double ad[3];     // amplitude direction: ow (vertex) or -w1 (reversed)
     double2 div;      // 1/w0p for the reversed entry, (1,0) otherwise
-    float dlo[3];     // x - box_min rounded to FP32 (mixed attenuation)
+    float q[3];       // signed distance to the nearer face in FP32 (mixed
+                      // attenuation): x - box_min, or -(box_max - x) when
+                      // the max face is nearer (near_face)
+    int qneg;         // bit j: the sign bit of q[j]
     float adf[3];     // ad rounded to FP32 (mixed HG amplitude)
     int kind;         // 0 first vertex, 1 later vertex, 2 reversed path
     int p;            // path of the entry
@@ -247,9 +265,10 @@ __device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64
         f.div = make_double2(pr.w0p.x / den, -pr.w0p.y / den);
     }
     for (int j = 0; j < 3; ++j) {
-        f.dlo[j] = j < DIM ? (float)(f.x[j] - P.bmin[j]) : 0.f;
+        f.q[j] = j < DIM ? near_face(f.x[j], P.bmin[j], P.bmax[j]) : 0.f;
         f.adf[j] = (float)f.ad[j];
     }
+    f.qneg = qneg_bits(f.q);
 }
 
 // Tabulated amplitude with the nearest-bin index formed in FP64, so the bin
@@ -264,13 +283,21 @@ __device__ __forceinline__ float2 amp_tab(double c, const DevPhase &ph)
 
 // Per-view constants of the mixed far-field E: the direction times
 // -1/lambda (the phase in cycles is then x . vn: the far-field phase is
-// k x.(-v)), and the distance over which the view leaves the box as an affine
-// function of the entry's x - box_min (dlo):
-//   (v_j > 0 ? W_j - dlo_j : dlo_j) / max(|v_j|, 1e-7) = fa_j dlo_j + fb_j,
-// W_j the box width (cubeDist.m:4-11 for a point inside the box).
+// k x.(-v)), and the distance over which the view leaves the box
+// (cubeDist.m:4-11 for a point inside the box), D_j = dlo_j / a_j when
+// v_j <= 0 and dhi_j / a_j when v_j > 0, with dlo = x - box_min,
+// dhi = box_max - x, a_j = max(|v_j|, 1e-7) and W_j = dlo_j + dhi_j the box
+// width.  An entry carries q_j = dlo_j when that face is nearer and -dhi_j
+// otherwise (near_face), and D_j = fa_j q_j + (q_j < 0 ? fbn_j : fbp_j):
+//   q >= 0, v_j <= 0:  q/a             q >= 0, v_j > 0:  (W - q)/a
+//   q <  0, v_j <= 0:  (W + q)/a       q <  0, v_j > 0:  -q/a
+// The only difference of two FP32 terms, W/a -|q|/a, has |q| <= W/2, so it
+// never cancels; writing every case from dlo alone (W/a - dlo/a) loses all
+// bits of dhi next to the max face of a wide box (a slab of lateral width
+// 1e5: an absolute error of ~8 distance units at |v_j| ~ 1e-3).
 struct FarView {
     double vn[3];
-    float fa[3], fb[3];
+    float fa[3], fbp[3], fbn[3];
 };
 template <int DIM>
 __device__ __forceinline__ FarView far_view(const DevProblem &P, const double *v)
@@ -280,29 +307,39 @@ __device__ __forceinline__ FarView far_view(const DevProblem &P, const double *v
         const double nd = j < DIM ? v[j] : 0.0;
         f.vn[j] = nd * -P.inv_lambda;
         const float inv = (float)(1.0 / fmax(fabs(nd), 1.0000000000000004e-07));
-        const float w = j < DIM ? (float)(P.bmax[j] - P.bmin[j]) : 0.f;
+        const float wi = j < DIM ? (float)(P.bmax[j] - P.bmin[j]) * inv : 0.f;
         f.fa[j] = nd > 0 ? -inv : inv;
-        f.fb[j] = nd > 0 ? w * inv : 0.f;
+        f.fbp[j] = nd > 0 ? wi : 0.f;
+        f.fbn[j] = nd > 0 ? 0.f : wi;
     }
     return f;
 }
 
+// qneg: bit j set when q_j carries a sign bit.  Every lane of a wave reads
+// the same entry, so the caller passes qneg as a wave-uniform (scalar)
+// value and each face costs one select with a scalar mask and one FMA.
+__device__ __forceinline__ float exit_dist(const FarView &fv, int j, float q, int qneg)
+{
+    return fmaf(fv.fa[j], q, ((qneg >> j) & 1) ? fv.fbn[j] : fv.fbp[j]);
+}
+
 // Far-field E(x, v) of the mixed variant: FP64 x.vn and range reduction to a
 // fraction of a cycle, hardware sin/cos (input in revolutions), FP32
-// attenuation over the distance to the face the view leaves by (one FMA per
-// face from the entry's dlo).  nhs = -sigma/2 log2(e).  lamp: log2 of a real
-// amplitude factor folded into the same exp2 (0 for none), so an isotropic
-// or HG amplitude costs no extra exponential and no complex multiply.
+// attenuation over the distance to the face the view leaves by (a select and
+// an FMA per face from the entry's q).  nhs = -sigma/2 log2(e).  lamp: log2
+// of a real amplitude factor folded into the same exp2 (0 for none), so an
+// isotropic or HG amplitude costs no extra exponential and no complex
+// multiply.
 template <int DIM>
-__device__ __forceinline__ float2 e_far_mixed(const double *x, const FarView &fv, const float *dlo, float nhs,
+__device__ __forceinline__ float2 e_far_mixed(const double *x, const FarView &fv, const float *q, int qneg, float nhs,
                                               float lamp = 0.f)
 {
     const double cyc = dot<DIM>(x, fv.vn);
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
     // the first face starts the minimum (a start at +inf costs a min per call)
-    float best = fmaf(fv.fa[0], dlo[0], fv.fb[0]);
-    for (int j = 1; j < DIM; ++j) best = fminf(best, fmaf(fv.fa[j], dlo[j], fv.fb[j]));
+    float best = exit_dist(fv, 0, q[0], qneg);
+    for (int j = 1; j < DIM; ++j) best = fminf(best, exit_dist(fv, j, q[j], qneg));
     const float mag = __builtin_amdgcn_exp2f(fmaf(nhs, best, lamp));
     return make_float2(mag * cs, mag * sn);
 }
@@ -418,9 +455,10 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             // wave first vertices: field_wave_first_kernel
             if (WAVE && kind == 0) continue;
             const double x[3] = {f.x[0], f.x[1], f.x[2]};
-            float dlo[3], adf[3];
+            float q[3], adf[3];
+            const int qneg = __builtin_amdgcn_readfirstlane(f.qneg);
             for (int j = 0; j < 3; ++j) {
-                dlo[j] = f.dlo[j];
+                q[j] = f.q[j];
                 adf[j] = f.adf[j];
             }
             double2 sce[NLQ];
@@ -442,7 +480,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     if (HET) {
                         evf = e_far_mixed_het<DIM>(P, M, x, mv[t], lamp);
                     } else {
-                        evf = e_far_mixed<DIM>(x, fv[t], dlo, nhs, lamp);
+                        evf = e_far_mixed<DIM>(x, fv[t], q, qneg, nhs, lamp);
                     }
                     if (kind != 0 && tab) {
                         const float2 am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
@@ -648,7 +686,8 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
                                                                        double2 *hpart)
 {
     __shared__ double sx[kWavePB][3];
-    __shared__ float sdlo[kWavePB][3];
+    __shared__ float sq[kWavePB][3];
+    __shared__ int sneg[kWavePB];
     __shared__ double2 scl[kWavePB][kWaveLG];
     const int v = blockIdx.x * kFieldViews + threadIdx.x;
     const bool vok = v < P.nv;
@@ -670,7 +709,12 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             const int i = t / 3, j = t % 3;
             const double xj = paths[pb + i].x1[j];
             sx[i][j] = xj;
-            sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
+            sq[i][j] = j < DIM ? near_face(xj, P.bmin[j], P.bmax[j]) : 0.f;
+        }
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            float q[3];
+            for (int j = 0; j < 3; ++j) q[j] = j < DIM ? near_face(paths[pb + i].x1[j], P.bmin[j], P.bmax[j]) : 0.f;
+            sneg[i] = qneg_bits(q);
         }
         for (int t = threadIdx.x; t < cnt * kWaveLG; t += blockDim.x) {
             const int i = t / kWaveLG, q = t % kWaveLG;
@@ -682,7 +726,7 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
             double2 ev;
             if (MIXED) {
-                const float2 e = e_far_mixed<DIM>(x, fv, sdlo[i], nhs);
+                const float2 e = e_far_mixed<DIM>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
                 ev = make_double2(e.x, e.y);
             } else {
                 ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
@@ -719,7 +763,8 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
                                                                     double2 *hpart)
 {
     __shared__ double sx[kWmPB][3];
-    __shared__ float sdlo[kWmPB][3];
+    __shared__ float sq[kWmPB][3];
+    __shared__ int sneg[kWmPB];
     constexpr int kWmLights = 32 * NB;
     __shared__ float2 scl[kWmPB][kWmLights];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -750,7 +795,12 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
             const int i = t / 3, j = t % 3;
             const double xj = paths[pb + i].x1[j];
             sx[i][j] = xj;
-            sdlo[i][j] = j < DIM ? (float)(xj - P.bmin[j]) : 0.f;
+            sq[i][j] = j < DIM ? near_face(xj, P.bmin[j], P.bmax[j]) : 0.f;
+        }
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            float q[3];
+            for (int j = 0; j < 3; ++j) q[j] = j < DIM ? near_face(paths[pb + i].x1[j], P.bmin[j], P.bmax[j]) : 0.f;
+            sneg[i] = qneg_bits(q);
         }
         for (int t = threadIdx.x; t < kWmPB * kWmLights; t += blockDim.x) {
             const int i = t / kWmLights, q = t % kWmLights;
@@ -765,7 +815,7 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
         for (int i = 0; i < cnt; ++i) {
             const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
             float2 ev = make_float2(0.f, 0.f);
-            if (vok) ev = e_far_mixed<DIM>(x, fv, sdlo[i], nhs);
+            if (vok) ev = e_far_mixed<DIM>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
             const float a1 = kk == 0 ? ev.x : -ev.y;
             const float a2 = kk == 0 ? ev.y : ev.x;
 #pragma unroll

@@ -61,6 +61,74 @@ def test_speckle_image_128_replay(oracle, precision, tol):
     assert err <= tol
 
 
+def paths_at_lateral_faces(ps, box_min, box_max, seed=5):
+    """The same walks, each moved laterally by its own offset so that it
+    touches a lateral face (x and y, min or max side, within 1e-3 .. 0.3) of
+    the 1e5-wide slab: views leave such a vertex through the lateral face
+    after a short distance, the case where an FP32 exit distance formed as
+    W/a - (x - box_min)/a would lose every bit."""
+    rng = np.random.default_rng(seed)
+    ps = {k: v.copy() for k, v in ps.items()}
+    off = ps["offset"]
+    x1, x = ps["x1"].reshape(-1, 3), ps["x"].reshape(-1, 3)
+    for p in range(off.size - 1):
+        k0, k1 = off[p], off[p + 1]
+        pts = np.vstack([x1[p:p + 1], x[k0:k1]])
+        for j in (0, 1):
+            gap = rng.uniform(1e-3, 0.3)
+            d = (box_max[j] - gap - pts[:, j].max()) if rng.integers(2) else (box_min[j] + gap - pts[:, j].min())
+            x1[p, j] += d
+            x[k0:k1, j] += d
+    return ps
+
+
+@pytest.mark.parametrize("cbs,nl", [(False, 1), (True, 3)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+def test_speckle_image_paths_at_lateral_faces(oracle, precision, tol, cbs, nl):
+    base = W.speckle_image_128()
+    l = np.array([[0.0, 0.05, -0.03], [0.0, 0.0, 0.04], [1.0, 1.0, 1.0]])[:, :nl]
+    l /= np.linalg.norm(l, axis=0)
+    spec = E.make_problem(base.sigt, base.albedo, base.box_min, base.box_max, l, W.view_grid(64, 10), 1, 1, 1.0, cbs,
+                          2, 3, 0.8, seed=3)
+    n = 200
+    say("speckle paths at lateral faces", precision, "cbs", cbs, "lights", nl)
+    ps = cached(("edge paths", cbs, nl), lambda: paths_at_lateral_faces(oracle.trace(spec, n), spec.box_min,
+                                                                        spec.box_max))
+    u0 = cached(("edge replay", cbs, nl), lambda: oracle.mcfield_replay(spec, ps, n))
+    u = E.mcfield_replay(spec, ps, n, precision)
+    err = rel(u, u0)
+    say("  err", err)
+    assert err <= tol
+
+
+@pytest.mark.parametrize("wave", [False, True])
+def test_speckle_image_mixed_vs_fp64_many_paths(wave):
+    """2e5 paths over the whole 1e5-wide slab (a few dozen vertices land
+    within a view's exit distance of a lateral face): mixed against FP64 on
+    the same paths, max |du| / max |u| <= 2e-5, for MCfield and for
+    MCfieldOnWave (its first vertices go through the matrix-core kernel)."""
+    base = W.speckle_image_128(seed=9)
+    v = W.view_grid(32, 10)
+    n = 200000
+    say("speckle mixed vs fp64, wave", wave)
+    if wave:
+        rng = np.random.default_rng(1)
+        l = np.vstack([rng.uniform(-0.1, 0.1, (2, 40)), np.ones((1, 40))])
+        l /= np.linalg.norm(l, axis=0)
+        wl = rng.normal(size=40) + 1j * rng.normal(size=40)
+        args = (base.sigt, base.albedo, base.box_min, base.box_max, l, v, 1, 1, n, 1.0, True, 2, 3, 0.8)
+        u64 = E.MCfieldOnWave(*args, Wl=wl, seed=9, precision="fp64")
+        um = E.MCfieldOnWave(*args, Wl=wl, seed=9, precision="mixed")
+    else:
+        spec = E.make_problem(base.sigt, base.albedo, base.box_min, base.box_max, [[0], [0], [1]], v, 1, 1, 1.0,
+                              False, 2, 3, 0.8, seed=9)
+        u64 = E.mcfield_spec(spec, n, "fp64")
+        um = E.mcfield_spec(spec, n, "mixed")
+    err = rel(um, u64)
+    say("  err", err)
+    assert err <= 2e-5
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-5)])
 def test_speckle_image_128_full_run(oracle, precision, tol):
     spec = W.speckle_image_128(seed=7)
