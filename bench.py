@@ -115,15 +115,18 @@ def serving_stream(n_req, rate, path_scale, seed=5):
     return W.request_stream(n_req, rate, path_scale, seed)
 
 
-def serve(n_req, rate, path_scale, max_active, shared=False, seed=5, timeline=None):
+def serve(n_req, rate, path_scale, max_active, mode="streams", seed=5, timeline=None):
     """Requests of the serving workload through the continuous-batching
     server, submitted at their Poisson arrival times (rate = inf: all at
-    once); latency is submit to result."""
+    once); latency is submit to result.  mode: "streams" (each request's
+    updates launched on its slot's streams), "shared" (small requests'
+    updates in shared launches) or "resident" (every update in one resident
+    loop kernel)."""
     import threading
     from mcspeckle_amd import engines as E
     stream = serving_stream(n_req, rate if np.isfinite(rate) else 1e9, path_scale, seed)
     lat, queued, work = [], [], 0.0
-    S = E.Server(max_active=max_active, shared_launches=shared)
+    S = E.Server(max_active=max_active, shared_launches=mode == "shared", resident_loop=mode == "resident")
     ids = []
     t0 = time.perf_counter()
 
@@ -156,7 +159,7 @@ def serve(n_req, rate, path_scale, max_active, shared=False, seed=5, timeline=No
            "mean_ms": float(lat.mean()), "queued_p50_ms": float(np.percentile(queued, 50)),
            "queued_p99_ms": float(np.percentile(queued, 99)), "throughput": work / t_end, "unit": "paths*entries/s",
            "requests_per_s": n_req / t_end, "makespan_s": t_end, "offered_span_s": stream[-1][0],
-           "shared_launches": bool(shared)}
+           "update_mode": mode, "shared_launches": mode == "shared"}
     if timeline is not None:
         # per request: arrival, admission and completion (s from the start), N, paths
         timeline.extend((ta, ta + q / 1e3, ta + l / 1e3, spec.nv * spec.nl, paths)
@@ -367,6 +370,58 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n}))
 
 
+def run_serving(args):
+    """Serving runs of the request workload: capacity (all submitted at once),
+    a sustained Poisson stream at a stated fraction of that capacity, and the
+    workload's 50 req/s as an overload figure (paths scaled down), with
+    separate launches and the modes named in --serve-modes."""
+    out = {}
+    tl = {} if args.timeline else None
+    # capacity: requests of the workload's distributions at their full path
+    # counts, all submitted at once; the server's completion rate is what
+    # the GPU sustains
+    modes = [m.strip() for m in args.serve_modes.split(",") if m.strip()]
+    cap = serve(args.capacity_requests, float("inf"), 1.0, args.serve_active, seed=11,
+                timeline=tl.setdefault("capacity", []) if tl is not None else None)
+    out["serving_capacity"] = cap
+    if "resident" in modes:
+        out["serving_capacity_resident"] = serve(
+            args.capacity_requests, float("inf"), 1.0, args.serve_active, mode="resident", seed=11,
+            timeline=tl.setdefault("capacity_resident", []) if tl is not None else None)
+    # a Poisson stream offering a stated fraction of that throughput
+    # (paths * entries per second; the request sizes do not depend on the
+    # rate, so the stream's own mean work per request sets the rate):
+    # latency when the server keeps up
+    probe = serving_stream(args.sustained_requests, 1.0, 1.0, 12)
+    mean_work = float(np.mean([p * float(sp.nv * sp.nl) ** 2 for _, sp, p in probe]))
+    rate = args.sustained_load * cap["throughput"] / mean_work
+    out["serving"] = serve(args.sustained_requests, rate, 1.0, args.serve_active, seed=12,
+                           timeline=tl.setdefault("sustained", []) if tl is not None else None)
+    out["serving"]["offered_fraction_of_capacity"] = args.sustained_load
+    if "resident" in modes:
+        # the same stream (same arrival times) with the resident loop
+        out["serving_resident"] = serve(args.sustained_requests, rate, 1.0, args.serve_active, mode="resident",
+                                        seed=12,
+                                        timeline=tl.setdefault("sustained_resident", []) if tl is not None else None)
+        out["serving_resident"]["offered_fraction_of_capacity"] = args.sustained_load
+    # the workload's stated arrival rate (50 req/s) is ~50x what one GPU
+    # completes: run it with paths scaled down, as an overload figure
+    out["serving_overload"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                    args.serve_active,
+                                    timeline=tl.setdefault("overload", []) if tl is not None else None)
+    # the same overload stream with the slots' small updates in shared launches
+    if "shared" in modes:
+        out["serving_overload_shared"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                               args.serve_active, mode="shared")
+    if "resident" in modes:
+        out["serving_overload_resident"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
+                                                 args.serve_active, mode="resident")
+    if tl is not None:
+        with open(args.timeline, "w") as f:
+            json.dump(tl, f)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,6 +444,8 @@ def main():
     ap.add_argument("--sustained-load", type=float, default=0.7)
     ap.add_argument("--timeline", default="", help="write the per-request serving timeline (JSON) here")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-modes", default="shared,resident",
+                    help="update modes measured beside separate launches: shared, resident")
     ap.add_argument("--field-paths", type=int, default=1000000)
     ap.add_argument("--quad-paths", type=int, default=5000000)
     ap.add_argument("--hetro-paths", type=int, default=200000)
@@ -423,6 +480,8 @@ def main():
         out = {}
         if not args.no_legs:
             out.update(run_legs(args))
+        if not args.no_serve:
+            out.update(run_serving(args))
         print(json.dumps(out))
         return
 
@@ -495,34 +554,7 @@ def main():
 
     # ---- serving: continuous batching, p50/p99 latency
     if not args.no_serve:
-        tl = {} if args.timeline else None
-        # capacity: requests of the workload's distributions at their full path
-        # counts, all submitted at once; the server's completion rate is what
-        # the GPU sustains
-        cap = serve(args.capacity_requests, float("inf"), 1.0, args.serve_active, seed=11,
-                    timeline=tl.setdefault("capacity", []) if tl is not None else None)
-        out["serving_capacity"] = cap
-        # a Poisson stream offering a stated fraction of that throughput
-        # (paths * entries per second; the request sizes do not depend on the
-        # rate, so the stream's own mean work per request sets the rate):
-        # latency when the server keeps up
-        probe = serving_stream(args.sustained_requests, 1.0, 1.0, 12)
-        mean_work = float(np.mean([p * float(sp.nv * sp.nl) ** 2 for _, sp, p in probe]))
-        rate = args.sustained_load * cap["throughput"] / mean_work
-        out["serving"] = serve(args.sustained_requests, rate, 1.0, args.serve_active, seed=12,
-                               timeline=tl.setdefault("sustained", []) if tl is not None else None)
-        out["serving"]["offered_fraction_of_capacity"] = args.sustained_load
-        # the workload's stated arrival rate (50 req/s) is ~50x what one GPU
-        # completes: run it with paths scaled down, as an overload figure
-        out["serving_overload"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
-                                        args.serve_active,
-                                        timeline=tl.setdefault("overload", []) if tl is not None else None)
-        # the same overload stream with the slots' small updates in shared launches
-        out["serving_overload_shared"] = serve(args.serve_requests, args.serve_rate, args.serve_path_scale,
-                                               args.serve_active, shared=True)
-        if tl is not None:
-            with open(args.timeline, "w") as f:
-                json.dump(tl, f)
+        out.update(run_serving(args))
 
     # ---- CPU baseline: FP64 restatement with the reference's loop structure
     if not args.no_cpu:

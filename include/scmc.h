@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SCMC_ABI_VERSION 2
+#define SCMC_ABI_VERSION 3
 
 enum scmc_status {
     SCMC_OK = 0,
@@ -263,6 +263,14 @@ typedef struct {
     int32_t shared_launches; /* 1: the mixed updates of small requests (N <= 1024)
                                 running at the same time share launches;
                                 0 (default): each request launches its own     */
+    int32_t resident_loop;   /* 1: the mixed updates of every request go to one
+                                resident update kernel that takes their work
+                                items from a job table in host memory as the
+                                requests publish them (no launch per update);
+                                0 (default): each request launches its own     */
+    int32_t resident_workgroups; /* workgroups of the resident kernel
+                                (0: one per CU, leaving the other half of each
+                                CU to the kernels that prepare the updates)    */
 } scmc_server_config;
 
 int scmc_server_start(const scmc_server_config *cfg);

@@ -89,7 +89,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("max_active", C.c_int32),
         ("hbm_budget_gb", C.c_double), ("paths_per_slice", C.c_int64),
-        ("shared_launches", C.c_int32),
+        ("shared_launches", C.c_int32), ("resident_loop", C.c_int32), ("resident_workgroups", C.c_int32),
     ]
 
 

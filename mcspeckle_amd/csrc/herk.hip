@@ -239,32 +239,15 @@ struct MixedGroup {
 
 // MAXG = 2 for a chunk's pair of problems, kMaxUpdateGroup for shared
 // launches (a smaller argument block and select chain for the common case)
-template <int MAXG>
-__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(MixedGroup<MAXG> G)
+// One work item (k-split s, tile t) of a mixed update, on a workgroup of 256
+// threads with NBUF * STAGEB bytes of LDS at lds.  The update kernel runs one
+// item per workgroup; the resident loop kernel runs items one after another.
+__device__ __forceinline__ void mixed_item(const uint4 *__restrict__ Z, double2 *__restrict__ acc,
+                                           const unsigned *__restrict__ bound, const int2 *__restrict__ order, int n,
+                                           int ntile, int ksplit, int64_t nks_total, int item, unsigned char *lds)
 {
     using namespace mixed;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // the problem of this work item (uniform); fields are picked with
-    // selects so the argument block is never indexed dynamically
-    int item = (int)blockIdx.x, g = 0;
-#pragma unroll
-    for (int q = 0; q + 1 < MAXG; ++q)
-        if (g == q && q + 1 < G.count && item >= G.u[q].nitems) {
-            item -= G.u[q].nitems;
-            g = q + 1;
-        }
-    MixedUpdate U = G.u[0];
-#pragma unroll
-    for (int q = 1; q < MAXG; ++q)
-        if (g == q) U = G.u[q];
-    // a chunk's pair shares N and the tile order (launch_herk_mixed_update)
-    const int n = MAXG == 2 ? G.u[0].n : U.n, ntile = MAXG == 2 ? G.u[0].ntile : U.ntile, ksplit = U.ksplit;
-    const int64_t nks_total = U.nks;
-    const uint4 *__restrict__ Z = U.Z;
-    double2 *__restrict__ acc = U.acc;
-    const unsigned *__restrict__ bound = U.bound;
-    const int2 *__restrict__ order = MAXG == 2 ? G.u[0].order : U.order;
     int s, t;
     item_decode(item, ksplit, ntile, s, t);
     const int ti = order[t].x, tj = order[t].y;
@@ -412,6 +395,162 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(MixedGroup<MAXG> G)
                     *p = o;
                 }
             }
+}
+
+template <int MAXG>
+__global__ __launch_bounds__(256, 2) void herk_mixed_kernel(MixedGroup<MAXG> G)
+{
+    using namespace mixed;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
+    // the problem of this work item (uniform); fields are picked with
+    // selects so the argument block is never indexed dynamically
+    int item = (int)blockIdx.x, g = 0;
+#pragma unroll
+    for (int q = 0; q + 1 < MAXG; ++q)
+        if (g == q && q + 1 < G.count && item >= G.u[q].nitems) {
+            item -= G.u[q].nitems;
+            g = q + 1;
+        }
+    MixedUpdate U = G.u[0];
+#pragma unroll
+    for (int q = 1; q < MAXG; ++q)
+        if (g == q) U = G.u[q];
+    // a chunk's pair shares N and the tile order (launch_herk_mixed_update)
+    const int n = MAXG == 2 ? G.u[0].n : U.n, ntile = MAXG == 2 ? G.u[0].ntile : U.ntile;
+    const int2 *order = MAXG == 2 ? G.u[0].order : U.order;
+    mixed_item(U.Z, U.acc, U.bound, order, n, ntile, U.ksplit, U.nks, item, lds);
+}
+
+// ------------------------------------------------------- resident loop
+// Serving: a persistent update kernel that takes the work items of the jobs
+// the host publishes (one mixed update each: one problem of one chunk of one
+// request), so the updates of every request in flight share one resident
+// launch instead of one launch each.
+//
+// Items are dealt to the eight XCDs as the update kernel's grid deals them:
+// item b of a job goes to XCD label b % 8 (a job's item count is padded to a
+// multiple of 8 with empty items), so item_decode gives each XCD the same
+// contiguous run of tiles per k-split as in a launch of its own.  The
+// workgroups of label x = blockIdx.x % 8 take items 8c + x from a per-job,
+// per-label counter, and a per-label cursor moves past jobs whose share is
+// taken.  A job's descriptor is read from host memory (system-scope loads,
+// uncached).
+//
+// Memory: the job's rows, accumulators and bound were written by other
+// kernels, some on other XCDs, so every wave starts an item with an
+// agent-scope acquire (L1/L2 invalidate) and ends it with an agent-scope
+// release (L2 write-back) before the item is counted; the last item of a job
+// sets done[j] with a system-scope release.
+//
+// Every wait is bounded.  A workgroup that finds no published work for
+// idle_ticks of the wall clock leaves, after announcing it: store
+// closing[inst & 1]; fence; load published.  The host publishes: store
+// published; fence; load closing[inst & 1].  So either this workgroup sees the
+// new job, or the host sees the announcement and queues another instance
+// behind this one on the loop's stream (the next instance continues from the
+// same counters).  `stop` ends idle workgroups at once.
+//
+// Helper instances (inst < 0) run the same loop but leave as soon as they
+// find nothing left to take and never announce it: launched on a request's
+// own stream when it publishes a chunk, they add workgroups wherever the
+// hardware finds room, as a launch of the update would, while the resident
+// instance keeps one workgroup per CU.
+__device__ __forceinline__ unsigned sys_peek(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ LoopJob read_job(const LoopJob *j)
+{
+    const uint64_t *w = (const uint64_t *)j;
+    uint64_t v[sizeof(LoopJob) / 8];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(LoopJob) / 8); ++i)
+        v[i] = __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return __builtin_bit_cast(LoopJob, v);
+}
+
+__global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, LoopDev *D, int inst, uint64_t idle_ticks)
+{
+    using namespace mixed;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
+    // the claimed (job, item) and the job's descriptor go to the other
+    // threads through the first bytes of the last stage, which an item writes
+    // only after its first barrier (two workgroups per CU leave no LDS for
+    // anything beside the stages); a barrier after reading them keeps them
+    // from being overwritten early
+    volatile int *claim = (volatile int *)(lds + (NBUF - 1) * STAGEB);
+    volatile uint64_t *jw = (volatile uint64_t *)(lds + (NBUF - 1) * STAGEB + 16);
+    constexpr int kJobWords = (int)(sizeof(LoopJob) / 8);
+    const unsigned x = blockIdx.x & 7;
+    unsigned p = 0;   // thread 0: this workgroup's cursor
+    if (threadIdx.x == 0) p = __hip_atomic_load(&D->active[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        // every thread is done with the previous item and its claim
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int job = -1, item = 0;
+            uint64_t t0 = wall_clock64();
+            for (;;) {
+                if (p < sys_peek(&H->published)) {
+                    const int n8 = (int)__hip_atomic_load(&H->jobs[p].nitems8, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_SYSTEM);
+                    const unsigned c = atomicAdd(&D->next[(size_t)p * 8 + x], 1u);
+                    if ((int64_t)c * 8 + x < n8) {
+                        job = (int)p;
+                        item = (int)(c * 8 + x);
+                        break;
+                    }
+                    // this label's share of job p is taken
+                    ++p;
+                    atomicMax(&D->active[x], p);
+                    t0 = wall_clock64();
+                    continue;
+                }
+                // a helper instance (inst < 0) leaves as soon as nothing is left
+                if (inst < 0 || sys_peek(&H->stop)) break;
+                if (wall_clock64() - t0 > idle_ticks) {
+                    __hip_atomic_store(&H->closing[inst & 1], 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __atomic_thread_fence(__ATOMIC_SEQ_CST);
+                    if (p < __hip_atomic_load(&H->published, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_SYSTEM)) {
+                        t0 = wall_clock64();
+                        continue;
+                    }
+                    break;
+                }
+                // about 20 us between looks at host memory
+                for (int q = 0; q < 6; ++q) __builtin_amdgcn_s_sleep(127);
+            }
+            claim[0] = job;
+            claim[1] = item;
+            if (job >= 0) {
+                const LoopJob J = read_job(&H->jobs[job]);
+                const uint64_t *w = (const uint64_t *)&J;
+#pragma unroll
+                for (int i = 0; i < kJobWords; ++i) jw[i] = w[i];
+            }
+        }
+        __syncthreads();
+        const int job = claim[0], item = claim[1];
+        if (job < 0) break;
+        uint64_t w[kJobWords];
+#pragma unroll
+        for (int i = 0; i < kJobWords; ++i) w[i] = jw[i];
+        const LoopJob J = __builtin_bit_cast(LoopJob, w);
+        __syncthreads();
+        if (item < J.nitems) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            mixed_item((const uint4 *)J.Z, J.acc, J.bound, (const int2 *)J.order, J.n, J.ntile, J.ksplit, J.nks, item,
+                       lds);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned f = atomicAdd(&D->finished[job], 1u);
+            if ((int)f == J.nitems8 - 1)
+                __hip_atomic_store(&H->done[job], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 // ------------------------------------------------------------- fp64
@@ -598,6 +737,28 @@ static void launch_group(const HerkProblem *const *hp, int count, hipStream_t s)
         grid += G.u[G.count++].nitems;
     }
     if (grid > 0) hipLaunchKernelGGL(herk_mixed_kernel<MAXG>, dim3((unsigned)grid), dim3(256), 0, s, G);
+}
+
+void loop_job_from(const HerkProblem &hp, LoopJob *out)
+{
+    const MixedUpdate u = mixed_desc(hp);
+    LoopJob j{};
+    j.nks = u.nks;
+    j.Z = u.Z;
+    j.acc = u.acc;
+    j.bound = u.bound;
+    j.order = u.order;
+    j.n = u.n;
+    j.ntile = u.ntile;
+    j.ksplit = u.ksplit;
+    j.nitems = hp.k > 0 ? u.nitems : 0;
+    j.nitems8 = (j.nitems + 7) / 8 * 8;
+    *out = j;
+}
+
+void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(herk_mixed_loop_kernel, dim3((unsigned)grid), dim3(256), 0, s, ctl_dev, dev, inst, idle_ticks);
 }
 
 void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s)

@@ -117,6 +117,41 @@ constexpr int kMaxUpdateGroup = 8;
 void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s);
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 
+// Resident update loop (serving).  One job is one mixed update (one problem
+// of a chunk, as launch_herk_mixed_update would run it); the host writes its
+// descriptor into LoopCtl (pinned, coherent host memory), then raises
+// `published`; the loop kernel sets done[j] once every item of job j has been
+// added into its accumulators.  LoopDev holds the kernel's own counters in
+// device memory (zeroed before the first instance).
+constexpr int kLoopJobs = 4096;
+struct LoopJob {
+    int64_t nks;              // K-steps of the expanded rows
+    const void *Z;            // expanded rows
+    double2 *acc;             // accumulator copies
+    const unsigned *bound;    // scale bound (may be null)
+    const void *order;        // tile order
+    int32_t n, ntile, ksplit, nitems;
+    int32_t nitems8;          // nitems rounded up to a multiple of 8
+    int32_t pad;
+};
+struct LoopCtl {
+    unsigned published;       // jobs published (host)
+    unsigned stop;            // host: idle workgroups leave
+    unsigned closing[2];      // kernel: a workgroup of instance i left (closing[i & 1])
+    unsigned pad[12];
+    unsigned done[kLoopJobs];
+    LoopJob jobs[kLoopJobs];
+};
+struct LoopDev {
+    unsigned active[8];                  // per XCD label: first job not yet exhausted
+    unsigned next[kLoopJobs * 8];        // per job and label: items taken / 8
+    unsigned finished[kLoopJobs];        // per job: items finished
+};
+void loop_job_from(const HerkProblem &hp, LoopJob *out);
+// One instance of the loop kernel (grid workgroups of 256 threads) on s; it
+// returns when idle for idle_ticks of the device wall clock, or on stop.
+void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s);
+
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the
 // optional per-row factor rowf[(v,l)] (single scattering amplitude), and write
 // the MATLAB nv x nv x nl x nl column-major layout (split or interleaved).

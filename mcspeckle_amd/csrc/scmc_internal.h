@@ -98,6 +98,49 @@ struct TraceChunk {
     int load(const scmc_pathset *ps, hipStream_t s);
 };
 
+// Serving: the resident update loop (herk_mixed_loop_kernel) and its host
+// side.  publish() hands one mixed update to the running loop (launching an
+// instance when none runs, or queueing the next one behind an instance that
+// has begun to leave) and returns a ticket; wait() returns once that job's
+// items are all in its accumulators.  Jobs are numbered within a generation
+// of kLoopJobs; when the table is full, publish() waits for every job,
+// stops the instance, clears the counters and starts a new generation (a
+// ticket of an older generation is done).  No call waits on the device
+// without a bound: the loop leaves after idle_ms without work.
+struct LoopTicket {
+    uint64_t gen = 0;
+    int idx = -1;   // -1: nothing published
+};
+struct ResidentLoop {
+    LoopCtl *H = nullptr;        // host view (pinned, coherent)
+    LoopCtl *Hd = nullptr;       // the same memory as the device sees it
+    LoopDev *D = nullptr;
+    hipStream_t stream = nullptr;
+    int grid = 0;
+    uint64_t idle_ticks = 0;
+    std::mutex mu;
+    unsigned njobs = 0;
+    uint64_t gen = 0;
+    int inst = -1;               // the newest instance queued (-1: none this generation)
+    int64_t instances = 0, jobs = 0;   // statistics
+    int init(int workgroups);    // workgroups <= 0: one per CU
+    int publish(const HerkProblem &hp, LoopTicket *t);
+    // A helper instance on s (the publishing request's stream) that takes
+    // items of any published job until none is left: up to `items`
+    // workgroups, at most two per CU.
+    int help(int items, hipStream_t s);
+    int cus = 0;
+    bool helpers = true;         // publish callers add helper instances
+    int wait(const LoopTicket &t);
+    int stop();
+    ~ResidentLoop();
+
+  private:
+    int launch_next();           // with mu held
+    int recycle();               // with mu held
+    int wait_index(unsigned j, uint64_t g);
+};
+
 // Factor buffers of one in-flight chunk (two alternate, so the factors of
 // the next chunk are built while the current one updates the accumulators).
 struct CovSlot {
@@ -108,6 +151,9 @@ struct CovSlot {
     bool used = false;       // events recorded and not yet harvested
     int64_t np = 0, k2 = 0;
     unsigned *bound_host = nullptr;   // pinned copy of bound[0], written before ev[5]
+    LoopTicket tk[2];                 // resident loop: the chunk's two jobs
+    bool looped = false;              // the update went to the resident loop
+    double loop_t0 = 0;               // resident loop: host time of publishing (ms)
 };
 
 // Covariance accumulation state across chunks.
@@ -147,6 +193,7 @@ struct UpdateBatcher {
 
 struct CovEngine {
     UpdateBatcher *batcher = nullptr;   // serving: shared update launches
+    ResidentLoop *loop = nullptr;       // serving: updates go to the resident loop
     bool mixed = false;
     int nv = 0, nl = 0, n = 0, tile = 0;
     int ks_m = 0, ks_s = 0;
@@ -170,6 +217,7 @@ struct CovEngine {
     int consume(hipStream_t sc);
     int harvest(CovSlot &S);
     int drain();
+    int wait_loop(CovSlot &S);          // the slot's resident-loop jobs are done
     int chunk(const ProblemState &ps, TraceChunk &tc, hipStream_t s);
     int finalize(const ProblemState &ps, const scmc_problem *pr, int64_t max_itr, double *ms_re, double *ms_im,
                  double *mm_re, double *mm_im, hipStream_t s);

@@ -12,6 +12,14 @@
 // paths, so the kernels of all admitted requests overlap on the GPU; a cancel takes effect at the next slice boundary and
 // frees the slot.  Results are normalised exactly as scmc_mccov and depend
 // only on the request (Philox paths, fixed slice schedule), not on the mix.
+//
+// Update modes for the mixed covariance updates: each chunk's update launched
+// on its slot's stream (default), the small requests' updates in shared
+// launches (shared_launches), or every update as a job of one resident loop
+// kernel that stays on the GPU and takes work items from a job table in host
+// memory (resident_loop, ResidentLoop in resident.cpp).  The results are the
+// same in all three (bit-identical for the resident loop); on the request
+// workload separate launches measured fastest, so they are the default.
 #include "scmc_internal.h"
 
 #include <cstdlib>
@@ -61,6 +69,7 @@ struct Server {
     bool stop = false;
     bool shared_updates = false;   // mixed updates of concurrent small requests in shared launches
     UpdateBatcher batcher;
+    std::unique_ptr<ResidentLoop> loop;   // mixed updates in one resident kernel (resident_loop)
     // A server still running when the process exits (no scmc_server_stop)
     // is stopped here: joinable threads would otherwise end the process in
     // std::terminate.  Running requests are cancelled at their next slice.
@@ -78,6 +87,7 @@ struct Server {
                 t.join();
                 joined = true;
             }
+        loop.reset();
         if (joined) reaper_stop();
     }
 };
@@ -156,8 +166,10 @@ static void run_request(Server *S, Request *r, Session &sess)
     r->rc = rc;
     if (rc) r->err = scmc_last_error();
     // every kernel of the request has finished: result() synchronised the
-    // session's stream after draining both (on an error path, wait here)
+    // session's stream after draining both (on an error path, wait here,
+    // resident-loop jobs included)
     if (rc) {
+        for (auto &sl : sess.ce.slot) (void)sess.ce.wait_loop(sl);
         if (sess.stream2) (void)hipStreamSynchronize(sess.stream2);
         if (sess.stream) (void)hipStreamSynchronize(sess.stream);
     }
@@ -197,6 +209,7 @@ static void slot_loop(Server *S)
         }
         if (!sess) sess.reset(new Session());
         sess->ce.batcher = S->shared_updates ? &S->batcher : nullptr;
+        sess->ce.loop = S->loop.get();
         run_request(S, r, *sess);
         if (r->rc || r->footprint * 2 * std::max(1, S->cfg.max_active) > S->budget) sess.reset();
         {
@@ -270,6 +283,11 @@ int scmc_server_start(const scmc_server_config *cfg)
     std::call_once(at_exit, [] { std::atexit([] { (void)scmc_server_stop(); }); });
     S->shared_updates = S->cfg.shared_launches != 0;
     if (S->shared_updates && (rc = S->batcher.init())) return rc;
+    if (S->cfg.resident_loop) {
+        if (S->shared_updates) return set_error(SCMC_E_ARG, "shared_launches and resident_loop exclude each other");
+        S->loop.reset(new ResidentLoop());
+        if ((rc = S->loop->init(S->cfg.resident_workgroups))) return rc;
+    }
     Server *raw = S.get();
     reaper_start();
     for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);
@@ -302,6 +320,11 @@ int scmc_server_stop(void)
     S->cv_work.notify_all();
     S->cv_done.notify_all();
     for (auto &t : S->slots) t.join();   // their sessions' buffers go to the reaper
+    if (S->loop) {
+        // every job is finished (the slots have drained): end the instance
+        (void)S->loop->stop();
+        S->loop.reset();
+    }
     reaper_stop();
     // every request is finished now: wake any waiter that is still asleep
     S->cv_done.notify_all();

@@ -555,11 +555,14 @@ atexit.register(_stop_server_at_exit)
 class Server:
     """Continuous-batching covariance server (scmc_server_* in scmc.h)."""
 
-    def __init__(self, max_active=4, hbm_budget_gb=0.0, paths_per_slice=0, device=0, shared_launches=False):
+    def __init__(self, max_active=4, hbm_budget_gb=0.0, paths_per_slice=0, device=0, shared_launches=False,
+                 resident_loop=False, resident_workgroups=0):
         cfg = N.ServerConfig()
         cfg.device, cfg.max_active = device, max_active
         cfg.hbm_budget_gb, cfg.paths_per_slice = hbm_budget_gb, paths_per_slice
         cfg.shared_launches = 1 if shared_launches else 0
+        cfg.resident_loop = 1 if resident_loop else 0
+        cfg.resident_workgroups = int(resident_workgroups)
         N.check(N.lib().scmc_server_start(C.byref(cfg)))
         self._specs = {}
 
