@@ -756,12 +756,7 @@ int CovEngine::consume(hipStream_t sc)
         if ((rc = loop->publish(hm, &S.tk[0])) || (rc = loop->publish(hs, &S.tk[1]))) return rc;
         // extra workgroups where the hardware finds room, as a launch would
         // have (they leave when no item is left, the resident ones stay)
-        if (loop->helpers) {
-            LoopJob a{}, b{};
-            loop_job_from(hm, &a);
-            loop_job_from(hs, &b);
-            if ((rc = loop->help(a.nitems8 + b.nitems8, sc))) return rc;
-        }
+        if (loop->helpers && (rc = loop->help(S.tk[0], S.tk[1], sc))) return rc;
     } else if (mixed && batcher && n <= kSharedUpdateMaxN) {
         // serving: one launch shared with the other small requests' chunks
         // (a larger problem fills the GPU alone and launches on its own

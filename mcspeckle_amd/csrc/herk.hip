@@ -450,11 +450,11 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_kernel(MixedGroup<MAXG> G)
 // behind this one on the loop's stream (the next instance continues from the
 // same counters).  `stop` ends idle workgroups at once.
 //
-// Helper instances (inst < 0) run the same loop but leave as soon as they
-// find nothing left to take and never announce it: launched on a request's
-// own stream when it publishes a chunk, they add workgroups wherever the
-// hardware finds room, as a launch of the update would, while the resident
-// instance keeps one workgroup per CU.
+// Helper instances (inst < 0) take items only of the jobs help_lo..help_hi
+// (a chunk's two updates), leave when those are all taken and never announce
+// it: launched on a request's own stream when it publishes a chunk, they add
+// workgroups wherever the hardware finds room, as a launch of the update
+// would, while the resident instance keeps one workgroup per CU.
 __device__ __forceinline__ unsigned sys_peek(const unsigned *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -470,7 +470,8 @@ __device__ __forceinline__ LoopJob read_job(const LoopJob *j)
     return __builtin_bit_cast(LoopJob, v);
 }
 
-__global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, LoopDev *D, int inst, uint64_t idle_ticks)
+__global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, LoopDev *D, int inst, uint64_t idle_ticks,
+                                                                 unsigned help_lo, unsigned help_hi)
 {
     using namespace mixed;
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
@@ -483,8 +484,9 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, Loo
     volatile uint64_t *jw = (volatile uint64_t *)(lds + (NBUF - 1) * STAGEB + 16);
     constexpr int kJobWords = (int)(sizeof(LoopJob) / 8);
     const unsigned x = blockIdx.x & 7;
-    unsigned p = 0;   // thread 0: this workgroup's cursor
-    if (threadIdx.x == 0) p = __hip_atomic_load(&D->active[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // thread 0: this workgroup's cursor (a helper walks only its own jobs)
+    unsigned p = help_lo;
+    if (threadIdx.x == 0 && inst >= 0) p = __hip_atomic_load(&D->active[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         // every thread is done with the previous item and its claim
         __syncthreads();
@@ -492,6 +494,7 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, Loo
             int job = -1, item = 0;
             uint64_t t0 = wall_clock64();
             for (;;) {
+                if (inst < 0 && p > help_hi) break;
                 if (p < sys_peek(&H->published)) {
                     const int n8 = (int)__hip_atomic_load(&H->jobs[p].nitems8, __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_SYSTEM);
@@ -503,11 +506,11 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, Loo
                     }
                     // this label's share of job p is taken
                     ++p;
-                    atomicMax(&D->active[x], p);
+                    if (inst >= 0) atomicMax(&D->active[x], p);
                     t0 = wall_clock64();
                     continue;
                 }
-                // a helper instance (inst < 0) leaves as soon as nothing is left
+                // a helper's jobs are published before it is launched
                 if (inst < 0 || sys_peek(&H->stop)) break;
                 if (wall_clock64() - t0 > idle_ticks) {
                     __hip_atomic_store(&H->closing[inst & 1], 1u, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -756,9 +759,11 @@ void loop_job_from(const HerkProblem &hp, LoopJob *out)
     *out = j;
 }
 
-void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s)
+void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s,
+                            unsigned help_lo, unsigned help_hi)
 {
-    hipLaunchKernelGGL(herk_mixed_loop_kernel, dim3((unsigned)grid), dim3(256), 0, s, ctl_dev, dev, inst, idle_ticks);
+    hipLaunchKernelGGL(herk_mixed_loop_kernel, dim3((unsigned)grid), dim3(256), 0, s, ctl_dev, dev, inst, idle_ticks,
+                       help_lo, help_hi);
 }
 
 void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s)

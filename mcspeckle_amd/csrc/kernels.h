@@ -149,8 +149,11 @@ struct LoopDev {
 };
 void loop_job_from(const HerkProblem &hp, LoopJob *out);
 // One instance of the loop kernel (grid workgroups of 256 threads) on s; it
-// returns when idle for idle_ticks of the device wall clock, or on stop.
-void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s);
+// returns when idle for idle_ticks of the device wall clock, or on stop.  A
+// helper instance (inst < 0) takes items of the published jobs
+// help_lo..help_hi only and returns when they are all taken.
+void launch_herk_mixed_loop(LoopCtl *ctl_dev, LoopDev *dev, int inst, uint64_t idle_ticks, int grid, hipStream_t s,
+                            unsigned help_lo = 0, unsigned help_hi = 0);
 
 // Sum the ksplit accumulators, restore the upper triangle, scale, apply the
 // optional per-row factor rowf[(v,l)] (single scattering amplitude), and write

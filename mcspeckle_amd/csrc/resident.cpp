@@ -97,12 +97,20 @@ int ResidentLoop::publish(const HerkProblem &hp, LoopTicket *t)
     return SCMC_OK;
 }
 
-int ResidentLoop::help(int items, hipStream_t s)
+int ResidentLoop::help(const LoopTicket &a, const LoopTicket &b, hipStream_t s)
 {
     int rc = SCMC_OK;
+    // the two tickets of one publish are consecutive jobs of one generation
+    int lo = -1, hi = -1, items = 0;
+    for (const LoopTicket *t : {&a, &b}) {
+        if (t->idx < 0) continue;
+        lo = lo < 0 ? t->idx : std::min(lo, t->idx);
+        hi = std::max(hi, t->idx);
+        items += H->jobs[t->idx].nitems8;
+    }
     const int g = std::min(items, 2 * std::max(cus, 1));
-    if (g <= 0) return SCMC_OK;
-    launch_herk_mixed_loop(Hd, D, -1, 0, g, s);
+    if (lo < 0 || g <= 0) return SCMC_OK;
+    launch_herk_mixed_loop(Hd, D, -1, 0, g, s, (unsigned)lo, (unsigned)hi);
     return hip_ok(hipGetLastError(), "resident loop helper", &rc) ? SCMC_OK : rc;
 }
 

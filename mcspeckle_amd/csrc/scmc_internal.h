@@ -126,9 +126,9 @@ struct ResidentLoop {
     int init(int workgroups);    // workgroups <= 0: one per CU
     int publish(const HerkProblem &hp, LoopTicket *t);
     // A helper instance on s (the publishing request's stream) that takes
-    // items of any published job until none is left: up to `items`
-    // workgroups, at most two per CU.
-    int help(int items, hipStream_t s);
+    // items of the published jobs a and b only (an empty ticket is skipped)
+    // until none is left: one workgroup per item, at most two per CU.
+    int help(const LoopTicket &a, const LoopTicket &b, hipStream_t s);
     int cus = 0;
     bool helpers = true;         // publish callers add helper instances
     int wait(const LoopTicket &t);
