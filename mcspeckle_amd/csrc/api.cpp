@@ -308,6 +308,16 @@ int ProblemState::setup(const scmc_problem *pr, hipStream_t s)
     P.path_offset = pr->path_offset;
     P.w0p_iso = 1.0 / sqrt(pow(2.0, dim - 1) * M_PI);
     P.ff_sign = pr->is_ff_v ? -1.0 : 1.0;
+    // a view's exit distance through the last axis's faces is at most
+    // W_last / max(|v_last|, 1e-7); through another axis j at least the
+    // point's distance to its nearer face j (|v_j| <= 1)
+    P.far_last_bound = INFINITY;
+    if (pr->is_ff_v && pr->nv > 0 && pr->v) {
+        double vmin = INFINITY;
+        for (int i = 0; i < pr->nv; ++i) vmin = std::min(vmin, fabs(pr->v[(size_t)i * dim + dim - 1]));
+        const double b = (P.bmax[dim - 1] - P.bmin[dim - 1]) / std::max(vmin, 1.0000000000000004e-07);
+        if (std::isfinite(b)) P.far_last_bound = (float)(b * (1 + 1e-5));
+    }
     // unmeanl / meanl: MCcov.m:82-83, MCfield.m:77-82
     double um[3] = {0, 0, 0};
     if (pr->has_lmean0) {

@@ -42,6 +42,11 @@ struct DevProblem {
     uint64_t path_offset;
     double w0p_iso;                   // 1/sqrt(2^(dim-1) pi)
     double ff_sign;                   // -1 far-field views, +1 near-field
+    // Far-field views: no view leaves the box farther than this through the
+    // faces of the last axis (W_last / min |v_last|, with a 1e-5 margin), so
+    // a point whose distance to the nearer face exceeds it on every other
+    // axis leaves every view through the last axis (+inf: no such bound)
+    float far_last_bound;
 };
 
 // ---------------------------------------------------------------- Philox

@@ -27,6 +27,8 @@
 #include "hetero.h"
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace scmc {
 
 constexpr int kFieldViews = 256;   // views per workgroup
@@ -219,10 +221,25 @@ __device__ __forceinline__ float near_face(double x, double lo, double hi)
     return dlo <= dhi ? (float)dlo : -(float)dhi;
 }
 
-__device__ __forceinline__ int qneg_bits(const float *q)
+// Bits 0-2: the sign bits of q.  Bit 3 (kLastAxisOnly): the point is farther
+// than P.far_last_bound from the nearer face on every axis but the last, so
+// every far-field view leaves it through the last axis's faces (the exit
+// distance through axis j is at least |q_j| since |v_j| <= 1, through the last
+// axis at most that bound); the minimum over the faces is then that face's
+// distance, exactly as the full minimum would find it (the bound carries a
+// margin over FP32 rounding).  The kernels test the bit for a whole batch of
+// entries (one barrier) and run the batch with only the last axis when every
+// entry has it -- in a slab whose lateral extent is far larger than its
+// thickness, nearly every batch.
+constexpr int kLastAxisOnly = 8;
+template <int DIM>
+__device__ __forceinline__ int qneg_bits(const DevProblem &P, const float *q)
 {
-    return (int)((unsigned)__float_as_int(q[0]) >> 31 | ((unsigned)__float_as_int(q[1]) >> 31) << 1 |
-                 ((unsigned)__float_as_int(q[2]) >> 31) << 2);
+    int b = (int)((unsigned)__float_as_int(q[0]) >> 31 | ((unsigned)__float_as_int(q[1]) >> 31) << 1 |
+                  ((unsigned)__float_as_int(q[2]) >> 31) << 2);
+    bool last = true;
+    for (int j = 0; j + 1 < DIM; ++j) last = last && fabsf(q[j]) > P.far_last_bound;
+    return last ? b | kLastAxisOnly : b;
 }
 
 struct FEntry {
@@ -268,7 +285,7 @@ __device__ __forceinline__ void load_entry(const DevProblem &P, int64_t e, int64
         f.q[j] = j < DIM ? near_face(f.x[j], P.bmin[j], P.bmax[j]) : 0.f;
         f.adf[j] = (float)f.ad[j];
     }
-    f.qneg = qneg_bits(f.q);
+    f.qneg = qneg_bits<DIM>(P, f.q);
 }
 
 // Tabulated amplitude with the nearest-bin index formed in FP64, so the bin
@@ -330,16 +347,23 @@ __device__ __forceinline__ float exit_dist(const FarView &fv, int j, float q, in
 // of a real amplitude factor folded into the same exp2 (0 for none), so an
 // isotropic or HG amplitude costs no extra exponential and no complex
 // multiply.
-template <int DIM>
+template <int DIM, bool LAST = false>
 __device__ __forceinline__ float2 e_far_mixed(const double *x, const FarView &fv, const float *q, int qneg, float nhs,
                                               float lamp = 0.f)
 {
     const double cyc = dot<DIM>(x, fv.vn);
     const float fr = (float)(cyc - rint(cyc));
     const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
-    // the first face starts the minimum (a start at +inf costs a min per call)
-    float best = exit_dist(fv, 0, q[0], qneg);
-    for (int j = 1; j < DIM; ++j) best = fminf(best, exit_dist(fv, j, q[j], qneg));
+    float best;
+    if (LAST) {
+        // the point is farther from the other axes' faces than any view's exit
+        // through the last axis (kLastAxisOnly, for every entry of the batch)
+        best = exit_dist(fv, DIM - 1, q[DIM - 1], qneg);
+    } else {
+        // the first face starts the minimum (a start at +inf costs a min per call)
+        best = exit_dist(fv, 0, q[0], qneg);
+        for (int j = 1; j < DIM; ++j) best = fminf(best, exit_dist(fv, j, q[j], qneg));
+    }
     const float mag = __builtin_amdgcn_exp2f(fmaf(nhs, best, lamp));
     return make_float2(mag * cs, mag * sn);
 }
@@ -365,8 +389,14 @@ __device__ __forceinline__ float2 e_far_mixed_het(const DevProblem &P, const Het
 // per-entry scalar work are shared between them.  AMPT: 0 reads the
 // amplitude type at run time; 3 is an instance for HG amplitudes only (the
 // uniform type tests then fold away).
+// Waves per SIMD the compiler must leave room for: five for the mixed HG
+// one-column instance with two views per lane (MCfield), whose batch loop
+// exists twice (all faces / last axis only) and would otherwise settle at four.
+constexpr int field_min_waves(bool mixed, bool het, int vpt, int ampt) { return mixed && !het && vpt == 2 && ampt == 3 ? 5 : 1; }
+
 template <int DIM, bool MIXED, bool WAVE, int NLQ, bool HET = false, int VPT = 1, int AMPT = 0>
-__global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t nent,
+__global__ __launch_bounds__(kFieldViews) __attribute__((amdgpu_waves_per_eu(field_min_waves(MIXED, HET, VPT, AMPT))))
+void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t nent,
                                                             int nsplit, int l0, int nlp, const PathRec *paths,
                                                             const VertexRec *verts, const int32_t *vpath,
                                                             const int64_t *offsets, const double2 *coef,
@@ -441,13 +471,18 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
             sc[ei][q] = c;
             if (F32ACC) scf[ei][q] = make_float2((float)c.x, (float)c.y);
         }
-        __syncthreads();
+        // mixed far field: the batch leaves every view through the last axis
+        int mine = 0;
+        if (MIXED && !HET) mine = threadIdx.x >= cnt || (se[threadIdx.x].qneg & kLastAxisOnly) != 0;
+        const bool batch_last = __syncthreads_and(mine) != 0;
         if (!vok[0]) continue;
         float2 accf[VPT][NLQ];
 #pragma unroll
         for (int t = 0; t < VPT; ++t)
 #pragma unroll
             for (int q = 0; q < NLQ; ++q) accf[t][q] = make_float2(0.f, 0.f);
+        auto entries = [&](auto last_tag) {
+        constexpr bool LAST = decltype(last_tag)::value;
         for (int ei = 0; ei < cnt; ++ei) {
             const FEntry &f = se[ei];
             // every lane reads the same entry: branch on its kind as a scalar
@@ -480,7 +515,7 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     if (HET) {
                         evf = e_far_mixed_het<DIM>(P, M, x, mv[t], lamp);
                     } else {
-                        evf = e_far_mixed<DIM>(x, fv[t], q, qneg, nhs, lamp);
+                        evf = e_far_mixed<DIM, LAST>(x, fv[t], q, qneg, nhs, lamp);
                     }
                     if (kind != 0 && tab) {
                         const float2 am = amp_tab<DIM>(dot<DIM>(f.ad, vd[t]), P.amp);
@@ -524,6 +559,15 @@ __global__ __launch_bounds__(kFieldViews) void field_kernel(DevProblem P, HetMed
                     }
                 }
             }
+        }
+        };
+        if constexpr (MIXED && !HET) {
+            if (batch_last)
+                entries(std::true_type{});
+            else
+                entries(std::false_type{});
+        } else {
+            entries(std::false_type{});
         }
         if (F32ACC) {
 #pragma unroll
@@ -711,33 +755,44 @@ __global__ __launch_bounds__(kFieldViews) void field_wave_first_kernel(DevProble
             sx[i][j] = xj;
             sq[i][j] = j < DIM ? near_face(xj, P.bmin[j], P.bmax[j]) : 0.f;
         }
+        int mine = 1;
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
             float q[3];
             for (int j = 0; j < 3; ++j) q[j] = j < DIM ? near_face(paths[pb + i].x1[j], P.bmin[j], P.bmax[j]) : 0.f;
-            sneg[i] = qneg_bits(q);
+            sneg[i] = qneg_bits<DIM>(P, q);
+            mine &= (sneg[i] & kLastAxisOnly) != 0;
         }
         for (int t = threadIdx.x; t < cnt * kWaveLG; t += blockDim.x) {
             const int i = t / kWaveLG, q = t % kWaveLG;
             scl[i][q] = q < nlg ? cl0[(size_t)(pb + i) * P.nl + l0 + q] : make_double2(0.0, 0.0);
         }
-        __syncthreads();
+        // the batch's first vertices leave every view through the last axis
+        const bool batch_last = __syncthreads_and(mine) != 0;
         if (!vok) continue;
-        for (int i = 0; i < cnt; ++i) {
-            const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
-            double2 ev;
-            if (MIXED) {
-                const float2 e = e_far_mixed<DIM>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
-                ev = make_double2(e.x, e.y);
-            } else {
-                ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
-            }
+        auto paths_of_batch = [&](auto last_tag) {
+            constexpr bool LAST = decltype(last_tag)::value;
+            for (int i = 0; i < cnt; ++i) {
+                const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
+                double2 ev;
+                if (MIXED) {
+                    const float2 e =
+                        e_far_mixed<DIM, LAST>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
+                    ev = make_double2(e.x, e.y);
+                } else {
+                    ev = phase_att64<DIM>(P, x, P.v + (size_t)v * DIM, P.ff_sign, P.is_ff_v);
+                }
 #pragma unroll
-            for (int q = 0; q < kWaveLG; ++q) {
-                const double2 c = scl[i][q];
-                acc[q].x += ev.x * c.x - ev.y * c.y;
-                acc[q].y += ev.x * c.y + ev.y * c.x;
+                for (int q = 0; q < kWaveLG; ++q) {
+                    const double2 c = scl[i][q];
+                    acc[q].x += ev.x * c.x - ev.y * c.y;
+                    acc[q].y += ev.x * c.y + ev.y * c.x;
+                }
             }
-        }
+        };
+        if (MIXED && batch_last)
+            paths_of_batch(std::true_type{});
+        else
+            paths_of_batch(std::false_type{});
     }
     if (vok)
 #pragma unroll
@@ -797,10 +852,12 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
             sx[i][j] = xj;
             sq[i][j] = j < DIM ? near_face(xj, P.bmin[j], P.bmax[j]) : 0.f;
         }
+        int mine = 1;
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
             float q[3];
             for (int j = 0; j < 3; ++j) q[j] = j < DIM ? near_face(paths[pb + i].x1[j], P.bmin[j], P.bmax[j]) : 0.f;
-            sneg[i] = qneg_bits(q);
+            sneg[i] = qneg_bits<DIM>(P, q);
+            mine &= (sneg[i] & kLastAxisOnly) != 0;
         }
         for (int t = threadIdx.x; t < kWmPB * kWmLights; t += blockDim.x) {
             const int i = t / kWmLights, q = t % kWmLights;
@@ -811,21 +868,29 @@ __global__ __launch_bounds__(256) void field_wave_first_mfma_kernel(DevProblem P
             }
             scl[i][q] = c;
         }
-        __syncthreads();
-        for (int i = 0; i < cnt; ++i) {
-            const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
-            float2 ev = make_float2(0.f, 0.f);
-            if (vok) ev = e_far_mixed<DIM>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
-            const float a1 = kk == 0 ? ev.x : -ev.y;
-            const float a2 = kk == 0 ? ev.y : ev.x;
+        // the batch's first vertices leave every view through the last axis
+        const bool batch_last = __syncthreads_and(mine) != 0;
+        auto paths_of_batch = [&](auto last_tag) {
+            constexpr bool LAST = decltype(last_tag)::value;
+            for (int i = 0; i < cnt; ++i) {
+                const double x[3] = {sx[i][0], sx[i][1], sx[i][2]};
+                float2 ev = make_float2(0.f, 0.f);
+                if (vok) ev = e_far_mixed<DIM, LAST>(x, fv, sq[i], __builtin_amdgcn_readfirstlane(sneg[i]), nhs);
+                const float a1 = kk == 0 ? ev.x : -ev.y;
+                const float a2 = kk == 0 ? ev.y : ev.x;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const float2 c = scl[i][b * 32 + m];
-                const float bv = kk == 0 ? c.x : c.y;
-                cr[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, cr[b], 0, 0, 0);
-                ci[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, bv, ci[b], 0, 0, 0);
+                for (int b = 0; b < NB; ++b) {
+                    const float2 c = scl[i][b * 32 + m];
+                    const float bv = kk == 0 ? c.x : c.y;
+                    cr[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, cr[b], 0, 0, 0);
+                    ci[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, bv, ci[b], 0, 0, 0);
+                }
             }
-        }
+        };
+        if (batch_last)
+            paths_of_batch(std::true_type{});
+        else
+            paths_of_batch(std::false_type{});
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
