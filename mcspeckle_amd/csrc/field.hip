@@ -544,7 +544,11 @@ void field_kernel(DevProblem P, HetMedium M, int64_t np, int64_t nvert, int64_t 
                     }
                 }
                 if (!F32ACC && kind == 2) ev = cmul(ev, f.div);
-                if (kind == 0) {
+                // one column: the first vertex's amplitude factor goes into ev,
+                // so every kind adds with the same four FMAs (one code path
+                // keeps the accumulators in place across the loop)
+                if (NLQ == 1 && kind == 0) ev = cmul(ev, afv[t][0]);
+                if (NLQ > 1 && kind == 0) {
                     for (int q = 0; q < NLQ; ++q) {
                         const double2 tt = cmul(ev, afv[t][q]), c = sce[q];
                         acc[t][q].x += tt.x * c.x - tt.y * c.y;

@@ -148,6 +148,11 @@ int ResidentLoop::recycle()
         if ((rc = wait_index(j, gen))) return rc;
     __atomic_store_n(&H->stop, 1u, __ATOMIC_SEQ_CST);
     if (!hip_ok(hipStreamSynchronize(stream), "resident loop stop", &rc)) return rc;
+    // helper instances run on the requests' own streams: one still queued
+    // there would start after the reset below and take an item of an old
+    // job (its job index is reused by the next generation), so wait for the
+    // whole device; all their jobs are done, so they end at once
+    if (!hip_ok(hipDeviceSynchronize(), "resident loop helpers", &rc)) return rc;
     if (!hip_ok(hipMemsetAsync(D, 0, sizeof(LoopDev), stream), "loop counters", &rc) ||
         !hip_ok(hipStreamSynchronize(stream), "loop counters", &rc))
         return rc;
