@@ -152,6 +152,7 @@ def serve(n_req, rate, path_scale, max_active, mode="streams", seed=5, timeline=
     t_end = time.perf_counter() - t0
     th.join()
     S.stop()
+    print(f"serving {mode}: {n_req} requests in {t_end:.1f} s", file=sys.stderr, flush=True)
     lat, queued = np.array(lat), np.array(queued)
     out = {"requests": n_req, "arrival_rate_per_s": rate if np.isfinite(rate) else None,
            "saturated": not np.isfinite(rate), "path_scale": path_scale, "max_active": max_active,
