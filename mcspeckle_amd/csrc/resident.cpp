@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include <vector>
 
 namespace scmc {
 
@@ -44,7 +45,15 @@ int ResidentLoop::init(int workgroups)
     memset(H, 0, sizeof(LoopCtl));
     if (!hip_ok(hipHostGetDevicePointer((void **)&Hd, H, 0), "loop control pointer", &rc)) return rc;
     if (!hip_ok(hipMalloc((void **)&D, sizeof(LoopDev)), "loop counters", &rc)) return rc;
-    if (!hip_ok(hipStreamCreate(&stream), "loop stream", &rc)) return rc;
+    // The loop's stream gets a hardware queue of its own: HIP maps a
+    // process's streams onto a few in-order hardware queues, and a request
+    // stream sharing the queue of the resident kernel would wait behind it
+    // until it leaves.  A stream with a CU mask (here every CU) is given its
+    // own queue.
+    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
+    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1;
+    if (!hip_ok(hipExtStreamCreateWithCUMask(&stream, (uint32_t)mask.size(), mask.data()), "loop stream", &rc))
+        return rc;
     if (!hip_ok(hipMemsetAsync(D, 0, sizeof(LoopDev), stream), "loop counters", &rc) ||
         !hip_ok(hipStreamSynchronize(stream), "loop counters", &rc))
         return rc;
