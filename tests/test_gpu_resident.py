@@ -11,7 +11,9 @@ Cauchy-Schwarz metric, and reported when it is not bit-identical):
     publish launches another;
   * more jobs than the job table holds (a new generation of the table);
   * a stream drawn from the serving workload against the FP64 one-shot engine
-    (<= 1e-4).
+    (<= 1e-4);
+  * cancel and server stop with requests in flight, and field requests served
+    beside the loop's covariance requests.
 """
 import sys
 import time
@@ -112,3 +114,53 @@ def test_resident_loop_serving_stream_matches_fp64():
         err = cs_err(Ms + Mm, Ms0 + Mm0)
         say("  request N", spec.nv * spec.nl, "paths", n, "err", err)
         assert err <= 1e-4, (spec.nv, spec.nl, err)
+
+
+def test_resident_loop_cancel_and_stop_with_requests_in_flight():
+    """Cancelling a running request and stopping the server with requests in
+    flight end cleanly with the resident loop: the cancelled and the
+    never-admitted requests report SCMC_E_CANCELLED, a request that finishes
+    in between is exact, and the next server starts normally."""
+    from mcspeckle_amd import _native as N
+    ref = run([MIX[0]], False, max_active=1)
+    S = E.Server(max_active=2, paths_per_slice=300, resident_loop=True)
+    try:
+        big = S.submit(req(9, 64, 32, 10.0, 0.9, seed=9), 200000, "mixed")
+        small = S.submit(req(0, 8, 8, 5.0, 0.5, seed=0), 2000, "mixed")
+        pending = [S.submit(req(10 + i, 48, 48, 8.0, 0.8, seed=10 + i), 100000, "mixed") for i in range(3)]
+        Ms, Mm, st = S.wait(small)
+        assert np.array_equal(Ms, ref[0][0]) and np.array_equal(Mm, ref[0][1])
+        time.sleep(0.2)
+        S.cancel(big)
+        with pytest.raises(N.ScmcError) as ei:
+            S.wait(big)
+        assert ei.value.code == N.SCMC_E_CANCELLED
+        say("cancelled the running request; stopping with", len(pending), "more in flight or pending")
+    finally:
+        S.stop()
+    for rid in pending:
+        with pytest.raises(N.ScmcError) as ei:
+            S.wait(rid)
+        assert ei.value.code in (N.SCMC_E_CANCELLED, N.SCMC_E_ARG)
+    again = run([MIX[0]], True, max_active=1)
+    assert np.array_equal(again[0][0], ref[0][0]) and np.array_equal(again[0][1], ref[0][1])
+
+
+def test_resident_loop_beside_field_requests():
+    """Field requests (not updates, so not loop jobs) served beside
+    covariance requests whose updates go to the resident loop: both kinds
+    equal their results without the loop."""
+    rng = np.random.default_rng(44)
+    field = E.make_problem(6.0, 0.97, [-5e4, -5e4, 0], [5e4, 5e4, 1], cone_dirs(2, 10, rng), cone_dirs(900, 12, rng),
+                           1, 1, 1.0, True, 2, 3, 0.8, seed=70)
+    out = {}
+    for resident in (False, True):
+        with E.Server(max_active=3, paths_per_slice=400, resident_loop=resident) as S:
+            fid = S.submit(field, 3000, "mixed", kind="field")
+            cids = {i: S.submit(req(i, nv, nl, od, g, seed=i, cbs=cbs), n, prec)
+                    for i, nv, nl, od, g, n, prec, cbs in MIX[:3]}
+            out[resident] = (S.wait(fid), {i: S.wait(r) for i, r in cids.items()})
+    (ua, _), ca = out[False]
+    (ub, _), cb = out[True]
+    assert np.array_equal(ua, ub)
+    compare(ca, cb, MIX[:3])
