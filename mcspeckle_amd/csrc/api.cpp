@@ -119,6 +119,62 @@ void reaper_stop()
     delete R;
 }
 
+// Idle device blocks kept across sessions and servers.  A session's pool
+// gives blocks up only once its request has synchronised (settle(), or the
+// session's end, after its streams drained), so no queued kernel can still
+// use them: they are kept here, up to kIdleCap bytes, for the next session or
+// call instead of going back to the driver.  Without this, a server started
+// after another one had freed its sessions' memory allocated all of it anew
+// while its other slots kept work queued, and with 16 slots those
+// allocations held every slot's HIP calls for up to 2.3 s.
+namespace {
+struct IdleCache {
+    std::mutex mu;
+    std::multimap<size_t, void *> blocks;   // by size
+    size_t bytes = 0;
+};
+IdleCache g_idle;
+constexpr size_t kIdleCap = (size_t)64 << 30;
+} // namespace
+
+static void idle_put(void *p, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> g(g_idle.mu);
+        if (g_idle.bytes + bytes <= kIdleCap) {
+            g_idle.blocks.emplace(bytes, p);
+            g_idle.bytes += bytes;
+            return;
+        }
+    }
+    device_free(p);
+}
+
+// smallest idle block of at least want bytes, at most twice that
+static void *idle_take(size_t want, size_t *got)
+{
+    std::lock_guard<std::mutex> g(g_idle.mu);
+    auto it = g_idle.blocks.lower_bound(want);
+    if (it == g_idle.blocks.end() || it->first > 2 * want) return nullptr;
+    void *p = it->second;
+    *got = it->first;
+    g_idle.bytes -= it->first;
+    g_idle.blocks.erase(it);
+    return p;
+}
+
+// every idle block back to the driver (an allocation failed)
+static void idle_trim()
+{
+    std::multimap<size_t, void *> b;
+    {
+        std::lock_guard<std::mutex> g(g_idle.mu);
+        b.swap(g_idle.blocks);
+        g_idle.bytes = 0;
+    }
+    for (const auto &kv : b) (void)hipFree(kv.second);
+}
+
 thread_local BlockPool *tls_block_pool = nullptr;
 
 void *BlockPool::take(size_t want, size_t *got)
@@ -146,7 +202,8 @@ void BlockPool::settle()
     std::sort(ready.begin(), ready.end(), [](const Blk &a, const Blk &b) { return a.bytes < b.bytes; });
     while (total > cap && !ready.empty()) {
         total -= ready.back().bytes;
-        device_free(ready.back().p);
+        held -= ready.back().bytes;
+        idle_put(ready.back().p, ready.back().bytes);
         ready.pop_back();
     }
 }
@@ -155,10 +212,11 @@ void BlockPool::trim(bool now)
 {
     for (const std::vector<Blk> *v : {&ready, &retired})
         for (const Blk &b : *v) {
+            held -= b.bytes;
             if (now)
                 (void)hipFree(b.p);
             else
-                device_free(b.p);
+                idle_put(b.p, b.bytes);
         }
     ready.clear();
     retired.clear();
@@ -203,20 +261,21 @@ bool DevBuf::ensure(size_t b, int *rc)
     size_t want = b ? b + b / 8 : 16;
     want = (want + 255) & ~(size_t)255;
     BlockPool *bp = tls_block_pool;
-    if (bp) {
-        size_t got = 0;
-        if (void *q = bp->take(want, &got)) {
-            p = q;
-            bytes = got;
-            pool = bp;
-            return true;
-        }
+    size_t got = 0;
+    void *q = bp ? bp->take(want, &got) : nullptr;
+    if (!q && (q = idle_take(want, &got)) && bp) bp->held += got;
+    if (q) {
+        p = q;
+        bytes = got;
+        pool = bp;
+        return true;
     }
     hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess && bp) {
+    if (e != hipSuccess) {
         // kept blocks may be what the driver is missing
         (void)hipGetLastError();
-        bp->trim(true);
+        if (bp) bp->trim(true);
+        idle_trim();
         e = hipMalloc(&p, want);
     }
     if (e != hipSuccess) {
@@ -226,7 +285,40 @@ bool DevBuf::ensure(size_t b, int *rc)
     }
     bytes = want;
     pool = bp;
+    if (bp) bp->held += want;
     return true;
+}
+
+thread_local bool tls_sleepy_waits = false;
+
+hipError_t event_wait(hipEvent_t e)
+{
+    if (!tls_sleepy_waits) return hipEventSynchronize(e);
+    // a few yields first (short waits stay short), then sleeps growing to 0.2 ms
+    for (int i = 0;; ++i) {
+        const hipError_t q = hipEventQuery(e);
+        if (q != hipErrorNotReady) return q;
+        if (i < 32)
+            std::this_thread::yield();
+        else
+            std::this_thread::sleep_for(std::chrono::microseconds(std::min(200, 10 * (i - 31))));
+    }
+}
+
+hipError_t stream_wait(hipStream_t s)
+{
+    if (!tls_sleepy_waits) return hipStreamSynchronize(s);
+    // one event per thread, destroyed when the thread ends (slot threads end
+    // in scmc_server_stop, before the runtime shuts down)
+    struct Holder {
+        hipEvent_t ev = nullptr;
+        ~Holder() { if (ev) (void)hipEventDestroy(ev); }
+    };
+    static thread_local Holder h;
+    hipError_t e = hipSuccess;
+    if (!h.ev && (e = hipEventCreateWithFlags(&h.ev, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventRecord(h.ev, s)) != hipSuccess) return e;
+    return event_wait(h.ev);
 }
 
 int ensure_device(int *dev_out)
@@ -354,7 +446,7 @@ int ProblemState::setup(const scmc_problem *pr, hipStream_t s)
         P.amp0.iso = P.w0p_iso;
     }
     // hold host copies alive until the async copies complete
-    return hip_ok(hipStreamSynchronize(s), "setup sync", &rc) ? SCMC_OK : rc;
+    return hip_ok(stream_wait(s), "setup sync", &rc) ? SCMC_OK : rc;
 }
 
 int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevBuf &icdf, DevPhase &out, hipStream_t s)
@@ -381,7 +473,7 @@ int ProblemState::upload_phase(const scmc_phase &ph, int type, DevBuf &amp, DevB
                 "copy sampleIcdf", &rc))
         return rc;
     launch_cos_table((double *)icdf.p, ph.n_icdf, s);
-    if (!hip_ok(hipGetLastError(), "cos table", &rc) || !hip_ok(hipStreamSynchronize(s), "table sync", &rc))
+    if (!hip_ok(hipGetLastError(), "cos table", &rc) || !hip_ok(stream_wait(s), "table sync", &rc))
         return rc;
     out.n_amp = ph.n_amp;
     out.amp = (const double2 *)amp.p;
@@ -417,7 +509,7 @@ int TraceChunk::run(const DevProblem &P, uint64_t p0, int64_t n, hipStream_t s, 
     if (!hip_ok(hipMemcpyAsync(&total, (int64_t *)offsets.p + n, sizeof total, hipMemcpyDeviceToHost, s), "copy total",
                 &rc))
         return rc;
-    if (!hip_ok(hipStreamSynchronize(s), "trace sync", &rc)) return rc;
+    if (!hip_ok(stream_wait(s), "trace sync", &rc)) return rc;
     nvert = total;
     if (!verts.ensure(sizeof(VertexRec) * (size_t)std::max<int64_t>(total, 1), &rc) ||
         !vpath.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), &rc))
@@ -533,7 +625,7 @@ int CovEngine::ensure_acc(DevBuf &acc, int &ks_have, int ks, hipStream_t s)
     launch_zero(nb.p, one * ks, s);
     if (acc.p && ks_have > 0)
         if (!hip_ok(hipMemcpyAsync(nb.p, acc.p, one * ks_have, hipMemcpyDeviceToDevice, s), "grow acc", &rc)) return rc;
-    if (!hip_ok(hipStreamSynchronize(s), "acc sync", &rc)) return rc;
+    if (!hip_ok(stream_wait(s), "acc sync", &rc)) return rc;
     std::swap(acc.p, nb.p);
     std::swap(acc.bytes, nb.bytes);
     std::swap(acc.pool, nb.pool);   // each block goes back where it came from
@@ -653,7 +745,7 @@ int UpdateBatcher::init()
 
 UpdateBatcher::~UpdateBatcher()
 {
-    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream) (void)stream_wait(stream);
     for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
     for (hipEvent_t e : spare) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -760,7 +852,7 @@ int CovEngine::consume(hipStream_t sc)
         // order) are complete, so the loop never waits on a kernel; the
         // previous chunk's jobs finished above
         (void)hipEventRecord(S.ev[6], sc);
-        if (!hip_ok(hipEventSynchronize(S.ev[6]), "update inputs", &rc)) return rc;
+        if (!hip_ok(event_wait(S.ev[6]), "update inputs", &rc)) return rc;
         S.loop_t0 = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
         S.looped = true;
         if ((rc = loop->publish(hm, &S.tk[0])) || (rc = loop->publish(hs, &S.tk[1]))) return rc;
@@ -821,7 +913,7 @@ int CovEngine::harvest(CovSlot &S)
     const bool looped = S.looped;
     S.looped = false;
     if ((rc = wait_loop(S))) return rc;
-    if (!hip_ok(hipEventSynchronize(S.ev[5]), "covariance sync", &rc)) return rc;
+    if (!hip_ok(event_wait(S.ev[5]), "covariance sync", &rc)) return rc;
     float tn = 0, tx = 0, th = 0, ts = 0;
     (void)hipEventElapsedTime(&tn, S.ev[0], S.ev[1]);
     (void)hipEventElapsedTime(&tx, S.ev[1], S.ev[2]);
@@ -863,7 +955,7 @@ int CovEngine::read_nonfinite(int64_t *out, hipStream_t s)
     unsigned long long h = 0;
     if (nonfinite.p &&
         (!hip_ok(hipMemcpyAsync(&h, nonfinite.p, sizeof h, hipMemcpyDeviceToHost, s), "copy nonfinite", &rc) ||
-         !hip_ok(hipStreamSynchronize(s), "nonfinite sync", &rc)))
+         !hip_ok(stream_wait(s), "nonfinite sync", &rc)))
         return rc;
     *out = (int64_t)h;
     return SCMC_OK;
@@ -931,7 +1023,7 @@ int CovEngine::finalize(const ProblemState &ps, const scmc_problem *pr, int64_t 
             if (!hip_ok(hipMemcpyAsync(d.re, o, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy out", &r2))
                 return r2;
         }
-        return hip_ok(hipStreamSynchronize(s), "finalize sync", &r2) ? SCMC_OK : r2;
+        return hip_ok(stream_wait(s), "finalize sync", &r2) ? SCMC_OK : r2;
     };
     // MCcov.m:293-299: Mm/2 with CBS, then /maxItr*V*sigt; Ms /maxItr*V*sigt
     if ((rc = emit(acc_s, ks_s, scale, far ? (const double2 *)af.p : nullptr, Dst{ms_re, ms_im}))) return rc;
@@ -945,7 +1037,7 @@ int CovEngine::finalize_mean(int64_t max_itr, double *re, double *im, hipStream_
     if (!re) return SCMC_OK;
     std::vector<double> h((size_t)n * 2);
     if (!hip_ok(hipMemcpyAsync(h.data(), mean.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s), "copy mean", &rc) ||
-        !hip_ok(hipStreamSynchronize(s), "mean sync", &rc))
+        !hip_ok(stream_wait(s), "mean sync", &rc))
         return rc;
     const double sc = 1.0 / (double)max_itr;   // MCCovHetro.m:276 with cs = 1
     for (int i = 0; i < n; ++i) {
@@ -1048,8 +1140,8 @@ int Session::init(const scmc_problem *pr, int kind_, int precision, const double
 
 Session::~Session()
 {
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (stream) (void)stream_wait(stream);
+    if (stream2) (void)stream_wait(stream2);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -1069,7 +1161,7 @@ int Session::run(int64_t first, int64_t n, bool settle)
         // the trace overwrites the path records the previous chunk's factors read
         if (kind == 0 && ce.nee_done && !hip_ok(hipStreamWaitEvent(ts, ce.nee_done, 0), "wait factors", &rc)) return rc;
         if ((rc = tc.run(ps.dp, (uint64_t)p0, np, ts, ps.het ? &ps.hm : nullptr))) return rc;
-        if (!hip_ok(hipStreamSynchronize(ts), "trace", &rc)) return rc;
+        if (!hip_ok(stream_wait(ts), "trace", &rc)) return rc;
         st.ms_trace += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         if (kind == 0) {
             // the tracer (a narrow kernel: one thread per path) runs on stream2

@@ -40,7 +40,7 @@ int FieldEngine::init(const ProblemState &ps, hipStream_t s, const double *wl_re
         }
         if (!wl.ensure(sizeof(double2) * (size_t)nl, &rc)) return rc;
         if (!hip_ok(hipMemcpyAsync(wl.p, h.data(), sizeof(double2) * (size_t)nl, hipMemcpyHostToDevice, s), "wl", &rc) ||
-            !hip_ok(hipStreamSynchronize(s), "wl sync", &rc))
+            !hip_ok(stream_wait(s), "wl sync", &rc))
             return rc;
     }
     return SCMC_OK;
@@ -59,7 +59,7 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
                      (const VertexRec *)tc.verts.p, (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p,
                      (double2 *)work.p, (double2 *)acc.p, mixed, wave ? (const double2 *)wl.p : nullptr,
                      (unsigned long long *)nonfinite.p, s);
-        if (!hip_ok(hipGetLastError(), "field launch", &rc) || !hip_ok(hipStreamSynchronize(s), "field sync", &rc))
+        if (!hip_ok(hipGetLastError(), "field launch", &rc) || !hip_ok(stream_wait(s), "field sync", &rc))
             return rc;
         *ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return SCMC_OK;
@@ -74,7 +74,7 @@ int FieldEngine::chunk(const ProblemState &ps, TraceChunk &tc, bool mixed, hipSt
                  wave ? (const double2 *)wl.p : nullptr, (unsigned long long *)nonfinite.p, s);
     (void)hipEventRecord(e1, s);
     if (!hip_ok(hipGetLastError(), "field launch", &rc)) return rc;
-    if (!hip_ok(hipEventSynchronize(e1), "field sync", &rc)) return rc;
+    if (!hip_ok(event_wait(e1), "field sync", &rc)) return rc;
     float t = 0;
     (void)hipEventElapsedTime(&t, e0, e1);
     *ms_accum += t;
@@ -89,7 +89,7 @@ int FieldEngine::read_nonfinite(int64_t *out, hipStream_t s)
     unsigned long long h = 0;
     if (nonfinite.p &&
         (!hip_ok(hipMemcpyAsync(&h, nonfinite.p, sizeof h, hipMemcpyDeviceToHost, s), "copy nonfinite", &rc) ||
-         !hip_ok(hipStreamSynchronize(s), "nonfinite sync", &rc)))
+         !hip_ok(stream_wait(s), "nonfinite sync", &rc)))
         return rc;
     *out = (int64_t)h;
     return SCMC_OK;
@@ -101,7 +101,7 @@ int FieldEngine::finalize(const scmc_problem *pr, int64_t max_itr, double *u_re,
     const size_t ne = (size_t)nv * ncol;
     std::vector<double> h(ne * 2);
     if (!hip_ok(hipMemcpyAsync(h.data(), acc.p, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy field", &rc) ||
-        !hip_ok(hipStreamSynchronize(s), "field sync", &rc))
+        !hip_ok(stream_wait(s), "field sync", &rc))
         return rc;
     // MCfield.m:253-254 (MCfieldOnWave.m:265-267); a voxel medium carries the
     // volume in px and scales by sqrt(1/maxItr) (MCsampleHetro.m:222)

@@ -151,7 +151,7 @@ int ProblemState::setup_medium(const scmc_problem *pr, const scmc_medium *m, hip
     null_basis(dp.meanl, dim, M.u);
     M.cap = 4 * (M.nb[0] + M.nb[1] + (dim == 3 ? M.nb[2] : 0)) + 16;
     het = true;
-    return hip_ok(hipStreamSynchronize(s), "medium sync", &rc) ? SCMC_OK : rc;
+    return hip_ok(stream_wait(s), "medium sync", &rc) ? SCMC_OK : rc;
 }
 
 } // namespace scmc

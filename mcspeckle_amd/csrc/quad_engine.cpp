@@ -205,7 +205,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                             "wl", &rc) ||
                     !hip_ok(hipMemcpyAsync(dwlk.p, hk.data(), sizeof(WaveLight) * hk.size(), hipMemcpyHostToDevice, s),
                             "wl", &rc) ||
-                    !hip_ok(hipStreamSynchronize(s), "wl sync", &rc))
+                    !hip_ok(stream_wait(s), "wl sync", &rc))
                     break;
                 g.wave = 1;
                 g.vsign = vsign;
@@ -239,7 +239,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                 if (rc) break;
                 unsigned long long hv = 0;
                 if (!hip_ok(hipMemcpyAsync(&hv, dcount.p, sizeof hv, hipMemcpyDeviceToHost, s), "vertex count", &rc) ||
-                    !hip_ok(hipStreamSynchronize(s), "quad scatter", &rc))
+                    !hip_ok(stream_wait(s), "quad scatter", &rc))
                     break;
                 st.ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
                 st.vertices = (int64_t)hv;
@@ -256,7 +256,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
                     if ((rc = tc.run(ps.dp, (uint64_t)p0, np, s))) break;
                     launch_quad_scatter(ps.dp, g, tc.nvert, (const PathRec *)tc.paths.p, (const VertexRec *)tc.verts.p,
                                         (const int32_t *)tc.vpath.p, (const int64_t *)tc.offsets.p, grid.p, fp32, s);
-                    if (!hip_ok(hipStreamSynchronize(s), "quad scatter", &rc)) break;
+                    if (!hip_ok(stream_wait(s), "quad scatter", &rc)) break;
                     st.ms_accum += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
                     st.paths += np;
                     st.vertices += tc.nvert;
@@ -325,7 +325,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             const double *hu = ws.host_u;
             if (!hip_ok(hipMemcpyAsync(ws.host_u, u.p, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, s), "copy u",
                         &rc) ||
-                !hip_ok(hipStreamSynchronize(s), "quad sync", &rc))
+                !hip_ok(stream_wait(s), "quad sync", &rc))
                 break;
             st.ms_nee += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
             if (u_im) {
@@ -338,7 +338,7 @@ static int fourier_quad_run(const scmc_problem *pr_in, const double *wl_re, cons
             }
         } while (0);
         // an error leaves queued work behind: finish it before the next call
-        if (rc) (void)hipStreamSynchronize(s);
+        if (rc) (void)stream_wait(s);
     }
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;

@@ -156,14 +156,14 @@ int ResidentLoop::recycle()
     for (unsigned j = 0; j < njobs; ++j)
         if ((rc = wait_index(j, gen))) return rc;
     __atomic_store_n(&H->stop, 1u, __ATOMIC_SEQ_CST);
-    if (!hip_ok(hipStreamSynchronize(stream), "resident loop stop", &rc)) return rc;
+    if (!hip_ok(stream_wait(stream), "resident loop stop", &rc)) return rc;
     // helper instances run on the requests' own streams: one still queued
     // there would start after the reset below and take an item of an old
     // job (its job index is reused by the next generation), so wait for the
     // whole device; all their jobs are done, so they end at once
     if (!hip_ok(hipDeviceSynchronize(), "resident loop helpers", &rc)) return rc;
     if (!hip_ok(hipMemsetAsync(D, 0, sizeof(LoopDev), stream), "loop counters", &rc) ||
-        !hip_ok(hipStreamSynchronize(stream), "loop counters", &rc))
+        !hip_ok(stream_wait(stream), "loop counters", &rc))
         return rc;
     for (unsigned j = 0; j < njobs; ++j) H->done[j] = 0;
     H->closing[0] = H->closing[1] = 0;
@@ -185,7 +185,7 @@ int ResidentLoop::stop()
         if ((rc = wait_index(j, gen))) break;
     __atomic_store_n(&H->stop, 1u, __ATOMIC_SEQ_CST);
     int r2 = SCMC_OK;
-    if (!hip_ok(hipStreamSynchronize(stream), "resident loop stop", &r2) && !rc) rc = r2;
+    if (!hip_ok(stream_wait(stream), "resident loop stop", &r2) && !rc) rc = r2;
     __atomic_store_n(&H->stop, 0u, __ATOMIC_SEQ_CST);
     inst = -1;
     // the counters stay: a later publish continues this generation

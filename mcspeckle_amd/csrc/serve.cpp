@@ -65,6 +65,7 @@ struct Server {
     std::deque<Request *> pending;
     std::map<int64_t, std::unique_ptr<Request>> all;
     std::vector<std::thread> slots;
+    std::vector<int64_t> held;   // device bytes each slot's session keeps (at its last request's end)
     int64_t next_id = 1;
     bool stop = false;
     bool shared_updates = false;   // mixed updates of concurrent small requests in shared launches
@@ -170,8 +171,8 @@ static void run_request(Server *S, Request *r, Session &sess)
     // resident-loop jobs included)
     if (rc) {
         for (auto &sl : sess.ce.slot) (void)sess.ce.wait_loop(sl);
-        if (sess.stream2) (void)hipStreamSynchronize(sess.stream2);
-        if (sess.stream) (void)hipStreamSynchronize(sess.stream);
+        if (sess.stream2) (void)stream_wait(sess.stream2);
+        if (sess.stream) (void)stream_wait(sess.stream);
     }
     sess.pool.settle();
     tls_block_pool = nullptr;
@@ -179,11 +180,17 @@ static void run_request(Server *S, Request *r, Session &sess)
 
 // A slot keeps one session between requests: its streams and device buffers
 // are reused (they only grow), so admitting a request allocates nothing in
-// the steady state and finishing one frees nothing -- hipFree synchronises
-// the whole device and would stall the other slots.  A session that served a
-// request larger than a 1/(2 max_active) share of the budget is released.
-static void slot_loop(Server *S)
+// the steady state and finishing one frees nothing.  Releasing a session
+// costs the whole server: hipHostFree and hipFree wait for the whole device,
+// and with many slots that is seconds of queued work, during which the
+// runtime holds every other slot's launches and stream creations.  So a
+// session is released only when its request failed or when the sessions of
+// all slots together keep more than half the budget (the bytes they hold,
+// not the footprint estimates, which assume full chunks): admitted requests
+// are charged against the budget, so device memory stays within 1.5 budgets.
+static void slot_loop(Server *S, int slot)
 {
+    tls_sleepy_waits = true;   // wait without keeping a core spinning
     std::unique_ptr<Session> sess;
     for (;;) {
         Request *r = nullptr;
@@ -211,7 +218,16 @@ static void slot_loop(Server *S)
         sess->ce.batcher = S->shared_updates ? &S->batcher : nullptr;
         sess->ce.loop = S->loop.get();
         run_request(S, r, *sess);
-        if (r->rc || r->footprint * 2 * std::max(1, S->cfg.max_active) > S->budget) sess.reset();
+        bool release = r->rc != 0;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->held[(size_t)slot] = (int64_t)sess->pool.held;
+            int64_t kept = 0;
+            for (int64_t h : S->held) kept += h;
+            if (2 * kept > S->budget) release = true;
+            if (release) S->held[(size_t)slot] = 0;
+        }
+        if (release) sess.reset();
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->used -= r->footprint;
@@ -290,7 +306,8 @@ int scmc_server_start(const scmc_server_config *cfg)
     }
     Server *raw = S.get();
     reaper_start();
-    for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw);
+    S->held.assign((size_t)S->cfg.max_active, 0);
+    for (int i = 0; i < S->cfg.max_active; ++i) S->slots.emplace_back(slot_loop, raw, i);
     g_srv = std::move(S);
     return SCMC_OK;
 }
