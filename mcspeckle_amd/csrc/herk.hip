@@ -591,20 +591,33 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
     const bool rvalid = gi < n && !(diag && T == 1);
     const int vv = rvalid ? gi % nv : 0, ll = rvalid ? gi / nv : 0;
 
-    auto stage_fill = [&](int st, int buf) {
+    // The raw factors of a stage are loaded into registers one stage ahead
+    // (stage_load) and turned into rows in LDS only after the current
+    // stage's MFMAs (stage_store), so the loads' latency hides behind them.
+    double2 fa[4], fb[4], fa2[4], fb2[4];
+    auto stage_load = [&](int st) {
         const int64_t k0 = kb + (int64_t)st * VPS;
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
+            const int64_t kk = k0 + hp * 4 + q4;
+            const bool ok = rvalid && kk < ke;
+            fa[q4] = ok ? A1[kk * nv + vv] : make_double2(0.0, 0.0);
+            fb[q4] = ok ? B1[kk * nl + ll] : make_double2(0.0, 0.0);
+            if (CBS) {
+                fa2[q4] = ok ? A2[kk * nv + vv] : make_double2(0.0, 0.0);
+                fb2[q4] = ok ? B2[kk * nl + ll] : make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto stage_store = [&](int buf) {
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
             const int q = hp * 4 + q4;
-            const int64_t kk = k0 + q;
-            double2 u = make_double2(0.0, 0.0);
-            if (rvalid && kk < ke) {
-                u = cmul(A1[kk * nv + vv], B1[kk * nl + ll]);
-                if (CBS) {
-                    double2 u2 = cmul(A2[kk * nv + vv], B2[kk * nl + ll]);
-                    u.x += u2.x;
-                    u.y += u2.y;
-                }
+            double2 u = cmul(fa[q4], fb[q4]);
+            if (CBS) {
+                const double2 u2 = cmul(fa2[q4], fb2[q4]);
+                u.x += u2.x;
+                u.y += u2.y;
             }
             if (!(diag && T == 1)) {
                 double *d = &lds[lidx(buf, T, q >> 1, r, (q & 1) * 2)];
@@ -630,11 +643,14 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
     const int btile = diag ? 0 : 1;
     const int nstage = (int)((ke - kb + VPS - 1) / VPS);
 
-    if (nstage > 0) stage_fill(0, 0);
+    if (nstage > 0) {
+        stage_load(0);
+        stage_store(0);
+    }
+    if (nstage > 1) stage_load(1);
     __syncthreads();
     for (int st = 0; st < nstage; ++st) {
         const int buf = st & 1;
-        if (st + 1 < nstage) stage_fill(st + 1, buf ^ 1);
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
             double a[2], ar[2], bb[2];
@@ -655,6 +671,10 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
                     accI[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rb], bb[cb], accI[rb][cb], 0, 0, 0);
                 }
         }
+        // buffer buf ^ 1 was last read in stage st - 1, which every wave
+        // finished before the previous barrier
+        if (st + 1 < nstage) stage_store(buf ^ 1);
+        if (st + 2 < nstage) stage_load(st + 2);
         __syncthreads();
     }
 
