@@ -725,6 +725,14 @@ int CovEngine::produce(const ProblemState &ps, TraceChunk &tc, hipStream_t sp)
         hs.zbuf = S.zbuf_s.p;
         launch_expand_mixed(hm, sp);
         launch_expand_mixed(hs, sp);
+    } else {
+        if (!S.zbuf_m.ensure(fp64_zbuf_bytes(std::max<int64_t>(k2, 1), n), &rc) ||
+            !S.zbuf_s.ensure(fp64_zbuf_bytes(std::max<int64_t>(np, 1), n), &rc))
+            return rc;
+        hm.zbuf = S.zbuf_m.p;
+        hs.zbuf = S.zbuf_s.p;
+        launch_expand_fp64(hm, sp);
+        launch_expand_fp64(hs, sp);
     }
     (void)hipEventRecord(S.ev[2], sp);
     S.np = np;
@@ -1055,7 +1063,8 @@ int CovEngine::finalize_mean(int64_t max_itr, double *re, double *im, hipStream_
 int64_t chunk_paths(int n, int64_t remaining, double mean_len, bool mixed)
 {
     // vertices per chunk: large enough to fill the GPU, bounded by the
-    // expanded-row buffer (8 B per row and vertex, ~4 GB) and factor memory
+    // expanded-row buffer (8 B per row and vertex mixed, 16 B FP64: ~4 and
+    // ~2.5 GB) and factor memory
     double cap = mixed ? 4.0e9 / (8.0 * n) : 2.5e9 / (16.0 * n);
     int64_t target_vertices = (int64_t)std::max(65536.0, std::min(4.0e6, cap));
     int64_t p = (int64_t)(target_vertices / std::max(mean_len, 1.0));

@@ -557,75 +557,92 @@ __global__ __launch_bounds__(256, 2) void herk_mixed_loop_kernel(LoopCtl *H, Loo
 }
 
 // ------------------------------------------------------------- fp64
+// The FP64 update in two steps, as the mixed one: the rows u_i = a_v b_l
+// (+ a2_v b2_l with CBS) of a chunk are expanded once into HBM, then the
+// Hermitian update streams them through LDS into v_mfma_f64_16x16x4f64.
+// Expanded layout: K-step t (vertices 2t, 2t+1), row i: 32 bytes
+// (Re u_i(2t), Im u_i(2t), Re u_i(2t+1), Im u_i(2t+1)); rows padded with
+// zeros to whole 64-row tiles, K-steps to whole stages of 8 vertices.
 namespace f64 {
-constexpr int TM = kTileF64;   // 64x64 tile per workgroup, 4 waves of 32x32
-constexpr int KS = 2;          // vertices per MFMA step (K = 4 doubles)
-constexpr int NKS = 4;         // steps per stage
-constexpr int VPS = KS * NKS;  // 8 vertices per stage
-// LDS: [stage 2][tile 2][NKS][TM rows][4 doubles]
-__device__ __forceinline__ int lidx(int stage, int tile, int ks, int row, int e)
-{
-    return (((stage * 2 + tile) * NKS + ks) * TM + row) * 4 + e;
-}
+constexpr int TM = kTileF64;    // 64x64 tile per workgroup, 4 waves of 32x32
+constexpr int SKS = 4;          // K-steps per stage
+constexpr int VPS = 2 * SKS;    // 8 vertices per stage
+// LDS stage: [tile 2 (rows, columns)][K-step SKS][TM rows][32 B]
+constexpr int STAGEB = 2 * SKS * TM * 32;
+// Two LDS stages of 16 KB: a stage's loads are issued as the MFMAs of the
+// stage before start and have those MFMAs' time to land.  At 32 KB of LDS
+// five workgroups fit on a CU, and with at least four waves per SIMD
+// required the compiler keeps the accumulators in VGPRs (91 in all), so
+// five waves per SIMD take turns on the matrix core.  Measured at N = 4096
+// (20 000 paths): 122 TFLOP/s algorithmic, 83 % MFMA-busy; three stages
+// (48 KB, three waves per SIMD) 101, four 86.
+constexpr int NBUF = 2;
+__host__ __device__ __forceinline__ int pad_rows(int n) { return (n + TM - 1) / TM * TM; }
+__host__ __device__ __forceinline__ int64_t nstages(int64_t k) { return (k + VPS - 1) / VPS; }
 } // namespace f64
 
+// One workgroup per stage and 256 rows: each thread forms its row's eight
+// products (the factors of a stage are a few KB and stay in L2) and writes
+// them as four 32-byte K-step entries; consecutive rows are consecutive in
+// every K-step, so the stores of a wave are contiguous.
 template <bool CBS>
-__global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, int64_t k, const double2 *__restrict__ A1,
-                                                        const double2 *__restrict__ B1, const double2 *__restrict__ A2,
-                                                        const double2 *__restrict__ B2, double2 *__restrict__ acc,
-                                                        int ksplit)
+__global__ __launch_bounds__(256) void expand_fp64_kernel(int nv, int nl, int n, int64_t k,
+                                                          const double2 *__restrict__ A1,
+                                                          const double2 *__restrict__ B1,
+                                                          const double2 *__restrict__ A2,
+                                                          const double2 *__restrict__ B2, double2 *__restrict__ Z)
 {
     using namespace f64;
-    __shared__ double lds[2 * 2 * NKS * TM * 4];
+    const int npad = pad_rows(n);
+    const int i = blockIdx.y * 256 + threadIdx.x;
+    if (i >= npad) return;
+    const int64_t st = blockIdx.x;
+    const bool row = i < n;
+    const int v = row ? i % nv : 0, l = row ? i / nv : 0;
+#pragma unroll
+    for (int t = 0; t < SKS; ++t) {
+        double2 u[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t kk = st * VPS + 2 * t + h;
+            u[h] = make_double2(0.0, 0.0);
+            if (row && kk < k) {
+                u[h] = cmul(A1[kk * nv + v], B1[kk * nl + l]);
+                if (CBS) {
+                    const double2 w = cmul(A2[kk * nv + v], B2[kk * nl + l]);
+                    u[h].x += w.x;
+                    u[h].y += w.y;
+                }
+            }
+        }
+        double2 *z = Z + ((st * SKS + t) * (int64_t)npad + i) * 2;
+        z[0] = u[0];
+        z[1] = u[1];
+    }
+}
+
+// The Hermitian update over the expanded rows.  Work item = (k-split s,
+// lower-triangle tile); s takes a contiguous run of stages.  Each stage's
+// rows go from HBM straight into LDS (global_load_lds_dwordx4: wave w moves
+// K-steps 2(w%2), 2(w%2)+1 of tile w/2 (rows, then columns), two 1 KB
+// chunks of 32 rows per K-step, lane order = LDS layout), NBUF - 1 stages
+// ahead of the matrix cores, with one workgroup barrier per stage.  Each
+// wave computes a 32x32 block: Re C += Z Z^T and Im C += Z' Z^T with
+// Z' = (Im, -Re) formed from the partner K element.
+template <int NBUF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void herk_fp64_kernel(const double *__restrict__ Z, int n, int64_t nst,
+                                                        double2 *__restrict__ acc, int ksplit)
+{
+    using namespace f64;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int item = blockIdx.x;
     const int s = item % ksplit;
     int ti, tj;
     tri_decode(item / ksplit, ti, tj);
-    const bool diag = (ti == tj);
-    const int64_t kb = k * s / ksplit, ke = k * (s + 1) / ksplit;
-
-    // staging: 128 row slots (64 row-tile + 64 column-tile), 2 threads each
-    const int T = tid >> 7, r = (tid & 127) >> 1, hp = tid & 1;
-    const int gi = (T ? tj : ti) * TM + r;
-    const bool rvalid = gi < n && !(diag && T == 1);
-    const int vv = rvalid ? gi % nv : 0, ll = rvalid ? gi / nv : 0;
-
-    // The raw factors of a stage are loaded into registers one stage ahead
-    // (stage_load) and turned into rows in LDS only after the current
-    // stage's MFMAs (stage_store), so the loads' latency hides behind them.
-    double2 fa[4], fb[4], fa2[4], fb2[4];
-    auto stage_load = [&](int st) {
-        const int64_t k0 = kb + (int64_t)st * VPS;
-#pragma unroll
-        for (int q4 = 0; q4 < 4; ++q4) {
-            const int64_t kk = k0 + hp * 4 + q4;
-            const bool ok = rvalid && kk < ke;
-            fa[q4] = ok ? A1[kk * nv + vv] : make_double2(0.0, 0.0);
-            fb[q4] = ok ? B1[kk * nl + ll] : make_double2(0.0, 0.0);
-            if (CBS) {
-                fa2[q4] = ok ? A2[kk * nv + vv] : make_double2(0.0, 0.0);
-                fb2[q4] = ok ? B2[kk * nl + ll] : make_double2(0.0, 0.0);
-            }
-        }
-    };
-    auto stage_store = [&](int buf) {
-#pragma unroll
-        for (int q4 = 0; q4 < 4; ++q4) {
-            const int q = hp * 4 + q4;
-            double2 u = cmul(fa[q4], fb[q4]);
-            if (CBS) {
-                const double2 u2 = cmul(fa2[q4], fb2[q4]);
-                u.x += u2.x;
-                u.y += u2.y;
-            }
-            if (!(diag && T == 1)) {
-                double *d = &lds[lidx(buf, T, q >> 1, r, (q & 1) * 2)];
-                d[0] = u.x;
-                d[1] = u.y;
-            }
-        }
-    };
+    const int64_t sb = nst * s / ksplit, se = nst * (s + 1) / ksplit;
+    const int nstage = (int)(se - sb);
+    const int npad = pad_rows(n);
 
     f64x4 accR[2][2], accI[2][2];
 #pragma unroll
@@ -638,31 +655,65 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
                 accI[a][c][e] = 0.0;
             }
 
+    // loading role
+    const int T = wave >> 1, KP = (wave & 1) * 2;
+    const double *src = Z + ((size_t)KP * npad + (size_t)((T ? tj : ti) * TM) + (lane >> 1)) * 4 + (lane & 1) * 2;
+    const size_t kstride = (size_t)npad * 4;   // doubles per K-step
+    unsigned char *ldst = lds + T * (SKS * TM * 32) + KP * (TM * 32);
+    typedef __attribute__((address_space(3))) void *lds_ptr;
+#define SCMC_ISSUE64(stage, buf)                                                                              \
+    do {                                                                                                      \
+        const double *g = src + (size_t)(stage) * SKS * kstride;                                              \
+        unsigned char *d = ldst + (buf) * STAGEB;                                                             \
+        __builtin_amdgcn_global_load_lds((const void *)g, (lds_ptr)d, 16, 0, 0);                              \
+        __builtin_amdgcn_global_load_lds((const void *)(g + 32 * 4), (lds_ptr)(d + 1024), 16, 0, 0);          \
+        __builtin_amdgcn_global_load_lds((const void *)(g + kstride), (lds_ptr)(d + TM * 32), 16, 0, 0);      \
+        __builtin_amdgcn_global_load_lds((const void *)(g + kstride + 32 * 4), (lds_ptr)(d + TM * 32 + 1024), \
+                                         16, 0, 0);                                                           \
+    } while (0)
+    // s_barrier orders execution only: the empty asm statements with a
+    // memory clobber on both sides keep every LDS read of a stage between the
+    // barriers that publish and retire it, without waiting for the loads of
+    // later stages (no s_waitcnt is added).
+#define SCMC_BARRIER64()                  \
+    do {                                  \
+        asm volatile("" ::: "memory");    \
+        __builtin_amdgcn_s_barrier();     \
+        asm volatile("" ::: "memory");    \
+    } while (0)
+    // vmcnt(4 (NBUF - 2)): every stage but the newest NBUF - 2 has landed
+    constexpr int kWait = 0xF70 | (4 * (NBUF - 2));
+    static_assert(4 * (NBUF - 2) <= 15, "vmcnt field");
+
     const int wr = wave >> 1, wc = wave & 1;
     const int kk = lane >> 4, l16 = lane & 15;
-    const int btile = diag ? 0 : 1;
-    const int nstage = (int)((ke - kb + VPS - 1) / VPS);
-
+    const int last = nstage - 1;
     if (nstage > 0) {
-        stage_load(0);
-        stage_store(0);
-    }
-    if (nstage > 1) stage_load(1);
-    __syncthreads();
-    for (int st = 0; st < nstage; ++st) {
-        const int buf = st & 1;
+        // every load stays inside the item: stages past the last reload the
+        // last one into a buffer that is never read again
 #pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
+        for (int p = 0; p < NBUF - 1; ++p) SCMC_ISSUE64(sb + (p < last ? p : last), p);
+        __builtin_amdgcn_s_waitcnt(kWait);
+        SCMC_BARRIER64();
+    }
+    for (int st = 0; st < nstage; ++st) {
+        const int nx = st + NBUF - 1;
+        // into the buffer of stage st - 1, which every wave finished before
+        // the last barrier
+        SCMC_ISSUE64(sb + (nx < last ? nx : last), nx % NBUF);
+        const double *bp = (const double *)(lds + (st % NBUF) * STAGEB);
+#pragma unroll
+        for (int ks = 0; ks < SKS; ++ks) {
             double a[2], ar[2], bb[2];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
-                const int row = wr * 32 + rb * 16 + l16;
-                a[rb] = lds[lidx(buf, 0, ks, row, kk)];
-                const double partner = lds[lidx(buf, 0, ks, row, kk ^ 1)];
+                const double *ra = bp + ((size_t)ks * TM + wr * 32 + rb * 16 + l16) * 4;
+                a[rb] = ra[kk];
+                const double partner = ra[kk ^ 1];
                 ar[rb] = (kk & 1) ? -partner : partner;
             }
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb) bb[cb] = lds[lidx(buf, btile, ks, wc * 32 + cb * 16 + l16, kk)];
+            for (int cb = 0; cb < 2; ++cb) bb[cb] = bp[((size_t)(SKS + ks) * TM + wc * 32 + cb * 16 + l16) * 4 + kk];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -671,12 +722,14 @@ __global__ __launch_bounds__(256) void herk_fp64_kernel(int nv, int nl, int n, i
                     accI[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rb], bb[cb], accI[rb][cb], 0, 0, 0);
                 }
         }
-        // buffer buf ^ 1 was last read in stage st - 1, which every wave
-        // finished before the previous barrier
-        if (st + 1 < nstage) stage_store(buf ^ 1);
-        if (st + 2 < nstage) stage_load(st + 2);
-        __syncthreads();
+        __builtin_amdgcn_s_waitcnt(kWait);   // stage st + 1 landed
+        SCMC_BARRIER64();
     }
+#undef SCMC_BARRIER64
+#undef SCMC_ISSUE64
+    // the clamped loads past the last stage may still be writing LDS: drain
+    // them before the workgroup can end and its LDS be handed to another
+    __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
 
     double2 *C = acc + (size_t)s * n * n;
 #pragma unroll
@@ -821,18 +874,28 @@ std::vector<int> blocked_tile_order(int nt)
 
 size_t mixed_zbuf_bytes(int64_t k, int n) { return (size_t)mixed::pad_ksteps(k) * (size_t)mixed::pad_rows(n) * 64; }
 
+size_t fp64_zbuf_bytes(int64_t k, int n) { return (size_t)f64::nstages(k) * (size_t)f64::pad_rows(n) * 128; }
+
+void launch_expand_fp64(const HerkProblem &hp, hipStream_t s)
+{
+    if (hp.k <= 0) return;
+    const dim3 g((unsigned)f64::nstages(hp.k), (unsigned)((f64::pad_rows(hp.n) + 255) / 256));
+    if (hp.A2)
+        hipLaunchKernelGGL(expand_fp64_kernel<true>, g, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const double2 *)hp.A1, (const double2 *)hp.B1, (const double2 *)hp.A2,
+                           (const double2 *)hp.B2, (double2 *)hp.zbuf);
+    else
+        hipLaunchKernelGGL(expand_fp64_kernel<false>, g, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
+                           (const double2 *)hp.A1, (const double2 *)hp.B1, nullptr, nullptr, (double2 *)hp.zbuf);
+}
+
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
     const int nt = ntiles(hp.n, kTileF64);
     const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
-    if (hp.A2)
-        hipLaunchKernelGGL(herk_fp64_kernel<true>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
-                           (const double2 *)hp.A1, (const double2 *)hp.B1, (const double2 *)hp.A2,
-                           (const double2 *)hp.B2, hp.acc, hp.ksplit);
-    else
-        hipLaunchKernelGGL(herk_fp64_kernel<false>, dim3(grid), dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
-                           (const double2 *)hp.A1, (const double2 *)hp.B1, nullptr, nullptr, hp.acc, hp.ksplit);
+    hipLaunchKernelGGL(herk_fp64_kernel<f64::NBUF>, dim3(grid), dim3(256), 0, s, (const double *)hp.zbuf, hp.n,
+                       f64::nstages(hp.k), hp.acc, hp.ksplit);
 }
 
 // ------------------------------------------------------------ finaliser

@@ -94,7 +94,7 @@ struct HerkProblem {
     double2 *acc;          // [ksplit][n][n] row-major FP64 accumulators
     const unsigned *bound; // mixed: float bits of max |u| (may be null)
     int ksplit;
-    void *zbuf = nullptr;          // mixed: expanded FP16 hi/lo rows (mixed_zbuf_bytes)
+    void *zbuf = nullptr;          // expanded rows: FP16 hi/lo (mixed_zbuf_bytes) or FP64 (fp64_zbuf_bytes)
     const void *order = nullptr;   // mixed: int2 tile order (blocked_tile_order)
 };
 // (ti, tj) pairs of the lower-triangle tiles in the blocked order used by
@@ -115,6 +115,9 @@ void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStr
 // accumulators; null or empty entries skipped) per launch, more in several.
 constexpr int kMaxUpdateGroup = 8;
 void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s);
+// FP64 in two steps as well: row expansion into hp.zbuf, then the update.
+size_t fp64_zbuf_bytes(int64_t k, int n);
+void launch_expand_fp64(const HerkProblem &hp, hipStream_t s);
 void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
 
 // Resident update loop (serving).  One job is one mixed update (one problem

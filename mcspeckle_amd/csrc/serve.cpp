@@ -113,7 +113,7 @@ int64_t request_footprint(int nv, int nl, int precision, bool near)
     const double cap = mixed ? 4.0e9 / (8.0 * n) : 2.5e9 / (16.0 * n);
     const double kchunk = std::max(65536.0, std::min(4.0e6, cap));
     const double cs = mixed ? 8.0 : 16.0;
-    double chunk = kchunk * ((mixed ? 8.0 * n : 0.0) + 2.0 * (nv + nl) * cs * 2.0 + 80.0);
+    double chunk = kchunk * ((mixed ? 8.0 : 16.0) * n + 2.0 * (nv + nl) * cs * 2.0 + 80.0);
     // near field: whole single-scatter rows, one per path (<= one per vertex)
     if (near) chunk += kchunk * n * cs;
     return (int64_t)(acc + chunk + 64.0 * 1024 * 1024);

@@ -120,6 +120,26 @@ def test_gpu_mccov_tile_edges(oracle, nv, nl, precision, tol):
         assert np.array_equal(C, C.conj().T)
 
 
+@pytest.mark.parametrize("nv,nl", [(8, 8), (9, 13)])
+def test_gpu_mccov_fp64_several_ksplits(oracle, nv, nl):
+    """Enough vertices (600 paths at optical depth 12) that the FP64 update
+    of one chunk runs as several k-splits, each a contiguous run of 8-vertex
+    stages of the expanded rows, the last one padded with zeros: equal to
+    the oracle to 1e-12."""
+    rng = np.random.default_rng(nv * 31 + nl)
+    l, v = far_dirs(nl, 15, rng), far_dirs(nv, 15, rng)
+    spec = E.make_problem(12.0, 0.97, [-50, -50, 0], [50, 50, 1], l, v, 1, 1, 1.0, True, 2, 3, 0.7, seed=11)
+    n = 600
+    ps = oracle.trace(spec, n)
+    nvert = int(ps["offset"][-1])
+    say("fp64 k-splits", nv, nl, "vertices", nvert)
+    assert nvert >= 2 * 2048   # at least two k-splits of >= 2048 vertices
+    Ms0, Mm0 = oracle.mccov_replay(spec, ps, n)
+    Ms, Mm = E.mccov_replay(spec, ps, n, "fp64")
+    assert cs_err(Ms, Ms0) <= 1e-12
+    assert cs_err(Mm, Mm0) <= 1e-12
+
+
 @pytest.mark.parametrize("kind,dim,cbs,smp,amp0", spec_cases())
 def test_gpu_mccov_full_run_same_seed(oracle, kind, dim, cbs, smp, amp0):
     spec = build(kind, dim, cbs, smp, amp0)
