@@ -128,21 +128,27 @@ void reaper_stop()
 // while its other slots kept work queued, and with 16 slots those
 // allocations held every slot's HIP calls for up to 2.3 s.
 namespace {
+struct IdleBlock {
+    int dev;   // the device the block was allocated on
+    void *p;
+};
 struct IdleCache {
     std::mutex mu;
-    std::multimap<size_t, void *> blocks;   // by size
+    std::multimap<size_t, IdleBlock> blocks;   // by size
     size_t bytes = 0;
 };
 IdleCache g_idle;
 constexpr size_t kIdleCap = (size_t)64 << 30;
 } // namespace
 
+// blocks go back on the device that is current, as they were allocated
 static void idle_put(void *p, size_t bytes)
 {
-    {
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
         std::lock_guard<std::mutex> g(g_idle.mu);
         if (g_idle.bytes + bytes <= kIdleCap) {
-            g_idle.blocks.emplace(bytes, p);
+            g_idle.blocks.emplace(bytes, IdleBlock{dev, p});
             g_idle.bytes += bytes;
             return;
         }
@@ -150,29 +156,41 @@ static void idle_put(void *p, size_t bytes)
     device_free(p);
 }
 
-// smallest idle block of at least want bytes, at most twice that
+// smallest idle block of the current device of at least want bytes, at most
+// twice that
 static void *idle_take(size_t want, size_t *got)
 {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> g(g_idle.mu);
-    auto it = g_idle.blocks.lower_bound(want);
-    if (it == g_idle.blocks.end() || it->first > 2 * want) return nullptr;
-    void *p = it->second;
-    *got = it->first;
-    g_idle.bytes -= it->first;
-    g_idle.blocks.erase(it);
-    return p;
+    for (auto it = g_idle.blocks.lower_bound(want); it != g_idle.blocks.end() && it->first <= 2 * want; ++it) {
+        if (it->second.dev != dev) continue;
+        void *p = it->second.p;
+        *got = it->first;
+        g_idle.bytes -= it->first;
+        g_idle.blocks.erase(it);
+        return p;
+    }
+    return nullptr;
 }
 
-// every idle block back to the driver (an allocation failed)
+// every idle block back to the driver (an allocation failed), each freed on
+// its own device
 static void idle_trim()
 {
-    std::multimap<size_t, void *> b;
+    std::multimap<size_t, IdleBlock> b;
     {
         std::lock_guard<std::mutex> g(g_idle.mu);
         b.swap(g_idle.blocks);
         g_idle.bytes = 0;
     }
-    for (const auto &kv : b) (void)hipFree(kv.second);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (const auto &kv : b) {
+        if (kv.second.dev != cur) (void)hipSetDevice(kv.second.dev);
+        (void)hipFree(kv.second.p);
+        if (kv.second.dev != cur) (void)hipSetDevice(cur);
+    }
 }
 
 thread_local BlockPool *tls_block_pool = nullptr;
