@@ -182,6 +182,17 @@ def c_problem(spec):
     return p
 
 
+def _cout(n):
+    """A complex output of n entries; _cp(a) passes it as the real pointer
+    with a NULL imaginary pointer, for which the C-ABI writes interleaved
+    (re, im) pairs (no separate real and imaginary arrays to combine)."""
+    return np.empty(n, dtype=np.complex128)
+
+
+def _cp(a):
+    return N.dptr(a.view(np.float64))
+
+
 def _prec(precision):
     if isinstance(precision, str):
         if precision not in N.PRECISIONS:
@@ -195,27 +206,26 @@ def mccov_spec(spec, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
     ne = (nv * nl) ** 2
-    ms_re, ms_im, mm_re, mm_im = (np.empty(ne) for _ in range(4))
+    ms, mm = _cout(ne), _cout(ne)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(L.scmc_mccov(C.byref(p), int(max_itr), _prec(precision), N.dptr(ms_re), N.dptr(ms_im),
-                         N.dptr(mm_re), N.dptr(mm_im), C.byref(st)))
+    N.check(L.scmc_mccov(C.byref(p), int(max_itr), _prec(precision), _cp(ms), None, _cp(mm), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
     shape = (nv, nv, nl, nl)
-    return ((ms_re + 1j * ms_im).reshape(shape, order="F"), (mm_re + 1j * mm_im).reshape(shape, order="F"))
+    return ms.reshape(shape, order="F"), mm.reshape(shape, order="F")
 
 
 def mcfield_spec(spec, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
-    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    u = _cout(nv * nl)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(L.scmc_mcfield(C.byref(p), int(max_itr), _prec(precision), N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+    N.check(L.scmc_mcfield(C.byref(p), int(max_itr), _prec(precision), _cp(u), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+    return u.reshape((nv, nl), order="F")
 
 
 def mcfield_noalloc_spec(spec, max_itr, precision="fp64", stats=None):
@@ -223,14 +233,13 @@ def mcfield_noalloc_spec(spec, max_itr, precision="fp64", stats=None):
     device buffer in the library's fixed scratch array (no device allocation)."""
     L = N.lib()
     nv, nl = spec.nv, spec.nl
-    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    u = _cout(nv * nl)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(L.scmc_mcfield_noalloc(C.byref(p), int(max_itr), _prec(precision), N.dptr(u_re), N.dptr(u_im),
-                                   C.byref(st)))
+    N.check(L.scmc_mcfield_noalloc(C.byref(p), int(max_itr), _prec(precision), _cp(u), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+    return u.reshape((nv, nl), order="F")
 
 
 def MCcov(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg, sct_type,
@@ -274,14 +283,14 @@ def MCfieldOnWave(sigt, albedo, box_min, box_max, l, v, is_ff_l, is_ff_v, maxItr
     if Wl is None:
         raise ScmcArgumentError("Wl (one weight per light) is required")
     wr, wi = _weights(Wl, spec.nl)
-    u_re, u_im = np.empty(spec.nv), np.empty(spec.nv)
+    u = _cout(spec.nv)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(N.lib().scmc_mcfield_wave(C.byref(p), N.dptr(wr), N.dptr(wi), int(maxItr), _prec(precision),
-                                      N.dptr(u_re), N.dptr(u_im), C.byref(st)))
+    N.check(N.lib().scmc_mcfield_wave(C.byref(p), N.dptr(wr), N.dptr(wi), int(maxItr), _prec(precision), _cp(u), None,
+                                      C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return u_re + 1j * u_im
+    return u
 
 
 def c_medium(spec, sigt, box_bin, albedo=None, lrad=None):
@@ -330,16 +339,15 @@ def MCCovHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_v, 
     m, keep = c_medium(spec, sig, box_bin, alb if alb.size > 1 else None, lrad)
     nv, nl = spec.nv, spec.nl
     ne, nf = nv * nv * nl * nl, nv * nl
-    bufs = [np.empty(ne), np.empty(ne), np.empty(ne), np.empty(ne), np.empty(nf), np.empty(nf)]
+    ms, mm, mean1 = _cout(ne), _cout(ne), _cout(nf)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(N.lib().scmc_mccov_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision),
-                                     *[N.dptr(b) for b in bufs], C.byref(st)))
+    N.check(N.lib().scmc_mccov_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision), _cp(ms), None, _cp(mm),
+                                     None, _cp(mean1), None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
     sh = (nv, nv, nl, nl)
-    return ((bufs[0] + 1j * bufs[1]).reshape(sh, order="F"), (bufs[2] + 1j * bufs[3]).reshape(sh, order="F"),
-            (bufs[4] + 1j * bufs[5]).reshape((nv, nl), order="F"))
+    return ms.reshape(sh, order="F"), mm.reshape(sh, order="F"), mean1.reshape((nv, nl), order="F")
 
 
 def MCsampleHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_v, maxItr, lambda_, doCBS, smpFlg,
@@ -357,14 +365,14 @@ def MCsampleHetro(sigt, albedo, box_min, box_max, box_bin, l, v, is_ff_l, is_ff_
         raise ScmcArgumentError("Invalid photons number")
     m, keep = c_medium(spec, sig, box_bin, alb if alb.size > 1 else None)
     nv, nl = spec.nv, spec.nl
-    re, im = np.empty(nv * nl), np.empty(nv * nl)
+    u = _cout(nv * nl)
     st = N.Stats()
     p = c_problem(spec)
-    N.check(N.lib().scmc_mcfield_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision), N.dptr(re), N.dptr(im),
+    N.check(N.lib().scmc_mcfield_hetro(C.byref(p), C.byref(m), int(maxItr), _prec(precision), _cp(u), None,
                                        C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (re + 1j * im).reshape((nv, nl), order="F")
+    return u.reshape((nv, nl), order="F")
 
 
 def quad_grid(box_min, box_max, v_max, v_stp, lambda_, wave=False):
@@ -465,30 +473,30 @@ def mccov_replay(spec, ps, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
     ne = (nv * nl) ** 2
-    ms_re, ms_im, mm_re, mm_im = (np.empty(ne) for _ in range(4))
+    ms, mm = _cout(ne), _cout(ne)
     p = c_problem(spec)
     cps = c_pathset(ps)
     st = N.Stats()
-    N.check(L.scmc_mccov_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(ms_re),
-                                N.dptr(ms_im), N.dptr(mm_re), N.dptr(mm_im), C.byref(st)))
+    N.check(L.scmc_mccov_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), _cp(ms), None, _cp(mm),
+                                None, C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
     shape = (nv, nv, nl, nl)
-    return ((ms_re + 1j * ms_im).reshape(shape, order="F"), (mm_re + 1j * mm_im).reshape(shape, order="F"))
+    return ms.reshape(shape, order="F"), mm.reshape(shape, order="F")
 
 
 def mcfield_replay(spec, ps, max_itr, precision="fp64", stats=None):
     L = N.lib()
     nv, nl = spec.nv, spec.nl
-    u_re, u_im = np.empty(nv * nl), np.empty(nv * nl)
+    u = _cout(nv * nl)
     p = c_problem(spec)
     cps = c_pathset(ps)
     st = N.Stats()
-    N.check(L.scmc_mcfield_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), N.dptr(u_re),
-                                  N.dptr(u_im), C.byref(st)))
+    N.check(L.scmc_mcfield_replay(C.byref(p), C.byref(cps), int(max_itr), _prec(precision), _cp(u), None,
+                                  C.byref(st)))
     if stats is not None:
         stats.update(st.as_dict())
-    return (u_re + 1j * u_im).reshape((nv, nl), order="F")
+    return u.reshape((nv, nl), order="F")
 
 
 class Session:
@@ -516,13 +524,13 @@ class Session:
         nv, nl = self.spec.nv, self.spec.nl
         if self.kind == "cov":
             ne = (nv * nl) ** 2
-            a = [np.empty(ne) for _ in range(4)]
-            N.check(N.lib().scmc_session_result(self.handle, int(max_itr), *[N.dptr(x) for x in a]))
+            ms, mm = _cout(ne), _cout(ne)
+            N.check(N.lib().scmc_session_result(self.handle, int(max_itr), _cp(ms), None, _cp(mm), None))
             shape = (nv, nv, nl, nl)
-            return (a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F")
-        re, im = np.empty(nv * nl), np.empty(nv * nl)
-        N.check(N.lib().scmc_session_result(self.handle, int(max_itr), N.dptr(re), N.dptr(im), None, None))
-        return (re + 1j * im).reshape((nv, nl), order="F")
+            return ms.reshape(shape, order="F"), mm.reshape(shape, order="F")
+        u = _cout(nv * nl)
+        N.check(N.lib().scmc_session_result(self.handle, int(max_itr), _cp(u), None, None, None))
+        return u.reshape((nv, nl), order="F")
 
     def close(self):
         if self.handle:
@@ -581,20 +589,19 @@ class Server:
         spec, kind = self._specs[rid]
         nv, nl = spec.nv, spec.nl
         ne = (nv * nl) ** 2 if kind == "cov" else nv * nl
-        a = [np.empty(ne) for _ in range(4 if kind == "cov" else 2)]
+        a = [_cout(ne) for _ in range(2 if kind == "cov" else 1)]
         st = N.Stats()
         fn = N.lib().scmc_wait if block else N.lib().scmc_poll
-        ptrs = [N.dptr(x) for x in a] + ([None, None] if kind != "cov" else [])
+        ptrs = [_cp(a[0]), None] + ([_cp(a[1]), None] if kind == "cov" else [None, None])
         rc = fn(int(rid), *ptrs, C.byref(st))
         if rc == N.SCMC_E_BUSY:
             return None
         self._specs.pop(rid, None)
         N.check(rc)
         if kind != "cov":
-            return (a[0] + 1j * a[1]).reshape((nv, nl), order="F"), st.as_dict()
+            return a[0].reshape((nv, nl), order="F"), st.as_dict()
         shape = (nv, nv, nl, nl)
-        return ((a[0] + 1j * a[1]).reshape(shape, order="F"), (a[2] + 1j * a[3]).reshape(shape, order="F"),
-                st.as_dict())
+        return a[0].reshape(shape, order="F"), a[1].reshape(shape, order="F"), st.as_dict()
 
     def poll(self, rid):
         """(Ms, Mm, stats) -- (u, stats) for a field request -- when finished,
