@@ -600,7 +600,7 @@ int CovEngine::init(const ProblemState &ps, bool mixed_, hipStream_t s)
     ms_herk = herk_flops = ms_expand = ms_accum = ms_nee = 0;
     herk_launches = 0;
     nee_done = nullptr;
-    if (order_n != n) order.release();
+    if (order_n != n || order_tile != tile) order.release();
     if (!nonfinite.ensure(sizeof(unsigned long long), &rc)) return rc;
     launch_zero(nonfinite.p, sizeof(unsigned long long), s);
     if (ps.het) {
@@ -852,11 +852,12 @@ int CovEngine::consume(hipStream_t sc)
     const int ksm = pick_ksplit(S.k2, n, tile), kss = pick_ksplit(S.np, n, tile);
     if ((rc = ensure_acc(acc_m, ks_m, std::max(ksm, ks_m), sc))) return rc;
     if ((rc = ensure_acc(acc_s, ks_s, std::max(kss, ks_s), sc))) return rc;
-    if (mixed && !order.p) {
+    if (!order.p) {
         // kept in order_host until the next N: the copy runs on the solve's
         // stream (a null-stream copy would wait for every other session)
         order_n = n;
-        order_host = blocked_tile_order((n + kTileMixed - 1) / kTileMixed);
+        order_tile = tile;
+        order_host = blocked_tile_order((n + tile - 1) / tile);
         if (!order.ensure(sizeof(int) * order_host.size(), &rc)) return rc;
         if (!hip_ok(hipMemcpyAsync(order.p, order_host.data(), sizeof(int) * order_host.size(), hipMemcpyHostToDevice,
                                    sc),

@@ -35,15 +35,6 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void tri_decode(int t, int &ti, int &tj)
-{
-    int a = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
-    while ((a + 1) * (a + 2) / 2 <= t) ++a;
-    while (a * (a + 1) / 2 > t) --a;
-    ti = a;
-    tj = t - a * (a + 1) / 2;
-}
-
 // Work item -> (k-split s, tile rank t).  Workgroups are dispatched to the 8
 // XCDs round robin (XCD = blockIdx % 8).  The splits run one after another
 // across the whole chip, so the expanded rows being read at any time are one
@@ -622,7 +613,9 @@ __global__ __launch_bounds__(256) void expand_fp64_kernel(int nv, int nl, int n,
 }
 
 // The Hermitian update over the expanded rows.  Work item = (k-split s,
-// lower-triangle tile); s takes a contiguous run of stages.  Each stage's
+// lower-triangle tile); s takes a contiguous run of stages.  Items are laid
+// out over the XCDs as the mixed update's (item_decode over the blocked tile
+// order), so the workgroups on one XCD share row panels in its L2.  Each stage's
 // rows go from HBM straight into LDS (global_load_lds_dwordx4: wave w moves
 // K-steps 2(w%2), 2(w%2)+1 of tile w/2 (rows, then columns), two 1 KB
 // chunks of 32 rows per K-step, lane order = LDS layout), NBUF - 1 stages
@@ -630,16 +623,16 @@ __global__ __launch_bounds__(256) void expand_fp64_kernel(int nv, int nl, int n,
 // wave computes a 32x32 block: Re C += Z Z^T and Im C += Z' Z^T with
 // Z' = (Im, -Re) formed from the partner K element.
 template <int NBUF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void herk_fp64_kernel(const double *__restrict__ Z, int n, int64_t nst,
-                                                        double2 *__restrict__ acc, int ksplit)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void herk_fp64_kernel(
+    const double *__restrict__ Z, int n, int64_t nst, double2 *__restrict__ acc, int ksplit,
+    const int2 *__restrict__ order, int ntile)
 {
     using namespace f64;
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int item = blockIdx.x;
-    const int s = item % ksplit;
-    int ti, tj;
-    tri_decode(item / ksplit, ti, tj);
+    int s, t;
+    item_decode(blockIdx.x, ksplit, ntile, s, t);
+    const int ti = order[t].x, tj = order[t].y;
     const int64_t sb = nst * s / ksplit, se = nst * (s + 1) / ksplit;
     const int nstage = (int)(se - sb);
     const int npad = pad_rows(n);
@@ -895,7 +888,7 @@ void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
     const int nt = ntiles(hp.n, kTileF64);
     const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
     hipLaunchKernelGGL(herk_fp64_kernel<f64::NBUF>, dim3(grid), dim3(256), 0, s, (const double *)hp.zbuf, hp.n,
-                       f64::nstages(hp.k), hp.acc, hp.ksplit);
+                       f64::nstages(hp.k), hp.acc, hp.ksplit, (const int2 *)hp.order, nt * (nt + 1) / 2);
 }
 
 // ------------------------------------------------------------ finaliser

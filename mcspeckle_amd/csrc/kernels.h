@@ -95,7 +95,7 @@ struct HerkProblem {
     const unsigned *bound; // mixed: float bits of max |u| (may be null)
     int ksplit;
     void *zbuf = nullptr;          // expanded rows: FP16 hi/lo (mixed_zbuf_bytes) or FP64 (fp64_zbuf_bytes)
-    const void *order = nullptr;   // mixed: int2 tile order (blocked_tile_order)
+    const void *order = nullptr;   // int2 tile order (blocked_tile_order) over the update's tile edge
 };
 // (ti, tj) pairs of the lower-triangle tiles in the blocked order used by
 // the mixed update, flattened.

@@ -212,7 +212,7 @@ struct CovEngine {
     double ms_herk = 0, herk_flops = 0, ms_expand = 0, ms_accum = 0, ms_nee = 0;
     int64_t herk_launches = 0;
     DevBuf acc_m, acc_s, order;
-    int order_n = 0;               // N the tile order was built for
+    int order_n = 0, order_tile = 0;   // N and tile edge the tile order was built for
     std::vector<int> order_host;   // its host copy (source of the async upload)
     DevBuf mean, mean_part;   // voxel medium: mean field (MCCovHetro's mean1), per-range partials
     DevBuf nonfinite;         // vertices with NaN/Inf factors in this solve (device counter)
