@@ -7,6 +7,9 @@
   1e-12 FP64) against the one-shot engine.
 * Admission by footprint: with a budget that fits one request at a time, the
   requests still all complete, one after another.
+* Under a budget that makes every session be released after its request
+  (its blocks going to the process-wide idle cache), results equal those of
+  a server that keeps its sessions, to 1e-12.
 * Cancel frees the slot and reports SCMC_E_CANCELLED.
 * Shared update launches (the chunks of concurrent requests in one
   matrix-core launch) give the results of separate launches, to 1e-12.
@@ -81,6 +84,26 @@ def test_shared_update_launches_match_separate_ones():
     for i, *_ in mix:
         Ms_a, Mm_a, _ = out[False][i]
         Ms_b, Mm_b, _ = out[True][i]
+        assert cs_err(Ms_b, Ms_a) <= 1e-12, i
+        assert cs_err(Mm_b, Mm_a) <= 1e-12, i
+
+
+def test_sessions_released_under_a_small_budget_give_the_same_results():
+    """A budget far below any request's footprint: requests run one at a time
+    (a request is admitted alone when nothing runs), and after each one the
+    slots keep more than half the budget, so every session is released and
+    its blocks go to the process-wide idle cache, from which the next
+    session takes them.  Results equal those of a server with the default
+    budget (sessions kept), to 1e-12, mixed and FP64 alike."""
+    out = {}
+    for budget in (0.0, 1e-3):
+        with E.Server(max_active=3, hbm_budget_gb=budget, paths_per_slice=500) as S:
+            ids = {i: S.submit(req(i, nv, nl, od, g, seed=i), n, prec) for i, nv, nl, od, g, n, prec in MIX}
+            say("submitted, budget", budget)
+            out[budget] = {i: S.wait(r) for i, r in ids.items()}
+    for i, *_ in MIX:
+        Ms_a, Mm_a, _ = out[0.0][i]
+        Ms_b, Mm_b, _ = out[1e-3][i]
         assert cs_err(Ms_b, Ms_a) <= 1e-12, i
         assert cs_err(Mm_b, Mm_a) <= 1e-12, i
 
