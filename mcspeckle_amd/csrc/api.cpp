@@ -871,8 +871,7 @@ int CovEngine::consume(hipStream_t sc)
     hs.acc = (double2 *)acc_s.p;
     hs.ksplit = kss;
     hs.order = order.p;
-    // mixed: one launch for both problems (ev[3]..ev[4]); FP64: the
-    // multiple-scattering launch between ev[3] and ev[4], then the single
+    // one launch for both problems, mixed or FP64 (ev[3]..ev[4])
     if (mixed && loop) {
         // serving with the resident loop: the chunk's two updates become jobs
         // once their inputs (factors, expansion, zeroed accumulators, tile
@@ -906,9 +905,8 @@ int CovEngine::consume(hipStream_t sc)
         if (mixed)
             launch_herk_mixed_update(hm, &hs, sc);
         else
-            launch_herk_fp64(hm, sc);
+            launch_herk_fp64(hm, &hs, sc);
         (void)hipEventRecord(S.ev[4], sc);
-        if (!mixed) launch_herk_fp64(hs, sc);
     }
     // the scale bound goes to pinned memory on the stream: a blocking copy on
     // the null stream would wait for the update in flight on the other stream
@@ -955,11 +953,11 @@ int CovEngine::harvest(CovSlot &S)
     }
     ms_nee += tn;
     ms_accum += tx + th + ts;
-    // the timed launch's rank-1 terms: both problems in mixed mode
-    const int64_t kt = S.k2 + (mixed ? S.np : 0);
+    // the timed launch's rank-1 terms: both problems
+    const int64_t kt = S.k2 + S.np;
     if (kt > 0) {
         ms_herk += th;
-        if (mixed) ms_expand += tx;
+        ms_expand += tx;
         herk_launches += 1;
         herk_flops += 8.0 * (double)n * (double)n * (double)kt;
     }

@@ -622,17 +622,40 @@ __global__ __launch_bounds__(256) void expand_fp64_kernel(int nv, int nl, int n,
 // ahead of the matrix cores, with one workgroup barrier per stage.  Each
 // wave computes a 32x32 block: Re C += Z Z^T and Im C += Z' Z^T with
 // Z' = (Im, -Re) formed from the partner K element.
+// One launch carries a chunk's two problems (multiple scattering, then the
+// short single-scattering items, which fill the workgroup slots the long
+// ones leave free at the end of the grid), each with its own rows,
+// accumulators and k-splits.
+struct F64Update {
+    const double *Z;       // expanded rows (fp64_zbuf_bytes)
+    double2 *acc;          // [ksplit][n][n] accumulator copies
+    const int2 *order;     // lower-triangle tiles in blocked order
+    int64_t nst;           // stages of the expanded rows
+    int n, ntile, ksplit;
+    int nitems;            // ntile * ksplit
+};
+struct F64Group {
+    F64Update u[2];
+    int count;
+};
+
 template <int NBUF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void herk_fp64_kernel(
-    const double *__restrict__ Z, int n, int64_t nst, double2 *__restrict__ acc, int ksplit,
-    const int2 *__restrict__ order, int ntile)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void herk_fp64_kernel(F64Group G)
 {
     using namespace f64;
     __shared__ __attribute__((aligned(16))) unsigned char lds[NBUF * STAGEB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int item = blockIdx.x;
+    const bool second = G.count > 1 && item >= G.u[0].nitems;
+    if (second) item -= G.u[0].nitems;
+    const F64Update U = second ? G.u[1] : G.u[0];
+    const double *__restrict__ Z = U.Z;
+    double2 *__restrict__ acc = U.acc;
+    const int n = U.n, ksplit = U.ksplit;
+    const int64_t nst = U.nst;
     int s, t;
-    item_decode(blockIdx.x, ksplit, ntile, s, t);
-    const int ti = order[t].x, tj = order[t].y;
+    item_decode(item, ksplit, U.ntile, s, t);
+    const int ti = U.order[t].x, tj = U.order[t].y;
     const int64_t sb = nst * s / ksplit, se = nst * (s + 1) / ksplit;
     const int nstage = (int)(se - sb);
     const int npad = pad_rows(n);
@@ -882,13 +905,26 @@ void launch_expand_fp64(const HerkProblem &hp, hipStream_t s)
                            (const double2 *)hp.A1, (const double2 *)hp.B1, nullptr, nullptr, (double2 *)hp.zbuf);
 }
 
-void launch_herk_fp64(const HerkProblem &hp, hipStream_t s)
+void launch_herk_fp64(const HerkProblem &a, const HerkProblem *b, hipStream_t s)
 {
-    if (hp.k <= 0) return;
-    const int nt = ntiles(hp.n, kTileF64);
-    const unsigned grid = (unsigned)(nt * (nt + 1) / 2 * hp.ksplit);
-    hipLaunchKernelGGL(herk_fp64_kernel<f64::NBUF>, dim3(grid), dim3(256), 0, s, (const double *)hp.zbuf, hp.n,
-                       f64::nstages(hp.k), hp.acc, hp.ksplit, (const int2 *)hp.order, nt * (nt + 1) / 2);
+    F64Group G{};
+    int64_t grid = 0;
+    for (const HerkProblem *p : {&a, b}) {
+        if (!p || p->k <= 0) continue;
+        F64Update &u = G.u[G.count++];
+        const int nt = ntiles(p->n, kTileF64);
+        u.Z = (const double *)p->zbuf;
+        u.acc = p->acc;
+        u.order = (const int2 *)p->order;
+        u.nst = f64::nstages(p->k);
+        u.n = p->n;
+        u.ntile = nt * (nt + 1) / 2;
+        u.ksplit = p->ksplit;
+        u.nitems = u.ntile * u.ksplit;
+        grid += u.nitems;
+    }
+    if (G.count == 0) return;
+    hipLaunchKernelGGL(herk_fp64_kernel<f64::NBUF>, dim3((unsigned)grid), dim3(256), 0, s, G);
 }
 
 // ------------------------------------------------------------ finaliser

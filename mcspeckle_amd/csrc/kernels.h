@@ -115,10 +115,12 @@ void launch_herk_mixed_update(const HerkProblem &a, const HerkProblem *b, hipStr
 // accumulators; null or empty entries skipped) per launch, more in several.
 constexpr int kMaxUpdateGroup = 8;
 void launch_herk_mixed_group(const HerkProblem *const *hp, int count, hipStream_t s);
-// FP64 in two steps as well: row expansion into hp.zbuf, then the update.
+// FP64 in two steps as well: row expansion into hp.zbuf, then the update,
+// which takes a second problem of the same N and tile order (b, may be null)
+// whose work items follow a's in the same grid.
 size_t fp64_zbuf_bytes(int64_t k, int n);
 void launch_expand_fp64(const HerkProblem &hp, hipStream_t s);
-void launch_herk_fp64(const HerkProblem &hp, hipStream_t s);
+void launch_herk_fp64(const HerkProblem &a, const HerkProblem *b, hipStream_t s);
 
 // Resident update loop (serving).  One job is one mixed update (one problem
 // of a chunk, as launch_herk_mixed_update would run it); the host writes its
