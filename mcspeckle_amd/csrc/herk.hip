@@ -612,6 +612,62 @@ __global__ __launch_bounds__(256) void expand_fp64_kernel(int nv, int nl, int n,
     }
 }
 
+// The same with one workgroup per stage for all rows: the stage's factors
+// (8 vertices x (Nv + Nl), both orders with CBS) are staged in LDS once, so
+// the rows' products read LDS instead of L2.  For factor sets of up to 64 KB.
+constexpr int kExpandFp64Lds = 64 * 1024;
+template <bool CBS>
+__global__ __launch_bounds__(256) void expand_fp64_stage_kernel(int nv, int nl, int n, int64_t k,
+                                                                const double2 *__restrict__ A1,
+                                                                const double2 *__restrict__ B1,
+                                                                const double2 *__restrict__ A2,
+                                                                const double2 *__restrict__ B2,
+                                                                double2 *__restrict__ Z)
+{
+    using namespace f64;
+    extern __shared__ double2 fd[];
+    double2 *a1 = fd, *b1 = a1 + VPS * nv, *a2 = b1 + VPS * nl, *b2 = a2 + VPS * nv;
+    const int npad = pad_rows(n);
+    const int64_t st = blockIdx.x;
+    for (int j = threadIdx.x; j < VPS * nv; j += blockDim.x) {
+        const int q = j / nv, v = j - q * nv;
+        const int64_t kk = st * VPS + q;
+        a1[j] = kk < k ? A1[kk * nv + v] : make_double2(0.0, 0.0);
+        if (CBS) a2[j] = kk < k ? A2[kk * nv + v] : make_double2(0.0, 0.0);
+    }
+    for (int j = threadIdx.x; j < VPS * nl; j += blockDim.x) {
+        const int q = j / nl, l = j - q * nl;
+        const int64_t kk = st * VPS + q;
+        b1[j] = kk < k ? B1[kk * nl + l] : make_double2(0.0, 0.0);
+        if (CBS) b2[j] = kk < k ? B2[kk * nl + l] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+        const bool row = i < n;
+        const int l = row ? i / nv : 0, v = row ? i - l * nv : 0;
+#pragma unroll
+        for (int t = 0; t < SKS; ++t) {
+            double2 u[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = 2 * t + h;
+                u[h] = make_double2(0.0, 0.0);
+                if (row) {
+                    u[h] = cmul(a1[q * nv + v], b1[q * nl + l]);
+                    if (CBS) {
+                        const double2 w = cmul(a2[q * nv + v], b2[q * nl + l]);
+                        u[h].x += w.x;
+                        u[h].y += w.y;
+                    }
+                }
+            }
+            double2 *z = Z + ((st * SKS + t) * (int64_t)npad + i) * 2;
+            z[0] = u[0];
+            z[1] = u[1];
+        }
+    }
+}
+
 // The Hermitian update over the expanded rows.  Work item = (k-split s,
 // lower-triangle tile); s takes a contiguous run of stages.  Items are laid
 // out over the XCDs as the mixed update's (item_decode over the blocked tile
@@ -895,6 +951,18 @@ size_t fp64_zbuf_bytes(int64_t k, int n) { return (size_t)f64::nstages(k) * (siz
 void launch_expand_fp64(const HerkProblem &hp, hipStream_t s)
 {
     if (hp.k <= 0) return;
+    const size_t lds = sizeof(double2) * f64::VPS * (size_t)(hp.nv + hp.nl) * (hp.A2 ? 2 : 1);
+    if (lds <= (size_t)kExpandFp64Lds) {
+        const dim3 g1((unsigned)f64::nstages(hp.k));
+        if (hp.A2)
+            hipLaunchKernelGGL(expand_fp64_stage_kernel<true>, g1, dim3(256), lds, s, hp.nv, hp.nl, hp.n, hp.k,
+                               (const double2 *)hp.A1, (const double2 *)hp.B1, (const double2 *)hp.A2,
+                               (const double2 *)hp.B2, (double2 *)hp.zbuf);
+        else
+            hipLaunchKernelGGL(expand_fp64_stage_kernel<false>, g1, dim3(256), lds, s, hp.nv, hp.nl, hp.n, hp.k,
+                               (const double2 *)hp.A1, (const double2 *)hp.B1, nullptr, nullptr, (double2 *)hp.zbuf);
+        return;
+    }
     const dim3 g((unsigned)f64::nstages(hp.k), (unsigned)((f64::pad_rows(hp.n) + 255) / 256));
     if (hp.A2)
         hipLaunchKernelGGL(expand_fp64_kernel<true>, g, dim3(256), 0, s, hp.nv, hp.nl, hp.n, hp.k,
